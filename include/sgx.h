@@ -1,0 +1,190 @@
+/*
+ * sgx.h - C ABI of libsgx, the MI355X (gfx950) HIP library behind the
+ * Sinograms.jl reconstruction hot path.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * an int status (SGX_OK = 0, negative on error) and records a message that
+ * sgx_last_error() returns.  Arrays follow Julia's column-major layout:
+ *   sino[ib, ia]        at ib + nb*ia
+ *   proj[is, it, ia]    at is + ns*(it + nt*ia)
+ *   image[ix, iy(, iz)] with x fastest.
+ *
+ * Pointer kinds.  Small per-view / per-axis arrays (angles, sin/cos tables,
+ * pixel-centre axes, filter response) are always host memory and are copied
+ * by the call.  The large arrays (sinogram / projections, image, mask) are
+ * device memory unless SGX_HOST is set in flags, in which case the library
+ * stages them through device buffers and copies the result back.
+ *
+ * Synchronisation.  Calls run on the context's stream and return after it
+ * has finished, matching the reference's synchronous semantics, unless
+ * SGX_ASYNC is set (device pointers only).
+ *
+ * Threading.  Contexts are independent; one context must not be used from
+ * two threads at once.
+ */
+#ifndef SGX_H
+#define SGX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct sgx_ctx sgx_ctx;
+typedef struct sgx_plan sgx_plan;
+
+/* status codes */
+enum {
+    SGX_OK = 0,
+    SGX_ERR_ARG = -1,         /* null pointer, negative size, bad enum           */
+    SGX_ERR_SIZE = -2,        /* reference throw("size mismatch"/"sino size")    */
+    SGX_ERR_VALUE = -3,       /* reference error(...), e.g. nb + extra > npad     */
+    SGX_ERR_DEVICE = -4,      /* HIP runtime failure                              */
+    SGX_ERR_UNSUPPORTED = -5  /* option not supported by this build               */
+};
+
+/* flags */
+#define SGX_HOST        0x1u   /* large arrays are host memory                    */
+#define SGX_ASYNC       0x2u   /* return without synchronising (device ptrs only) */
+#define SGX_Q_BF16      0x10u  /* filtered data held in bfloat16 (device ptrs)    */
+
+/* geometry kinds (the reference's SinoPar, SinoFanArc, SinoFanFlat,
+ * CtFanArc, CtFanFlat; src/geom/type2.jl:95-159, type3.jl:145-184) */
+enum {
+    SGX_SINO_PAR = 0,
+    SGX_SINO_FAN_ARC = 1,
+    SGX_SINO_FAN_FLAT = 2,
+    SGX_CT_FAN_ARC = 3,
+    SGX_CT_FAN_FLAT = 4
+};
+
+/* precision forms of a plan */
+enum {
+    SGX_PREC_FP32 = 0,        /* FP32 filter (FP32 MFMA), FP32 filtered storage  */
+    SGX_PREC_BF16 = 1         /* split-BF16x3 MFMA filter, BF16 filtered storage  */
+};
+
+/* ---- context ------------------------------------------------------------ */
+
+int sgx_version(void);
+int sgx_ctx_create(int device, sgx_ctx** out);
+void sgx_ctx_destroy(sgx_ctx* ctx);
+const char* sgx_last_error(const sgx_ctx* ctx);
+/* Run subsequent calls on an existing HIP stream (hipStream_t, may be NULL for
+ * the default stream).  The context never destroys a stream it did not make. */
+int sgx_ctx_set_stream(sgx_ctx* ctx, void* hip_stream);
+int sgx_ctx_synchronize(sgx_ctx* ctx);
+
+/* ---- backprojectors ------------------------------------------------------ */
+
+/* fbp_back_par!(image, sino, sinphi, cosphi, ds, offset, xc, yc, mask; ia_skip)
+ * Replaces src/fbp2/back-par.jl:206-253 (+ fbp_back_par_xy :276-315).
+ * sinphi, cosphi: length ceil(na / ia_skip) (already subset, as in the
+ * reference).  mask may be NULL (all pixels).  Pixels outside the mask are 0.
+ * Returns SGX_ERR_SIZE for the reference's throw("size mismatch"). */
+int sgx_back_par(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_t ia_skip,
+                 const double* sinphi, const double* cosphi, double ds, double offset,
+                 const double* xc, int64_t nx, const double* yc, int64_t ny,
+                 const uint8_t* mask, float* image, uint32_t flags);
+
+/* fbp_back_fan!(image, sino, sinb, cosb, dsd, dso, source_offset, is_arc,
+ *               ds, offset, xc, yc, mask; ia_skip)
+ * Replaces src/fbp2/back-fan.jl:215-254 (+ fbp_back_fan_xy :280-336). */
+int sgx_back_fan(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_t ia_skip,
+                 const double* sinb, const double* cosb, double dsd, double dso,
+                 double source_offset, int is_arc, double ds, double offset,
+                 const double* xc, int64_t nx, const double* yc, int64_t ny,
+                 const uint8_t* mask, float* image, uint32_t flags);
+
+/* cbct_back_fan!(image, proj, betas, dsd, dso, ds, dt, offset_s, offset_t,
+ *                is_arc, xc, yc, zc, mask; ia_skip)
+ * Replaces src/fbp3/cbct-back.jl:95-144 (+ cbct_back_fan_voxel :150-213).
+ * betas has length na (the full view list; ia_skip selects from it, as in
+ * the reference).  Per-view sin/cos are formed on the device in FP64.
+ * With SGX_Q_BF16, proj points to bfloat16 data. */
+int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                  int64_t ia_skip, const double* betas, double dsd, double dso,
+                  double ds, double dt, double offset_s, double offset_t, int is_arc,
+                  const double* xc, int64_t nx, const double* yc, int64_t ny,
+                  const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                  uint32_t flags);
+
+/* ---- row filter ------------------------------------------------------------ */
+
+/* fbp_sino_filter(sino, filter; extra)
+ * Replaces src/fbp/filter.jl:122-147.  in is (nb, ncols), ncols = product of
+ * the trailing dimensions; out is (nb + extra, ncols) with the extra rows 0.
+ * Computed as the Toeplitz contraction out[n,c] = sum_m g[(n-m) mod npad] in[m,c]
+ * with g = real(ifft(Hk)), on MFMA.  prec selects SGX_PREC_FP32 (FP32 output)
+ * or SGX_PREC_BF16 (split-BF16x3 products, FP32 accumulation, FP32 output).
+ * Returns SGX_ERR_VALUE for the reference's error("nb + extra > npad"). */
+int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
+                    const float* Hk, int64_t npad, int64_t extra, int prec,
+                    float* out, uint32_t flags);
+
+/* parker_weight_fan_short(nb, na, orbit, orbit_short, ar, gam, gamma_max)
+ * Replaces src/fbp/parker.jl:62-108.  Writes the (nb, na) Float32 weights,
+ * evaluated in FP64 on the device. */
+int sgx_parker_fan_short(sgx_ctx* ctx, int64_t nb, int64_t na, const double* ar,
+                         const double* gam, double gamma_max, float* out, uint32_t flags);
+
+/* ---- plans: the fused whole path ----------------------------------------- */
+
+/* Geometry and weighting of a plan (plan_fbp, src/fbp2/plan2.jl:106-138 and
+ * src/fbp3/plan3.jl:60-80).  Angles are the caller's _ar(rg) in radians. */
+typedef struct {
+    int32_t kind;          /* SGX_SINO_* or SGX_CT_*                              */
+    int32_t parker;        /* 1: short-scan Parker weights (parker.jl:62-108)     */
+    int64_t ns;            /* detector bins: nb (2-D) or ns (3-D)                 */
+    int64_t nt;            /* detector rows (1 for 2-D)                           */
+    int64_t na;            /* views                                               */
+    double ds;             /* bin spacing: rg.d (2-D) or rg.ds (3-D)              */
+    double dt;             /* row spacing (3-D; ignored for 2-D)                  */
+    double offset_s;       /* rg.offset or rg.offset_s                            */
+    double offset_t;       /* rg.offset_t (3-D)                                   */
+    double dsd, dso;       /* fan / cone distances (ignored for SGX_SINO_PAR)     */
+    double source_offset;  /* 2-D fan only (cbct_back ignores it, as reference)   */
+    double gamma_max;      /* _gamma_max(rg), for Parker                          */
+    double angle_weight;   /* _angle_weights(_ar(rg)) (d-angle.jl:209-220)        */
+} sgx_geom;
+
+/* Create a plan.  betas: na angles (radians).  Image grid: axes xc, yc (, zc
+ * for 3-D; pass nz = 1 and zc = NULL for 2-D) and an optional host mask.
+ * Hk: the npad filter response of fbp_filter (host, Float32).  prec:
+ * SGX_PREC_FP32 or SGX_PREC_BF16.  The plan owns device copies of all of
+ * these plus the filtered-data buffer. */
+int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
+                    const double* xc, int64_t nx, const double* yc, int64_t ny,
+                    const double* zc, int64_t nz, const uint8_t* mask,
+                    const float* Hk, int64_t npad, int prec, sgx_plan** out);
+void sgx_plan_destroy(sgx_plan* plan);
+
+/* Per-view weight (na Float32, host) multiplied into the filter's operand,
+ * e.g. parker_weight_par for a parallel-beam orbit that is not a multiple of
+ * 180 degrees (src/fbp/parker.jl:26-55).  NULL removes it. */
+int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview);
+
+/* fbp(plan, sino) / fdk(plan, proj): weight -> filter -> backproject.
+ * Replaces src/fbp2/fbp.jl:24-86 and src/fbp3/fdk.jl:65-81.  sino is
+ * (ns, nt, na) Float32 (nt = 1 for 2-D); image is (nx, ny, nz). */
+int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t flags);
+
+/* The filtered, weighted data of the last execute (device pointer, layout
+ * (ns, nt, na), Float32 or bfloat16 per the plan's precision). */
+const void* sgx_plan_filtered(const sgx_plan* plan);
+
+/* The (ns, nt, na) view weights the plan applies, as the reference's
+ * plan.view_weight broadcast to full size (host Float32 out). */
+int sgx_plan_view_weight(sgx_plan* plan, float* out);
+
+/* Device time (ms) of the backprojection and filter launches of the last
+ * execute, measured with HIP events on the plan's stream. */
+int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SGX_H */

@@ -1,0 +1,172 @@
+// K1 / K2: pixel-driven 2-D backprojection, parallel beam and fan beam
+// (arc and flat detectors), for gfx950.
+//
+// Replaces fbp_back_par_xy (src/fbp2/back-par.jl:276-315) and
+// fbp_back_fan_xy (src/fbp2/back-fan.jl:280-336).  Per pixel (x, y) / ds
+// and per view:
+//   parallel: bb = x cos + y sin + wb,                     w2 = 1
+//   fan:      d = dso + x sin - y cos,  r = x cos + y sin - source_offset
+//             arc:  bb = dsd atan(r, d) + wb,  w2 = dsd^2 / (d^2 + r^2)
+//             flat: m = dsd / d, bb = m r + wb, w2 = m^2
+//   il = floor(bb); if 1 <= il and il + 1 <= nb: add w2 * lerp(q[il], q[il+1])
+//
+// Layout.  A block owns a 16 x 16 pixel tile (one pixel per lane) and a
+// group of views.  The views of the group are streamed through LDS in chunks
+// (every view row of the chunk is read from HBM once per block, coalesced),
+// the two taps of each update are LDS reads, per-view cos/sin are
+// wave-uniform scalar loads and the sum is kept in a register.  When there
+// are too few pixel tiles to fill the GPU the views are split over G groups;
+// each group writes a partial image and a second kernel adds the G partials
+// in a fixed order (deterministic, no atomics).
+
+#include "sgx_internal.h"
+#include "sgx.h"
+
+namespace sgx {
+namespace {
+
+constexpr int kTile = 16;
+constexpr int kThreads = kTile * kTile;
+constexpr int kLdsBytes = 48 * 1024;
+
+int chunk_views(int nb) { return std::max(1, std::min(64, kLdsBytes / (4 * nb))); }
+
+template <int GEOM>  // 0 parallel, 1 fan arc, 2 fan flat
+__global__ __launch_bounds__(kThreads) void back2d_kernel(
+    const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ xds,
+    const float* __restrict__ yds, int nx, int ny, const uint8_t* __restrict__ mask,
+    float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, float wb,
+    float dsd_ds, float dso_ds, float so_ds)
+{
+    extern __shared__ float lds[];
+    const int tiles_x = (nx + kTile - 1) / kTile;
+    const int ix = (blockIdx.x % tiles_x) * kTile + (threadIdx.x % kTile);
+    const int iy = (blockIdx.x / tiles_x) * kTile + (threadIdx.x / kTile);
+    const int64_t pix = ix + (int64_t)nx * iy;
+    const bool inside = ix < nx && iy < ny;
+    const bool active = inside && (mask == nullptr || mask[pix]);
+    float* dst = out + out_group_stride * blockIdx.y;
+
+    if (!__syncthreads_or(active)) {          // whole tile outside the mask
+        if (inside) dst[pix] = 0.f;
+        return;
+    }
+
+    const int v_begin = blockIdx.y * views_per_group;
+    const int v_end = min(nviews, v_begin + views_per_group);
+    const float x = inside ? xds[ix] : 0.f;
+    const float y = inside ? yds[iy] : 0.f;
+    const float nb1 = (float)nb;               // valid: 1 <= bb < nb
+    const float dsd2 = dsd_ds * dsd_ds;
+    float acc = 0.f;
+
+    for (int c0 = v_begin; c0 < v_end; c0 += vc) {
+        const int nv = min(vc, v_end - c0);
+        __syncthreads();
+        for (int v = 0; v < nv; ++v) {
+            const float* src = q + view_stride * (int64_t)(c0 + v);
+            for (int b = threadIdx.x; b < nb; b += kThreads) lds[v * nb + b] = src[b];
+        }
+        __syncthreads();
+        if (active) {
+            for (int v = 0; v < nv; ++v) {
+                const ViewCS t = cs[c0 + v];
+                float bb, w2 = 1.f;
+                if (GEOM == 0) {
+                    bb = x * t.c + y * t.s + wb;
+                } else {
+                    const float d = dso_ds + x * t.s - y * t.c;
+                    const float r = x * t.c + y * t.s - so_ds;
+                    if (GEOM == 1) {
+                        bb = dsd_ds * atan2f(r, d) + wb;
+                        w2 = dsd2 / (d * d + r * r);
+                    } else {
+                        const float m = dsd_ds / d;
+                        bb = m * r + wb;
+                        w2 = m * m;
+                    }
+                }
+                // 1 <= floor(bb) <= nb - 1; out of range: clamped address, weight 0
+                const float w = (bb >= 1.f && bb < nb1) ? w2 : 0.f;
+                const float bbc = fminf(fmaxf(bb, 1.f), nb1 - 1.f);
+                const float ilf = floorf(bbc);
+                const float wr = bbc - ilf;
+                const float* p = lds + v * nb + ((int)ilf - 1);
+                const float a = p[0], b = p[1];
+                acc = fmaf(w, fmaf(wr, b - a, a), acc);
+            }
+        }
+    }
+    if (inside) dst[pix] = active ? acc : 0.f;
+}
+
+__global__ void sum_partials_kernel(const float* __restrict__ part, int groups, int64_t n,
+                                    const uint8_t* __restrict__ mask, float* __restrict__ image)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float s = 0.f;
+    for (int g = 0; g < groups; ++g) s += part[g * n + i];
+    image[i] = (mask == nullptr || mask[i]) ? s : 0.f;
+}
+
+int groups_for(int nb, int nviews, int nx, int ny)
+{
+    const int tiles = ((nx + kTile - 1) / kTile) * ((ny + kTile - 1) / kTile);
+    const int vc = chunk_views(nb);
+    const int max_groups = (nviews + vc - 1) / vc;
+    const int want = (2048 + tiles - 1) / tiles;
+    return std::max(1, std::min(want, max_groups));
+}
+
+}  // namespace
+
+int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny)
+{
+    const int g = groups_for(nb, nviews, nx, ny);
+    return g > 1 ? (int64_t)g * nx * ny : 0;
+}
+
+void launch_back2d(const Back2dArgs& a, hipStream_t st)
+{
+    if ((int64_t)a.nb * 4 > kLdsBytes)
+        throw Error(SGX_ERR_UNSUPPORTED, "back2d: nb > 12288 bins is not supported");
+    const int vc = chunk_views(a.nb);
+    int groups = groups_for(a.nb, a.nviews, a.nx, a.ny);
+    const int64_t npix = (int64_t)a.nx * a.ny;
+    if (groups > 1 && (a.partial == nullptr || a.partial_cap < groups * npix)) groups = 1;
+    // views per group: a whole number of LDS chunks
+    const int chunks = (a.nviews + vc - 1) / vc;
+    const int vpg = ((chunks + groups - 1) / groups) * vc;
+    groups = (a.nviews + vpg - 1) / vpg;
+    const int tiles = ((a.nx + kTile - 1) / kTile) * ((a.ny + kTile - 1) / kTile);
+    float* out = groups > 1 ? a.partial : a.image;
+    dim3 grid(tiles, groups), block(kThreads);
+    const size_t lds = (size_t)vc * a.nb * sizeof(float);
+    switch (a.geom) {
+    case 0:
+        hipLaunchKernelGGL(back2d_kernel<0>, grid, block, lds, st, a.q, a.nb, a.nviews,
+                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
+                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    case 1:
+        hipLaunchKernelGGL(back2d_kernel<1>, grid, block, lds, st, a.q, a.nb, a.nviews,
+                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
+                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    default:
+        hipLaunchKernelGGL(back2d_kernel<2>, grid, block, lds, st, a.q, a.nb, a.nviews,
+                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
+                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    }
+    SGX_HIP(hipGetLastError());
+    if (groups > 1) {
+        hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256),
+                           0, st, a.partial, groups, npix, a.mask, a.image);
+        SGX_HIP(hipGetLastError());
+    }
+}
+
+}  // namespace sgx

@@ -1,0 +1,318 @@
+// K4 + K5: ramp filtering of sinogram / projection rows as a batched Toeplitz
+// contraction on MFMA, with the pre-weights (fan sinogram weight, FDK
+// cylinder weight, Parker short-scan weight in FP64, angular weight) fused
+// into the operand load.  For gfx950.
+//
+// Replaces fbp_sino_filter (src/fbp/filter.jl:122-147): the reference zero-pads
+// every column to npad, multiplies its FFT by the real, even Hk and keeps the
+// real part of rows 1..nb.  That is exactly the circular convolution
+//   out[n, c] = sum_{m < nb} g[(n - m) mod npad] in[m, c],   g = real(ifft(Hk)),
+// i.e. out = T * B with the nb x nb Toeplitz matrix T[n, m] = g[(n-m) mod npad]
+// and B the (weighted) input, nb x ncols, column-major.
+//
+// Tiling.  A block (4 waves) owns 32 columns and a band of row tiles of 32;
+// each wave keeps RT 32x32 accumulators.  The input is streamed in chunks of
+// 64 rows (m): one coalesced pass per chunk from HBM, weighted, then staged in
+// LDS (FP32, or pre-split into bf16 hi / lo halves).  T is never stored: the
+// block keeps the band of g it needs in LDS and every A fragment is read from
+// it at index n - m.
+//   FP32 form:  v_mfma_f32_32x32x2_f32.
+//   BF16 form:  x = hi + lo with hi = bf16 truncation, lo = bf16(x - hi);
+//               T*B ~ Th*Bh + Th*Bl + Tl*Bh on v_mfma_f32_32x32x16_bf16 with
+//               FP32 accumulation (split-BF16x3).
+// Output rows n >= nb (the reference's `extra` rows) are written as 0.
+
+#include "sgx_internal.h"
+#include "sgx.h"
+
+namespace sgx {
+namespace {
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kKC = 64;            // input rows staged per chunk
+constexpr int kBPadF = kKC + 1;    // FP32 staging stride (conflict-free column reads)
+constexpr int kBPadH = kKC + 8;    // bf16 staging stride (16-byte aligned fragments)
+
+struct Params {
+    const float* in;
+    int nb;
+    int64_t ncols;
+    int nt;
+    const float* g;
+    int npad;
+    int nrows_out;          // nb + extra
+    void* out;
+    float scale;
+    const float* wbin;
+    const float* wview;
+    int cyl;
+    float cyl_dsd, cyl_dso;
+    const float* s_axis;
+    const float* t_axis;
+    const double* gam;
+    const double* bet;
+    double gamma_max;
+    int row_tiles_per_block;
+};
+
+__device__ __forceinline__ double parker_w(double gg, double bb, double gm)
+{
+    // parker_weight_fan_short (src/fbp/parker.jl:96-106), regions in the
+    // reference's order; 0 outside, and 0 at exactly bb == pi - 2 gg.
+    const double pi = 3.14159265358979323846;
+    double w = 0.0;
+    if (bb < 2.0 * (gm - gg)) {
+        const double s = sin(pi / 2.0 * (bb / (2.0 * (gm - gg))));
+        w = s * s;
+    } else if (bb < pi - 2.0 * gg) {
+        w = 1.0;
+    }
+    if (pi - 2.0 * gg < bb && bb <= pi + 2.0 * gm) {
+        const double s = sin(pi / 2.0 * ((pi + 2.0 * gm - bb) / (2.0 * (gm + gg))));
+        w = s * s;
+    }
+    return w;
+}
+
+__device__ __forceinline__ float weight(const Params& p, int m, int64_t c)
+{
+    float w = p.scale;
+    if (p.wbin) w *= p.wbin[m];
+    if (p.cyl) {
+        const int t = (int)(c % p.nt);
+        const float s = p.s_axis[m], tt = p.t_axis[t];
+        if (p.cyl == 1) {
+            w *= p.cyl_dso / sqrtf(s * s + tt * tt + p.cyl_dsd * p.cyl_dsd);
+        } else {
+            const float r = tt / p.cyl_dsd;
+            w *= (p.cyl_dso / p.cyl_dsd) * cosf(s / p.cyl_dsd) / sqrtf(r * r + 1.f);
+        }
+    }
+    if (p.wview) w *= p.wview[c / p.nt];
+    if (p.gam) w *= (float)parker_w(p.gam[m], p.bet[c / p.nt], p.gamma_max);
+    return w;
+}
+
+__device__ __forceinline__ __bf16 bf16_trunc(float x)
+{
+    return __builtin_bit_cast(__bf16, (uint16_t)(__float_as_uint(x) >> 16));
+}
+
+__device__ __forceinline__ __bf16 bf16_rne(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (uint16_t)(u >> 16));
+}
+
+__device__ __forceinline__ float bf16_to_f(__bf16 h)
+{
+    return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
+}
+
+template <bool SPLIT, bool OUT_BF16, int RT>
+__global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
+{
+    __shared__ float gl[4 * 8 * 32 + 2048 + 64];
+    __shared__ __attribute__((aligned(16))) unsigned char bstage[SPLIT ? 2 * 32 * kBPadH * 2
+                                                                        : 32 * kBPadF * 4];
+    float* bs_f = reinterpret_cast<float*>(bstage);
+    __bf16* bs_hi = reinterpret_cast<__bf16*>(bstage);
+    __bf16* bs_lo = bs_hi + 32 * kBPadH;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t c0 = (int64_t)blockIdx.x * 32;
+    const int tile0 = blockIdx.y * 4 * RT;              // first row tile of the block
+    const int n_lo = tile0 * 32;
+    const int kp = ((p.nb + kKC - 1) / kKC) * kKC;       // padded K
+    const int glen = 4 * RT * 32 + kp - 1;
+
+    // g band: gl[j] = g[(n_lo - (kp - 1) + j) mod npad];  T[n, m] = gl[n - n_lo - m + kp - 1]
+    for (int j = tid; j < glen; j += 256) {
+        int k = (n_lo - (kp - 1) + j) % p.npad;
+        if (k < 0) k += p.npad;
+        gl[j] = p.g[k];
+    }
+
+    floatx16 acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r)
+#pragma unroll
+        for (int i = 0; i < 16; ++i) acc[r][i] = 0.f;
+
+    const int col = lane & 31, half = lane >> 5;
+    const int scol = tid >> 3, skk = (tid & 7) * 8;      // staging: column, row offset
+
+    for (int m0 = 0; m0 < p.nb; m0 += kKC) {
+        __syncthreads();
+        {
+            const int64_t c = c0 + scol;
+            float v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int m = m0 + skk + e;
+                v[e] = (m < p.nb && c < p.ncols) ? p.in[m + (int64_t)p.nb * c] * weight(p, m, c)
+                                                 : 0.f;
+            }
+            if (SPLIT) {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const __bf16 h = bf16_trunc(v[e]);
+                    bs_hi[scol * kBPadH + skk + e] = h;
+                    bs_lo[scol * kBPadH + skk + e] = bf16_rne(v[e] - bf16_to_f(h));
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) bs_f[scol * kBPadF + skk + e] = v[e];
+            }
+        }
+        __syncthreads();
+
+        if (SPLIT) {
+#pragma unroll
+            for (int ks = 0; ks < kKC / 16; ++ks) {
+                const int kb = ks * 16 + half * 8;
+                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(bs_hi + col * kBPadH + kb);
+                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(bs_lo + col * kBPadH + kb);
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    const int n = (tile0 + wave * RT + r) * 32 + col;
+                    const float* ga = gl + (n - n_lo) - (m0 + kb) + kp - 1;
+                    bf16x8 ah, al;
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) {
+                        const float x = ga[-j];
+                        const __bf16 h = bf16_trunc(x);
+                        ah[j] = h;
+                        al[j] = bf16_rne(x - bf16_to_f(h));
+                    }
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[r], 0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[r], 0, 0, 0);
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[r], 0, 0, 0);
+                }
+            }
+        } else {
+#pragma unroll 4
+            for (int ks = 0; ks < kKC / 2; ++ks) {
+                const int k = ks * 2 + half;
+                const float b = bs_f[col * kBPadF + k];
+#pragma unroll
+                for (int r = 0; r < RT; ++r) {
+                    const int n = (tile0 + wave * RT + r) * 32 + col;
+                    const float a = gl[(n - n_lo) - (m0 + k) + kp - 1];
+                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[r], 0, 0, 0);
+                }
+            }
+        }
+    }
+
+    // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
+    const int64_t c = c0 + col;
+    if (c < p.ncols) {
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int nbase = (tile0 + wave * RT + r) * 32;
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
+                if (n < p.nrows_out) {
+                    const float v = n < p.nb ? acc[r][i] : 0.f;
+                    const int64_t o = n + (int64_t)p.nrows_out * c;
+                    if (OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
+                    else static_cast<float*>(p.out)[o] = v;
+                }
+            }
+        }
+    }
+}
+
+template <bool SPLIT, bool OUT_BF16, int RT>
+void launch_rt(const Params& p, hipStream_t st)
+{
+    const int row_tiles = (p.nrows_out + 31) / 32;
+    const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
+    dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
+    hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT_BF16, RT>), grid, dim3(256), 0, st, p);
+    SGX_HIP(hipGetLastError());
+}
+
+template <bool SPLIT, bool OUT_BF16>
+void launch_mode(const Params& p, hipStream_t st)
+{
+    const int row_tiles = (p.nrows_out + 31) / 32;
+    const int64_t col_blocks = (p.ncols + 31) / 32;
+    // Prefer one block per column band (the input is read and weighted once);
+    // split the rows when the columns alone cannot fill the GPU.
+    int rt = (row_tiles + 3) / 4;
+    if (col_blocks < 1024) rt = 1;
+    if (rt <= 1) launch_rt<SPLIT, OUT_BF16, 1>(p, st);
+    else if (rt <= 2) launch_rt<SPLIT, OUT_BF16, 2>(p, st);
+    else if (rt <= 4) launch_rt<SPLIT, OUT_BF16, 4>(p, st);
+    else launch_rt<SPLIT, OUT_BF16, 8>(p, st);
+}
+
+__global__ void parker_kernel(const double* __restrict__ gam, const double* __restrict__ bet,
+                              int nb, int na, double gm, float* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < (int64_t)nb * na) out[i] = (float)parker_w(gam[i % nb], bet[i / nb], gm);
+}
+
+__global__ void weights_kernel(Params p, float* __restrict__ out)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < (int64_t)p.nb * p.ncols) out[i] = weight(p, (int)(i % p.nb), i / p.nb);
+}
+
+Params make_params(const FilterArgs& a)
+{
+    Params p;
+    p.in = a.in; p.nb = a.nb; p.ncols = a.ncols; p.nt = a.nt; p.g = a.g; p.npad = a.npad;
+    p.nrows_out = a.nb + a.extra; p.out = a.out; p.scale = a.scale; p.wbin = a.wbin;
+    p.wview = a.wview;
+    p.cyl = a.cyl; p.cyl_dsd = a.cyl_dsd; p.cyl_dso = a.cyl_dso; p.s_axis = a.s_axis;
+    p.t_axis = a.t_axis; p.gam = a.gam; p.bet = a.bet; p.gamma_max = a.gamma_max;
+    p.row_tiles_per_block = 0;
+    return p;
+}
+
+}  // namespace
+
+void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st)
+{
+    const Params p = make_params(a);
+    const int64_t n = (int64_t)a.nb * a.ncols;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p,
+                       out);
+    SGX_HIP(hipGetLastError());
+}
+
+void launch_sino_filter(const FilterArgs& a, hipStream_t st)
+{
+    if (a.ncols <= 0) return;
+    if (a.nb > 2048)
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
+    const Params p = make_params(a);
+    if (a.split_bf16) {
+        if (a.out_bf16) launch_mode<true, true>(p, st);
+        else launch_mode<true, false>(p, st);
+    } else {
+        if (a.out_bf16) launch_mode<false, true>(p, st);
+        else launch_mode<false, false>(p, st);
+    }
+}
+
+void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,
+                   float* out, hipStream_t st)
+{
+    const int64_t n = (int64_t)nb * na;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(parker_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, gam,
+                       bet, nb, na, gamma_max, out);
+    SGX_HIP(hipGetLastError());
+}
+
+}  // namespace sgx

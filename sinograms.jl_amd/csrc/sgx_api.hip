@@ -1,0 +1,729 @@
+// C ABI of libsgx: contexts, argument checking, staging of host arrays,
+// per-call geometry tables and the fused plan (weight -> filter -> back).
+//
+// The argument checks reproduce the reference's error paths:
+//   throw("size mismatch")  src/fbp2/back-par.jl:220, back-fan.jl:232,
+//                           src/fbp3/cbct-back.jl:116        -> SGX_ERR_SIZE
+//   error("nb=... + extra=... > npad=...")  src/fbp/filter.jl:131 -> SGX_ERR_VALUE
+
+#include "sgx_internal.h"
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <sstream>
+
+using sgx::Error;
+
+struct sgx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    std::string err;
+    struct Buf {
+        void* p = nullptr;
+        size_t n = 0;
+    };
+    Buf ws[12];
+
+    // Grow-only device workspace, one slot per role.  Reuse is ordered by the
+    // context's stream; a grow waits for the stream before freeing.
+    void* get(int slot, size_t bytes)
+    {
+        Buf& b = ws[slot];
+        if (bytes == 0) return nullptr;
+        if (b.n < bytes) {
+            if (b.p) {
+                SGX_HIP(hipStreamSynchronize(stream));
+                SGX_HIP(hipFree(b.p));
+                b.p = nullptr;
+                b.n = 0;
+            }
+            SGX_HIP(hipMalloc(&b.p, bytes));
+            b.n = bytes;
+        }
+        return b.p;
+    }
+    ~sgx_ctx()
+    {
+        for (Buf& b : ws)
+            if (b.p) (void)hipFree(b.p);
+        if (own_stream && stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+enum Slot {
+    W_TABLES = 0,   // per-call view / axis tables
+    W_PARTIAL,      // 2-D view-group partial images
+    W_IN,           // staged input
+    W_OUT,          // staged output
+    W_MASK,         // staged mask
+    W_G,            // filter kernel g
+    W_EXTRA,
+};
+
+template <class F>
+int guard(sgx_ctx* ctx, F&& f)
+{
+    if (!ctx) return SGX_ERR_ARG;
+    try {
+        SGX_HIP(hipSetDevice(ctx->device));
+        f();
+        ctx->err.clear();
+        return SGX_OK;
+    } catch (const Error& e) {
+        ctx->err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        ctx->err = e.what();
+        return SGX_ERR_DEVICE;
+    }
+}
+
+void require(bool ok, int code, const char* msg)
+{
+    if (!ok) throw Error(code, msg);
+}
+
+// Host -> device copy of a small host table into a region of a workspace slot.
+struct TablePack {
+    std::vector<unsigned char> host;
+    size_t add(const void* src, size_t bytes)
+    {
+        const size_t off = (host.size() + 255) & ~size_t(255);
+        host.resize(off + bytes);
+        std::memcpy(host.data() + off, src, bytes);
+        return off;
+    }
+    unsigned char* upload(sgx_ctx* ctx, int slot)
+    {
+        auto* d = static_cast<unsigned char*>(ctx->get(slot, host.size()));
+        SGX_HIP(hipMemcpyAsync(d, host.data(), host.size(), hipMemcpyHostToDevice, ctx->stream));
+        // the pack's host memory must outlive the copy
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        return d;
+    }
+};
+
+std::vector<float> axis_over(const double* a, int64_t n, double ds)
+{
+    std::vector<float> v(n);
+    for (int64_t i = 0; i < n; ++i) v[i] = (float)((float)a[i] / (float)ds);
+    return v;
+}
+
+// Staging of a large array: device pointer as is, or a copy of host memory.
+const void* stage_in(sgx_ctx* ctx, int slot, const void* p, size_t bytes, uint32_t flags)
+{
+    if (!(flags & SGX_HOST) || p == nullptr) return p;
+    void* d = ctx->get(slot, bytes);
+    SGX_HIP(hipMemcpyAsync(d, p, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return d;
+}
+
+void finish(sgx_ctx* ctx, void* host_out, const void* dev_out, size_t bytes, uint32_t flags)
+{
+    if (flags & SGX_HOST) {
+        SGX_HIP(hipMemcpyAsync(host_out, dev_out, bytes, hipMemcpyDeviceToHost, ctx->stream));
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+    } else if (!(flags & SGX_ASYNC)) {
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+    }
+}
+
+std::vector<double> filter_kernel_g(const float* Hk, int64_t npad)
+{
+    // g = real(ifft(Hk)): g[k] = (1/npad) sum_j Hk[j] cos(2 pi j k / npad)
+    std::vector<double> cs(npad), g(npad);
+    const double w = 2.0 * M_PI / (double)npad;
+    for (int64_t j = 0; j < npad; ++j) cs[j] = std::cos(w * (double)j);
+    for (int64_t k = 0; k < npad; ++k) {
+        double acc = 0;
+        int64_t ph = 0;
+        for (int64_t j = 0; j < npad; ++j) {
+            acc += (double)Hk[j] * cs[ph];
+            ph += k;
+            if (ph >= npad) ph -= npad;
+        }
+        g[k] = acc / (double)npad;
+    }
+    return g;
+}
+
+void back2d_common(sgx_ctx* ctx, int geom, const float* sino, int64_t nb, int64_t na,
+                   int64_t ia_skip, const double* sinb, const double* cosb, double dsd,
+                   double dso, double source_offset, double ds, double offset, const double* xc,
+                   int64_t nx, const double* yc, int64_t ny, const uint8_t* mask, float* image,
+                   uint32_t flags)
+{
+    require(sino && sinb && cosb && xc && yc && image, SGX_ERR_ARG, "null pointer argument");
+    require(nb > 1 && na > 0 && nx > 0 && ny > 0 && ia_skip >= 1, SGX_ERR_ARG, "bad size");
+    require(nb < (1 << 30) && nx * ny < (int64_t(1) << 31), SGX_ERR_UNSUPPORTED, "too large");
+    require(ds != 0.0, SGX_ERR_ARG, "ds must be nonzero");
+    const int64_t nv = (na + ia_skip - 1) / ia_skip;
+
+    TablePack pack;
+    std::vector<sgx::ViewCS> cs(nv);
+    for (int64_t i = 0; i < nv; ++i) cs[i] = {(float)cosb[i], (float)sinb[i]};
+    const size_t o_cs = pack.add(cs.data(), cs.size() * sizeof(sgx::ViewCS));
+    const auto xv = axis_over(xc, nx, ds), yv = axis_over(yc, ny, ds);
+    const size_t o_x = pack.add(xv.data(), xv.size() * 4);
+    const size_t o_y = pack.add(yv.data(), yv.size() * 4);
+    unsigned char* d = pack.upload(ctx, W_TABLES);
+
+    const float* q = static_cast<const float*>(
+        stage_in(ctx, W_IN, sino, sizeof(float) * nb * na, flags));
+    const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nx * ny, flags));
+    float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nx * ny)) : image;
+
+    sgx::Back2dArgs a{};
+    a.q = q;
+    a.nb = (int)nb;
+    a.nviews = (int)nv;
+    a.view_stride = nb * ia_skip;
+    a.cs = reinterpret_cast<const sgx::ViewCS*>(d + o_cs);
+    a.xds = reinterpret_cast<const float*>(d + o_x);
+    a.yds = reinterpret_cast<const float*>(d + o_y);
+    a.nx = (int)nx;
+    a.ny = (int)ny;
+    a.mask = m;
+    a.image = img;
+    a.geom = geom;
+    a.wb = (float)((nb + 1) / 2.0 + (double)(float)offset);
+    a.dsd_ds = (float)dsd / (float)ds;
+    a.dso_ds = (float)dso / (float)ds;
+    a.so_ds = (float)source_offset / (float)ds;
+    const int64_t pf = sgx::back2d_partial_floats(a.nb, a.nviews, a.nx, a.ny);
+    a.partial = static_cast<float*>(ctx->get(W_PARTIAL, 4 * pf));
+    a.partial_cap = pf;
+    sgx::launch_back2d(a, ctx->stream);
+    finish(ctx, image, img, 4 * nx * ny, flags);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sgx_version(void) { return 100; }
+
+int sgx_ctx_create(int device, sgx_ctx** out)
+{
+    if (!out) return SGX_ERR_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return SGX_ERR_DEVICE;
+    auto* c = new (std::nothrow) sgx_ctx();
+    if (!c) return SGX_ERR_DEVICE;
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return SGX_ERR_DEVICE;
+    }
+    c->own_stream = true;
+    *out = c;
+    return SGX_OK;
+}
+
+void sgx_ctx_destroy(sgx_ctx* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx;
+}
+
+const char* sgx_last_error(const sgx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int sgx_ctx_set_stream(sgx_ctx* ctx, void* hip_stream)
+{
+    return guard(ctx, [&] {
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        if (ctx->own_stream && ctx->stream) SGX_HIP(hipStreamDestroy(ctx->stream));
+        ctx->stream = static_cast<hipStream_t>(hip_stream);
+        ctx->own_stream = false;
+    });
+}
+
+int sgx_ctx_synchronize(sgx_ctx* ctx)
+{
+    return guard(ctx, [&] { SGX_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+int sgx_back_par(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_t ia_skip,
+                 const double* sinphi, const double* cosphi, double ds, double offset,
+                 const double* xc, int64_t nx, const double* yc, int64_t ny,
+                 const uint8_t* mask, float* image, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        back2d_common(ctx, 0, sino, nb, na, ia_skip, sinphi, cosphi, 0, 0, 0, ds, offset, xc,
+                      nx, yc, ny, mask, image, flags);
+    });
+}
+
+int sgx_back_fan(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_t ia_skip,
+                 const double* sinb, const double* cosb, double dsd, double dso,
+                 double source_offset, int is_arc, double ds, double offset,
+                 const double* xc, int64_t nx, const double* yc, int64_t ny,
+                 const uint8_t* mask, float* image, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        back2d_common(ctx, is_arc ? 1 : 2, sino, nb, na, ia_skip, sinb, cosb, dsd, dso,
+                      source_offset, ds, offset, xc, nx, yc, ny, mask, image, flags);
+    });
+}
+
+int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                  int64_t ia_skip, const double* betas, double dsd, double dso,
+                  double ds, double dt, double offset_s, double offset_t, int is_arc,
+                  const double* xc, int64_t nx, const double* yc, int64_t ny,
+                  const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                  uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(proj && betas && xc && yc && zc && image, SGX_ERR_ARG, "null pointer argument");
+        require(ns > 1 && nt > 1 && na > 0 && nx > 0 && ny > 0 && nz > 0 && ia_skip >= 1,
+                SGX_ERR_ARG, "bad size");
+        require(ds != 0.0 && dt != 0.0, SGX_ERR_ARG, "ds and dt must be nonzero");
+        require(!((flags & SGX_Q_BF16) && (flags & SGX_HOST)), SGX_ERR_UNSUPPORTED,
+                "bf16 projections must be device memory");
+        require(nx * ny < (int64_t(1) << 31) / std::max<int64_t>(nz, 1), SGX_ERR_UNSUPPORTED,
+                "volume too large");
+        const int64_t nv = (na + ia_skip - 1) / ia_skip;
+
+        TablePack pack;
+        std::vector<double> b(nv);
+        for (int64_t i = 0; i < nv; ++i) b[i] = betas[i * ia_skip];
+        const size_t o_b = pack.add(b.data(), b.size() * 8);
+        const auto xv = axis_over(xc, nx, ds), yv = axis_over(yc, ny, ds),
+                   zv = axis_over(zc, nz, ds);
+        const size_t o_x = pack.add(xv.data(), xv.size() * 4);
+        const size_t o_y = pack.add(yv.data(), yv.size() * 4);
+        const size_t o_z = pack.add(zv.data(), zv.size() * 4);
+        const size_t o_cs = pack.add(b.data(), nv * sizeof(sgx::ViewCS));   // space only
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        auto* cs = reinterpret_cast<sgx::ViewCS*>(d + o_cs);
+        sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), nv, 1, cs, ctx->stream);
+
+        const size_t esz = (flags & SGX_Q_BF16) ? 2 : 4;
+        const void* q = stage_in(ctx, W_IN, proj, esz * ns * nt * na, flags);
+        const int64_t nvox = nx * ny * nz;
+        const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nvox, flags));
+        float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
+
+        sgx::CbctArgs a{};
+        a.q = q;
+        a.q_bf16 = (flags & SGX_Q_BF16) != 0;
+        a.ns = (int)ns;
+        a.nt = (int)nt;
+        a.nviews = (int)nv;
+        a.view_stride = ns * nt * ia_skip;
+        a.cs = cs;
+        a.xds = reinterpret_cast<const float*>(d + o_x);
+        a.yds = reinterpret_cast<const float*>(d + o_y);
+        a.zds = reinterpret_cast<const float*>(d + o_z);
+        a.nx = (int)nx;
+        a.ny = (int)ny;
+        a.nz = (int)nz;
+        a.mask = m;
+        a.image = img;
+        a.dsd_ds = (float)dsd / (float)ds;
+        a.dso_ds = (float)dso / (float)ds;
+        a.ws = (float)((ns + 1) / 2.0 + (double)(float)offset_s);
+        a.wt = (float)((nt + 1) / 2.0 + (double)(float)offset_t);
+        a.inv_dt_ds = 1.0f / ((float)dt / (float)ds);
+        a.is_arc = is_arc != 0;
+        sgx::launch_cbct_back(a, ctx->stream);
+        finish(ctx, image, img, 4 * nvox, flags);
+    });
+}
+
+int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
+                    const float* Hk, int64_t npad, int64_t extra, int prec,
+                    float* out, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(in && Hk && out, SGX_ERR_ARG, "null pointer argument");
+        require(nb > 0 && ncols >= 0 && npad > 0 && extra >= 0, SGX_ERR_ARG, "bad size");
+        require(prec == SGX_PREC_FP32 || prec == SGX_PREC_BF16, SGX_ERR_ARG, "bad precision");
+        if (nb + extra > npad) {
+            std::ostringstream os;
+            os << "nb=" << nb << " + extra=" << extra << " > npad=" << npad;
+            throw Error(SGX_ERR_VALUE, os.str());
+        }
+        const auto gd = filter_kernel_g(Hk, npad);
+        std::vector<float> gf(gd.begin(), gd.end());
+        float* g = static_cast<float*>(ctx->get(W_G, 4 * npad));
+        SGX_HIP(hipMemcpyAsync(g, gf.data(), 4 * npad, hipMemcpyHostToDevice, ctx->stream));
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        const int64_t nout = (nb + extra) * ncols;
+        const float* din = static_cast<const float*>(
+            stage_in(ctx, W_IN, in, 4 * nb * ncols, flags));
+        float* dout = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nout)) : out;
+
+        sgx::FilterArgs a{};
+        a.in = din;
+        a.nb = (int)nb;
+        a.ncols = ncols;
+        a.nt = 1;
+        a.g = g;
+        a.npad = (int)npad;
+        a.extra = (int)extra;
+        a.out = dout;
+        a.out_bf16 = false;
+        a.split_bf16 = prec == SGX_PREC_BF16;
+        a.scale = 1.0f;
+        sgx::launch_sino_filter(a, ctx->stream);
+        finish(ctx, out, dout, 4 * nout, flags);
+    });
+}
+
+int sgx_parker_fan_short(sgx_ctx* ctx, int64_t nb, int64_t na, const double* ar,
+                         const double* gam, double gamma_max, float* out, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(ar && gam && out, SGX_ERR_ARG, "null pointer argument");
+        require(nb > 0 && na > 1, SGX_ERR_ARG, "bad size");
+        // equal spacing check (parker.jl:73-74)
+        const double da0 = ar[1] - ar[0];
+        for (int64_t i = 1; i + 1 < na; ++i)
+            require(std::fabs((ar[i + 1] - ar[i]) - da0) <= 1e-4 * std::fabs(da0) + 1e-12,
+                    SGX_ERR_VALUE, "Unequal angles not yet supported");
+        double amin = ar[0];
+        for (int64_t i = 1; i < na; ++i) amin = std::min(amin, ar[i]);
+        TablePack pack;
+        std::vector<double> bet(na);
+        for (int64_t i = 0; i < na; ++i) bet[i] = std::max(ar[i] - amin, 0.0);
+        const size_t o_g = pack.add(gam, nb * 8);
+        const size_t o_b = pack.add(bet.data(), na * 8);
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        float* o = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nb * na)) : out;
+        sgx::launch_parker(reinterpret_cast<const double*>(d + o_g),
+                           reinterpret_cast<const double*>(d + o_b), (int)nb, (int)na, gamma_max,
+                           o, ctx->stream);
+        finish(ctx, out, o, 4 * nb * na, flags);
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// plans
+// ---------------------------------------------------------------------------
+
+struct sgx_plan {
+    sgx_ctx* ctx = nullptr;
+    sgx_geom geom{};
+    int prec = SGX_PREC_FP32;
+    int64_t nx = 0, ny = 0, nz = 1;
+    int64_t npad = 0;
+    void* tables = nullptr;       // device: cs, gam, bet, axes, g, wbin, wview, s/t axes
+    uint8_t* mask = nullptr;      // device (or null)
+    void* q = nullptr;            // filtered data (ns, nt, na)
+    float* partial = nullptr;
+    int64_t partial_cap = 0;
+    // table offsets
+    sgx::ViewCS* cs = nullptr;
+    double* gam = nullptr;
+    double* bet = nullptr;
+    float *xds = nullptr, *yds = nullptr, *zds = nullptr, *g = nullptr, *wbin = nullptr,
+          *wview = nullptr, *s_axis = nullptr, *t_axis = nullptr;
+    float* wview_owned = nullptr;
+    hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+    bool timed = false;
+
+    ~sgx_plan()
+    {
+        for (hipEvent_t& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (wview_owned) (void)hipFree(wview_owned);
+        if (tables) (void)hipFree(tables);
+        if (mask) (void)hipFree(mask);
+        if (q) (void)hipFree(q);
+        if (partial) (void)hipFree(partial);
+    }
+    bool is3d() const { return geom.kind == SGX_CT_FAN_ARC || geom.kind == SGX_CT_FAN_FLAT; }
+    bool is_arc() const { return geom.kind == SGX_SINO_FAN_ARC || geom.kind == SGX_CT_FAN_ARC; }
+
+    sgx::FilterArgs filter_args(const float* in, void* out) const
+    {
+        const sgx_geom& G = geom;
+        sgx::FilterArgs a{};
+        a.in = in;
+        a.nb = (int)G.ns;
+        a.nt = (int)G.nt;
+        a.ncols = G.nt * G.na;
+        a.g = g;
+        a.npad = (int)npad;
+        a.extra = 0;
+        a.out = out;
+        a.out_bf16 = prec == SGX_PREC_BF16;
+        a.split_bf16 = prec == SGX_PREC_BF16;
+        a.scale = (float)G.angle_weight;
+        a.wbin = wbin;
+        a.wview = wview;
+        if (is3d()) {
+            a.cyl = is_arc() ? 2 : 1;
+            a.cyl_dsd = (float)G.dsd;
+            a.cyl_dso = (float)G.dso;
+            a.s_axis = s_axis;
+            a.t_axis = t_axis;
+        }
+        if (G.parker) {
+            a.gam = gam;
+            a.bet = bet;
+            a.gamma_max = G.gamma_max;
+        }
+        return a;
+    }
+};
+
+extern "C" {
+
+int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
+                    const double* xc, int64_t nx, const double* yc, int64_t ny,
+                    const double* zc, int64_t nz, const uint8_t* mask,
+                    const float* Hk, int64_t npad, int prec, sgx_plan** out)
+{
+    return guard(ctx, [&] {
+        require(geom && betas && xc && yc && Hk && out, SGX_ERR_ARG, "null pointer argument");
+        *out = nullptr;
+        const sgx_geom& G = *geom;
+        require(G.kind >= SGX_SINO_PAR && G.kind <= SGX_CT_FAN_FLAT, SGX_ERR_ARG, "bad kind");
+        const bool is3d = G.kind >= SGX_CT_FAN_ARC;
+        require(G.ns > 1 && G.na > 0 && nx > 0 && ny > 0, SGX_ERR_ARG, "bad size");
+        require(!is3d || (zc && nz > 0 && G.nt > 1), SGX_ERR_ARG, "3-D plan needs zc, nz, nt");
+        require(is3d || G.nt == 1, SGX_ERR_ARG, "2-D plan needs nt == 1");
+        require(prec == SGX_PREC_FP32 || prec == SGX_PREC_BF16, SGX_ERR_ARG, "bad precision");
+        require(!(prec == SGX_PREC_BF16 && !is3d), SGX_ERR_UNSUPPORTED,
+                "the bf16 storage form is provided for cone-beam plans");
+        require(G.ds != 0.0 && (!is3d || G.dt != 0.0), SGX_ERR_ARG, "ds / dt must be nonzero");
+        if (G.ns > npad) {
+            std::ostringstream os;
+            os << "nb=" << G.ns << " + extra=0 > npad=" << npad;
+            throw Error(SGX_ERR_VALUE, os.str());
+        }
+        auto P = std::make_unique<sgx_plan>();
+        P->ctx = ctx;
+        P->geom = G;
+        P->prec = prec;
+        P->nx = nx;
+        P->ny = ny;
+        P->nz = is3d ? nz : 1;
+        P->npad = npad;
+
+        const int64_t ns = G.ns, nt = G.nt, na = G.na;
+        TablePack pack;
+        std::vector<double> b(betas, betas + na);
+        const size_t o_b = pack.add(b.data(), na * 8);
+        const size_t o_cs = pack.add(b.data(), na * sizeof(sgx::ViewCS));
+        const auto xv = axis_over(xc, nx, G.ds), yv = axis_over(yc, ny, G.ds);
+        const size_t o_x = pack.add(xv.data(), nx * 4);
+        const size_t o_y = pack.add(yv.data(), ny * 4);
+        size_t o_z = 0;
+        if (is3d) {
+            const auto zv = axis_over(zc, nz, G.ds);
+            o_z = pack.add(zv.data(), nz * 4);
+        }
+        const auto gd = filter_kernel_g(Hk, npad);
+        std::vector<float> gf(gd.begin(), gd.end());
+        const size_t o_g = pack.add(gf.data(), npad * 4);
+        // detector axis s (and t), fan angles gamma
+        std::vector<double> s(ns), gamv(ns);
+        for (int64_t i = 0; i < ns; ++i)
+            s[i] = G.ds * ((double)i - ((ns - 1) / 2.0 + (double)(float)G.offset_s));
+        for (int64_t i = 0; i < ns; ++i) gamv[i] = (G.kind == SGX_SINO_FAN_ARC || G.kind == SGX_CT_FAN_ARC)
+                                                       ? s[i] / G.dsd : std::atan(s[i] / G.dsd);
+        size_t o_wbin = 0, o_gam = 0, o_bet = 0, o_s = 0, o_t = 0;
+        if (G.kind == SGX_SINO_FAN_ARC || G.kind == SGX_SINO_FAN_FLAT) {
+            // fbp_sino_weight (src/fbp2/sino-weight.jl:12-16)
+            std::vector<float> w(ns);
+            for (int64_t i = 0; i < ns; ++i)
+                w[i] = (float)(std::fabs(G.dso * std::cos(gamv[i]) -
+                                         G.source_offset * std::sin(gamv[i])) / G.dsd);
+            o_wbin = pack.add(w.data(), ns * 4);
+        }
+        if (G.parker) {
+            require(G.kind != SGX_SINO_PAR, SGX_ERR_ARG,
+                    "parallel-beam Parker weights are passed per view");
+            double amin = betas[0];
+            for (int64_t i = 1; i < na; ++i) amin = std::min(amin, betas[i]);
+            std::vector<double> bet(na);
+            for (int64_t i = 0; i < na; ++i) bet[i] = std::max(betas[i] - amin, 0.0);
+            o_gam = pack.add(gamv.data(), ns * 8);
+            o_bet = pack.add(bet.data(), na * 8);
+        }
+        if (is3d) {
+            std::vector<float> sf(ns), tf(nt);
+            for (int64_t i = 0; i < ns; ++i) sf[i] = (float)s[i];
+            for (int64_t i = 0; i < nt; ++i)
+                tf[i] = (float)(G.dt * ((double)i - ((nt - 1) / 2.0 + (double)(float)G.offset_t)));
+            o_s = pack.add(sf.data(), ns * 4);
+            o_t = pack.add(tf.data(), nt * 4);
+        }
+        SGX_HIP(hipMalloc(&P->tables, pack.host.size()));
+        SGX_HIP(hipMemcpy(P->tables, pack.host.data(), pack.host.size(), hipMemcpyHostToDevice));
+        auto* d = static_cast<unsigned char*>(P->tables);
+        P->cs = reinterpret_cast<sgx::ViewCS*>(d + o_cs);
+        P->xds = reinterpret_cast<float*>(d + o_x);
+        P->yds = reinterpret_cast<float*>(d + o_y);
+        if (is3d) P->zds = reinterpret_cast<float*>(d + o_z);
+        P->g = reinterpret_cast<float*>(d + o_g);
+        if (o_wbin) P->wbin = reinterpret_cast<float*>(d + o_wbin);
+        if (G.parker) {
+            P->gam = reinterpret_cast<double*>(d + o_gam);
+            P->bet = reinterpret_cast<double*>(d + o_bet);
+        }
+        if (is3d) {
+            P->s_axis = reinterpret_cast<float*>(d + o_s);
+            P->t_axis = reinterpret_cast<float*>(d + o_t);
+        }
+        sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), na, 1, P->cs,
+                              ctx->stream);
+        const int64_t nvox = nx * ny * P->nz;
+        if (mask) {
+            SGX_HIP(hipMalloc(&P->mask, nvox));
+            SGX_HIP(hipMemcpy(P->mask, mask, nvox, hipMemcpyHostToDevice));
+        }
+        const size_t esz = prec == SGX_PREC_BF16 ? 2 : 4;
+        SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na));
+        if (!is3d) {
+            P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny);
+            if (P->partial_cap) SGX_HIP(hipMalloc(&P->partial, 4 * P->partial_cap));
+        }
+        for (hipEvent_t& e : P->ev) SGX_HIP(hipEventCreate(&e));
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        *out = P.release();
+    });
+}
+
+void sgx_plan_destroy(sgx_plan* plan)
+{
+    if (!plan) return;
+    (void)hipSetDevice(plan->ctx->device);
+    (void)hipStreamSynchronize(plan->ctx->stream);
+    delete plan;
+}
+
+int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview)
+{
+    if (!plan) return SGX_ERR_ARG;
+    return guard(plan->ctx, [&] {
+        sgx_plan& P = *plan;
+        const int64_t na = P.geom.na;
+        if (!wview) {
+            P.wview = nullptr;
+            return;
+        }
+        if (!P.wview_owned) {
+            void* p = nullptr;
+            SGX_HIP(hipMalloc(&p, 4 * na));
+            P.wview_owned = static_cast<float*>(p);
+        }
+        SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
+        SGX_HIP(hipMemcpy(P.wview_owned, wview, 4 * na, hipMemcpyHostToDevice));
+        P.wview = P.wview_owned;
+    });
+}
+
+int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t flags)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(sino && image, SGX_ERR_ARG, "null pointer argument");
+        sgx_plan& P = *plan;
+        const sgx_geom& G = P.geom;
+        const int64_t nin = G.ns * G.nt * G.na, nvox = P.nx * P.ny * P.nz;
+        const float* in = static_cast<const float*>(stage_in(ctx, W_IN, sino, 4 * nin, flags));
+        float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
+        hipStream_t st = ctx->stream;
+
+        SGX_HIP(hipEventRecord(P.ev[0], st));
+        sgx::launch_sino_filter(P.filter_args(in, P.q), st);
+        SGX_HIP(hipEventRecord(P.ev[1], st));
+        if (P.is3d()) {
+            sgx::CbctArgs a{};
+            a.q = P.q;
+            a.q_bf16 = P.prec == SGX_PREC_BF16;
+            a.ns = (int)G.ns;
+            a.nt = (int)G.nt;
+            a.nviews = (int)G.na;
+            a.view_stride = G.ns * G.nt;
+            a.cs = P.cs;
+            a.xds = P.xds;
+            a.yds = P.yds;
+            a.zds = P.zds;
+            a.nx = (int)P.nx;
+            a.ny = (int)P.ny;
+            a.nz = (int)P.nz;
+            a.mask = P.mask;
+            a.image = img;
+            a.dsd_ds = (float)G.dsd / (float)G.ds;
+            a.dso_ds = (float)G.dso / (float)G.ds;
+            a.ws = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+            a.wt = (float)((G.nt + 1) / 2.0 + (double)(float)G.offset_t);
+            a.inv_dt_ds = 1.0f / ((float)G.dt / (float)G.ds);
+            a.is_arc = P.is_arc();
+            sgx::launch_cbct_back(a, st);
+        } else {
+            sgx::Back2dArgs a{};
+            a.q = static_cast<const float*>(P.q);
+            a.nb = (int)G.ns;
+            a.nviews = (int)G.na;
+            a.view_stride = G.ns;
+            a.cs = P.cs;
+            a.xds = P.xds;
+            a.yds = P.yds;
+            a.nx = (int)P.nx;
+            a.ny = (int)P.ny;
+            a.mask = P.mask;
+            a.image = img;
+            a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+            a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+            a.dsd_ds = (float)G.dsd / (float)G.ds;
+            a.dso_ds = (float)G.dso / (float)G.ds;
+            a.so_ds = (float)G.source_offset / (float)G.ds;
+            a.partial = P.partial;
+            a.partial_cap = P.partial_cap;
+            sgx::launch_back2d(a, st);
+        }
+        SGX_HIP(hipEventRecord(P.ev[2], st));
+        P.timed = true;
+        finish(ctx, image, img, 4 * nvox, flags);
+    });
+}
+
+const void* sgx_plan_filtered(const sgx_plan* plan) { return plan ? plan->q : nullptr; }
+
+int sgx_plan_view_weight(sgx_plan* plan, float* out)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(out, SGX_ERR_ARG, "null pointer argument");
+        const sgx_geom& G = plan->geom;
+        const int64_t n = G.ns * G.nt * G.na;
+        float* d = static_cast<float*>(ctx->get(W_OUT, 4 * n));
+        sgx::launch_filter_weights(plan->filter_args(nullptr, nullptr), d, ctx->stream);
+        SGX_HIP(hipMemcpyAsync(out, d, 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms)
+{
+    if (!plan) return SGX_ERR_ARG;
+    return guard(plan->ctx, [&] {
+        require(plan->timed, SGX_ERR_ARG, "no execute recorded yet");
+        SGX_HIP(hipEventSynchronize(plan->ev[2]));
+        float a = 0, b = 0;
+        SGX_HIP(hipEventElapsedTime(&a, plan->ev[0], plan->ev[1]));
+        SGX_HIP(hipEventElapsedTime(&b, plan->ev[1], plan->ev[2]));
+        if (filter_ms) *filter_ms = a;
+        if (back_ms) *back_ms = b;
+    });
+}
+
+}  // extern "C"

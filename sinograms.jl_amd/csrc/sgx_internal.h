@@ -1,0 +1,114 @@
+// Internal declarations shared by the libsgx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "sgx.h"
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace sgx {
+
+// Error carrying an sgx status code; converted to a return value at the C ABI.
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define SGX_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::sgx::Error(SGX_ERR_DEVICE, std::string(#call) + ": " +             \
+                                                   hipGetErrorString(e_));             \
+    } while (0)
+
+// Per-view geometry of the backprojectors, formed on the device in FP64 and
+// stored as FP32 for the inner loops.
+struct ViewCS {
+    float c, s;
+};
+
+// --- launch wrappers (defined in the .hip kernel files) -------------------
+
+// Per-view cos/sin from angles (FP64 on the device).
+void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* out,
+                      hipStream_t st);
+
+// K3: cone-beam voxel-driven backprojection.
+struct CbctArgs {
+    const void* q;          // (ns, nt, na) float or bf16
+    bool q_bf16;
+    int ns, nt;
+    int nviews;             // views used (after ia_skip)
+    int64_t view_stride;    // elements between used views
+    const ViewCS* cs;       // nviews
+    const float* xds;       // nx, x / ds
+    const float* yds;       // ny
+    const float* zds;       // nz
+    int nx, ny, nz;
+    const uint8_t* mask;    // nx*ny*nz or null
+    float* image;
+    float dsd_ds, dso_ds, ws, wt, inv_dt_ds;
+    bool is_arc;
+};
+void launch_cbct_back(const CbctArgs& a, hipStream_t st);
+
+// K1/K2: 2-D pixel-driven backprojection with view chunks in LDS.
+struct Back2dArgs {
+    const float* q;         // (nb, na) float
+    int nb;
+    int nviews;             // views used (after ia_skip)
+    int64_t view_stride;    // elements between used views
+    const ViewCS* cs;       // nviews
+    const float* xds;       // nx
+    const float* yds;       // ny
+    int nx, ny;
+    const uint8_t* mask;    // nx*ny or null
+    float* image;
+    int geom;               // 0 par, 1 fan arc, 2 fan flat
+    float wb;               // (nb+1)/2 + offset
+    float dsd_ds, dso_ds, so_ds;
+    float* partial;         // workspace for view-group partial images (may be null)
+    int64_t partial_cap;    // floats available in partial
+};
+void launch_back2d(const Back2dArgs& a, hipStream_t st);
+int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny);
+
+// K4 (+K5): Toeplitz row filter with fused weights on MFMA.
+struct FilterArgs {
+    const float* in;        // (nb, ncols)
+    int nb;
+    int64_t ncols;
+    int nt;                 // rows per view (ncols = nt * na); 1 for 2-D
+    const float* g;         // npad, g = real(ifft(Hk))
+    int npad;
+    int extra;              // zero rows appended to each output column
+    void* out;              // (nb + extra, ncols), float or bf16
+    bool out_bf16;
+    bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
+    // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
+    float scale;
+    const float* wbin;      // nb or null
+    const float* wview;     // na or null (per-view weight, e.g. parallel-beam Parker)
+    int cyl;                // 0 none, 1 flat, 2 arc
+    float cyl_dsd, cyl_dso;
+    const float* s_axis;    // nb (cyl)
+    const float* t_axis;    // nt (cyl)
+    const double* gam;      // nb (parker) or null
+    const double* bet;      // na, beta - min(beta), clamped >= 0 (parker)
+    double gamma_max;
+};
+void launch_sino_filter(const FilterArgs& a, hipStream_t st);
+// The fused pre-weights of a filter launch, (nb, ncols) FP32.
+void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st);
+
+// Parker weights (nb, na) in FP64 on the device, stored FP32.
+void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,
+                   float* out, hipStream_t st);
+
+}  // namespace sgx

@@ -1,0 +1,160 @@
+"""ctypes binding of libsgx.so (the C ABI declared in include/sgx.h).
+
+The library is the only compute path: if it cannot be loaded, every call
+raises.  There is no CPU fallback.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("SGX_LIB", os.path.join(os.path.dirname(_PKG), "libsgx.so"))
+
+SGX_OK = 0
+SGX_ERR_ARG = -1
+SGX_ERR_SIZE = -2
+SGX_ERR_VALUE = -3
+SGX_ERR_DEVICE = -4
+SGX_ERR_UNSUPPORTED = -5
+
+SGX_HOST = 0x1
+SGX_ASYNC = 0x2
+SGX_Q_BF16 = 0x10
+
+SGX_SINO_PAR = 0
+SGX_SINO_FAN_ARC = 1
+SGX_SINO_FAN_FLAT = 2
+SGX_CT_FAN_ARC = 3
+SGX_CT_FAN_FLAT = 4
+
+SGX_PREC_FP32 = 0
+SGX_PREC_BF16 = 1
+
+# Every symbol include/sgx.h declares (checked by the CPU test suite).
+EXPORTS = (
+    "sgx_version", "sgx_ctx_create", "sgx_ctx_destroy", "sgx_last_error",
+    "sgx_ctx_set_stream", "sgx_ctx_synchronize", "sgx_back_par", "sgx_back_fan",
+    "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
+    "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
+    "sgx_plan_view_weight", "sgx_plan_last_times",
+)
+
+
+class SgxGeom(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32), ("parker", ctypes.c_int32),
+        ("ns", ctypes.c_int64), ("nt", ctypes.c_int64), ("na", ctypes.c_int64),
+        ("ds", ctypes.c_double), ("dt", ctypes.c_double),
+        ("offset_s", ctypes.c_double), ("offset_t", ctypes.c_double),
+        ("dsd", ctypes.c_double), ("dso", ctypes.c_double),
+        ("source_offset", ctypes.c_double), ("gamma_max", ctypes.c_double),
+        ("angle_weight", ctypes.c_double),
+    ]
+
+
+class SgxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def load():
+    """Load libsgx.so, declaring argument types.  Raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"libsgx.so not found at {LIB_PATH}; build it with "
+                "`python sinograms.jl_amd/build.py` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        _declare(L)
+        _lib = L
+    return _lib
+
+
+def _declare(L):
+    i64, f64, i32, u32 = ctypes.c_int64, ctypes.c_double, ctypes.c_int, ctypes.c_uint32
+    vp, cp = ctypes.c_void_p, ctypes.c_char_p
+    L.sgx_version.restype = i32
+    L.sgx_ctx_create.argtypes = [i32, ctypes.POINTER(vp)]
+    L.sgx_ctx_destroy.argtypes = [vp]
+    L.sgx_ctx_destroy.restype = None
+    L.sgx_last_error.argtypes = [vp]
+    L.sgx_last_error.restype = cp
+    L.sgx_ctx_set_stream.argtypes = [vp, vp]
+    L.sgx_ctx_synchronize.argtypes = [vp]
+    L.sgx_back_par.argtypes = [vp, vp, i64, i64, i64, vp, vp, f64, f64, vp, i64, vp, i64,
+                               vp, vp, u32]
+    L.sgx_back_fan.argtypes = [vp, vp, i64, i64, i64, vp, vp, f64, f64, f64, i32, f64, f64,
+                               vp, i64, vp, i64, vp, vp, u32]
+    L.sgx_cbct_back.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64, f64,
+                                i32, vp, i64, vp, i64, vp, i64, vp, vp, u32]
+    L.sgx_sino_filter.argtypes = [vp, vp, i64, i64, vp, i64, i64, i32, vp, u32]
+    L.sgx_parker_fan_short.argtypes = [vp, i64, i64, vp, vp, f64, vp, u32]
+    L.sgx_plan_create.argtypes = [vp, ctypes.POINTER(SgxGeom), vp, vp, i64, vp, i64, vp, i64,
+                                  vp, vp, i64, i32, ctypes.POINTER(vp)]
+    L.sgx_plan_destroy.argtypes = [vp]
+    L.sgx_plan_destroy.restype = None
+    L.sgx_plan_set_view_weight.argtypes = [vp, vp]
+    L.sgx_plan_execute.argtypes = [vp, vp, vp, u32]
+    L.sgx_plan_filtered.argtypes = [vp]
+    L.sgx_plan_filtered.restype = vp
+    L.sgx_plan_view_weight.argtypes = [vp, vp]
+    L.sgx_plan_last_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
+                                      ctypes.POINTER(ctypes.c_float)]
+
+
+class Context:
+    """One libsgx context (device + stream)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.sgx_ctx_create(device, ctypes.byref(h))
+        if rc != SGX_OK:
+            raise SgxError(rc, f"sgx_ctx_create({device}) failed ({rc})")
+        self.h = h
+        self.device = device
+        if stream is not None:
+            self.check(L.sgx_ctx_set_stream(self.h, ctypes.c_void_p(stream)))
+
+    def check(self, rc):
+        if rc != SGX_OK:
+            raise SgxError(rc, load().sgx_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            load().sgx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_contexts = {}
+
+
+def context(device: int = 0) -> Context:
+    """The shared context of a device, running on torch's current stream."""
+    c = _contexts.get(device)
+    if c is None:
+        import torch
+        with torch.cuda.device(device):
+            stream = torch.cuda.current_stream(device).cuda_stream
+        c = Context(device, stream)
+        _contexts[device] = c
+    return c

@@ -1,0 +1,318 @@
+"""GPU parity tests: libsgx (through its C ABI) against the CPU oracle.
+
+Bars: rel-RMSE <= 1e-5 against the oracle's reference-precision (Float32)
+mode for the FP32 paths; the BF16-storage form is checked exactly against
+the oracle run on the same bf16-rounded data (so only accumulation order
+differs) and its storage error against Float64 is reported.
+"""
+
+import math
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import rel_rmse
+
+pytestmark = pytest.mark.gpu
+
+TOL32 = 1e-5
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def torch_cuda():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a GPU"
+    return torch
+
+
+def _grid(n, d=1.0, off=0.0):
+    return d * (np.arange(n) - ((n - 1) / 2 + off))
+
+
+def _bf16_round(x):
+    u = np.ascontiguousarray(np.asarray(x, np.float32)).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).reshape(np.shape(x))
+
+
+# --- K3 cone-beam backprojection -------------------------------------------------------
+
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("dims,ia_skip,masked", [((16, 16, 8), 1, False),
+                                                 ((19, 13, 11), 2, True),
+                                                 ((9, 24, 35), 1, True)])
+def test_cbct_back_fp32_vs_oracle(sg, orc, torch_cuda, arc, dims, ia_skip, masked):
+    _say("cbct_back", arc, dims, ia_skip, masked)
+    rng = np.random.default_rng(10)
+    ns, nt, na = 40, 30, 24
+    proj = rng.standard_normal((ns, nt, na)).astype(np.float32)
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=ns, nt=nt, ds=1.1, dt=1.3, na=na,
+                                                 offset_s=0.25, offset_t=-0.5, dsd=120.0,
+                                                 dod=50.0)
+    ig = sg.ImageGeom(dims, (1.0, 0.9, 1.1), (0.3, 0.0, -0.2))
+    if masked:
+        m = rng.random(dims) > 0.3
+        ig = ig.with_mask(m)
+    got = sg.cbct_back(proj, rg, ig, ia_skip=ia_skip)
+    xc, yc, zc = ig.axes()
+    ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, rg.offset_s,
+                        rg.offset_t, arc, xc, yc, zc, ig.mask, ia_skip=ia_skip,
+                        dtype=np.float32)
+    assert got.shape == dims
+    assert np.all(got[~ig.mask] == 0)
+    assert rel_rmse(got, ref) < TOL32
+    # device pointers give the same answer as host pointers
+    t = torch_cuda.from_numpy(np.ascontiguousarray(proj.transpose(2, 1, 0))).cuda().permute(2, 1, 0)
+    got_d = sg.cbct_back(t, rg, ig, ia_skip=ia_skip).cpu().numpy()
+    assert np.array_equal(got_d, got)
+
+
+def test_cbct_back_detector_edges(sg, orc, torch_cuda):
+    # voxels projecting beyond every detector edge, including NaN-free far field
+    _say("cbct_back edges")
+    rng = np.random.default_rng(11)
+    rg = sg.CtFanFlat(ns=12, nt=10, ds=2.0, na=8, dsd=60.0, dod=20.0)
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    ig = sg.ImageGeom((32, 30, 10), (1.0, 1.0, 2.0))
+    got = sg.cbct_back(proj, rg, ig)
+    xc, yc, zc = ig.axes()
+    ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False,
+                        xc, yc, zc, ig.mask, dtype=np.float32)
+    assert np.isfinite(got).all()
+    assert rel_rmse(got, ref) < TOL32
+    assert np.any(ref == 0)          # some voxels see no detector at all
+
+
+def test_cbct_back_bf16_storage(sg, orc, torch_cuda):
+    _say("cbct_back bf16")
+    torch = torch_cuda
+    rng = np.random.default_rng(12)
+    rg = sg.CtFanArc(ns=48, nt=32, ds=1.0, dt=1.2, na=30, dsd=150.0, dod=60.0)
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    pb = _bf16_round(proj)
+    tb = torch.from_numpy(np.ascontiguousarray(proj.transpose(2, 1, 0))).cuda().to(
+        torch.bfloat16)
+    ig = sg.ImageGeom((20, 18, 12))
+    from sinograms_amd import _lib
+    import ctypes
+    L = _lib.load()
+    ctx = _lib.context(0)
+    out = torch.empty((12, 18, 20), device="cuda")
+    xc, yc, zc = (np.ascontiguousarray(v) for v in ig.axes())
+    betas = np.ascontiguousarray(rg.ar(f32=True))
+    p = lambda a: ctypes.c_void_p(a.ctypes.data)
+    ctx.check(L.sgx_cbct_back(ctx.h, ctypes.c_void_p(tb.data_ptr()), rg.ns, rg.nt, rg.na, 1,
+                              p(betas), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, 1, p(xc), 20,
+                              p(yc), 18, p(zc), 12, None, ctypes.c_void_p(out.data_ptr()),
+                              _lib.SGX_Q_BF16))
+    got = out.permute(2, 1, 0).cpu().numpy()
+    ref = orc.cbct_back(pb, betas, rg.dsd, rg.dso, 1.0, 1.2, 0.0, 0.0, True, xc, yc, zc,
+                        ig.mask, dtype=np.float32)
+    assert rel_rmse(got, ref) < TOL32
+
+
+# --- K1 / K2 2-D backprojection ----------------------------------------------------------
+
+@pytest.mark.parametrize("kind", ["par", "arc", "flat"])
+@pytest.mark.parametrize("nx,ny,na,ia_skip", [(16, 12, 30, 1), (37, 41, 50, 3),
+                                              (128, 128, 180, 1)])
+def test_back2d_fp32_vs_oracle(sg, orc, torch_cuda, kind, nx, ny, na, ia_skip):
+    _say("back2d", kind, nx, ny, na, ia_skip)
+    rng = np.random.default_rng(20)
+    nb = 61
+    if kind == "par":
+        rg = sg.SinoPar(nb=nb, d=1.3, na=na, offset=0.25)
+    else:
+        rg = (sg.SinoFanArc if kind == "arc" else sg.SinoFanFlat)(
+            nb=nb, d=1.3, na=na, offset=0.25, dsd=300.0, dod=100.0, source_offset=0.7)
+    sino = rng.standard_normal(rg.dims()).astype(np.float32)
+    ig = sg.ImageGeom((nx, ny), (0.8, 0.9))
+    ig = ig.with_mask(rng.random((nx, ny)) > 0.2)
+    got = sg.fbp_back(rg, ig, sino, ia_skip=ia_skip)
+    xc, yc = ig.axes()
+    ar32 = rg.ar(f32=True)[::ia_skip].astype(np.float32)
+    sn, cs = np.sin(ar32), np.cos(ar32)
+    if kind == "par":
+        ref = orc.back_par(sino, sn, cs, rg.d, rg.offset, xc, yc, ig.mask, ia_skip=ia_skip,
+                           dtype=np.float32)
+    else:
+        ref = orc.back_fan(sino, sn, cs, rg.dsd, rg.dso, rg.source_offset, kind == "arc",
+                           rg.d, rg.offset, xc, yc, ig.mask, ia_skip=ia_skip,
+                           dtype=np.float32)
+    assert np.all(got[~ig.mask] == 0)
+    assert rel_rmse(got, ref) < TOL32
+
+
+# --- K4 filter and K5 Parker ------------------------------------------------------------
+
+@pytest.mark.parametrize("nb,ncols,npad,extra", [(37, 5, 64, 3), (367, 360, 512, 0),
+                                                 (888, 70, 1024, 0), (512, 1000, 1024, 2)])
+def test_sino_filter_fp32_vs_oracle(sg, orc, torch_cuda, nb, ncols, npad, extra):
+    _say("filter fp32", nb, ncols, npad, extra)
+    rng = np.random.default_rng(30)
+    x = rng.standard_normal((nb, ncols)).astype(np.float32)
+    h, _ = orc.ramp_arc(npad, 1.0, 900.0)
+    Hk = orc.fbp_filter(h, npad, 1.0, window=orc.fbp_window("hann", npad)).astype(np.float32)
+    got = sg.fbp_sino_filter(x, Hk, extra=extra)
+    ref = orc.sino_filter(x, Hk, extra=extra, dtype=np.float32)
+    assert got.shape == (nb + extra, ncols)
+    assert np.all(got[nb:] == 0)
+    assert rel_rmse(got, ref) < 2e-6
+
+
+@pytest.mark.parametrize("nb,ncols,npad", [(37, 5, 64), (367, 360, 512), (512, 1000, 1024)])
+def test_sino_filter_split_bf16x3(sg, orc, torch_cuda, nb, ncols, npad):
+    _say("filter bf16x3", nb, ncols, npad)
+    rng = np.random.default_rng(31)
+    x = rng.standard_normal((nb, ncols)).astype(np.float32)
+    h, _ = orc.ramp_flat(npad, 1.0)
+    Hk = orc.fbp_filter(h, npad, 1.0).astype(np.float32)
+    got = sg.fbp_sino_filter(x, Hk, precision="bf16x3")
+    ref = orc.sino_filter(x.astype(np.float64), Hk.astype(np.float64))
+    err = rel_rmse(got, ref)
+    _say("  split-bf16x3 rel-RMSE vs Float64:", err)
+    assert err < 3e-5
+
+
+def test_parker_device_vs_oracle(sg, orc, torch_cuda):
+    _say("parker")
+    rg = sg.SinoFanArc.short(nb=888, d=1.0239, na=984, dsd=949.0, dod=408.0)
+    pw = sg.parker_weight(rg)
+    ref = orc.parker_fan_short(rg.gamma(), rg.ar(f32=True), rg.gamma_max())
+    assert pw.shape == (888, 642)
+    assert np.abs(pw - ref).max() < 1e-6
+    ct = sg.CtFanFlat.short(ns=64, nt=10, na=100)
+    assert sg.parker_weight(ct).shape == (64, 1, ct.na)
+
+
+# --- fused plans (fbp / fdk) -----------------------------------------------------------------
+
+def _fan_case(sg, arc, short, n=96, nb=120):
+    cls = sg.SinoFanArc if arc else sg.SinoFanFlat
+    kw = dict(nb=nb, d=1.0, dsd=400.0, dod=150.0)
+    rg = cls.short(na=200, **kw) if short else cls(na=200, **kw)
+    ig = sg.ImageGeom((n, n), (1.0, 1.0))
+    return rg, ig.with_mask(ig.circle(rg.rfov()))
+
+
+@pytest.mark.parametrize("arc", [True, False])
+@pytest.mark.parametrize("short", [False, True])
+def test_fbp_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
+    _say("fbp fan plan", arc, short)
+    rg, ig = _fan_case(sg, arc, short)
+    ell = sg.phantom.random_ellipses(5, 6, 96.0)
+    sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
+    plan = sg.plan_fbp(rg, ig, window=sg.Window(sg.Hann()))
+    got = sg.fbp(plan, sino)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.angle_weight(ar) * (orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())
+                                 if short else 1.0)
+    sw = orc.fan_sino_weight(rg.gamma(), rg.dso, rg.dsd)
+    xc, yc = ig.axes()
+    args = (plan.filter, vw, sw, ar, rg.dsd, rg.dso, 0.0, arc, rg.d, 0.0, xc, yc, ig.mask)
+    ref32 = orc.fbp_fan(sino, *args, dtype=np.float32)
+    ref64 = orc.fbp_fan(sino.astype(np.float64), *args)
+    e32, e64 = rel_rmse(got, ref32), rel_rmse(got, ref64)
+    _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs Float64 oracle {e64:.3g}")
+    assert e32 < TOL32
+    # applied pre-weights equal the reference's sino weight .* view_weight
+    w = plan.applied_weights()[:, 0, :]
+    assert np.allclose(w, sw[:, None] * vw, rtol=2e-6, atol=1e-9)
+    # stack path: recon3[:, :, 1] == recon (test/fbp2/fbp-fan.jl:36-38)
+    r3 = sg.fbp(plan, sino.reshape(sino.shape + (1,), order="F"))
+    assert np.array_equal(r3[:, :, 0], got)
+
+
+def test_fbp_par_plan_vs_oracle_and_disk_area(sg, orc, torch_cuda):
+    _say("fbp par plan")
+    rg = sg.SinoPar(nb=367, d=1.0, na=360)
+    ig = sg.ImageGeom((256, 256), (1.0, 1.0))
+    ell = sg.phantom.random_ellipses(0, 10, 256.0)
+    sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
+    plan = sg.plan_fbp(rg, ig, window=sg.Window(sg.Hann()))
+    got = sg.fbp(plan, sino)
+    ar = rg.ar(f32=True).astype(np.float32)
+    xc, yc = ig.axes()
+    args = (plan.filter, orc.angle_weight(ar), ar, rg.d, 0.0, xc, yc, ig.mask)
+    e32 = rel_rmse(got, orc.fbp_par(sino, *args, dtype=np.float32))
+    e64 = rel_rmse(got, orc.fbp_par(sino.astype(np.float64), *args))
+    _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs Float64 oracle {e64:.3g}")
+    assert e32 < TOL32
+    # test/fbp2/fbp-par.jl:12-33 through the convenience form
+    nr, na = 64, 128
+    dr = 10 / nr
+    r = (np.arange(nr) - (nr - 1) / 2) * dr
+    phi = np.arange(na) / na * math.pi
+    tau = (r[:, None] - (2 * np.cos(phi) + 1 * np.sin(phi))[None, :]) / 2
+    s = (2 * np.where(np.abs(tau) < 1, 2 * np.sqrt(np.clip(1 - tau ** 2, 0, None)), 0)
+         ).astype(np.float32)
+    recon = sg.fbp(s, dr=dr)
+    area = recon.sum() * dr * dr
+    assert abs(area - 4 * math.pi) / (4 * math.pi) < 2e-4
+    # stack of two slices (5x the second), fbp-par.jl:44-49
+    s3 = np.stack([s, 5 * s], axis=2)
+    rg2 = sg.SinoPar(nb=nr, na=na, d=dr)
+    ig2 = sg.ImageGeom((nr, nr), (dr, dr))
+    ig2 = ig2.with_mask(ig2.circle(rg2.rfov()))
+    r3 = sg.fbp(sg.plan_fbp(rg2, ig2), s3)
+    areas = r3.sum(axis=(0, 1)) * dr * dr / np.array([1, 5])
+    assert np.all(np.abs(areas - 4 * math.pi) / (4 * math.pi) < 5e-4)
+
+
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("short", [False, True])
+def test_fdk_plan_vs_oracle_and_volume(sg, orc, torch_cuda, arc, short):
+    # test/fbp3/fdk.jl:14-34 geometry and tolerances
+    _say("fdk plan", arc, short)
+    cls = sg.CtFanArc if arc else sg.CtFanFlat
+    kw = dict(ns=64, nt=40, ds=1.0, dt=1.2)
+    rg = cls.short(na=59, **kw) if short else cls(na=32, **kw)
+    ig = sg.ImageGeom((30, 28, 26), (1.0, 1.0, 1.0))
+    ell = np.array([[2.0, 3.0, 4.0, 8.0, 7.0, 6.0, 1.0]])
+    proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
+    plan = sg.plan_fbp(rg, ig)
+    got = sg.fdk(plan, proj)
+    vt = 4 / 3 * math.pi * 8 * 7 * 6
+    assert abs(got.sum() - vt) / vt < (3e-3 if not short else 2e-2)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, arc)[:, :, None] * \
+        orc.angle_weight(ar)
+    if short:
+        vw = vw * orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())[:, None, :]
+    xc, yc, zc = ig.axes()
+    args = (plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, arc, xc, yc, zc,
+            ig.mask)
+    e32 = rel_rmse(got, orc.fdk(proj, *args, dtype=np.float32))
+    e64 = rel_rmse(got, orc.fdk(proj.astype(np.float64), *args))
+    _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs Float64 oracle {e64:.3g}")
+    assert e32 < TOL32
+    w = plan.applied_weights()
+    assert np.allclose(w, np.broadcast_to(vw, w.shape), rtol=2e-6, atol=1e-9)
+
+
+def test_fdk_bf16_plan_error(sg, orc, torch_cuda):
+    _say("fdk bf16 plan")
+    rg = sg.CtFanFlat(ns=128, nt=96, ds=1.2, dt=1.2, na=90, dsd=1200.0, dod=400.0)
+    ig = sg.ImageGeom((64, 64, 32), (1.0 * 2, 1.0 * 2, 1.0 * 2))
+    ell = sg.phantom.random_ellipsoids(3, 12, 128.0)
+    proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
+    p32 = sg.plan_fbp(rg, ig)
+    pbf = sg.plan_fbp(rg, ig, precision="bf16")
+    a = sg.fdk(p32, proj)
+    b = sg.fdk(pbf, proj)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
+        orc.angle_weight(ar)
+    xc, yc, zc = ig.axes()
+    ref64 = orc.fdk(proj.astype(np.float64), p32.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt,
+                    0.0, 0.0, False, xc, yc, zc, ig.mask)
+    ea, eb = rel_rmse(a, ref64), rel_rmse(b, ref64)
+    _say(f"  FDK rel-RMSE vs Float64: fp32 {ea:.3g}  bf16 storage {eb:.3g}")
+    assert ea < 1e-5
+    assert eb < 1e-3
