@@ -1,0 +1,236 @@
+"""Benchmark: FDK (cone-beam, flat panel) on one MI355X.
+
+Workload (the configuration the headline metric is quoted on):
+  CtFanFlat, 512 x 384 detector at 1.2 mm, dso = 800 mm, dsd = 1200 mm,
+  360 views over 360 deg; 256 x 256 x 128 volume at 1 mm; 12 random
+  ellipsoids (seed 0) with analytic cone-beam projections (Float64 -> Float32).
+
+A step is one fdk(plan, proj): fused weights + Toeplitz ramp filter + voxel-
+driven backprojection, with the projections already resident in HBM.  value
+is voxel-view updates per second of the whole step for the plain FP32 form;
+the BF16-storage form is reported beside it.  The roofline object describes
+the dominant kernel (the backprojector) under the gather-equivalent model:
+16 B per voxel-view update (4 taps x 4 B) for FP32 storage, 8 B for BF16.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "sinograms.jl_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+METRIC = ("FDK backproj Gupdates/s & % HBM peak; mixed vs plain FP32 at "
+          "rel-RMSE≤1e-4")
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_problem(device):
+    import numpy as np
+    import sinograms_amd as sg
+    rg = sg.CtFanFlat(ns=512, nt=384, ds=1.2, dt=1.2, na=360, dsd=1200.0, dod=400.0)
+    ig = sg.ImageGeom((256, 256, 128), (1.0, 1.0, 1.0))
+    fov = 256.0
+    ell = sg.phantom.random_ellipsoids(0, 12, fov)
+    proj = sg.phantom.proj_ellipsoids(ell, rg, device=device).float()   # (ns, nt, na) col-major
+    return sg, rg, ig, ell, proj
+
+
+def time_steps(fn, steps, warmup):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_times(plan, proj, out, n):
+    """Average (filter_ms, back_ms) over n executes, from HIP events recorded
+    on the stream the kernels run on."""
+    f = b = 0.0
+    for _ in range(n):
+        plan.execute(proj, out=out)
+        fm, bm = plan.times()
+        f += fm
+        b += bm
+    return f / n, b / n
+
+
+def cpu_leg(sg, rg, ig, proj, gpu32, gpubf, nslab):
+    """Oracle timing on a bounded sample and the accuracy of both GPU forms on
+    it.  Sample: the nslab central z-slices of the volume (all 256 x 256
+    columns, all 360 views)."""
+    import numpy as np
+    import oracle
+    oracle.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ar = rg.ar(f32=True).astype(np.float32)
+    xc, yc, zc = ig.axes()
+    z0 = ig.dims[2] // 2 - nslab // 2
+    zs = zc[z0:z0 + nslab]
+    mask = np.ones((len(xc), len(yc), nslab), bool)
+    p = proj.double().cpu().permute(2, 1, 0).numpy().transpose(2, 1, 0)   # (ns, nt, na) F-order
+    vw = oracle.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
+        oracle.angle_weight(ar)
+    Hk = gpu32.filter
+    # Float64 oracle: weight, filter (direct Toeplitz), backproject
+    t = time.perf_counter()
+    q64 = oracle.sino_filter(p * vw, Hk.astype(np.float64))
+    t_filter64 = time.perf_counter() - t
+    ref64 = oracle.cbct_back(q64, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False, xc, yc, zs,
+                             mask)
+    # reference-precision (Float32) oracle; its backprojection is the timed baseline
+    q32 = oracle.sino_filter((p * vw).astype(np.float32), Hk, dtype=np.float32)
+    t = time.perf_counter()
+    ref32 = oracle.cbct_back(q32, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False, xc, yc,
+                             zs, mask, dtype=np.float32)
+    t_back32 = time.perf_counter() - t
+    updates = ref32.size * rg.na
+
+    def err(x, r):
+        return float(np.sqrt(((x - r) ** 2).sum() / (r ** 2).sum()))
+    g32 = gpu32[:, :, z0:z0 + nslab]
+    gbf = gpubf[:, :, z0:z0 + nslab]
+    acc = {
+        "sample": f"z-slices {z0}..{z0 + nslab - 1} of 128 (all 256x256 columns, 360 views)",
+        "fp32_vs_f32_oracle": err(g32, ref32),
+        "fp32_vs_f64_oracle": err(g32, ref64),
+        "bf16_vs_f64_oracle": err(gbf, ref64),
+        "f32_oracle_vs_f64_oracle": err(ref32, ref64),
+    }
+    cpu = {
+        "value": updates / t_back32 / 1e9,
+        "unit": "Gupdates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle Float32 cbct_back (C++/OpenMP restatement, not the reference) "
+                   f"on {nslab} of 128 z-slices: {updates} voxel-view updates in "
+                   f"{t_back32:.2f} s; CPU {cpu_model()}; filter excluded "
+                   f"(Float64 direct filter took {t_filter64:.1f} s)"),
+    }
+    return cpu, acc
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-slices", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    dev = torch.device("cuda:0")
+    log("generating projections ...")
+    sg, rg, ig, ell, proj = make_problem(dev)
+    torch.cuda.synchronize()
+    updates = int(np.prod(ig.dims)) * rg.na
+
+    log("planning ...")
+    p32 = sg.plan_fbp(rg, ig)
+    pbf = sg.plan_fbp(rg, ig, precision="bf16")
+    out32 = sg.recon._empty_colmajor(ig.dims, dev)
+    outbf = sg.recon._empty_colmajor(ig.dims, dev)
+
+    log("timing FP32 ...")
+    t32 = time_steps(lambda: p32.execute(proj, out=out32), args.steps, args.warmup)
+    log("timing BF16 storage ...")
+    tbf = time_steps(lambda: pbf.execute(proj, out=outbf), args.steps, args.warmup)
+    nk = max(3, min(args.steps, 10))
+    f32ms, b32ms = kernel_times(p32, proj, out32, nk)
+    fbfms, bbfms = kernel_times(pbf, proj, outbf, nk)
+    log(f"FP32: step {t32 * 1e3:.3f} ms (filter {f32ms:.3f} ms, back {b32ms:.3f} ms); "
+        f"BF16: step {tbf * 1e3:.3f} ms (filter {fbfms:.3f} ms, back {bbfms:.3f} ms)")
+
+    g32 = out32.cpu().numpy() if False else out32.permute(2, 1, 0).contiguous().cpu().numpy(
+    ).transpose(2, 1, 0)
+    gbf = outbf.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
+
+    bytes32 = 16.0 * updates
+    achieved = bytes32 / (b32ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "cbct_back FP32 (gather-equivalent 16 B/update)",
+            "kernel_ms": b32ms, "filter_ms": f32ms}
+    nsamp = 2 * ig.dims[0] * ig.dims[1] * rg.nt * rg.na * rg.ns   # MAC x2 of the Toeplitz
+    filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
+    mixed = {
+        "value": updates / tbf / 1e9, "unit": "Gupdates/s", "ms_per_step": tbf * 1e3,
+        "speedup_vs_fp32": t32 / tbf,
+        "back_kernel_ms": bbfms, "filter_ms": fbfms,
+        "back_achieved_GBs_8B_model": 8.0 * updates / (bbfms * 1e-3) / 1e9,
+        "back_speedup_vs_fp32": b32ms / bbfms,
+        "filter_TFLOPs_algorithmic": filt_flops / (fbfms * 1e-3) / 1e12,
+    }
+    result = {
+        "metric": METRIC,
+        "value": updates / t32 / 1e9,
+        "unit": "Gupdates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: 12 random ellipsoids (seed 0), analytic Float64 cone-beam "
+                "projections cast to Float32",
+        "config": {"workload": "FDK CtFanFlat 256x256x128 volume, 512x384 detector, 360 views",
+                   "updates_per_step": updates, "projections_per_s": rg.na / t32,
+                   "filter_TFLOPs_algorithmic_fp32": filt_flops / (f32ms * 1e-3) / 1e12},
+        "roofline": roof,
+        "mixed_bf16": mixed,
+    }
+    if not args.no_cpu:
+        log("CPU oracle leg ...")
+        cpu, acc = cpu_leg(sg, rg, ig, proj, p32_with_filter(p32, g32), gbf, args.cpu_slices)
+        result["cpu_baseline"] = cpu
+        result["accuracy"] = acc
+    print(json.dumps(result), flush=True)
+
+
+class p32_with_filter:
+    """The FP32 result volume plus the plan's filter response (for the CPU leg)."""
+
+    def __init__(self, plan, vol):
+        self.filter = plan.filter
+        self.vol = vol
+
+    def __getitem__(self, k):
+        return self.vol[k]
+
+
+if __name__ == "__main__":
+    main()

@@ -88,8 +88,9 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                     }
                 }
                 // 1 <= floor(bb) <= nb - 1; out of range: clamped address, weight 0
-                const float w = (bb >= 1.f && bb < nb1) ? w2 : 0.f;
-                const float bbc = fminf(fmaxf(bb, 1.f), nb1 - 1.f);
+                const bool ok = bb >= 1.f && bb < nb1;
+                const float w = ok ? w2 : 0.f;
+                const float bbc = ok ? bb : 1.f;
                 const float ilf = floorf(bbc);
                 const float wr = bbc - ilf;
                 const float* p = lds + v * nb + ((int)ilf - 1);

@@ -87,7 +87,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 
     const float x = xds[ix], y = yds[iy];
     const float dsd2 = dsd_ds * dsd_ds;
-    const float nsf = (float)ns, ntf = (float)nt, ntm1 = (float)(nt - 1);
+    const float nsf = (float)ns, ntf = (float)nt;
 
     const QT* qv = q;
     for (int ia = 0; ia < nviews; ++ia, qv += view_stride) {
@@ -118,8 +118,9 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #pragma unroll
         for (int j = 0; j < ZB; ++j) {
             const float bt = fmaf(mz, z[j], wt);
-            wz[j] = (bt >= 1.f && bt < ntf) ? w2 : 0.f;   // 1 <= floor(bt) < nt
-            const float btc = fminf(fmaxf(bt, 1.f), ntm1);
+            const bool ok = bt >= 1.f && bt < ntf;          // 1 <= floor(bt) < nt
+            wz[j] = ok ? w2 : 0.f;
+            const float btc = ok ? bt : 1.f;
             const float itf = floorf(btc);
             wu[j] = btc - itf;
             const QT* r0 = base + (int64_t)((int)itf - 1) * ns;

@@ -469,11 +469,19 @@ class _Plan:
     def in_dims(self):
         return self.rg.dims()
 
-    def execute(self, sino):
+    def execute(self, sino, out=None):
+        """Weight, filter and backproject ``sino``; ``out`` may be a
+        preallocated column-major CUDA tensor of the image size."""
         a = _Arr(sino)
         if a.host and self.device != 0:
             raise ValueError("host arrays run on device 0")
-        img, iptr = a.out(self.ig.dims)
+        if out is not None:
+            if a.host or tuple(out.shape) != tuple(self.ig.dims) or \
+                    not out.permute(*range(out.dim() - 1, -1, -1)).is_contiguous():
+                raise ValueError("out must be a column-major device tensor of the image size")
+            img, iptr = out, out.data_ptr()
+        else:
+            img, iptr = a.out(self.ig.dims)
         self._ctx.check(_lib.load().sgx_plan_execute(self.h, a.ptr, iptr, a.flags))
         return img
 
