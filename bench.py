@@ -71,7 +71,7 @@ def kernel_times(plan, proj, out, n):
     return f / n, b / n
 
 
-def cpu_leg(sg, rg, ig, proj, gpu32, gpubf, nslab):
+def cpu_leg(sg, rg, ig, proj, Hk, gpu32, gpubf, nslab):
     """Oracle timing on a bounded sample and the accuracy of both GPU forms on
     it.  Sample: the nslab central z-slices of the volume (all 256 x 256
     columns, all 360 views)."""
@@ -87,7 +87,6 @@ def cpu_leg(sg, rg, ig, proj, gpu32, gpubf, nslab):
     p = proj.double().cpu().permute(2, 1, 0).numpy().transpose(2, 1, 0)   # (ns, nt, na) F-order
     vw = oracle.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
         oracle.angle_weight(ar)
-    Hk = gpu32.filter
     # Float64 oracle: weight, filter (direct Toeplitz), backproject
     t = time.perf_counter()
     q64 = oracle.sino_filter(p * vw, Hk.astype(np.float64))
@@ -142,7 +141,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=4)
+    ap.add_argument("--cpu-slices", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -173,8 +172,7 @@ def main():
     log(f"FP32: step {t32 * 1e3:.3f} ms (filter {f32ms:.3f} ms, back {b32ms:.3f} ms); "
         f"BF16: step {tbf * 1e3:.3f} ms (filter {fbfms:.3f} ms, back {bbfms:.3f} ms)")
 
-    g32 = out32.cpu().numpy() if False else out32.permute(2, 1, 0).contiguous().cpu().numpy(
-    ).transpose(2, 1, 0)
+    g32 = out32.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
     gbf = outbf.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
 
     bytes32 = 16.0 * updates
@@ -183,7 +181,6 @@ def main():
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "cbct_back FP32 (gather-equivalent 16 B/update)",
             "kernel_ms": b32ms, "filter_ms": f32ms}
-    nsamp = 2 * ig.dims[0] * ig.dims[1] * rg.nt * rg.na * rg.ns   # MAC x2 of the Toeplitz
     filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
     mixed = {
         "value": updates / tbf / 1e9, "unit": "Gupdates/s", "ms_per_step": tbf * 1e3,
@@ -215,21 +212,10 @@ def main():
     }
     if not args.no_cpu:
         log("CPU oracle leg ...")
-        cpu, acc = cpu_leg(sg, rg, ig, proj, p32_with_filter(p32, g32), gbf, args.cpu_slices)
+        cpu, acc = cpu_leg(sg, rg, ig, proj, p32.filter, g32, gbf, args.cpu_slices)
         result["cpu_baseline"] = cpu
         result["accuracy"] = acc
     print(json.dumps(result), flush=True)
-
-
-class p32_with_filter:
-    """The FP32 result volume plus the plan's filter response (for the CPU leg)."""
-
-    def __init__(self, plan, vol):
-        self.filter = plan.filter
-        self.vol = vol
-
-    def __getitem__(self, k):
-        return self.vol[k]
 
 
 if __name__ == "__main__":
