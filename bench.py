@@ -71,8 +71,8 @@ def kernel_times(plan, proj, out, n):
     return f / n, b / n
 
 
-def cpu_leg(sg, rg, ig, proj, Hk, gpu32, gpubf, nslab):
-    """Oracle timing on a bounded sample and the accuracy of both GPU forms on
+def cpu_leg(sg, rg, ig, proj, Hk, vols, nslab):
+    """Oracle timing on a bounded sample and the accuracy of every GPU form on
     it.  Sample: the nslab central z-slices of the volume (all 256 x 256
     columns, all 360 views)."""
     import numpy as np
@@ -103,13 +103,13 @@ def cpu_leg(sg, rg, ig, proj, Hk, gpu32, gpubf, nslab):
 
     def err(x, r):
         return float(np.sqrt(((x - r) ** 2).sum() / (r ** 2).sum()))
-    g32 = gpu32[:, :, z0:z0 + nslab]
-    gbf = gpubf[:, :, z0:z0 + nslab]
+    g = {f: v[:, :, z0:z0 + nslab] for f, v in vols.items()}
     acc = {
         "sample": f"z-slices {z0}..{z0 + nslab - 1} of 128 (all 256x256 columns, 360 views)",
-        "fp32_vs_f32_oracle": err(g32, ref32),
-        "fp32_vs_f64_oracle": err(g32, ref64),
-        "bf16_vs_f64_oracle": err(gbf, ref64),
+        "fp32_vs_f32_oracle": err(g["fp32"], ref32),
+        "fp32_vs_f64_oracle": err(g["fp32"], ref64),
+        "f16s_vs_f64_oracle": err(g["f16s"], ref64),
+        "bf16_vs_f64_oracle": err(g["bf16"], ref64),
         "f32_oracle_vs_f64_oracle": err(ref32, ref64),
     }
     cpu = {
@@ -141,7 +141,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-slices", type=int, default=16)
+    ap.add_argument("--cpu-slices", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -157,39 +157,37 @@ def main():
     updates = int(np.prod(ig.dims)) * rg.na
 
     log("planning ...")
-    p32 = sg.plan_fbp(rg, ig)
-    pbf = sg.plan_fbp(rg, ig, precision="bf16")
-    out32 = sg.recon._empty_colmajor(ig.dims, dev)
-    outbf = sg.recon._empty_colmajor(ig.dims, dev)
+    forms = ("fp32", "bf16", "f16s")
+    plans = {f: sg.plan_fbp(rg, ig, precision=f) for f in forms}
+    outs = {f: sg.recon._empty_colmajor(ig.dims, dev) for f in forms}
+    step, kt = {}, {}
+    for f in forms:
+        log(f"timing {f} ...")
+        step[f] = time_steps(lambda: plans[f].execute(proj, out=outs[f]), args.steps,
+                             args.warmup)
+        kt[f] = kernel_times(plans[f], proj, outs[f], max(3, min(args.steps, 10)))
+        log(f"  {f}: step {step[f] * 1e3:.3f} ms (filter {kt[f][0]:.3f} ms, "
+            f"back {kt[f][1]:.3f} ms)")
+    vols = {f: outs[f].permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
+            for f in forms}
 
-    log("timing FP32 ...")
-    t32 = time_steps(lambda: p32.execute(proj, out=out32), args.steps, args.warmup)
-    log("timing BF16 storage ...")
-    tbf = time_steps(lambda: pbf.execute(proj, out=outbf), args.steps, args.warmup)
-    nk = max(3, min(args.steps, 10))
-    f32ms, b32ms = kernel_times(p32, proj, out32, nk)
-    fbfms, bbfms = kernel_times(pbf, proj, outbf, nk)
-    log(f"FP32: step {t32 * 1e3:.3f} ms (filter {f32ms:.3f} ms, back {b32ms:.3f} ms); "
-        f"BF16: step {tbf * 1e3:.3f} ms (filter {fbfms:.3f} ms, back {bbfms:.3f} ms)")
-
-    g32 = out32.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
-    gbf = outbf.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
-
-    bytes32 = 16.0 * updates
-    achieved = bytes32 / (b32ms * 1e-3) / 1e9
+    t32 = step["fp32"]
+    f32ms, b32ms = kt["fp32"]
+    achieved = 16.0 * updates / (b32ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
             "kernel": "cbct_back FP32 (gather-equivalent 16 B/update)",
             "kernel_ms": b32ms, "filter_ms": f32ms}
     filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
-    mixed = {
-        "value": updates / tbf / 1e9, "unit": "Gupdates/s", "ms_per_step": tbf * 1e3,
-        "speedup_vs_fp32": t32 / tbf,
-        "back_kernel_ms": bbfms, "filter_ms": fbfms,
-        "back_achieved_GBs_8B_model": 8.0 * updates / (bbfms * 1e-3) / 1e9,
-        "back_speedup_vs_fp32": b32ms / bbfms,
-        "filter_TFLOPs_algorithmic": filt_flops / (fbfms * 1e-3) / 1e12,
-    }
+
+    def form(f, bpu):
+        fm, bm = kt[f]
+        return {"value": updates / step[f] / 1e9, "unit": "Gupdates/s",
+                "ms_per_step": step[f] * 1e3, "speedup_vs_fp32": t32 / step[f],
+                "back_kernel_ms": bm, "filter_ms": fm,
+                "back_achieved_GBs_model": bpu * updates / (bm * 1e-3) / 1e9,
+                "back_speedup_vs_fp32": b32ms / bm,
+                "filter_TFLOPs_algorithmic": filt_flops / (fm * 1e-3) / 1e12}
     result = {
         "metric": METRIC,
         "value": updates / t32 / 1e9,
@@ -208,11 +206,12 @@ def main():
                    "updates_per_step": updates, "projections_per_s": rg.na / t32,
                    "filter_TFLOPs_algorithmic_fp32": filt_flops / (f32ms * 1e-3) / 1e12},
         "roofline": roof,
-        "mixed_bf16": mixed,
+        "mixed_f16s": form("f16s", 8.0),
+        "mixed_bf16": form("bf16", 8.0),
     }
     if not args.no_cpu:
         log("CPU oracle leg ...")
-        cpu, acc = cpu_leg(sg, rg, ig, proj, p32.filter, g32, gbf, args.cpu_slices)
+        cpu, acc = cpu_leg(sg, rg, ig, proj, plans["fp32"].filter, vols, args.cpu_slices)
         result["cpu_baseline"] = cpu
         result["accuracy"] = acc
     print(json.dumps(result), flush=True)

@@ -63,7 +63,9 @@ enum {
 /* precision forms of a plan */
 enum {
     SGX_PREC_FP32 = 0,        /* FP32 filter (FP32 MFMA), FP32 filtered storage  */
-    SGX_PREC_BF16 = 1         /* split-BF16x3 MFMA filter, BF16 filtered storage  */
+    SGX_PREC_BF16 = 1,        /* split-BF16x3 MFMA filter, BF16 filtered storage  */
+    SGX_PREC_F16S = 2         /* split-BF16x3 MFMA filter, FP16 filtered storage
+                                 with a per-view scale (view max -> 2^15)         */
 };
 
 /* ---- context ------------------------------------------------------------ */
@@ -174,6 +176,13 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
 /* The filtered, weighted data of the last execute (device pointer, layout
  * (ns, nt, na), Float32 or bfloat16 per the plan's precision). */
 const void* sgx_plan_filtered(const sgx_plan* plan);
+
+/* Copy the stored filtered data of the last execute to host memory: `host`
+ * receives ns*nt*na elements of the plan's storage type (4 bytes for
+ * SGX_PREC_FP32, 2 bytes otherwise) and `view_scales` (na floats, may be
+ * NULL) the per-view factor that multiplies the stored values (1 except for
+ * SGX_PREC_F16S). */
+int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales);
 
 /* The (ns, nt, na) view weights the plan applies, as the reference's
  * plan.view_weight broadcast to full size (host Float32 out). */

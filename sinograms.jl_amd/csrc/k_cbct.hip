@@ -20,11 +20,13 @@
 // scalar cache.  Accumulation is in registers (FP32); each voxel is written
 // once, so no atomics.
 //
-// Storage.  q is FP32, or bfloat16 (the precision form): a pair of adjacent
-// bins is one 4-byte (bf16) or 8-byte (FP32) load.
+// Storage.  q is FP32, bfloat16, or FP16 with a per-view scale (the 16-bit
+// forms): a pair of adjacent bins is one 8-byte (FP32) or 4-byte load.
 
 #include "sgx_internal.h"
 #include "sgx.h"
+
+#include <hip/hip_fp16.h>
 
 namespace sgx {
 namespace {
@@ -32,31 +34,85 @@ namespace {
 constexpr int kWaves = 4;      // waves per block, each on its own z-slab
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
-__device__ __forceinline__ float2 load_pair(const float* p)
+// Storage formats of the filtered projections.
+enum { F_F32 = 0, F_BF16 = 1, F_F16S = 2 };
+
+// Raw buffer resource over one view: the 32-bit offsets of the gathers are
+// VGPR byte offsets and the second detector row is reached through the
+// wave-uniform SGPR offset, so both taps of a row pair share one address.
+// Loads at offsets beyond num_records return 0 instead of faulting.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const char* p, uint32_t bytes)
 {
-    float2 v;
-    __builtin_memcpy(&v, p, sizeof(float2));
-    return v;
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(p), 0, (int)bytes, 0x00020000);
 }
 
-__device__ __forceinline__ float2 load_pair(const __hip_bfloat16* p)
+// 16-bit storage: the pair (is, is+1) starts at a 2-byte-aligned offset, so
+// it is taken from the dword-aligned 8-byte window that contains it (one
+// aligned load plus one v_alignbit).  The window can reach 4 bytes past the
+// pair: every 16-bit buffer handed to this kernel has 4 bytes of padding
+// after its last view (plan buffers are allocated so; the direct entry point
+// stages into a padded copy) and the views' resource covers them.  When the
+// row pitch is a multiple of 4 bytes the second row shares the first row's
+// window offset (through the SGPR offset); otherwise its offset is formed
+// separately.
+__device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                           uint32_t soff)
 {
-    uint32_t u;
-    __builtin_memcpy(&u, p, sizeof(uint32_t));
-    float2 v;
-    v.x = __uint_as_float(u << 16);
-    v.y = __uint_as_float(u & 0xffff0000u);
-    return v;
+    const auto u = __builtin_amdgcn_raw_buffer_load_b64(r, voff & ~3u, soff, 0);
+    return __builtin_amdgcn_alignbit(u[1], u[0], (voff & 2u) << 3);
 }
 
-template <int ZB, bool ARC, typename QT>
+__device__ __forceinline__ float2 dec_bf16(uint32_t u)
+{
+    return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+}
+
+__device__ __forceinline__ float2 dec_f16(uint32_t u)
+{
+    return make_float2(__half2float(__ushort_as_half((unsigned short)(u & 0xffffu))),
+                       __half2float(__ushort_as_half((unsigned short)(u >> 16))));
+}
+
+// Rows `it` and `it+1` of one view as two (is, is+1) pairs.
+template <int FMT, bool PITCH4>
+__device__ __forceinline__ void row_pairs(__amdgpu_buffer_rsrc_t r, uint32_t voff,
+                                          uint32_t rowb, float2& lo, float2& hi)
+{
+    if (FMT == F_F32) {
+        const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+        const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, voff, rowb, 0);
+        lo = make_float2(__uint_as_float(a[0]), __uint_as_float(a[1]));
+        hi = make_float2(__uint_as_float(b[0]), __uint_as_float(b[1]));
+    } else {
+        const uint32_t a = pair16(r, voff, 0);
+        const uint32_t b = PITCH4 ? pair16(r, voff, rowb) : pair16(r, voff + rowb, 0);
+        lo = FMT == F_BF16 ? dec_bf16(a) : dec_f16(a);
+        hi = FMT == F_BF16 ? dec_bf16(b) : dec_f16(b);
+    }
+}
+
+template <int FMT> struct Tap {
+    static constexpr uint32_t bytes = FMT == F_F32 ? 4 : 2;
+};
+
+// Per voxel the loop below costs one FMA (bt), a truncating convert, a
+// fract, one 24-bit multiply-add for the byte offset shared by both rows,
+// two paired buffer loads (row it+1 through the SGPR offset) and seven
+// FMA-class ops for the bilinear blend and the accumulation.
+// The bounds tests are hoisted: bt is linear in z, so when bt at the lane's
+// lowest and highest z are both in [1, nt) every voxel of the lane is in
+// range; only lanes at the detector's top or bottom edge take the checked
+// path.
+template <int ZB, bool ARC, int FMT, bool PITCH4>
 __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
-    const QT* __restrict__ q, int ns, int nt, int nviews, int64_t view_stride,
-    const ViewCS* __restrict__ cs, const float* __restrict__ xds,
-    const float* __restrict__ yds, const float* __restrict__ zds, int nx, int ny, int nz,
-    const uint8_t* __restrict__ mask, float* __restrict__ image, float dsd_ds, float dso_ds,
-    float ws, float wt, float inv_dt_ds)
+    const void* __restrict__ q, int ns, int nt, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds)
 {
+    using T = Tap<FMT>;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (nx + kPatch - 1) / kPatch;
@@ -75,7 +131,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         const int iz = iz0 + j;
         const bool in = iz < nz && (mask == nullptr || mask[col + nxy * iz]);
         any |= in;
-        z[j] = iz < nz ? zds[iz] : 0.f;
+        z[j] = zds[min(iz, nz - 1)];
         acc[j] = 0.f;
     }
     if (!any) {
@@ -84,48 +140,70 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
             if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = 0.f;
         return;
     }
+    float zlo = z[0], zhi = z[0];
+#pragma unroll
+    for (int j = 1; j < ZB; ++j) {
+        zlo = fminf(zlo, z[j]);
+        zhi = fmaxf(zhi, z[j]);
+    }
 
     const float x = xds[ix], y = yds[iy];
     const float dsd2 = dsd_ds * dsd_ds;
     const float nsf = (float)ns, ntf = (float)nt;
+    const uint32_t rowb = (uint32_t)ns * T::bytes;           // bytes per detector row
+    // bytes per view (+4 padding bytes reachable by the 16-bit window loads)
+    const uint32_t vbytes = (uint32_t)nt * rowb + (T::bytes == 2 ? 4u : 0u);
+    const int64_t viewb = view_stride * (int64_t)T::bytes;
 
-    const QT* qv = q;
-    for (int ia = 0; ia < nviews; ++ia, qv += view_stride) {
+    const char* v0 = static_cast<const char*>(q);
+    for (int ia = 0; ia < nviews; ++ia, v0 += viewb) {
         const ViewCS v = cs[ia];
         const float xb = x * v.c + y * v.s;
         const float yb = dso_ds - (-x * v.s + y * v.c);
-        const float mag = dsd_ds / yb;
+        const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
         float sp, w2;
         if (ARC) {
             sp = dsd_ds * atan2f(xb, yb);
-            w2 = dsd2 / (xb * xb + yb * yb);
+            w2 = dsd2 * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
         } else {
             sp = mag * xb;
             w2 = mag * mag;
         }
+        if (FMT == F_F16S) w2 *= vscale[ia];
         const float bs = sp + ws;
         // 1 <= floor(bs) < ns  <=>  1 <= bs < ns (also false for NaN)
-        if (!(bs >= 1.f && bs < nsf)) continue;  // the column misses the detector
+        if (!(bs >= 1.f && bs < nsf)) continue;          // the column misses the detector
         const float isf = floorf(bs);
         const float wr = bs - isf;
-        const int i0 = (int)isf - 1;
+        // byte offset of (is-1, it-1) is it*rowb + (is-1)*bytes - rowb
+        const uint32_t colb = ((uint32_t)isf - 1u) * T::bytes - rowb;
         const float mz = mag * inv_dt_ds;
-        const QT* base = qv + i0;
-        // Branch-free over z so that all 2*ZB paired loads issue back to back:
-        // an out-of-range row is clamped to a valid address and weighted 0.
+        const __amdgpu_buffer_rsrc_t rs = view_rsrc(v0, vbytes);
+
         float2 lo[ZB], hi[ZB];
         float wu[ZB], wz[ZB];
+        const float blo = fmaf(mz, zlo, wt), bhi = fmaf(mz, zhi, wt);
+        if (blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf) {
 #pragma unroll
-        for (int j = 0; j < ZB; ++j) {
-            const float bt = fmaf(mz, z[j], wt);
-            const bool ok = bt >= 1.f && bt < ntf;          // 1 <= floor(bt) < nt
-            wz[j] = ok ? w2 : 0.f;
-            const float btc = ok ? bt : 1.f;
-            const float itf = floorf(btc);
-            wu[j] = btc - itf;
-            const QT* r0 = base + (int64_t)((int)itf - 1) * ns;
-            lo[j] = load_pair(r0);
-            hi[j] = load_pair(r0 + ns);
+            for (int j = 0; j < ZB; ++j) {
+                const float bt = fmaf(mz, z[j], wt);
+                const uint32_t it = (uint32_t)bt;               // floor, bt >= 1
+                wu[j] = __builtin_amdgcn_fractf(bt);
+                wz[j] = w2;
+                const uint32_t boff = __umul24(it, rowb) + colb;
+                row_pairs<FMT, PITCH4>(rs, boff, rowb, lo[j], hi[j]);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < ZB; ++j) {
+                const float bt = fmaf(mz, z[j], wt);
+                const bool ok = bt >= 1.f && bt < ntf;          // 1 <= floor(bt) < nt
+                const uint32_t it = (uint32_t)(ok ? bt : 1.f);  // clamped address when out
+                wu[j] = ok ? __builtin_amdgcn_fractf(bt) : 0.f;
+                wz[j] = ok ? w2 : 0.f;
+                const uint32_t boff = __umul24(it, rowb) + colb;
+                row_pairs<FMT, PITCH4>(rs, boff, rowb, lo[j], hi[j]);
+            }
         }
 #pragma unroll
         for (int j = 0; j < ZB; ++j) {
@@ -145,16 +223,16 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     }
 }
 
-template <int ZB, bool ARC, typename QT>
+template <int ZB, bool ARC, int FMT, bool PITCH4>
 void launch(const CbctArgs& a, hipStream_t st)
 {
     const int tiles = ((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
     const int zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
     dim3 grid(tiles, zblocks), block(64 * kWaves);
-    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, QT>), grid, block, 0, st,
-                       static_cast<const QT*>(a.q), a.ns, a.nt, a.nviews, a.view_stride, a.cs,
-                       a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds,
-                       a.dso_ds, a.ws, a.wt, a.inv_dt_ds);
+    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, PITCH4>), grid, block, 0, st, a.q, a.ns, a.nt,
+                       a.nviews, a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx,
+                       a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt,
+                       a.inv_dt_ds);
     SGX_HIP(hipGetLastError());
 }
 
@@ -184,12 +262,29 @@ void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* ou
 void launch_cbct_back(const CbctArgs& a, hipStream_t st)
 {
     constexpr int ZB = 8;
-    if (a.q_bf16) {
-        if (a.is_arc) launch<ZB, true, __hip_bfloat16>(a, st);
-        else launch<ZB, false, __hip_bfloat16>(a, st);
+    // Byte offsets within one view are 32-bit.
+    if ((int64_t)a.ns * a.nt * 4 >= (int64_t(1) << 31))
+        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: one view must be below 2 GiB");
+    const bool pitch4 = (a.ns % 2) == 0;       // 16-bit rows a multiple of 4 bytes
+    if (a.fmt == SGX_Q_FMT_BF16) {
+        if (pitch4) {
+            if (a.is_arc) launch<ZB, true, F_BF16, true>(a, st);
+            else launch<ZB, false, F_BF16, true>(a, st);
+        } else {
+            if (a.is_arc) launch<ZB, true, F_BF16, false>(a, st);
+            else launch<ZB, false, F_BF16, false>(a, st);
+        }
+    } else if (a.fmt == SGX_Q_FMT_F16S) {
+        if (pitch4) {
+            if (a.is_arc) launch<ZB, true, F_F16S, true>(a, st);
+            else launch<ZB, false, F_F16S, true>(a, st);
+        } else {
+            if (a.is_arc) launch<ZB, true, F_F16S, false>(a, st);
+            else launch<ZB, false, F_F16S, false>(a, st);
+        }
     } else {
-        if (a.is_arc) launch<ZB, true, float>(a, st);
-        else launch<ZB, false, float>(a, st);
+        if (a.is_arc) launch<ZB, true, F_F32, true>(a, st);
+        else launch<ZB, false, F_F32, true>(a, st);
     }
 }
 

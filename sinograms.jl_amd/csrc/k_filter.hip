@@ -25,6 +25,8 @@
 #include "sgx_internal.h"
 #include "sgx.h"
 
+#include <hip/hip_fp16.h>
+
 namespace sgx {
 namespace {
 
@@ -54,6 +56,7 @@ struct Params {
     const double* gam;
     const double* bet;
     double gamma_max;
+    unsigned int* vmax;     // per-view max |out| as float bits (optional)
     int row_tiles_per_block;
 };
 
@@ -210,6 +213,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
     const int64_t c = c0 + col;
+    float amax = 0.f;
     if (c < p.ncols) {
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
@@ -219,11 +223,24 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                 const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
                 if (n < p.nrows_out) {
                     const float v = n < p.nb ? acc[r][i] : 0.f;
+                    amax = fmaxf(amax, fabsf(v));
                     const int64_t o = n + (int64_t)p.nrows_out * c;
                     if (OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
                     else static_cast<float*>(p.out)[o] = v;
                 }
             }
+        }
+    }
+    if (p.vmax) {
+        // per-view max |out|: one atomic per wave when its 32 columns share a view
+        const int64_t cl = c < p.ncols ? c : p.ncols - 1;
+        const int view = (int)(cl / p.nt);
+        const int view0 = __shfl(view, 0), view31 = __shfl(view, 31);
+        if (view0 == view31) {
+            for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+            if (lane == 0) atomicMax(p.vmax + view0, __float_as_uint(amax));
+        } else if (c < p.ncols) {
+            atomicMax(p.vmax + view, __float_as_uint(amax));
         }
     }
 }
@@ -251,6 +268,30 @@ void launch_mode(const Params& p, hipStream_t st)
     else if (rt <= 2) launch_rt<SPLIT, OUT_BF16, 2>(p, st);
     else if (rt <= 4) launch_rt<SPLIT, OUT_BF16, 4>(p, st);
     else launch_rt<SPLIT, OUT_BF16, 8>(p, st);
+}
+
+// FP16 storage with a per-view scale: view max maps to 2^15.
+__global__ void quantize_f16s_kernel(const float* __restrict__ in, int64_t per_view, int na,
+                                     const unsigned int* __restrict__ vmax,
+                                     __half* __restrict__ out, float* __restrict__ vscale)
+{
+    const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
+    const int64_t n = per_view * na;
+    for (int k = 0; k < 4; ++k) {
+        const int64_t i = i0 + k;
+        if (i >= n) break;
+        const int view = (int)(i / per_view);
+        const float m = __uint_as_float(vmax[view]);
+        const float inv = m > 0.f ? 32768.f / m : 0.f;
+        out[i] = __float2half_rn(in[i] * inv);
+        if (i % per_view == 0) vscale[view] = m / 32768.f;
+    }
+}
+
+__global__ void zero_u32_kernel(unsigned int* p, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 __global__ void parker_kernel(const double* __restrict__ gam, const double* __restrict__ bet,
@@ -303,6 +344,25 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
         if (a.out_bf16) launch_mode<false, true>(p, st);
         else launch_mode<false, false>(p, st);
     }
+}
+
+void launch_quantize_f16s(const float* in, int64_t per_view, int na, unsigned int* vmax,
+                          void* out, float* vscale, hipStream_t st)
+{
+    const int64_t n = per_view * na;
+    if (n <= 0) return;
+    const int64_t threads = (n + 3) / 4;
+    hipLaunchKernelGGL(quantize_f16s_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256),
+                       0, st, in, per_view, na, vmax, static_cast<__half*>(out), vscale);
+    SGX_HIP(hipGetLastError());
+}
+
+void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t st)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p,
+                       n);
+    SGX_HIP(hipGetLastError());
 }
 
 void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,

@@ -308,13 +308,23 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
 
         const size_t esz = (flags & SGX_Q_BF16) ? 2 : 4;
         const void* q = stage_in(ctx, W_IN, proj, esz * ns * nt * na, flags);
+        if (flags & SGX_Q_BF16) {
+            // the 16-bit gathers need readable padding after the last view
+            void* d = ctx->get(W_IN, esz * ns * nt * na + sgx::kQ16Pad);
+            SGX_HIP(hipMemsetAsync(static_cast<char*>(d) + esz * ns * nt * na, 0, sgx::kQ16Pad,
+                                   ctx->stream));
+            SGX_HIP(hipMemcpyAsync(d, proj, esz * ns * nt * na, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+            q = d;
+        }
         const int64_t nvox = nx * ny * nz;
         const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nvox, flags));
         float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
 
         sgx::CbctArgs a{};
         a.q = q;
-        a.q_bf16 = (flags & SGX_Q_BF16) != 0;
+        a.fmt = (flags & SGX_Q_BF16) ? sgx::SGX_Q_FMT_BF16 : sgx::SGX_Q_FMT_F32;
+        a.vscale = nullptr;
         a.ns = (int)ns;
         a.nt = (int)nt;
         a.nviews = (int)nv;
@@ -421,6 +431,9 @@ struct sgx_plan {
     void* tables = nullptr;       // device: cs, gam, bet, axes, g, wbin, wview, s/t axes
     uint8_t* mask = nullptr;      // device (or null)
     void* q = nullptr;            // filtered data (ns, nt, na)
+    float* qtmp = nullptr;        // FP32 filtered data before FP16 scaling (F16S form)
+    unsigned int* vmax = nullptr; // per-view max |filtered| (F16S form)
+    float* vscale = nullptr;      // per-view FP16 scale (F16S form)
     float* partial = nullptr;
     int64_t partial_cap = 0;
     // table offsets
@@ -441,6 +454,9 @@ struct sgx_plan {
         if (tables) (void)hipFree(tables);
         if (mask) (void)hipFree(mask);
         if (q) (void)hipFree(q);
+        if (qtmp) (void)hipFree(qtmp);
+        if (vmax) (void)hipFree(vmax);
+        if (vscale) (void)hipFree(vscale);
         if (partial) (void)hipFree(partial);
     }
     bool is3d() const { return geom.kind == SGX_CT_FAN_ARC || geom.kind == SGX_CT_FAN_FLAT; }
@@ -459,7 +475,8 @@ struct sgx_plan {
         a.extra = 0;
         a.out = out;
         a.out_bf16 = prec == SGX_PREC_BF16;
-        a.split_bf16 = prec == SGX_PREC_BF16;
+        a.split_bf16 = prec == SGX_PREC_BF16 || prec == SGX_PREC_F16S;
+        a.vmax = prec == SGX_PREC_F16S ? vmax : nullptr;
         a.scale = (float)G.angle_weight;
         a.wbin = wbin;
         a.wview = wview;
@@ -495,9 +512,10 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         require(G.ns > 1 && G.na > 0 && nx > 0 && ny > 0, SGX_ERR_ARG, "bad size");
         require(!is3d || (zc && nz > 0 && G.nt > 1), SGX_ERR_ARG, "3-D plan needs zc, nz, nt");
         require(is3d || G.nt == 1, SGX_ERR_ARG, "2-D plan needs nt == 1");
-        require(prec == SGX_PREC_FP32 || prec == SGX_PREC_BF16, SGX_ERR_ARG, "bad precision");
-        require(!(prec == SGX_PREC_BF16 && !is3d), SGX_ERR_UNSUPPORTED,
-                "the bf16 storage form is provided for cone-beam plans");
+        require(prec == SGX_PREC_FP32 || prec == SGX_PREC_BF16 || prec == SGX_PREC_F16S,
+                SGX_ERR_ARG, "bad precision");
+        require(prec == SGX_PREC_FP32 || is3d, SGX_ERR_UNSUPPORTED,
+                "the 16-bit storage forms are provided for cone-beam plans");
         require(G.ds != 0.0 && (!is3d || G.dt != 0.0), SGX_ERR_ARG, "ds / dt must be nonzero");
         if (G.ns > npad) {
             std::ostringstream os;
@@ -586,8 +604,14 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             SGX_HIP(hipMalloc(&P->mask, nvox));
             SGX_HIP(hipMemcpy(P->mask, mask, nvox, hipMemcpyHostToDevice));
         }
-        const size_t esz = prec == SGX_PREC_BF16 ? 2 : 4;
-        SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na));
+        const size_t esz = prec == SGX_PREC_FP32 ? 4 : 2;
+        SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na + sgx::kQ16Pad));
+        SGX_HIP(hipMemset(static_cast<char*>(P->q) + esz * ns * nt * na, 0, sgx::kQ16Pad));
+        if (prec == SGX_PREC_F16S) {
+            SGX_HIP(hipMalloc(&P->qtmp, 4 * ns * nt * na));
+            SGX_HIP(hipMalloc(&P->vmax, 4 * na));
+            SGX_HIP(hipMalloc(&P->vscale, 4 * na));
+        }
         if (!is3d) {
             P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny);
             if (P->partial_cap) SGX_HIP(hipMalloc(&P->partial, 4 * P->partial_cap));
@@ -641,12 +665,21 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
         hipStream_t st = ctx->stream;
 
         SGX_HIP(hipEventRecord(P.ev[0], st));
-        sgx::launch_sino_filter(P.filter_args(in, P.q), st);
+        if (P.prec == SGX_PREC_F16S) {
+            sgx::launch_zero_u32(P.vmax, G.na, st);
+            sgx::launch_sino_filter(P.filter_args(in, P.qtmp), st);
+            sgx::launch_quantize_f16s(P.qtmp, G.ns * G.nt, (int)G.na, P.vmax, P.q, P.vscale, st);
+        } else {
+            sgx::launch_sino_filter(P.filter_args(in, P.q), st);
+        }
         SGX_HIP(hipEventRecord(P.ev[1], st));
         if (P.is3d()) {
             sgx::CbctArgs a{};
             a.q = P.q;
-            a.q_bf16 = P.prec == SGX_PREC_BF16;
+            a.fmt = P.prec == SGX_PREC_BF16   ? sgx::SGX_Q_FMT_BF16
+                    : P.prec == SGX_PREC_F16S ? sgx::SGX_Q_FMT_F16S
+                                              : sgx::SGX_Q_FMT_F32;
+            a.vscale = P.vscale;
             a.ns = (int)G.ns;
             a.nt = (int)G.nt;
             a.nviews = (int)G.na;
@@ -696,6 +729,28 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
 }
 
 const void* sgx_plan_filtered(const sgx_plan* plan) { return plan ? plan->q : nullptr; }
+
+int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(host, SGX_ERR_ARG, "null pointer argument");
+        const sgx_geom& G = plan->geom;
+        const size_t esz = plan->prec == SGX_PREC_FP32 ? 4 : 2;
+        SGX_HIP(hipMemcpyAsync(host, plan->q, esz * G.ns * G.nt * G.na, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        if (view_scales) {
+            if (plan->prec == SGX_PREC_F16S) {
+                SGX_HIP(hipMemcpyAsync(view_scales, plan->vscale, 4 * G.na,
+                                       hipMemcpyDeviceToHost, ctx->stream));
+            } else {
+                for (int64_t i = 0; i < G.na; ++i) view_scales[i] = 1.0f;
+            }
+        }
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
 
 int sgx_plan_view_weight(sgx_plan* plan, float* out)
 {

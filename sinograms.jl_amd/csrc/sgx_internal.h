@@ -40,9 +40,12 @@ void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* ou
                       hipStream_t st);
 
 // K3: cone-beam voxel-driven backprojection.
+enum { SGX_Q_FMT_F32 = 0, SGX_Q_FMT_BF16 = 1, SGX_Q_FMT_F16S = 2 };
 struct CbctArgs {
-    const void* q;          // (ns, nt, na) float or bf16
-    bool q_bf16;
+    const void* q;          // (ns, nt, na) float, bf16 or scaled fp16; 16-bit data must be
+                            // followed by >= 4 bytes of readable padding
+    int fmt;                // SGX_Q_FMT_*
+    const float* vscale;    // per used view, for SGX_Q_FMT_F16S (else null)
     int ns, nt;
     int nviews;             // views used (after ia_skip)
     int64_t view_stride;    // elements between used views
@@ -57,6 +60,8 @@ struct CbctArgs {
     bool is_arc;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
+// Bytes of readable padding the 16-bit forms need after the last view.
+constexpr size_t kQ16Pad = 16;
 
 // K1/K2: 2-D pixel-driven backprojection with view chunks in LDS.
 struct Back2dArgs {
@@ -102,10 +107,17 @@ struct FilterArgs {
     const double* gam;      // nb (parker) or null
     const double* bet;      // na, beta - min(beta), clamped >= 0 (parker)
     double gamma_max;
+    unsigned int* vmax;     // per-view max |out| (float bits), atomically raised; or null
 };
 void launch_sino_filter(const FilterArgs& a, hipStream_t st);
 // The fused pre-weights of a filter launch, (nb, ncols) FP32.
 void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st);
+
+// FP16 with a per-view scale from vmax (view max -> 2^15); writes vscale.
+void launch_quantize_f16s(const float* in, int64_t per_view, int na, unsigned int* vmax,
+                          void* out, float* vscale, hipStream_t st);
+// Fill with zeros by a kernel (usable under stream capture).
+void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t st);
 
 // Parker weights (nb, na) in FP64 on the device, stored FP32.
 void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,

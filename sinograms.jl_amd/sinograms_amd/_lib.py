@@ -32,6 +32,7 @@ SGX_CT_FAN_FLAT = 4
 
 SGX_PREC_FP32 = 0
 SGX_PREC_BF16 = 1
+SGX_PREC_F16S = 2
 
 # Every symbol include/sgx.h declares (checked by the CPU test suite).
 EXPORTS = (
@@ -39,7 +40,7 @@ EXPORTS = (
     "sgx_ctx_set_stream", "sgx_ctx_synchronize", "sgx_back_par", "sgx_back_fan",
     "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
-    "sgx_plan_view_weight", "sgx_plan_last_times",
+    "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
 )
 
 
@@ -111,6 +112,7 @@ def _declare(L):
     L.sgx_plan_filtered.argtypes = [vp]
     L.sgx_plan_filtered.restype = vp
     L.sgx_plan_view_weight.argtypes = [vp, vp]
+    L.sgx_plan_read_filtered.argtypes = [vp, vp, vp]
     L.sgx_plan_last_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_float)]
 

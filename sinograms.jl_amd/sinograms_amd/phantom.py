@@ -97,32 +97,33 @@ def image_ellipses(ell, x, y, device="cpu"):
     return out
 
 
-def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None):
+def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None, chunk: int = 64):
     """Exact cone-beam projections (ns, nt, na) of axis-aligned ellipsoids,
-    as a column-major float64 tensor.  ``views`` selects a subset of views."""
+    as a column-major float64 tensor.  ``views`` selects a subset of views.
+    Views are processed in chunks (one batched pass per ellipsoid)."""
     torch = _torch()
     s = torch.as_tensor(rg.s(), device=device)
     t = torch.as_tensor(rg.t(), device=device)
-    beta_all = rg.ar()
+    beta_all = np.asarray(rg.ar(), np.float64)
     idx = np.arange(rg.na) if views is None else np.asarray(views)
     out = torch.zeros((len(idx), rg.nt, rg.ns), dtype=torch.float64, device=device)
-    S, T = s[None, :], t[:, None]                      # (1, ns), (nt, 1)
+    S, T = s[None, None, :], t[None, :, None]          # (1, 1, ns), (1, nt, 1)
     if rg.is_arc:
         g = S / rg.dsd
         lx, ly = rg.dsd * torch.sin(g), rg.dso - rg.dsd * torch.cos(g)
     else:
         lx, ly = S, torch.full_like(S, -rg.dod)
     ell = np.asarray(ell, np.float64)
-    for k, ia in enumerate(idx):
-        b = float(beta_all[ia])
-        cb, sb = math.cos(b), math.sin(b)
-        px, py = -rg.dso * sb, rg.dso * cb             # source
-        dx = (lx * cb - ly * sb) - px                  # (1, ns)
+    for k0 in range(0, len(idx), chunk):
+        b = torch.as_tensor(beta_all[idx[k0:k0 + chunk]], device=device)[:, None, None]
+        cb, sb = torch.cos(b), torch.sin(b)
+        px, py = -rg.dso * sb, rg.dso * cb             # source, (v, 1, 1)
+        dx = (lx * cb - ly * sb) - px                  # (v, 1, ns)
         dy = (lx * sb + ly * cb) - py
-        dz = T                                         # (nt, 1)
+        dz = T                                         # (1, nt, 1)
         norm = torch.sqrt(dx * dx + dy * dy + dz * dz)
-        ux, uy, uz = dx / norm, dy / norm, dz / norm   # (nt, ns)
-        acc = out[k]
+        ux, uy, uz = dx / norm, dy / norm, dz / norm   # (v, nt, ns)
+        acc = out[k0:k0 + chunk]
         for cx, cy, cz, a, bb, c, rho in ell:
             qx, qy, qz = (px - cx) / a, (py - cy) / bb, (0.0 - cz) / c
             vx, vy, vz = ux / a, uy / bb, uz / c

@@ -452,7 +452,8 @@ class _Plan:
         zc = xyz[2] if is3 else None
         mask = None if ig.mask.all() else np.asfortranarray(ig.mask.astype(np.uint8))
         Hk = np.ascontiguousarray(np.asarray(filter, dtype=np.float32))
-        prec = {"fp32": _lib.SGX_PREC_FP32, "bf16": _lib.SGX_PREC_BF16}[precision]
+        prec = {"fp32": _lib.SGX_PREC_FP32, "bf16": _lib.SGX_PREC_BF16,
+                "f16s": _lib.SGX_PREC_F16S}[precision]
         h = ctypes.c_void_p()
         self._ctx.check(_lib.load().sgx_plan_create(
             self._ctx.h, ctypes.byref(G), _dptr(_f64(self.ar)), _dptr(xyz[0]), len(xyz[0]),
@@ -502,6 +503,26 @@ class _Plan:
     def filtered_ptr(self) -> int:
         return _lib.load().sgx_plan_filtered(self.h)
 
+    def read_filtered(self):
+        """(stored filtered data as float32 (ns, nt, na) column-major, per-view
+        scale) of the last execute; the data are the stored values (bf16 /
+        fp16 widened exactly), the scale multiplies them."""
+        G = self.geom
+        n = G.ns * G.nt * G.na
+        if self.precision == "fp32":
+            raw = np.empty(n, np.float32)
+        else:
+            raw = np.empty(n, np.uint16)
+        sc = np.empty(G.na, np.float32)
+        self._ctx.check(_lib.load().sgx_plan_read_filtered(self.h, _dptr(raw), _dptr(sc)))
+        if self.precision == "bf16":
+            val = (raw.astype(np.uint32) << 16).view(np.float32)
+        elif self.precision == "f16s":
+            val = raw.view(np.float16).astype(np.float32)
+        else:
+            val = raw
+        return val.reshape((G.ns, G.nt, G.na), order="F"), sc
+
     def __del__(self):
         try:
             if getattr(self, "h", None):
@@ -543,8 +564,10 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
              filter=None, precision: str = "fp32", device: int = 0, **_ignored):
     """``plan_fbp`` (plan2.jl:106-138 for SinoGeom, plan3.jl:60-72 for CtGeom).
 
-    precision: "fp32" (FP32 filter and storage) or "bf16" (cone-beam only:
-    split-BF16x3 MFMA filter, BF16 filtered projections, FP32 accumulation).
+    precision: "fp32" (FP32 filter and storage), or for cone-beam plans the
+    16-bit storage forms "bf16" (split-BF16x3 MFMA filter, BF16 filtered
+    projections) and "f16s" (split-BF16x3 filter, FP16 filtered projections
+    with a per-view scale); backprojection always accumulates in FP32.
     """
     nb = rg.nb if isinstance(rg, SinoGeom) else rg.ns
     npad = npad or _nextpow2(nb + 1)
