@@ -309,12 +309,27 @@ __global__ void weights_kernel(Params p, float* __restrict__ out)
 
 Params make_params(const FilterArgs& a)
 {
-    Params p;
-    p.in = a.in; p.nb = a.nb; p.ncols = a.ncols; p.nt = a.nt; p.g = a.g; p.npad = a.npad;
-    p.nrows_out = a.nb + a.extra; p.out = a.out; p.scale = a.scale; p.wbin = a.wbin;
+    Params p{};   // every field not set below is zero / null
+    p.in = a.in;
+    p.nb = a.nb;
+    p.ncols = a.ncols;
+    p.nt = a.nt;
+    p.g = a.g;
+    p.npad = a.npad;
+    p.nrows_out = a.nb + a.extra;
+    p.out = a.out;
+    p.scale = a.scale;
+    p.wbin = a.wbin;
     p.wview = a.wview;
-    p.cyl = a.cyl; p.cyl_dsd = a.cyl_dsd; p.cyl_dso = a.cyl_dso; p.s_axis = a.s_axis;
-    p.t_axis = a.t_axis; p.gam = a.gam; p.bet = a.bet; p.gamma_max = a.gamma_max;
+    p.cyl = a.cyl;
+    p.cyl_dsd = a.cyl_dsd;
+    p.cyl_dso = a.cyl_dso;
+    p.s_axis = a.s_axis;
+    p.t_axis = a.t_axis;
+    p.gam = a.gam;
+    p.bet = a.bet;
+    p.gamma_max = a.gamma_max;
+    p.vmax = a.vmax;
     p.row_tiles_per_block = 0;
     return p;
 }
