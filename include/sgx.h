@@ -132,6 +132,30 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
 int sgx_parker_fan_short(sgx_ctx* ctx, int64_t nb, int64_t na, const double* ar,
                          const double* gam, double gamma_max, float* out, uint32_t flags);
 
+/* ---- distance-driven projector --------------------------------------------- */
+
+/* project_bdd(phantom, geo)
+ * Replaces src/sys2/bdd_2d.jl:17-110 (+ integrate1D :229-241, map2x/map2y
+ * :249-263).  phantom is (nPix, nPix) column-major, Float64, or Float32 with
+ * SGX_IN_F32; sino is (nangle, nDet) column-major Float64 (the reference's
+ * output layout and type).  angle: nangle radians (host).  Computed in FP64. */
+#define SGX_IN_F32      0x20u  /* project_bdd: phantom is Float32             */
+int sgx_project_bdd(sgx_ctx* ctx, const void* phantom, int64_t nPix, int64_t nDet,
+                    const double* angle, int64_t nangle, double DSD, double DS0, double pSize,
+                    double dSize, double* sino, uint32_t flags);
+
+/* backproject_bdd(sinogram, geo)
+ * Replaces src/sys2/bdd_2d.jl:119-220.  sino is (nangle, nDet) Float64
+ * column-major, image (nPix, nPix) Float64.  ref_alias = 1 reproduces the
+ * reference exactly: bdd_2d.jl:139-140 binds reconImgTmp to the initial
+ * reconImg, so the first view's per-row buffer is added once more; 0 gives
+ * the plain sum over views (the exact adjoint of sgx_project_bdd).  Views
+ * are summed with FP64 atomics (order not fixed). */
+int sgx_backproject_bdd(sgx_ctx* ctx, const double* sino, int64_t nPix, int64_t nDet,
+                        const double* angle, int64_t nangle, double DSD, double DS0,
+                        double pSize, double dSize, int ref_alias, double* image,
+                        uint32_t flags);
+
 /* ---- plans: the fused whole path ----------------------------------------- */
 
 /* Geometry and weighting of a plan (plan_fbp, src/fbp2/plan2.jl:106-138 and

@@ -416,6 +416,55 @@ int sgx_parker_fan_short(sgx_ctx* ctx, int64_t nb, int64_t na, const double* ar,
     });
 }
 
+int sgx_project_bdd(sgx_ctx* ctx, const void* phantom, int64_t nPix, int64_t nDet,
+                    const double* angle, int64_t nangle, double DSD, double DS0, double pSize,
+                    double dSize, double* sino, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(phantom && angle && sino, SGX_ERR_ARG, "null pointer argument");
+        require(nPix > 0 && nDet > 0 && nangle > 0, SGX_ERR_ARG, "bad size");
+        require(nPix * nPix < (int64_t(1) << 31), SGX_ERR_UNSUPPORTED, "image too large");
+        const bool f32 = (flags & SGX_IN_F32) != 0;
+        const size_t isz = f32 ? 4 : 8;
+        TablePack pack;
+        const size_t o_a = pack.add(angle, 8 * nangle);
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        const void* img = stage_in(ctx, W_IN, phantom, isz * nPix * nPix, flags);
+        const int64_t nout = nangle * nDet;
+        double* out = (flags & SGX_HOST) ? static_cast<double*>(ctx->get(W_OUT, 8 * nout)) : sino;
+        double* partial = static_cast<double*>(
+            ctx->get(W_PARTIAL, 8 * sgx::bdd_project_partial_doubles((int)nPix, (int)nDet,
+                                                                    (int)nangle)));
+        sgx::launch_bdd_project(img, f32, (int)nPix, (int)nDet,
+                                reinterpret_cast<const double*>(d + o_a), (int)nangle, DSD, DS0,
+                                pSize, dSize, partial, out, ctx->stream);
+        finish(ctx, sino, out, 8 * nout, flags);
+    });
+}
+
+int sgx_backproject_bdd(sgx_ctx* ctx, const double* sino, int64_t nPix, int64_t nDet,
+                        const double* angle, int64_t nangle, double DSD, double DS0,
+                        double pSize, double dSize, int ref_alias, double* image,
+                        uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(sino && angle && image, SGX_ERR_ARG, "null pointer argument");
+        require(nPix > 0 && nDet > 1 && nangle > 0, SGX_ERR_ARG, "bad size");
+        require(nPix * nPix < (int64_t(1) << 31), SGX_ERR_UNSUPPORTED, "image too large");
+        TablePack pack;
+        const size_t o_a = pack.add(angle, 8 * nangle);
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        const double* sn = static_cast<const double*>(
+            stage_in(ctx, W_IN, sino, 8 * nangle * nDet, flags));
+        const int64_t nout = nPix * nPix;
+        double* out = (flags & SGX_HOST) ? static_cast<double*>(ctx->get(W_OUT, 8 * nout)) : image;
+        sgx::launch_bdd_backproject(sn, (int)nPix, (int)nDet,
+                                    reinterpret_cast<const double*>(d + o_a), (int)nangle, DSD,
+                                    DS0, pSize, dSize, ref_alias != 0, out, ctx->stream);
+        finish(ctx, image, out, 8 * nout, flags);
+    });
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------

@@ -119,6 +119,17 @@ void launch_quantize_f16s(const float* in, int64_t per_view, int na, unsigned in
 // Fill with zeros by a kernel (usable under stream capture).
 void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t st);
 
+// K6: distance-driven projector (FP64).
+int64_t bdd_project_partial_doubles(int nPix, int nDet, int nangle);
+void launch_bdd_project(const void* img, bool img_f32, int nPix, int nDet,
+                        const double* angle, int nangle, double DSD, double DS0, double pSize,
+                        double dSize, double* partial, double* sino, hipStream_t st);
+
+// K6 adjoint: distance-driven backprojector (FP64, atomics over views).
+void launch_bdd_backproject(const double* sino, int nPix, int nDet, const double* angle,
+                            int nangle, double DSD, double DS0, double pSize, double dSize,
+                            bool ref_alias, double* image, hipStream_t st);
+
 // Parker weights (nb, na) in FP64 on the device, stored FP32.
 void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,
                    float* out, hipStream_t st);

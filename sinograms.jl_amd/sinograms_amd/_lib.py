@@ -23,6 +23,7 @@ SGX_ERR_UNSUPPORTED = -5
 SGX_HOST = 0x1
 SGX_ASYNC = 0x2
 SGX_Q_BF16 = 0x10
+SGX_IN_F32 = 0x20
 
 SGX_SINO_PAR = 0
 SGX_SINO_FAN_ARC = 1
@@ -41,6 +42,7 @@ EXPORTS = (
     "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
+    "sgx_project_bdd", "sgx_backproject_bdd",
 )
 
 
@@ -113,6 +115,9 @@ def _declare(L):
     L.sgx_plan_filtered.restype = vp
     L.sgx_plan_view_weight.argtypes = [vp, vp]
     L.sgx_plan_read_filtered.argtypes = [vp, vp, vp]
+    L.sgx_project_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, vp, u32]
+    L.sgx_backproject_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, i32, vp,
+                                      u32]
     L.sgx_plan_last_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_float)]
 

@@ -69,7 +69,8 @@ class _Arr:
     def __init__(self, x, dtype=np.float32):
         if _is_cuda(x):
             import torch
-            tdt = {np.float32: torch.float32, np.uint8: torch.uint8}[dtype]
+            tdt = {np.float32: torch.float32, np.float64: torch.float64,
+                   np.uint8: torch.uint8}[dtype]
             t = _colmajor(x if x.dtype == tdt else x.to(tdt))
             self.keep, self.ptr, self.device, self.host = t, t.data_ptr(), t.device, False
             self.shape = tuple(t.shape)

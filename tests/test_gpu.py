@@ -335,3 +335,43 @@ def test_fdk_f16s_storage_exact_vs_oracle(sg, orc, torch_cuda):
     ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
                         xc, yc, zc, ig.mask, dtype=np.float32)
     assert rel_rmse(got, ref) < TOL32
+
+
+# --- K6 distance-driven projector ------------------------------------------------------
+
+@pytest.mark.parametrize("nPix,nDet,na,f32", [(32, 57, 24, False), (128, 226, 90, True),
+                                              (64, 101, 360, False)])
+def test_project_bdd_vs_oracle(sg, orc, torch_cuda, nPix, nDet, na, f32):
+    _say("project_bdd", nPix, nDet, na, f32)
+    geo = dict(DSD=949.0, DS0=541.0, pSize=512.0 / nPix, dSize=1.0239 * 910 / nDet,
+               nPix=nPix, nDet=nDet, angle=np.arange(na) * 2 * math.pi / na)
+    rng = np.random.default_rng(40)
+    x = rng.random((nPix, nPix)) - 0.3
+    if f32:
+        x = x.astype(np.float32)
+    got = sg.project_bdd(x, geo)
+    ref = orc.project_bdd(x.astype(np.float64), **geo)
+    assert got.shape == (na, nDet) and got.dtype == np.float64
+    assert rel_rmse(got, ref) < 1e-12
+    # device arrays in, device array out
+    xt = torch_cuda.from_numpy(np.ascontiguousarray(x.T)).cuda().T
+    got_d = sg.project_bdd(xt, geo).cpu().numpy()
+    assert np.allclose(got_d, got, rtol=0, atol=1e-12 * np.abs(got).max())
+
+
+@pytest.mark.parametrize("nPix,nDet,na", [(32, 57, 24), (128, 226, 90)])
+def test_backproject_bdd_vs_oracle_and_adjoint(sg, orc, torch_cuda, nPix, nDet, na):
+    _say("backproject_bdd", nPix, nDet, na)
+    geo = dict(DSD=949.0, DS0=541.0, pSize=512.0 / nPix, dSize=1.0239 * 910 / nDet,
+               nPix=nPix, nDet=nDet, angle=np.arange(na) * 2 * math.pi / na)
+    rng = np.random.default_rng(41)
+    y = rng.random((na, nDet))
+    for alias in (True, False):
+        got = sg.backproject_bdd(y, geo, ref_alias=alias)
+        ref = orc.backproject_bdd(y, **geo, ref_alias=alias)
+        assert rel_rmse(got, ref) < 1e-12
+    # adjoint identity of the GPU pair (without the reference's aliasing quirk)
+    x = rng.random((nPix, nPix))
+    lhs = np.dot(sg.project_bdd(x, geo).ravel(), y.ravel())
+    rhs = np.dot(x.ravel(), sg.backproject_bdd(y, geo, ref_alias=False).ravel())
+    assert lhs == pytest.approx(rhs, rel=1e-12)
