@@ -64,8 +64,8 @@ enum {
 enum {
     SGX_PREC_FP32 = 0,        /* FP32 filter (FP32 MFMA), FP32 filtered storage  */
     SGX_PREC_BF16 = 1,        /* split-BF16x3 MFMA filter, BF16 filtered storage  */
-    SGX_PREC_F16S = 2         /* split-BF16x3 MFMA filter, FP16 filtered storage
-                                 with a per-view scale (view max -> 2^15)         */
+    SGX_PREC_F16S = 2         /* FP32 MFMA filter, FP16 filtered storage with a
+                                 per-view scale (view max -> 2^15)                */
 };
 
 /* ---- context ------------------------------------------------------------ */

@@ -524,7 +524,7 @@ struct sgx_plan {
         a.extra = 0;
         a.out = out;
         a.out_bf16 = prec == SGX_PREC_BF16;
-        a.split_bf16 = prec == SGX_PREC_BF16 || prec == SGX_PREC_F16S;
+        a.split_bf16 = prec == SGX_PREC_BF16;
         a.vmax = prec == SGX_PREC_F16S ? vmax : nullptr;
         a.scale = (float)G.angle_weight;
         a.wbin = wbin;
