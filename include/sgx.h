@@ -197,8 +197,10 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview);
  * (ns, nt, na) Float32 (nt = 1 for 2-D); image is (nx, ny, nz). */
 int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t flags);
 
-/* The filtered, weighted data of the last execute (device pointer, layout
- * (ns, nt, na), Float32 or bfloat16 per the plan's precision). */
+/* The filtered, weighted data of the last execute (device pointer).  2-D
+ * plans: column-major (ns, na) Float32.  Cone-beam plans: the row-pair gather
+ * layout, where entry (is, it, ia) holds the two values q[is, it, ia] and
+ * q[is, it+1, ia] (Float32, bfloat16 or scaled FP16 per the precision). */
 const void* sgx_plan_filtered(const sgx_plan* plan);
 
 /* Copy the stored filtered data of the last execute to host memory: `host`

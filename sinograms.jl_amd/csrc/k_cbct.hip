@@ -73,12 +73,27 @@ __device__ __forceinline__ float2 dec_f16(uint32_t u)
                        __half2float(__ushort_as_half((unsigned short)(u >> 16))));
 }
 
-// Rows `it` and `it+1` of one view as two (is, is+1) pairs.
-template <int FMT, bool PITCH4>
+// Rows `it` and `it+1` of one view as two (is, is+1) pairs, from the
+// column-major layout (two row loads) or from the row-pair layout, where the
+// entry (is, it) holds (q[is, it], q[is, it+1]) so all four taps are one
+// load: 16 bytes (FP32) or 8 bytes (16-bit).
+template <int FMT, bool PITCH4, bool RP>
 __device__ __forceinline__ void row_pairs(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                           uint32_t rowb, float2& lo, float2& hi)
 {
-    if (FMT == F_F32) {
+    if (RP) {
+        if (FMT == F_F32) {
+            const auto u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+            lo = make_float2(__uint_as_float(u[0]), __uint_as_float(u[2]));
+            hi = make_float2(__uint_as_float(u[1]), __uint_as_float(u[3]));
+        } else {
+            const auto u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
+            const float2 e0 = FMT == F_BF16 ? dec_bf16(u[0]) : dec_f16(u[0]);
+            const float2 e1 = FMT == F_BF16 ? dec_bf16(u[1]) : dec_f16(u[1]);
+            lo = make_float2(e0.x, e1.x);
+            hi = make_float2(e0.y, e1.y);
+        }
+    } else if (FMT == F_F32) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
         const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, voff, rowb, 0);
         lo = make_float2(__uint_as_float(a[0]), __uint_as_float(a[1]));
@@ -91,8 +106,11 @@ __device__ __forceinline__ void row_pairs(__amdgpu_buffer_rsrc_t r, uint32_t vof
     }
 }
 
-template <int FMT> struct Tap {
-    static constexpr uint32_t bytes = FMT == F_F32 ? 4 : 2;
+template <int FMT, bool RP> struct Tap {
+    // bytes per stored sample (column-major) or per row-pair entry
+    static constexpr uint32_t bytes = (FMT == F_F32 ? 4 : 2) * (RP ? 2 : 1);
+    // readable padding the gathers may touch past the last view
+    static constexpr uint32_t pad = (FMT != F_F32 && !RP) ? 4 : 0;
 };
 
 // Per voxel the loop below costs one FMA (bt), a truncating convert, a
@@ -103,7 +121,7 @@ template <int FMT> struct Tap {
 // lowest and highest z are both in [1, nt) every voxel of the lane is in
 // range; only lanes at the detector's top or bottom edge take the checked
 // path.
-template <int ZB, bool ARC, int FMT, bool PITCH4>
+template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
 __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const void* __restrict__ q, int ns, int nt, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
@@ -112,7 +130,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds)
 {
-    using T = Tap<FMT>;
+    using T = Tap<FMT, RP>;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (nx + kPatch - 1) / kPatch;
@@ -151,9 +169,9 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const float dsd2 = dsd_ds * dsd_ds;
     const float nsf = (float)ns, ntf = (float)nt;
     const uint32_t rowb = (uint32_t)ns * T::bytes;           // bytes per detector row
-    // bytes per view (+4 padding bytes reachable by the 16-bit window loads)
-    const uint32_t vbytes = (uint32_t)nt * rowb + (T::bytes == 2 ? 4u : 0u);
-    const int64_t viewb = view_stride * (int64_t)T::bytes;
+    // bytes per view (+ padding reachable by the 16-bit window loads)
+    const uint32_t vbytes = (uint32_t)nt * rowb + T::pad;
+    const int64_t viewb = view_stride * (int64_t)(FMT == F_F32 ? 4 : 2) * (RP ? 2 : 1);
 
     const char* v0 = static_cast<const char*>(q);
     for (int ia = 0; ia < nviews; ++ia, v0 += viewb) {
@@ -191,7 +209,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
                 wu[j] = __builtin_amdgcn_fractf(bt);
                 wz[j] = w2;
                 const uint32_t boff = __umul24(it, rowb) + colb;
-                row_pairs<FMT, PITCH4>(rs, boff, rowb, lo[j], hi[j]);
+                row_pairs<FMT, PITCH4, RP>(rs, boff, rowb, lo[j], hi[j]);
             }
         } else {
 #pragma unroll
@@ -202,7 +220,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
                 wu[j] = ok ? __builtin_amdgcn_fractf(bt) : 0.f;
                 wz[j] = ok ? w2 : 0.f;
                 const uint32_t boff = __umul24(it, rowb) + colb;
-                row_pairs<FMT, PITCH4>(rs, boff, rowb, lo[j], hi[j]);
+                row_pairs<FMT, PITCH4, RP>(rs, boff, rowb, lo[j], hi[j]);
             }
         }
 #pragma unroll
@@ -223,13 +241,14 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     }
 }
 
-template <int ZB, bool ARC, int FMT, bool PITCH4>
+template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
 void launch(const CbctArgs& a, hipStream_t st)
 {
     const int tiles = ((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
     const int zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
     dim3 grid(tiles, zblocks), block(64 * kWaves);
-    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, PITCH4>), grid, block, 0, st, a.q, a.ns, a.nt,
+    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, PITCH4, RP>), grid, block, 0, st, a.q, a.ns,
+                       a.nt,
                        a.nviews, a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx,
                        a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt,
                        a.inv_dt_ds);
@@ -263,28 +282,33 @@ void launch_cbct_back(const CbctArgs& a, hipStream_t st)
 {
     constexpr int ZB = 8;
     // Byte offsets within one view are 32-bit.
-    if ((int64_t)a.ns * a.nt * 4 >= (int64_t(1) << 31))
+    if ((int64_t)a.ns * a.nt * 8 >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: one view must be below 2 GiB");
     const bool pitch4 = (a.ns % 2) == 0;       // 16-bit rows a multiple of 4 bytes
-    if (a.fmt == SGX_Q_FMT_BF16) {
-        if (pitch4) {
-            if (a.is_arc) launch<ZB, true, F_BF16, true>(a, st);
-            else launch<ZB, false, F_BF16, true>(a, st);
+    if (a.row_pair) {
+        if (a.fmt == SGX_Q_FMT_BF16) {
+            if (a.is_arc) launch<ZB, true, F_BF16, true, true>(a, st);
+            else launch<ZB, false, F_BF16, true, true>(a, st);
+        } else if (a.fmt == SGX_Q_FMT_F16S) {
+            if (a.is_arc) launch<ZB, true, F_F16S, true, true>(a, st);
+            else launch<ZB, false, F_F16S, true, true>(a, st);
         } else {
-            if (a.is_arc) launch<ZB, true, F_BF16, false>(a, st);
-            else launch<ZB, false, F_BF16, false>(a, st);
+            if (a.is_arc) launch<ZB, true, F_F32, true, true>(a, st);
+            else launch<ZB, false, F_F32, true, true>(a, st);
+        }
+    } else if (a.fmt == SGX_Q_FMT_BF16) {
+        if (pitch4) {
+            if (a.is_arc) launch<ZB, true, F_BF16, true, false>(a, st);
+            else launch<ZB, false, F_BF16, true, false>(a, st);
+        } else {
+            if (a.is_arc) launch<ZB, true, F_BF16, false, false>(a, st);
+            else launch<ZB, false, F_BF16, false, false>(a, st);
         }
     } else if (a.fmt == SGX_Q_FMT_F16S) {
-        if (pitch4) {
-            if (a.is_arc) launch<ZB, true, F_F16S, true>(a, st);
-            else launch<ZB, false, F_F16S, true>(a, st);
-        } else {
-            if (a.is_arc) launch<ZB, true, F_F16S, false>(a, st);
-            else launch<ZB, false, F_F16S, false>(a, st);
-        }
+        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: scaled FP16 data come in the row-pair layout");
     } else {
-        if (a.is_arc) launch<ZB, true, F_F32, true>(a, st);
-        else launch<ZB, false, F_F32, true>(a, st);
+        if (a.is_arc) launch<ZB, true, F_F32, true, false>(a, st);
+        else launch<ZB, false, F_F32, true, false>(a, st);
     }
 }
 

@@ -34,6 +34,10 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kKC = 64;            // input rows staged per chunk
+// output layouts: column-major (nb + extra, ncols), or the row-pair gather
+// layout of the cone-beam backprojector (entry (n, t) of a view holds the
+// pair q[n, t], q[n, t+1])
+enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F32_RP = 2, OUT_BF16_RP = 3 };
 constexpr int kBPadF = kKC + 1;    // FP32 staging stride (conflict-free column reads)
 constexpr int kBPadH = kKC + 8;    // bf16 staging stride (16-byte aligned fragments)
 
@@ -115,7 +119,7 @@ __device__ __forceinline__ float bf16_to_f(__bf16 h)
     return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
 }
 
-template <bool SPLIT, bool OUT_BF16, int RT>
+template <bool SPLIT, int OUT, int RT>
 __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 {
     __shared__ float gl[4 * 8 * 32 + 2048 + 64];
@@ -224,9 +228,26 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                 if (n < p.nrows_out) {
                     const float v = n < p.nb ? acc[r][i] : 0.f;
                     amax = fmaxf(amax, fabsf(v));
-                    const int64_t o = n + (int64_t)p.nrows_out * c;
-                    if (OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
-                    else static_cast<float*>(p.out)[o] = v;
+                    if (OUT == OUT_F32 || OUT == OUT_BF16) {
+                        const int64_t o = n + (int64_t)p.nrows_out * c;
+                        if (OUT == OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
+                        else static_cast<float*>(p.out)[o] = v;
+                    } else {
+                        // row-pair layout: entry (n, t) of a view holds (q[n,t], q[n,t+1]);
+                        // q[n,t] is .x of entry (n,t) and .y of entry (n,t-1)
+                        const int64_t t = c % p.nt;
+                        const int64_t e = n + (int64_t)p.nrows_out * c;   // entry index
+                        if (OUT == OUT_F32_RP) {
+                            float* o = static_cast<float*>(p.out);
+                            o[2 * e] = v;
+                            if (t > 0) o[2 * (e - p.nrows_out) + 1] = v;
+                        } else {
+                            __bf16* o = static_cast<__bf16*>(p.out);
+                            const __bf16 h = bf16_rne(v);
+                            o[2 * e] = h;
+                            if (t > 0) o[2 * (e - p.nrows_out) + 1] = h;
+                        }
+                    }
                 }
             }
         }
@@ -245,17 +266,17 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     }
 }
 
-template <bool SPLIT, bool OUT_BF16, int RT>
+template <bool SPLIT, int OUT, int RT>
 void launch_rt(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
     const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
     dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
-    hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT_BF16, RT>), grid, dim3(256), 0, st, p);
+    hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT, RT>), grid, dim3(256), 0, st, p);
     SGX_HIP(hipGetLastError());
 }
 
-template <bool SPLIT, bool OUT_BF16>
+template <bool SPLIT, int OUT>
 void launch_mode(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
@@ -264,15 +285,17 @@ void launch_mode(const Params& p, hipStream_t st)
     // split the rows when the columns alone cannot fill the GPU.
     int rt = (row_tiles + 3) / 4;
     if (col_blocks < 1024) rt = 1;
-    if (rt <= 1) launch_rt<SPLIT, OUT_BF16, 1>(p, st);
-    else if (rt <= 2) launch_rt<SPLIT, OUT_BF16, 2>(p, st);
-    else if (rt <= 4) launch_rt<SPLIT, OUT_BF16, 4>(p, st);
-    else launch_rt<SPLIT, OUT_BF16, 8>(p, st);
+    if (rt <= 1) launch_rt<SPLIT, OUT, 1>(p, st);
+    else if (rt <= 2) launch_rt<SPLIT, OUT, 2>(p, st);
+    else if (rt <= 4) launch_rt<SPLIT, OUT, 4>(p, st);
+    else launch_rt<SPLIT, OUT, 8>(p, st);
 }
 
-// FP16 storage with a per-view scale: view max maps to 2^15.
-__global__ void quantize_f16s_kernel(const float* __restrict__ in, int64_t per_view, int na,
-                                     const unsigned int* __restrict__ vmax,
+// FP16 storage with a per-view scale (view max maps to 2^15), written in the
+// row-pair gather layout: q[n, t] goes to .x of entry (n, t) and .y of entry
+// (n, t-1); per_view = ns * nt, row = ns.
+__global__ void quantize_f16s_kernel(const float* __restrict__ in, int64_t per_view, int row,
+                                     int na, const unsigned int* __restrict__ vmax,
                                      __half* __restrict__ out, float* __restrict__ vscale)
 {
     const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
@@ -283,7 +306,9 @@ __global__ void quantize_f16s_kernel(const float* __restrict__ in, int64_t per_v
         const int view = (int)(i / per_view);
         const float m = __uint_as_float(vmax[view]);
         const float inv = m > 0.f ? 32768.f / m : 0.f;
-        out[i] = __float2half_rn(in[i] * inv);
+        const __half h = __float2half_rn(in[i] * inv);
+        out[2 * i] = h;
+        if ((i % per_view) >= row) out[2 * (i - row) + 1] = h;
         if (i % per_view == 0) vscale[view] = m / 32768.f;
     }
 }
@@ -352,23 +377,35 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     if (a.nb > 2048)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
     const Params p = make_params(a);
+    if (a.row_pair && a.extra != 0)
+        throw Error(SGX_ERR_ARG, "sino_filter: the row-pair layout has no extra rows");
+    const int out = a.row_pair ? (a.out_bf16 ? OUT_BF16_RP : OUT_F32_RP)
+                               : (a.out_bf16 ? OUT_BF16 : OUT_F32);
     if (a.split_bf16) {
-        if (a.out_bf16) launch_mode<true, true>(p, st);
-        else launch_mode<true, false>(p, st);
+        switch (out) {
+        case OUT_F32: launch_mode<true, OUT_F32>(p, st); break;
+        case OUT_BF16: launch_mode<true, OUT_BF16>(p, st); break;
+        case OUT_F32_RP: launch_mode<true, OUT_F32_RP>(p, st); break;
+        default: launch_mode<true, OUT_BF16_RP>(p, st); break;
+        }
     } else {
-        if (a.out_bf16) launch_mode<false, true>(p, st);
-        else launch_mode<false, false>(p, st);
+        switch (out) {
+        case OUT_F32: launch_mode<false, OUT_F32>(p, st); break;
+        case OUT_BF16: launch_mode<false, OUT_BF16>(p, st); break;
+        case OUT_F32_RP: launch_mode<false, OUT_F32_RP>(p, st); break;
+        default: launch_mode<false, OUT_BF16_RP>(p, st); break;
+        }
     }
 }
 
-void launch_quantize_f16s(const float* in, int64_t per_view, int na, unsigned int* vmax,
-                          void* out, float* vscale, hipStream_t st)
+void launch_quantize_f16s(const float* in, int64_t per_view, int row, int na,
+                          unsigned int* vmax, void* out, float* vscale, hipStream_t st)
 {
     const int64_t n = per_view * na;
     if (n <= 0) return;
     const int64_t threads = (n + 3) / 4;
     hipLaunchKernelGGL(quantize_f16s_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256),
-                       0, st, in, per_view, na, vmax, static_cast<__half*>(out), vscale);
+                       0, st, in, per_view, row, na, vmax, static_cast<__half*>(out), vscale);
     SGX_HIP(hipGetLastError());
 }
 

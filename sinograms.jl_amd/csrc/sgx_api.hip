@@ -524,6 +524,7 @@ struct sgx_plan {
         a.extra = 0;
         a.out = out;
         a.out_bf16 = prec == SGX_PREC_BF16;
+        a.row_pair = is3d() && out == q;
         a.split_bf16 = prec == SGX_PREC_BF16;
         a.vmax = prec == SGX_PREC_F16S ? vmax : nullptr;
         a.scale = (float)G.angle_weight;
@@ -653,9 +654,11 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             SGX_HIP(hipMalloc(&P->mask, nvox));
             SGX_HIP(hipMemcpy(P->mask, mask, nvox, hipMemcpyHostToDevice));
         }
-        const size_t esz = prec == SGX_PREC_FP32 ? 4 : 2;
+        // cone-beam plans store the filtered data in the row-pair gather layout
+        // (two values per entry); 2-D plans column-major
+        const size_t esz = (prec == SGX_PREC_FP32 ? 4 : 2) * (is3d ? 2 : 1);
         SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na + sgx::kQ16Pad));
-        SGX_HIP(hipMemset(static_cast<char*>(P->q) + esz * ns * nt * na, 0, sgx::kQ16Pad));
+        SGX_HIP(hipMemset(P->q, 0, esz * ns * nt * na + sgx::kQ16Pad));
         if (prec == SGX_PREC_F16S) {
             SGX_HIP(hipMalloc(&P->qtmp, 4 * ns * nt * na));
             SGX_HIP(hipMalloc(&P->vmax, 4 * na));
@@ -717,7 +720,8 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
         if (P.prec == SGX_PREC_F16S) {
             sgx::launch_zero_u32(P.vmax, G.na, st);
             sgx::launch_sino_filter(P.filter_args(in, P.qtmp), st);
-            sgx::launch_quantize_f16s(P.qtmp, G.ns * G.nt, (int)G.na, P.vmax, P.q, P.vscale, st);
+            sgx::launch_quantize_f16s(P.qtmp, G.ns * G.nt, (int)G.ns, (int)G.na, P.vmax, P.q,
+                                      P.vscale, st);
         } else {
             sgx::launch_sino_filter(P.filter_args(in, P.q), st);
         }
@@ -729,6 +733,7 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
                     : P.prec == SGX_PREC_F16S ? sgx::SGX_Q_FMT_F16S
                                               : sgx::SGX_Q_FMT_F32;
             a.vscale = P.vscale;
+            a.row_pair = true;
             a.ns = (int)G.ns;
             a.nt = (int)G.nt;
             a.nviews = (int)G.na;
@@ -787,8 +792,14 @@ int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales)
         require(host, SGX_ERR_ARG, "null pointer argument");
         const sgx_geom& G = plan->geom;
         const size_t esz = plan->prec == SGX_PREC_FP32 ? 4 : 2;
-        SGX_HIP(hipMemcpyAsync(host, plan->q, esz * G.ns * G.nt * G.na, hipMemcpyDeviceToHost,
-                               ctx->stream));
+        const int64_t n = G.ns * G.nt * G.na;
+        if (plan->is3d()) {
+            // row-pair layout: the column-major value is the first half of each entry
+            SGX_HIP(hipMemcpy2DAsync(host, esz, plan->q, 2 * esz, esz, n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        } else {
+            SGX_HIP(hipMemcpyAsync(host, plan->q, esz * n, hipMemcpyDeviceToHost, ctx->stream));
+        }
         if (view_scales) {
             if (plan->prec == SGX_PREC_F16S) {
                 SGX_HIP(hipMemcpyAsync(view_scales, plan->vscale, 4 * G.na,

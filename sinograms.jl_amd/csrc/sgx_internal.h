@@ -45,6 +45,7 @@ struct CbctArgs {
     const void* q;          // (ns, nt, na) float, bf16 or scaled fp16; 16-bit data must be
                             // followed by >= 4 bytes of readable padding
     int fmt;                // SGX_Q_FMT_*
+    bool row_pair;          // q in the row-pair gather layout (2 values per entry)
     const float* vscale;    // per used view, for SGX_Q_FMT_F16S (else null)
     int ns, nt;
     int nviews;             // views used (after ia_skip)
@@ -95,6 +96,7 @@ struct FilterArgs {
     int extra;              // zero rows appended to each output column
     void* out;              // (nb + extra, ncols), float or bf16
     bool out_bf16;
+    bool row_pair;          // write the row-pair gather layout (entry (n,t) = q[n,t], q[n,t+1])
     bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
     // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
     float scale;
@@ -113,9 +115,10 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st);
 // The fused pre-weights of a filter launch, (nb, ncols) FP32.
 void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st);
 
-// FP16 with a per-view scale from vmax (view max -> 2^15); writes vscale.
-void launch_quantize_f16s(const float* in, int64_t per_view, int na, unsigned int* vmax,
-                          void* out, float* vscale, hipStream_t st);
+// FP16 with a per-view scale from vmax (view max -> 2^15), row-pair layout;
+// writes vscale.  per_view = ns * nt, row = ns.
+void launch_quantize_f16s(const float* in, int64_t per_view, int row, int na,
+                          unsigned int* vmax, void* out, float* vscale, hipStream_t st);
 // Fill with zeros by a kernel (usable under stream capture).
 void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t st);
 
