@@ -152,28 +152,73 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     const int col = lane & 31, half = lane >> 5;
     const int scol = tid >> 3, skk = (tid & 7) * 8;      // staging: column, row offset
 
+    // Per-thread staging column: its view / row index and every factor of
+    // the fused weight that does not depend on the bin m, formed once.
+    const int64_t sc = c0 + scol;
+    const bool cvalid = sc < p.ncols;
+    int sview = 0, st = 0;
+    if (cvalid) {
+        sview = (int)(sc / p.nt);
+        st = (int)(sc - (int64_t)sview * p.nt);
+    }
+    float colw = p.scale;
+    if (cvalid && p.wview) colw *= p.wview[sview];
+    float cyl_c = 0.f, cyl_k = 0.f;
+    if (cvalid && p.cyl == 1) {            // dso / sqrt(s^2 + t^2 + dsd^2)
+        const float tt = p.t_axis[st];
+        cyl_c = tt * tt + p.cyl_dsd * p.cyl_dsd;
+    } else if (cvalid && p.cyl == 2) {     // (dso/dsd) cos(s/dsd) / sqrt((t/dsd)^2 + 1)
+        const float r = p.t_axis[st] / p.cyl_dsd;
+        cyl_k = (p.cyl_dso / p.cyl_dsd) / sqrtf(r * r + 1.f);
+    }
+    const double sbet = (cvalid && p.gam) ? p.bet[sview] : 0.0;
+    const float* src = p.in + (int64_t)p.nb * (cvalid ? sc : 0);
+
+    auto load8 = [&](int m0, float (&v)[8]) {
+        const int m = m0 + skk;
+        if (cvalid && m + 8 <= p.nb) {
+            __builtin_memcpy(v, src + m, sizeof(float) * 8);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = (cvalid && m + e < p.nb) ? src[m + e] : 0.f;
+        }
+    };
+    auto weight_of = [&](int m) -> float {
+        float w = colw;
+        if (p.wbin) w *= p.wbin[m];
+        if (p.cyl == 1) {
+            const float s = p.s_axis[m];
+            w *= p.cyl_dso * rsqrtf(s * s + cyl_c);
+        } else if (p.cyl == 2) {
+            w *= cyl_k * cosf(p.s_axis[m] / p.cyl_dsd);
+        }
+        if (p.gam) w *= (float)parker_w(p.gam[m], sbet, p.gamma_max);
+        return w;
+    };
+
+    float vnext[8];
+    load8(0, vnext);
     for (int m0 = 0; m0 < p.nb; m0 += kKC) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = vnext[e];
+        if (m0 + kKC < p.nb) load8(m0 + kKC, vnext);     // next chunk in flight meanwhile
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int m = m0 + skk + e;
+            v[e] = (cvalid && m < p.nb) ? v[e] * weight_of(m) : 0.f;
+        }
         __syncthreads();
-        {
-            const int64_t c = c0 + scol;
-            float v[8];
+        if (SPLIT) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const int m = m0 + skk + e;
-                v[e] = (m < p.nb && c < p.ncols) ? p.in[m + (int64_t)p.nb * c] * weight(p, m, c)
-                                                 : 0.f;
+                const __bf16 h = bf16_trunc(v[e]);
+                bs_hi[scol * kBPadH + skk + e] = h;
+                bs_lo[scol * kBPadH + skk + e] = bf16_rne(v[e] - bf16_to_f(h));
             }
-            if (SPLIT) {
+        } else {
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const __bf16 h = bf16_trunc(v[e]);
-                    bs_hi[scol * kBPadH + skk + e] = h;
-                    bs_lo[scol * kBPadH + skk + e] = bf16_rne(v[e] - bf16_to_f(h));
-                }
-            } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e) bs_f[scol * kBPadF + skk + e] = v[e];
-            }
+            for (int e = 0; e < 8; ++e) bs_f[scol * kBPadF + skk + e] = v[e];
         }
         __syncthreads();
 
