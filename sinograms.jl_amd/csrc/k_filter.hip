@@ -263,35 +263,88 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
     const int64_t c = c0 + col;
     float amax = 0.f;
-    if (c < p.ncols) {
+    if (OUT == OUT_F32 || OUT == OUT_BF16) {
+        if (c < p.ncols) {
+#pragma unroll
+            for (int r = 0; r < RT; ++r) {
+                const int nbase = (tile0 + wave * RT + r) * 32;
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
+                    if (n < p.nrows_out) {
+                        const float v = n < p.nb ? acc[r][i] : 0.f;
+                        amax = fmaxf(amax, fabsf(v));
+                        const int64_t o = n + (int64_t)p.nrows_out * c;
+                        if (OUT == OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
+                        else static_cast<float*>(p.out)[o] = v;
+                    }
+                }
+            }
+        }
+    } else {
+        // Row-pair layout: entry (n, c) holds (q[n, c], q[n, c+1]) within a view.
+        // The right neighbour column comes from the next lane; the block's last
+        // column writes only its first halves, its first column also fills the
+        // second halves of the previous column's entries (the block to the left).
+        const int t = (int)(c % p.nt);
+        const bool cval = c < p.ncols;
+        const bool full = cval && col < 31 && t < p.nt - 1 && c + 1 < p.ncols;
+        const bool left = cval && col == 0 && t > 0;
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             const int nbase = (tile0 + wave * RT + r) * 32;
 #pragma unroll
-            for (int i = 0; i < 16; ++i) {
-                const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
-                if (n < p.nrows_out) {
-                    const float v = n < p.nb ? acc[r][i] : 0.f;
-                    amax = fmaxf(amax, fabsf(v));
-                    if (OUT == OUT_F32 || OUT == OUT_BF16) {
-                        const int64_t o = n + (int64_t)p.nrows_out * c;
-                        if (OUT == OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
-                        else static_cast<float*>(p.out)[o] = v;
+            for (int g = 0; g < 4; ++g) {
+                const int n0 = nbase + 8 * g + 4 * half;
+                float v[4], y[4];
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    v[k] = acc[r][4 * g + k];
+                    y[k] = __shfl_down(v[k], 1, 32);
+                }
+                if (!cval) continue;
+                const int64_t e0 = n0 + (int64_t)p.nb * c;
+                if (OUT == OUT_F32_RP) {
+                    float* o = static_cast<float*>(p.out);
+                    if (full && n0 + 4 <= p.nb) {
+                        const float w8[8] = {v[0], y[0], v[1], y[1], v[2], y[2], v[3], y[3]};
+                        __builtin_memcpy(o + 2 * e0, w8, sizeof(w8));
                     } else {
-                        // row-pair layout: entry (n, t) of a view holds (q[n,t], q[n,t+1]);
-                        // q[n,t] is .x of entry (n,t) and .y of entry (n,t-1)
-                        const int64_t t = c % p.nt;
-                        const int64_t e = n + (int64_t)p.nrows_out * c;   // entry index
-                        if (OUT == OUT_F32_RP) {
-                            float* o = static_cast<float*>(p.out);
-                            o[2 * e] = v;
-                            if (t > 0) o[2 * (e - p.nrows_out) + 1] = v;
-                        } else {
-                            __bf16* o = static_cast<__bf16*>(p.out);
-                            const __bf16 h = bf16_rne(v);
-                            o[2 * e] = h;
-                            if (t > 0) o[2 * (e - p.nrows_out) + 1] = h;
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            if (n0 + k >= p.nb) break;
+                            o[2 * (e0 + k)] = v[k];
+                            if (full) o[2 * (e0 + k) + 1] = y[k];
                         }
+                    }
+                    if (left) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (n0 + k < p.nb) o[2 * (e0 + k - p.nb) + 1] = v[k];
+                    }
+                } else {
+                    __bf16* o = static_cast<__bf16*>(p.out);
+                    __bf16 hv[4], hy[4];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) {
+                        hv[k] = bf16_rne(v[k]);
+                        hy[k] = bf16_rne(y[k]);
+                    }
+                    if (full && n0 + 4 <= p.nb) {
+                        const __bf16 w8[8] = {hv[0], hy[0], hv[1], hy[1], hv[2], hy[2], hv[3], hy[3]};
+                        __builtin_memcpy(o + 2 * e0, w8, sizeof(w8));
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k) {
+                            if (n0 + k >= p.nb) break;
+                            o[2 * (e0 + k)] = hv[k];
+                            if (full) o[2 * (e0 + k) + 1] = hy[k];
+                        }
+                    }
+                    if (left) {
+#pragma unroll
+                        for (int k = 0; k < 4; ++k)
+                            if (n0 + k < p.nb) o[2 * (e0 + k - p.nb) + 1] = hv[k];
                     }
                 }
             }
