@@ -198,17 +198,25 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview);
 int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t flags);
 
 /* The filtered, weighted data of the last execute (device pointer).  2-D
- * plans: column-major (ns, na) Float32.  Cone-beam plans: the row-pair gather
- * layout, where entry (is, it, ia) holds the two values q[is, it, ia] and
- * q[is, it+1, ia] (Float32, bfloat16 or scaled FP16 per the precision). */
+ * plans: column-major (ns, na) Float32.  Cone-beam plans: the row-pair
+ * difference layout, where entry (is, it, ia) holds the two values
+ * q[is, it, ia] and q[is, it+1, ia] - q[is, it, ia] (0 for it = nt-1), as
+ * Float32, bfloat16 or scaled FP16 per the precision. */
 const void* sgx_plan_filtered(const sgx_plan* plan);
 
 /* Copy the stored filtered data of the last execute to host memory: `host`
  * receives ns*nt*na elements of the plan's storage type (4 bytes for
- * SGX_PREC_FP32, 2 bytes otherwise) and `view_scales` (na floats, may be
- * NULL) the per-view factor that multiplies the stored values (1 except for
- * SGX_PREC_F16S). */
+ * SGX_PREC_FP32, 2 bytes otherwise; for cone-beam plans the first value of
+ * each entry) and `view_scales` (na floats, may be NULL) the per-view factor
+ * that multiplies the stored values (1 except for SGX_PREC_F16S). */
 int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales);
+
+/* Size in bytes of the stored filtered data (the layout of sgx_plan_filtered). */
+int64_t sgx_plan_stored_bytes(const sgx_plan* plan);
+
+/* Copy the stored filtered data as-is (`bytes` must equal
+ * sgx_plan_stored_bytes) to host memory. */
+int sgx_plan_read_stored(sgx_plan* plan, void* host, int64_t bytes);
 
 /* The (ns, nt, na) view weights the plan applies, as the reference's
  * plan.view_weight broadcast to full size (host Float32 out). */

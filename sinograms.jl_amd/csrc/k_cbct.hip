@@ -15,18 +15,24 @@
 // z-independent work (rotation, the one divide, the arc atan2, w2, bs, the s
 // weights and the row base address) is done once per column and view and
 // amortised over ZB voxels; per voxel the loop is one FMA for bt, the floor /
-// bounds logic, two paired loads (taps is and is+1 of rows it and it+1) and
-// the bilinear blend.  Per-view cos/sin are wave-uniform and read through the
-// scalar cache.  Accumulation is in registers (FP32); each voxel is written
-// once, so no atomics.
+// bounds logic, one row-pair load (taps is and is+1 of rows it and it+1) and
+// four FMAs.  Per-view cos/sin are wave-uniform and read through the scalar
+// cache.  Accumulation is in registers (FP32); each voxel is written once, so
+// no atomics.
 //
 // Storage.  q is FP32, bfloat16, or FP16 with a per-view scale (the 16-bit
-// forms): a pair of adjacent bins is one 8-byte (FP32) or 4-byte load.
+// forms), either column-major (the direct entry point) or in the row-pair
+// difference layout the plan's filter writes: entry (is, it) of a view holds
+// (q[is, it], q[is, it+1] - q[is, it]), so the four taps of an update are one
+// 16-byte (FP32) or 8-byte (16-bit) load and the t interpolation is one FMA
+// per column.
 
 #include "sgx_internal.h"
 #include "sgx.h"
 
 #include <hip/hip_fp16.h>
+
+#include <cstdlib>
 
 namespace sgx {
 namespace {
@@ -73,27 +79,29 @@ __device__ __forceinline__ float2 dec_f16(uint32_t u)
                        __half2float(__ushort_as_half((unsigned short)(u >> 16))));
 }
 
-// Rows `it` and `it+1` of one view as two (is, is+1) pairs, from the
-// column-major layout (two row loads) or from the row-pair layout, where the
-// entry (is, it) holds (q[is, it], q[is, it+1]) so all four taps are one
-// load: 16 bytes (FP32) or 8 bytes (16-bit).
+// The four taps of one update as (a0, d0, a1, d1): a = q[is, it] and
+// d = q[is, it+1] - q[is, it] for the two columns is (0) and is+1 (1).
+// Row-pair layout: entry (is, it) of a view already holds (a, d) for column
+// is, so the two entries (is, it), (is+1, it) are one 16-byte (FP32) or
+// 8-byte (16-bit) load.  Column-major layout: two row loads (row it+1
+// through the SGPR offset) and the differences formed here.
 template <int FMT, bool PITCH4, bool RP>
-__device__ __forceinline__ void row_pairs(__amdgpu_buffer_rsrc_t r, uint32_t voff,
-                                          uint32_t rowb, float2& lo, float2& hi)
+__device__ __forceinline__ float4 taps(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t rowb)
 {
     if (RP) {
         if (FMT == F_F32) {
             const auto u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-            lo = make_float2(__uint_as_float(u[0]), __uint_as_float(u[2]));
-            hi = make_float2(__uint_as_float(u[1]), __uint_as_float(u[3]));
+            return make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]),
+                               __uint_as_float(u[2]), __uint_as_float(u[3]));
         } else {
             const auto u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
             const float2 e0 = FMT == F_BF16 ? dec_bf16(u[0]) : dec_f16(u[0]);
             const float2 e1 = FMT == F_BF16 ? dec_bf16(u[1]) : dec_f16(u[1]);
-            lo = make_float2(e0.x, e1.x);
-            hi = make_float2(e0.y, e1.y);
+            return make_float4(e0.x, e0.y, e1.x, e1.y);
         }
-    } else if (FMT == F_F32) {
+    }
+    float2 lo, hi;
+    if (FMT == F_F32) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
         const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, voff, rowb, 0);
         lo = make_float2(__uint_as_float(a[0]), __uint_as_float(a[1]));
@@ -104,6 +112,7 @@ __device__ __forceinline__ void row_pairs(__amdgpu_buffer_rsrc_t r, uint32_t vof
         lo = FMT == F_BF16 ? dec_bf16(a) : dec_f16(a);
         hi = FMT == F_BF16 ? dec_bf16(b) : dec_f16(b);
     }
+    return make_float4(lo.x, hi.x - lo.x, lo.y, hi.y - lo.y);
 }
 
 template <int FMT, bool RP> struct Tap {
@@ -113,14 +122,31 @@ template <int FMT, bool RP> struct Tap {
     static constexpr uint32_t pad = (FMT != F_F32 && !RP) ? 4 : 0;
 };
 
-// Per voxel the loop below costs one FMA (bt), a truncating convert, a
-// fract, one 24-bit multiply-add for the byte offset shared by both rows,
-// two paired buffer loads (row it+1 through the SGPR offset) and seven
-// FMA-class ops for the bilinear blend and the accumulation.
+// Byte offset that no view's buffer resource covers (every view is below
+// 2 GiB): a buffer load there returns zeros without touching memory.
+constexpr uint32_t kMissOffset = 0x80000000u;
+
+// Block order.  Workgroups are handed to the eight XCDs round-robin, and
+// each XCD has its own L2.  The physical block index is renumbered so that
+// XCD k runs one contiguous run of logical blocks (a compact z-slab and
+// y-band of the volume), whose footprint on each view is a fraction of the
+// detector, instead of every eighth tile of the whole volume.
+__device__ __forceinline__ int xcd_order(int phys, int nblocks)
+{
+    const int q = nblocks >> 3, r = nblocks & 7;
+    const int k = phys & 7, slot = phys >> 3;
+    return k * q + min(k, r) + slot;
+}
+
+// Per update the loop below costs one FMA (bt), a truncating convert, a
+// fract, one 24-bit multiply-add for the byte offset, one buffer load (row
+// pair layout) and four FMAs: p0 = a0 + wu d0, p1 = a1 + wu d1, then
+// acc += wa p0 + wb p1 with the s weights and w2 folded per view
+// (wa = w2 (1 - wr), wb = w2 wr).
 // The bounds tests are hoisted: bt is linear in z, so when bt at the lane's
 // lowest and highest z are both in [1, nt) every voxel of the lane is in
 // range; only lanes at the detector's top or bottom edge take the checked
-// path.
+// path, where a voxel out of range loads zeros (offset kMissOffset).
 template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
 __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const void* __restrict__ q, int ns, int nt, int nviews, int64_t view_stride,
@@ -128,15 +154,18 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds)
+    float inv_dt_ds, int remap)
 {
     using T = Tap<FMT, RP>;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (nx + kPatch - 1) / kPatch;
-    const int ix = (blockIdx.x % tiles_x) * kPatch + (lane & (kPatch - 1));
-    const int iy = (blockIdx.x / tiles_x) * kPatch + (lane / kPatch);
-    const int iz0 = (blockIdx.y * kWaves + wave) * ZB;
+    const int tiles = tiles_x * ((ny + kPatch - 1) / kPatch);
+    const int blk = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    const int tile = blk % tiles, zblk = blk / tiles;
+    const int ix = (tile % tiles_x) * kPatch + (lane & (kPatch - 1));
+    const int iy = (tile / tiles_x) * kPatch + (lane / kPatch);
+    const int iz0 = (zblk * kWaves + wave) * ZB;
     if (ix >= nx || iy >= ny || iz0 >= nz) return;
 
     const int64_t nxy = (int64_t)nx * ny;
@@ -192,14 +221,15 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         // 1 <= floor(bs) < ns  <=>  1 <= bs < ns (also false for NaN)
         if (!(bs >= 1.f && bs < nsf)) continue;          // the column misses the detector
         const float isf = floorf(bs);
-        const float wr = bs - isf;
+        const float wb = w2 * (bs - isf);
+        const float wa = w2 - wb;
         // byte offset of (is-1, it-1) is it*rowb + (is-1)*bytes - rowb
         const uint32_t colb = ((uint32_t)isf - 1u) * T::bytes - rowb;
         const float mz = mag * inv_dt_ds;
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(v0, vbytes);
 
-        float2 lo[ZB], hi[ZB];
-        float wu[ZB], wz[ZB];
+        float4 tp[ZB];
+        float wu[ZB];
         const float blo = fmaf(mz, zlo, wt), bhi = fmaf(mz, zhi, wt);
         if (blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf) {
 #pragma unroll
@@ -207,27 +237,24 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
                 const float bt = fmaf(mz, z[j], wt);
                 const uint32_t it = (uint32_t)bt;               // floor, bt >= 1
                 wu[j] = __builtin_amdgcn_fractf(bt);
-                wz[j] = w2;
-                const uint32_t boff = __umul24(it, rowb) + colb;
-                row_pairs<FMT, PITCH4, RP>(rs, boff, rowb, lo[j], hi[j]);
+                tp[j] = taps<FMT, PITCH4, RP>(rs, __umul24(it, rowb) + colb, rowb);
             }
         } else {
 #pragma unroll
             for (int j = 0; j < ZB; ++j) {
                 const float bt = fmaf(mz, z[j], wt);
                 const bool ok = bt >= 1.f && bt < ntf;          // 1 <= floor(bt) < nt
-                const uint32_t it = (uint32_t)(ok ? bt : 1.f);  // clamped address when out
+                const uint32_t it = (uint32_t)(ok ? bt : 1.f);
                 wu[j] = ok ? __builtin_amdgcn_fractf(bt) : 0.f;
-                wz[j] = ok ? w2 : 0.f;
-                const uint32_t boff = __umul24(it, rowb) + colb;
-                row_pairs<FMT, PITCH4, RP>(rs, boff, rowb, lo[j], hi[j]);
+                const uint32_t boff = ok ? __umul24(it, rowb) + colb : kMissOffset;
+                tp[j] = taps<FMT, PITCH4, RP>(rs, boff, rowb);
             }
         }
 #pragma unroll
         for (int j = 0; j < ZB; ++j) {
-            const float p1 = fmaf(wr, lo[j].y - lo[j].x, lo[j].x);
-            const float p2 = fmaf(wr, hi[j].y - hi[j].x, hi[j].x);
-            acc[j] = fmaf(wz[j], fmaf(wu[j], p2 - p1, p1), acc[j]);
+            const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
+            const float p1 = fmaf(wu[j], tp[j].w, tp[j].z);
+            acc[j] = fmaf(wb, p1, fmaf(wa, p0, acc[j]));
         }
     }
 
@@ -244,14 +271,15 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
 void launch(const CbctArgs& a, hipStream_t st)
 {
-    const int tiles = ((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
-    const int zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
-    dim3 grid(tiles, zblocks), block(64 * kWaves);
+    const int64_t tiles = (int64_t)((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
+    const int64_t zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
+    if (tiles * zblocks >= (int64_t(1) << 31))
+        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
+    dim3 grid((unsigned)(tiles * zblocks)), block(64 * kWaves);
     hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, PITCH4, RP>), grid, block, 0, st, a.q, a.ns,
-                       a.nt,
-                       a.nviews, a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx,
+                       a.nt, a.nviews, a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx,
                        a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt,
-                       a.inv_dt_ds);
+                       a.inv_dt_ds, a.xcd_order ? 1 : 0);
     SGX_HIP(hipGetLastError());
 }
 
@@ -278,9 +306,17 @@ void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* ou
     SGX_HIP(hipGetLastError());
 }
 
-void launch_cbct_back(const CbctArgs& a, hipStream_t st)
+void launch_cbct_back(const CbctArgs& args, hipStream_t st)
 {
     constexpr int ZB = 8;
+    // SGX_CBCT_XCD_ORDER=1 selects the XCD-grouped block order (a tuning
+    // switch; on the 256^2 x 128 FDK workload the plain order measured faster).
+    static const bool env_xcd = [] {
+        const char* e = std::getenv("SGX_CBCT_XCD_ORDER");
+        return e && e[0] == '1';
+    }();
+    CbctArgs a = args;
+    if (env_xcd) a.xcd_order = true;
     // Byte offsets within one view are 32-bit.
     if ((int64_t)a.ns * a.nt * 8 >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: one view must be below 2 GiB");

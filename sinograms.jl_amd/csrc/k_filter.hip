@@ -21,6 +21,13 @@
 //               T*B ~ Th*Bh + Th*Bl + Tl*Bh on v_mfma_f32_32x32x16_bf16 with
 //               FP32 accumulation (split-BF16x3).
 // Output rows n >= nb (the reference's `extra` rows) are written as 0.
+//
+// Row-pair difference layout (cone-beam plans).  Entry (n, t) of a view holds
+// (q[n, t], q[n, t+1] - q[n, t]) in FP32, BF16, or FP16 scaled by a per-view
+// power of two (F16S), the layout the backprojector gathers from.  Columns
+// t and t+1 of a view are neighbouring lanes of one block; so that every
+// lane's right neighbour is in its block, row-pair blocks advance by 31
+// columns and overlap by one (the 32nd column only feeds its left neighbour).
 
 #include "sgx_internal.h"
 #include "sgx.h"
@@ -34,10 +41,9 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kKC = 64;            // input rows staged per chunk
-// output layouts: column-major (nb + extra, ncols), or the row-pair gather
-// layout of the cone-beam backprojector (entry (n, t) of a view holds the
-// pair q[n, t], q[n, t+1])
-enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F32_RP = 2, OUT_BF16_RP = 3 };
+// output layouts: column-major (nb + extra, ncols), or the row-pair
+// difference layout of the cone-beam backprojector
+enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F32_RP = 2, OUT_BF16_RP = 3, OUT_F16S_RP = 4 };
 constexpr int kBPadF = kKC + 1;    // FP32 staging stride (conflict-free column reads)
 constexpr int kBPadH = kKC + 8;    // bf16 staging stride (16-byte aligned fragments)
 
@@ -60,9 +66,23 @@ struct Params {
     const double* gam;
     const double* bet;
     double gamma_max;
-    unsigned int* vmax;     // per-view max |out| as float bits (optional)
+    const unsigned int* vbound;   // F16S: per-view max |weighted input| (float bits)
+    float g1;                     // F16S: sum |g|
+    float* vscale;                // F16S: per-view output scale (written)
     int row_tiles_per_block;
 };
+
+// F16S output scale of a view: the power of two s >= bound * g1 / 2^14, so
+// that |q| / s <= 2^14 (|q| <= bound * g1) and differences of neighbours stay
+// below 2^15, inside FP16's range.  Scaling by a power of two is exact.
+__device__ __forceinline__ float f16s_scale(unsigned int bound_bits, float g1)
+{
+    const float b = __uint_as_float(bound_bits) * g1;
+    if (!(b > 0.f)) return 1.f;
+    int e;
+    (void)frexpf(b, &e);                 // b <= 2^e
+    return ldexpf(1.f, e - 14);
+}
 
 __device__ __forceinline__ double parker_w(double gg, double bb, double gm)
 {
@@ -130,7 +150,8 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     __bf16* bs_lo = bs_hi + 32 * kBPadH;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t c0 = (int64_t)blockIdx.x * 32;
+    constexpr bool RPOUT = OUT == OUT_F32_RP || OUT == OUT_BF16_RP || OUT == OUT_F16S_RP;
+    const int64_t c0 = (int64_t)blockIdx.x * (RPOUT ? 31 : 32);
     const int tile0 = blockIdx.y * 4 * RT;              // first row tile of the block
     const int n_lo = tile0 * 32;
     const int kp = ((p.nb + kKC - 1) / kKC) * kKC;       // padded K
@@ -262,8 +283,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
     const int64_t c = c0 + col;
-    float amax = 0.f;
-    if (OUT == OUT_F32 || OUT == OUT_BF16) {
+    if (!RPOUT) {
         if (c < p.ncols) {
 #pragma unroll
             for (int r = 0; r < RT; ++r) {
@@ -273,7 +293,6 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                     const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
                     if (n < p.nrows_out) {
                         const float v = n < p.nb ? acc[r][i] : 0.f;
-                        amax = fmaxf(amax, fabsf(v));
                         const int64_t o = n + (int64_t)p.nrows_out * c;
                         if (OUT == OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
                         else static_cast<float*>(p.out)[o] = v;
@@ -282,84 +301,69 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
             }
         }
     } else {
-        // Row-pair layout: entry (n, c) holds (q[n, c], q[n, c+1]) within a view.
-        // The right neighbour column comes from the next lane; the block's last
-        // column writes only its first halves, its first column also fills the
-        // second halves of the previous column's entries (the block to the left).
+        // Row-pair difference layout: entry (n, t) = (q[n, t], q[n, t+1] - q[n, t]).
+        // Column t+1 is the next lane (col + 1 <= 31 within the block); the
+        // second half of a view's last row is written as 0 (never gathered).
         const int t = (int)(c % p.nt);
-        const bool cval = c < p.ncols;
-        const bool full = cval && col < 31 && t < p.nt - 1 && c + 1 < p.ncols;
-        const bool left = cval && col == 0 && t > 0;
+        const bool own = c < p.ncols && col < 31;
+        const bool has_next = t < p.nt - 1;
+        float inv = 1.f;
+        if (OUT == OUT_F16S_RP && c < p.ncols) {
+            const int view = (int)(c / p.nt);
+            const float sc = f16s_scale(p.vbound[view], p.g1);
+            inv = 1.f / sc;                                   // exact: a power of two
+            if (own && t == 0 && wave == 0 && half == 0 && blockIdx.y == 0)
+                p.vscale[view] = sc;
+        }
 #pragma unroll
         for (int r = 0; r < RT; ++r) {
             const int nbase = (tile0 + wave * RT + r) * 32;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nbase + 8 * g + 4 * half;
-                float v[4], y[4];
+                float v[4], d[4];
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    v[k] = acc[r][4 * g + k];
-                    y[k] = __shfl_down(v[k], 1, 32);
+                    v[k] = acc[r][4 * g + k] * inv;
+                    const float y = __shfl_down(v[k], 1, 32);
+                    d[k] = has_next ? y - v[k] : 0.f;
                 }
-                if (!cval) continue;
+                if (!own || n0 >= p.nb) continue;
                 const int64_t e0 = n0 + (int64_t)p.nb * c;
                 if (OUT == OUT_F32_RP) {
-                    float* o = static_cast<float*>(p.out);
-                    if (full && n0 + 4 <= p.nb) {
-                        const float w8[8] = {v[0], y[0], v[1], y[1], v[2], y[2], v[3], y[3]};
-                        __builtin_memcpy(o + 2 * e0, w8, sizeof(w8));
+                    float* o = static_cast<float*>(p.out) + 2 * e0;
+                    if (n0 + 4 <= p.nb) {
+                        const float w8[8] = {v[0], d[0], v[1], d[1], v[2], d[2], v[3], d[3]};
+                        __builtin_memcpy(o, w8, sizeof(w8));
                     } else {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            if (n0 + k >= p.nb) break;
-                            o[2 * (e0 + k)] = v[k];
-                            if (full) o[2 * (e0 + k) + 1] = y[k];
+                        for (int k = 0; k < 4 && n0 + k < p.nb; ++k) {
+                            o[2 * k] = v[k];
+                            o[2 * k + 1] = d[k];
                         }
-                    }
-                    if (left) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            if (n0 + k < p.nb) o[2 * (e0 + k - p.nb) + 1] = v[k];
                     }
                 } else {
-                    __bf16* o = static_cast<__bf16*>(p.out);
-                    __bf16 hv[4], hy[4];
+                    uint16_t h8[8];
 #pragma unroll
                     for (int k = 0; k < 4; ++k) {
-                        hv[k] = bf16_rne(v[k]);
-                        hy[k] = bf16_rne(y[k]);
-                    }
-                    if (full && n0 + 4 <= p.nb) {
-                        const __bf16 w8[8] = {hv[0], hy[0], hv[1], hy[1], hv[2], hy[2], hv[3], hy[3]};
-                        __builtin_memcpy(o + 2 * e0, w8, sizeof(w8));
-                    } else {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k) {
-                            if (n0 + k >= p.nb) break;
-                            o[2 * (e0 + k)] = hv[k];
-                            if (full) o[2 * (e0 + k) + 1] = hy[k];
+                        if (OUT == OUT_BF16_RP) {
+                            h8[2 * k] = __builtin_bit_cast(uint16_t, bf16_rne(v[k]));
+                            h8[2 * k + 1] = __builtin_bit_cast(uint16_t, bf16_rne(d[k]));
+                        } else {
+                            h8[2 * k] = __half_as_ushort(__float2half_rn(v[k]));
+                            h8[2 * k + 1] = __half_as_ushort(__float2half_rn(d[k]));
                         }
                     }
-                    if (left) {
-#pragma unroll
-                        for (int k = 0; k < 4; ++k)
-                            if (n0 + k < p.nb) o[2 * (e0 + k - p.nb) + 1] = hv[k];
+                    uint16_t* o = static_cast<uint16_t*>(p.out) + 2 * e0;
+                    if (n0 + 4 <= p.nb) {
+                        __builtin_memcpy(o, h8, sizeof(h8));
+                    } else {
+                        for (int k = 0; k < 4 && n0 + k < p.nb; ++k) {
+                            o[2 * k] = h8[2 * k];
+                            o[2 * k + 1] = h8[2 * k + 1];
+                        }
                     }
                 }
             }
-        }
-    }
-    if (p.vmax) {
-        // per-view max |out|: one atomic per wave when its 32 columns share a view
-        const int64_t cl = c < p.ncols ? c : p.ncols - 1;
-        const int view = (int)(cl / p.nt);
-        const int view0 = __shfl(view, 0), view31 = __shfl(view, 31);
-        if (view0 == view31) {
-            for (int o = 32; o >= 1; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
-            if (lane == 0) atomicMax(p.vmax + view0, __float_as_uint(amax));
-        } else if (c < p.ncols) {
-            atomicMax(p.vmax + view, __float_as_uint(amax));
         }
     }
 }
@@ -369,7 +373,9 @@ void launch_rt(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
     const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
-    dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
+    // row-pair blocks advance by 31 columns (one column of overlap)
+    const int64_t cstep = (OUT == OUT_F32_RP || OUT == OUT_BF16_RP || OUT == OUT_F16S_RP) ? 31 : 32;
+    dim3 grid((unsigned)((p.ncols + cstep - 1) / cstep), (unsigned)row_blocks);
     hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT, RT>), grid, dim3(256), 0, st, p);
     SGX_HIP(hipGetLastError());
 }
@@ -389,25 +395,26 @@ void launch_mode(const Params& p, hipStream_t st)
     else launch_rt<SPLIT, OUT, 8>(p, st);
 }
 
-// FP16 storage with a per-view scale (view max maps to 2^15), written in the
-// row-pair gather layout: q[n, t] goes to .x of entry (n, t) and .y of entry
-// (n, t-1); per_view = ns * nt, row = ns.
-__global__ void quantize_f16s_kernel(const float* __restrict__ in, int64_t per_view, int row,
-                                     int na, const unsigned int* __restrict__ vmax,
-                                     __half* __restrict__ out, float* __restrict__ vscale)
+// Per-view max |weighted input|: block (view, y) covers kBoundCols columns of
+// one view, threads stride over the bins (coalesced), one atomic per block.
+constexpr int kBoundCols = 16;
+__global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __restrict__ vbound)
 {
-    const int64_t i0 = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) * 4;
-    const int64_t n = per_view * na;
-    for (int k = 0; k < 4; ++k) {
-        const int64_t i = i0 + k;
-        if (i >= n) break;
-        const int view = (int)(i / per_view);
-        const float m = __uint_as_float(vmax[view]);
-        const float inv = m > 0.f ? 32768.f / m : 0.f;
-        const __half h = __float2half_rn(in[i] * inv);
-        out[2 * i] = h;
-        if ((i % per_view) >= row) out[2 * (i - row) + 1] = h;
-        if (i % per_view == 0) vscale[view] = m / 32768.f;
+    const int view = blockIdx.x;
+    const int t0 = blockIdx.y * kBoundCols;
+    float m = 0.f;
+    for (int t = t0; t < min(t0 + kBoundCols, p.nt); ++t) {
+        const int64_t c = (int64_t)view * p.nt + t;
+        const float* col = p.in + (int64_t)p.nb * c;
+        for (int n = threadIdx.x; n < p.nb; n += 256) m = fmaxf(m, fabsf(col[n] * weight(p, n, c)));
+    }
+    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    __shared__ float red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(vbound + view, __float_as_uint(m));      // m >= 0: bit order = value order
     }
 }
 
@@ -452,7 +459,9 @@ Params make_params(const FilterArgs& a)
     p.gam = a.gam;
     p.bet = a.bet;
     p.gamma_max = a.gamma_max;
-    p.vmax = a.vmax;
+    p.vbound = a.vbound;
+    p.g1 = a.g1;
+    p.vscale = a.vscale;
     p.row_tiles_per_block = 0;
     return p;
 }
@@ -477,7 +486,9 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     const Params p = make_params(a);
     if (a.row_pair && a.extra != 0)
         throw Error(SGX_ERR_ARG, "sino_filter: the row-pair layout has no extra rows");
-    const int out = a.row_pair ? (a.out_bf16 ? OUT_BF16_RP : OUT_F32_RP)
+    if (a.out_f16s && (!a.row_pair || a.split_bf16 || !a.vbound || !a.vscale))
+        throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output is row-pair, FP32 products");
+    const int out = a.row_pair ? (a.out_f16s ? OUT_F16S_RP : a.out_bf16 ? OUT_BF16_RP : OUT_F32_RP)
                                : (a.out_bf16 ? OUT_BF16 : OUT_F32);
     if (a.split_bf16) {
         switch (out) {
@@ -491,19 +502,23 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
         case OUT_F32: launch_mode<false, OUT_F32>(p, st); break;
         case OUT_BF16: launch_mode<false, OUT_BF16>(p, st); break;
         case OUT_F32_RP: launch_mode<false, OUT_F32_RP>(p, st); break;
-        default: launch_mode<false, OUT_BF16_RP>(p, st); break;
+        case OUT_BF16_RP: launch_mode<false, OUT_BF16_RP>(p, st); break;
+        default: launch_mode<false, OUT_F16S_RP>(p, st); break;
         }
     }
 }
 
-void launch_quantize_f16s(const float* in, int64_t per_view, int row, int na,
-                          unsigned int* vmax, void* out, float* vscale, hipStream_t st)
+void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t st)
 {
-    const int64_t n = per_view * na;
-    if (n <= 0) return;
-    const int64_t threads = (n + 3) / 4;
-    hipLaunchKernelGGL(quantize_f16s_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256),
-                       0, st, in, per_view, row, na, vmax, static_cast<__half*>(out), vscale);
+    if (a.ncols <= 0) return;
+    const int64_t na = a.ncols / a.nt;
+    launch_zero_u32(vbound, na, st);
+    const Params p = make_params(a);
+    if ((a.nt + kBoundCols - 1) / kBoundCols > 65535)
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many rows per view");
+    hipLaunchKernelGGL(bound_kernel, dim3((unsigned)na,
+                                          (unsigned)((a.nt + kBoundCols - 1) / kBoundCols)),
+                       dim3(256), 0, st, p, vbound);
     SGX_HIP(hipGetLastError());
 }
 

@@ -480,9 +480,9 @@ struct sgx_plan {
     void* tables = nullptr;       // device: cs, gam, bet, axes, g, wbin, wview, s/t axes
     uint8_t* mask = nullptr;      // device (or null)
     void* q = nullptr;            // filtered data (ns, nt, na)
-    float* qtmp = nullptr;        // FP32 filtered data before FP16 scaling (F16S form)
-    unsigned int* vmax = nullptr; // per-view max |filtered| (F16S form)
-    float* vscale = nullptr;      // per-view FP16 scale (F16S form)
+    unsigned int* vbound = nullptr; // per-view max |weighted input| (F16S form)
+    float* vscale = nullptr;        // per-view FP16 scale (F16S form)
+    float g1 = 0.f;                 // sum |g| (F16S output bound)
     float* partial = nullptr;
     int64_t partial_cap = 0;
     // table offsets
@@ -503,8 +503,7 @@ struct sgx_plan {
         if (tables) (void)hipFree(tables);
         if (mask) (void)hipFree(mask);
         if (q) (void)hipFree(q);
-        if (qtmp) (void)hipFree(qtmp);
-        if (vmax) (void)hipFree(vmax);
+        if (vbound) (void)hipFree(vbound);
         if (vscale) (void)hipFree(vscale);
         if (partial) (void)hipFree(partial);
     }
@@ -526,7 +525,12 @@ struct sgx_plan {
         a.out_bf16 = prec == SGX_PREC_BF16;
         a.row_pair = is3d() && out == q;
         a.split_bf16 = prec == SGX_PREC_BF16;
-        a.vmax = prec == SGX_PREC_F16S ? vmax : nullptr;
+        if (prec == SGX_PREC_F16S) {
+            a.out_f16s = true;
+            a.vbound = vbound;
+            a.g1 = g1;
+            a.vscale = vscale;
+        }
         a.scale = (float)G.angle_weight;
         a.wbin = wbin;
         a.wview = wview;
@@ -597,6 +601,9 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         const auto gd = filter_kernel_g(Hk, npad);
         std::vector<float> gf(gd.begin(), gd.end());
         const size_t o_g = pack.add(gf.data(), npad * 4);
+        double g1 = 0.0;
+        for (float v : gf) g1 += std::fabs((double)v);
+        P->g1 = (float)g1;   // |q| <= g1 max|input|; FP16 keeps 2x headroom above the bound
         // detector axis s (and t), fan angles gamma
         std::vector<double> s(ns), gamv(ns);
         for (int64_t i = 0; i < ns; ++i)
@@ -660,8 +667,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na + sgx::kQ16Pad));
         SGX_HIP(hipMemset(P->q, 0, esz * ns * nt * na + sgx::kQ16Pad));
         if (prec == SGX_PREC_F16S) {
-            SGX_HIP(hipMalloc(&P->qtmp, 4 * ns * nt * na));
-            SGX_HIP(hipMalloc(&P->vmax, 4 * na));
+            SGX_HIP(hipMalloc(&P->vbound, 4 * na));
             SGX_HIP(hipMalloc(&P->vscale, 4 * na));
         }
         if (!is3d) {
@@ -717,14 +723,9 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
         hipStream_t st = ctx->stream;
 
         SGX_HIP(hipEventRecord(P.ev[0], st));
-        if (P.prec == SGX_PREC_F16S) {
-            sgx::launch_zero_u32(P.vmax, G.na, st);
-            sgx::launch_sino_filter(P.filter_args(in, P.qtmp), st);
-            sgx::launch_quantize_f16s(P.qtmp, G.ns * G.nt, (int)G.ns, (int)G.na, P.vmax, P.q,
-                                      P.vscale, st);
-        } else {
-            sgx::launch_sino_filter(P.filter_args(in, P.q), st);
-        }
+        const sgx::FilterArgs fa = P.filter_args(in, P.q);
+        if (P.prec == SGX_PREC_F16S) sgx::launch_filter_bound(fa, P.vbound, st);
+        sgx::launch_sino_filter(fa, st);
         SGX_HIP(hipEventRecord(P.ev[1], st));
         if (P.is3d()) {
             sgx::CbctArgs a{};
@@ -808,6 +809,26 @@ int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales)
                 for (int64_t i = 0; i < G.na; ++i) view_scales[i] = 1.0f;
             }
         }
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int64_t sgx_plan_stored_bytes(const sgx_plan* plan)
+{
+    if (!plan) return 0;
+    const sgx_geom& G = plan->geom;
+    const int64_t esz = (plan->prec == SGX_PREC_FP32 ? 4 : 2) * (plan->is3d() ? 2 : 1);
+    return esz * G.ns * G.nt * G.na;
+}
+
+int sgx_plan_read_stored(sgx_plan* plan, void* host, int64_t bytes)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(host, SGX_ERR_ARG, "null pointer argument");
+        require(bytes == sgx_plan_stored_bytes(plan), SGX_ERR_SIZE, "size mismatch");
+        SGX_HIP(hipMemcpyAsync(host, plan->q, bytes, hipMemcpyDeviceToHost, ctx->stream));
         SGX_HIP(hipStreamSynchronize(ctx->stream));
     });
 }

@@ -45,7 +45,7 @@ struct CbctArgs {
     const void* q;          // (ns, nt, na) float, bf16 or scaled fp16; 16-bit data must be
                             // followed by >= 4 bytes of readable padding
     int fmt;                // SGX_Q_FMT_*
-    bool row_pair;          // q in the row-pair gather layout (2 values per entry)
+    bool row_pair;          // q in the row-pair difference layout (2 values per entry)
     const float* vscale;    // per used view, for SGX_Q_FMT_F16S (else null)
     int ns, nt;
     int nviews;             // views used (after ia_skip)
@@ -59,6 +59,7 @@ struct CbctArgs {
     float* image;
     float dsd_ds, dso_ds, ws, wt, inv_dt_ds;
     bool is_arc;
+    bool xcd_order = false; // renumber blocks so each XCD runs a compact sub-volume
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 // Bytes of readable padding the 16-bit forms need after the last view.
@@ -96,7 +97,9 @@ struct FilterArgs {
     int extra;              // zero rows appended to each output column
     void* out;              // (nb + extra, ncols), float or bf16
     bool out_bf16;
-    bool row_pair;          // write the row-pair gather layout (entry (n,t) = q[n,t], q[n,t+1])
+    bool out_f16s;          // scaled FP16 (row-pair only): q / vscale[view]
+    bool row_pair;          // write the row-pair difference layout
+                            // (entry (n,t) = q[n,t], q[n,t+1] - q[n,t])
     bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
     // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
     float scale;
@@ -109,16 +112,20 @@ struct FilterArgs {
     const double* gam;      // nb (parker) or null
     const double* bet;      // na, beta - min(beta), clamped >= 0 (parker)
     double gamma_max;
-    unsigned int* vmax;     // per-view max |out| (float bits), atomically raised; or null
+    // scaled FP16: vbound[view] = max |weighted input| of the view (float
+    // bits); the output scale is the power of two vscale[view] >= that bound
+    // times g1 / 2^14, so |q| / vscale <= 2^14 and the differences fit FP16
+    const unsigned int* vbound;
+    float g1;               // sum |g| over the band the filter uses
+    float* vscale;          // written per view
 };
 void launch_sino_filter(const FilterArgs& a, hipStream_t st);
 // The fused pre-weights of a filter launch, (nb, ncols) FP32.
 void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st);
 
-// FP16 with a per-view scale from vmax (view max -> 2^15), row-pair layout;
-// writes vscale.  per_view = ns * nt, row = ns.
-void launch_quantize_f16s(const float* in, int64_t per_view, int row, int na,
-                          unsigned int* vmax, void* out, float* vscale, hipStream_t st);
+// Per-view max |weighted input| of a filter launch (float bits) into
+// vbound[na], zeroed first: the bound the scaled-FP16 output is sized by.
+void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t st);
 // Fill with zeros by a kernel (usable under stream capture).
 void launch_zero_u32(unsigned int* p, int64_t n, hipStream_t st);
 

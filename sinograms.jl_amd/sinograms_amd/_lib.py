@@ -42,6 +42,7 @@ EXPORTS = (
     "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
+    "sgx_plan_stored_bytes", "sgx_plan_read_stored",
     "sgx_project_bdd", "sgx_backproject_bdd",
 )
 
@@ -115,6 +116,9 @@ def _declare(L):
     L.sgx_plan_filtered.restype = vp
     L.sgx_plan_view_weight.argtypes = [vp, vp]
     L.sgx_plan_read_filtered.argtypes = [vp, vp, vp]
+    L.sgx_plan_stored_bytes.argtypes = [vp]
+    L.sgx_plan_stored_bytes.restype = ctypes.c_int64
+    L.sgx_plan_read_stored.argtypes = [vp, vp, ctypes.c_int64]
     L.sgx_project_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, vp, u32]
     L.sgx_backproject_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, i32, vp,
                                       u32]

@@ -319,21 +319,33 @@ def test_fdk_16bit_storage_forms_error(sg, orc, torch_cuda):
     assert errs["bf16"] < 5e-3          # BF16 storage: recorded, above 1e-4
 
 
-def test_fdk_f16s_storage_exact_vs_oracle(sg, orc, torch_cuda):
-    # The FP16 form's backprojection reads exactly the quantized data the
-    # oracle gets: dequantize the plan's stored projections and compare.
-    _say("fdk f16s dequantized parity")
+@pytest.mark.parametrize("prec", ["fp32", "f16s", "bf16"])
+def test_fdk_stored_pairs_exact_vs_oracle(sg, orc, torch_cuda, prec):
+    # The backprojection reads exactly the stored row-pair difference data:
+    # take the plan's stored (value, difference) pairs, widen them exactly and
+    # backproject them with the oracle's pair-reading checker.
+    _say("fdk stored pairs parity", prec)
     rg = sg.CtFanArc(ns=64, nt=48, ds=1.0, dt=1.2, na=40, dsd=300.0, dod=100.0)
     ig = sg.ImageGeom((24, 24, 16))
     rng = np.random.default_rng(7)
     proj = rng.standard_normal(rg.dims()).astype(np.float32)
-    plan = sg.plan_fbp(rg, ig, precision="f16s")
+    plan = sg.plan_fbp(rg, ig, precision=prec)
     got = sg.fdk(plan, proj)
-    q, scale = plan.read_filtered()
-    q = q.astype(np.float64) * scale[None, None, :]
+    a, d, scale = plan.read_pairs()
+    assert np.all(d[:, -1, :] == 0)
+    if prec == "fp32":
+        # layout: d is the Float32 difference of the stored neighbours
+        assert np.array_equal(d[:, :-1, :], a[:, 1:, :] - a[:, :-1, :])
+    else:
+        assert scale.dtype == np.float32 and np.all(scale > 0)
+        if prec == "f16s":      # powers of two
+            assert np.all(np.frexp(scale)[0] == 0.5)
+            assert np.abs(a).max() <= 2.0 ** 14 and np.abs(d).max() <= 2.0 ** 15
+    a = a * scale[None, None, :]
+    d = d * scale[None, None, :]
     xc, yc, zc = ig.axes()
-    ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
-                        xc, yc, zc, ig.mask, dtype=np.float32)
+    ref = orc.cbct_back_pairs(a, d, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, True,
+                              xc, yc, zc, ig.mask, dtype=np.float32)
     assert rel_rmse(got, ref) < TOL32
 
 
