@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slices", type=int, default=128)
+    ap.add_argument("--cbct-kernel", choices=("staged", "gather"), default="staged",
+                    help="cone-beam backprojector (LDS-staged windows or global gathers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -157,6 +159,7 @@ def main():
     updates = int(np.prod(ig.dims)) * rg.na
 
     log("planning ...")
+    sg._lib.context(0).cbct_kernel = args.cbct_kernel
     forms = ("fp32", "bf16", "f16s")
     plans = {f: sg.plan_fbp(rg, ig, precision=f) for f in forms}
     outs = {f: sg.recon._empty_colmajor(ig.dims, dev) for f in forms}
@@ -176,7 +179,7 @@ def main():
     achieved = 16.0 * updates / (b32ms * 1e-3) / 1e9
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "cbct_back FP32 (gather-equivalent 16 B/update)",
+            "kernel": f"cbct_back FP32, {args.cbct_kernel} (gather-equivalent 16 B/update)",
             "kernel_ms": b32ms, "filter_ms": f32ms}
     filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
 

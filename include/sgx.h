@@ -79,6 +79,14 @@ const char* sgx_last_error(const sgx_ctx* ctx);
 int sgx_ctx_set_stream(sgx_ctx* ctx, void* hip_stream);
 int sgx_ctx_synchronize(sgx_ctx* ctx);
 
+/* Kernel choices of a context (no counterpart in the reference; the results
+ * agree to FP32 rounding whatever the choice).
+ *   SGX_OPT_CBCT_KERNEL: 0 = LDS-staged cone-beam backprojector (default),
+ *                        1 = gather backprojector. */
+enum { SGX_OPT_CBCT_KERNEL = 1 };
+int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value);
+int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value);
+
 /* ---- backprojectors ------------------------------------------------------ */
 
 /* fbp_back_par!(image, sino, sinphi, cosphi, ds, offset, xc, yc, mask; ia_skip)
@@ -197,21 +205,25 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview);
  * (ns, nt, na) Float32 (nt = 1 for 2-D); image is (nx, ny, nz). */
 int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t flags);
 
-/* The filtered, weighted data of the last execute (device pointer).  2-D
- * plans: column-major (ns, na) Float32.  Cone-beam plans: the row-pair
- * difference layout, where entry (is, it, ia) holds the two values
- * q[is, it, ia] and q[is, it+1, ia] - q[is, it, ia] (0 for it = nt-1), as
- * Float32, bfloat16 or scaled FP16 per the precision. */
+/* The filtered, weighted data of the last execute (device pointer):
+ * column-major (pitch, nt, na) (nt = 1 for 2-D plans) of the plan's storage
+ * type (Float32, bfloat16 or scaled FP16 per the precision), where pitch =
+ * sgx_plan_pitch (ns for 2-D plans, ns rounded up to a multiple of 4 for
+ * cone-beam plans; the pad bins are 0). */
 const void* sgx_plan_filtered(const sgx_plan* plan);
+
+/* Elements per detector row of the stored filtered data. */
+int64_t sgx_plan_pitch(const sgx_plan* plan);
 
 /* Copy the stored filtered data of the last execute to host memory: `host`
  * receives ns*nt*na elements of the plan's storage type (4 bytes for
- * SGX_PREC_FP32, 2 bytes otherwise; for cone-beam plans the first value of
- * each entry) and `view_scales` (na floats, may be NULL) the per-view factor
+ * SGX_PREC_FP32, 2 bytes otherwise; column-major, pad bins dropped) and
+ * `view_scales` (na floats, may be NULL) the per-view factor
  * that multiplies the stored values (1 except for SGX_PREC_F16S). */
 int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales);
 
-/* Size in bytes of the stored filtered data (the layout of sgx_plan_filtered). */
+/* Size in bytes of the stored filtered data (the layout of sgx_plan_filtered,
+ * pad bins included). */
 int64_t sgx_plan_stored_bytes(const sgx_plan* plan);
 
 /* Copy the stored filtered data as-is (`bytes` must equal

@@ -9,23 +9,36 @@
 //   t' = mag z,  bs = s' + ws,  bt = t' / (dt/ds) + wt
 //   if 1 <= floor(bs) < ns and 1 <= floor(bt) < nt: add w2 * bilinear(q)
 //
-// Mapping.  One lane owns one (x, y) column and ZB voxels along z; a
-// wavefront owns an 8 x 8 patch of columns, so the 64 lanes of every gather
-// touch a ~17-bin x 2-row window of one view (a few cache lines).  All
-// z-independent work (rotation, the one divide, the arc atan2, w2, bs, the s
-// weights and the row base address) is done once per column and view and
-// amortised over ZB voxels; per voxel the loop is one FMA for bt, the floor /
-// bounds logic, one row-pair load (taps is and is+1 of rows it and it+1) and
-// four FMAs.  Per-view cos/sin are wave-uniform and read through the scalar
-// cache.  Accumulation is in registers (FP32); each voxel is written once, so
-// no atomics.
+// Storage.  q is column-major (ns, nt, na) with a row pitch of `pitch`
+// elements (>= ns), in FP32, bfloat16, or FP16 with a per-view scale folded
+// into w2 (the 16-bit forms).
 //
-// Storage.  q is FP32, bfloat16, or FP16 with a per-view scale (the 16-bit
-// forms), either column-major (the direct entry point) or in the row-pair
-// difference layout the plan's filter writes: entry (is, it) of a view holds
-// (q[is, it], q[is, it+1] - q[is, it]), so the four taps of an update are one
-// 16-byte (FP32) or 8-byte (16-bit) load and the t interpolation is one FMA
-// per column.
+// Two kernels share the per-voxel arithmetic:
+//
+// * cbct_staged_kernel (default).  A block (4 waves) owns a 16 x 16 patch of
+//   (x, y) columns and 16 voxels along z; every lane owns one column and the
+//   block's 16 z.  For every view the block's footprint on the detector (a
+//   window of at most kWmax bins x kHmax rows, taken from a per-(block, view)
+//   table that cbct_window_kernel computes from the block's corners) is
+//   staged in LDS as FP32 pairs (q[is, it], q[is, it+1] - q[is, it]), with
+//   16-byte coalesced loads and the next view's window loaded while the
+//   current view is backprojected (two LDS buffers, one barrier per view).
+//   An update is then one ds_read2_b64 (taps is and is+1 of rows it and
+//   it+1), an FMA per column for the t interpolation and two FMAs into the
+//   accumulator with the s weights and w2 folded per view.  The texture
+//   addresser sees about 12 wide loads per block and view instead of one
+//   64-lane gather per 64 updates, which is what bounds the gather kernel.
+//   A lane whose taps fall outside the staged window (which the corner bound
+//   rules out up to rounding, or when the window is larger than the LDS
+//   buffer) and lanes at the detector's top or bottom edge take the gather
+//   path below for that view, so the result never depends on the window.
+//
+// * cbct_back_kernel (gather).  Each wave owns an 8 x 8 patch of columns and
+//   8 voxels along z; taps are gathered from global memory, two row loads
+//   per update through one buffer resource per view (the second row through
+//   the SGPR offset).
+//
+// Both accumulate in FP32 registers and write each voxel once (no atomics).
 
 #include "sgx_internal.h"
 #include "sgx.h"
@@ -37,30 +50,30 @@
 namespace sgx {
 namespace {
 
-constexpr int kWaves = 4;      // waves per block, each on its own z-slab
-constexpr int kPatch = 8;      // 8 x 8 columns per wave
-
 // Storage formats of the filtered projections.
 enum { F_F32 = 0, F_BF16 = 1, F_F16S = 2 };
 
-// Raw buffer resource over one view: the 32-bit offsets of the gathers are
-// VGPR byte offsets and the second detector row is reached through the
-// wave-uniform SGPR offset, so both taps of a row pair share one address.
-// Loads at offsets beyond num_records return 0 instead of faulting.
+template <int FMT> struct Fmt {
+    static constexpr uint32_t esz = FMT == F_F32 ? 4 : 2;    // bytes per sample
+    static constexpr uint32_t pad = FMT == F_F32 ? 0 : 4;    // readable bytes past a view
+};
+
+// Raw buffer resource over one view.  Loads at offsets at or beyond
+// num_records return 0 without touching memory.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t view_rsrc(const char* p, uint32_t bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(p), 0, (int)bytes, 0x00020000);
 }
 
+// Byte offset that no view's buffer resource covers (every view is below
+// 2 GiB): a buffer load there returns zeros.
+constexpr uint32_t kMissOffset = 0x80000000u;
+
 // 16-bit storage: the pair (is, is+1) starts at a 2-byte-aligned offset, so
 // it is taken from the dword-aligned 8-byte window that contains it (one
 // aligned load plus one v_alignbit).  The window can reach 4 bytes past the
-// pair: every 16-bit buffer handed to this kernel has 4 bytes of padding
-// after its last view (plan buffers are allocated so; the direct entry point
-// stages into a padded copy) and the views' resource covers them.  When the
-// row pitch is a multiple of 4 bytes the second row shares the first row's
-// window offset (through the SGPR offset); otherwise its offset is formed
-// separately.
+// pair: every 16-bit buffer handed to these kernels has kQ16Pad bytes of
+// padding after its last view and the views' resource covers 4 of them.
 __device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t r, uint32_t voff,
                                            uint32_t soff)
 {
@@ -68,38 +81,25 @@ __device__ __forceinline__ uint32_t pair16(__amdgpu_buffer_rsrc_t r, uint32_t vo
     return __builtin_amdgcn_alignbit(u[1], u[0], (voff & 2u) << 3);
 }
 
-__device__ __forceinline__ float2 dec_bf16(uint32_t u)
+__device__ __forceinline__ float bf16f(uint32_t h) { return __uint_as_float(h << 16); }
+__device__ __forceinline__ float f16f(uint32_t h)
 {
-    return make_float2(__uint_as_float(u << 16), __uint_as_float(u & 0xffff0000u));
+    return __half2float(__ushort_as_half((unsigned short)h));
 }
 
-__device__ __forceinline__ float2 dec_f16(uint32_t u)
+template <int FMT> __device__ __forceinline__ float2 dec2(uint32_t u)
 {
-    return make_float2(__half2float(__ushort_as_half((unsigned short)(u & 0xffffu))),
-                       __half2float(__ushort_as_half((unsigned short)(u >> 16))));
+    if (FMT == F_BF16) return make_float2(bf16f(u & 0xffffu), bf16f(u >> 16));
+    return make_float2(f16f(u & 0xffffu), f16f(u >> 16));
 }
 
-// The four taps of one update as (a0, d0, a1, d1): a = q[is, it] and
-// d = q[is, it+1] - q[is, it] for the two columns is (0) and is+1 (1).
-// Row-pair layout: entry (is, it) of a view already holds (a, d) for column
-// is, so the two entries (is, it), (is+1, it) are one 16-byte (FP32) or
-// 8-byte (16-bit) load.  Column-major layout: two row loads (row it+1
-// through the SGPR offset) and the differences formed here.
-template <int FMT, bool PITCH4, bool RP>
+// The four taps of one update from global memory as (a0, d0, a1, d1):
+// a = q[is, it] and d = q[is, it+1] - q[is, it] for the columns is (0) and
+// is+1 (1); voff is the byte offset of (is, it) in the view, rowb the row
+// pitch in bytes (a multiple of 8 for the 16-bit forms).
+template <int FMT>
 __device__ __forceinline__ float4 taps(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t rowb)
 {
-    if (RP) {
-        if (FMT == F_F32) {
-            const auto u = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
-            return make_float4(__uint_as_float(u[0]), __uint_as_float(u[1]),
-                               __uint_as_float(u[2]), __uint_as_float(u[3]));
-        } else {
-            const auto u = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
-            const float2 e0 = FMT == F_BF16 ? dec_bf16(u[0]) : dec_f16(u[0]);
-            const float2 e1 = FMT == F_BF16 ? dec_bf16(u[1]) : dec_f16(u[1]);
-            return make_float4(e0.x, e0.y, e1.x, e1.y);
-        }
-    }
     float2 lo, hi;
     if (FMT == F_F32) {
         const auto a = __builtin_amdgcn_raw_buffer_load_b64(r, voff, 0, 0);
@@ -107,62 +107,109 @@ __device__ __forceinline__ float4 taps(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
         lo = make_float2(__uint_as_float(a[0]), __uint_as_float(a[1]));
         hi = make_float2(__uint_as_float(b[0]), __uint_as_float(b[1]));
     } else {
-        const uint32_t a = pair16(r, voff, 0);
-        const uint32_t b = PITCH4 ? pair16(r, voff, rowb) : pair16(r, voff + rowb, 0);
-        lo = FMT == F_BF16 ? dec_bf16(a) : dec_f16(a);
-        hi = FMT == F_BF16 ? dec_bf16(b) : dec_f16(b);
+        lo = dec2<FMT>(pair16(r, voff, 0));
+        hi = dec2<FMT>(pair16(r, voff, rowb));
     }
     return make_float4(lo.x, hi.x - lo.x, lo.y, hi.y - lo.y);
 }
 
-template <int FMT, bool RP> struct Tap {
-    // bytes per stored sample (column-major) or per row-pair entry
-    static constexpr uint32_t bytes = (FMT == F_F32 ? 4 : 2) * (RP ? 2 : 1);
-    // readable padding the gathers may touch past the last view
-    static constexpr uint32_t pad = (FMT != F_F32 && !RP) ? 4 : 0;
+// Per-view, z-independent part of one column's update.
+struct ColView {
+    bool hit;        // 1 <= floor(bs) < ns
+    float isf;       // floor(bs)
+    float wa, wb;    // w2 (1 - wr), w2 wr
+    float mz;        // dbt / dz
 };
 
-// Byte offset that no view's buffer resource covers (every view is below
-// 2 GiB): a buffer load there returns zeros without touching memory.
-constexpr uint32_t kMissOffset = 0x80000000u;
-
-// Block order.  Workgroups are handed to the eight XCDs round-robin, and
-// each XCD has its own L2.  The physical block index is renumbered so that
-// XCD k runs one contiguous run of logical blocks (a compact z-slab and
-// y-band of the volume), whose footprint on each view is a fraction of the
-// detector, instead of every eighth tile of the whole volume.
-__device__ __forceinline__ int xcd_order(int phys, int nblocks)
+template <bool ARC, int FMT>
+__device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float dsd_ds,
+                                            float dso_ds, float ws, float nsf, float inv_dt_ds,
+                                            float vs)
 {
-    const int q = nblocks >> 3, r = nblocks & 7;
-    const int k = phys & 7, slot = phys >> 3;
-    return k * q + min(k, r) + slot;
+    ColView r;
+    const float xb = x * v.c + y * v.s;
+    const float yb = dso_ds - (-x * v.s + y * v.c);
+    const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
+    float sp, w2;
+    if (ARC) {
+        sp = dsd_ds * atan2f(xb, yb);
+        w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
+    } else {
+        sp = mag * xb;
+        w2 = mag * mag;
+    }
+    if (FMT == F_F16S) w2 *= vs;
+    const float bs = sp + ws;
+    r.hit = bs >= 1.f && bs < nsf;          // 1 <= floor(bs) < ns (false for NaN)
+    r.isf = floorf(bs);
+    r.wb = w2 * (bs - r.isf);
+    r.wa = w2 - r.wb;
+    r.mz = mag * inv_dt_ds;
+    return r;
 }
 
-// Per update the loop below costs one FMA (bt), a truncating convert, a
-// fract, one 24-bit multiply-add for the byte offset, one buffer load (row
-// pair layout) and four FMAs: p0 = a0 + wu d0, p1 = a1 + wu d1, then
-// acc += wa p0 + wb p1 with the s weights and w2 folded per view
-// (wa = w2 (1 - wr), wb = w2 wr).
-// The bounds tests are hoisted: bt is linear in z, so when bt at the lane's
-// lowest and highest z are both in [1, nt) every voxel of the lane is in
-// range; only lanes at the detector's top or bottom edge take the checked
-// path, where a voxel out of range loads zeros (offset kMissOffset).
-template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
+// One view of ZB voxels from global memory.  `fast`: every voxel's bt is in
+// [1, nt); otherwise voxels out of range load zeros (offset kMissOffset)
+// and weigh 0.
+template <int ZB, int FMT>
+__device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[ZB], float wt,
+                                            float ntf, bool fast, __amdgpu_buffer_rsrc_t rs,
+                                            uint32_t rowb, float (&acc)[ZB])
+{
+    // byte offset of (is-1, it-1) is it*rowb + (is-1)*esz - rowb
+    const uint32_t colb = ((uint32_t)cv.isf - 1u) * Fmt<FMT>::esz - rowb;
+    constexpr int C = ZB < 8 ? ZB : 8;                 // voxels per batch of loads
+#pragma unroll
+    for (int h = 0; h < ZB; h += C) {
+        float4 tp[C];
+        float wu[C];
+        if (fast) {
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const float bt = fmaf(cv.mz, z[h + j], wt);
+                wu[j] = __builtin_amdgcn_fractf(bt);
+                tp[j] = taps<FMT>(rs, __umul24((uint32_t)bt, rowb) + colb, rowb);
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                const float bt = fmaf(cv.mz, z[h + j], wt);
+                const bool ok = bt >= 1.f && bt < ntf;       // 1 <= floor(bt) < nt
+                wu[j] = ok ? __builtin_amdgcn_fractf(bt) : 0.f;
+                const uint32_t boff = ok ? __umul24((uint32_t)bt, rowb) + colb : kMissOffset;
+                tp[j] = taps<FMT>(rs, boff, rowb);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < C; ++j) {
+            const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
+            const float p1 = fmaf(wu[j], tp[j].w, tp[j].z);
+            acc[h + j] = fmaf(cv.wb, p1, fmaf(cv.wa, p0, acc[h + j]));
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// gather kernel
+// ---------------------------------------------------------------------------
+
+constexpr int kWaves = 4;      // waves per block, each on its own z-slab
+constexpr int kPatch = 8;      // 8 x 8 columns per wave
+
+template <int ZB, bool ARC, int FMT>
 __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
-    const void* __restrict__ q, int ns, int nt, int nviews, int64_t view_stride,
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds, int remap)
+    float inv_dt_ds)
 {
-    using T = Tap<FMT, RP>;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (nx + kPatch - 1) / kPatch;
     const int tiles = tiles_x * ((ny + kPatch - 1) / kPatch);
-    const int blk = remap ? xcd_order(blockIdx.x, gridDim.x) : blockIdx.x;
-    const int tile = blk % tiles, zblk = blk / tiles;
+    const int tile = blockIdx.x % tiles, zblk = blockIdx.x / tiles;
     const int ix = (tile % tiles_x) * kPatch + (lane & (kPatch - 1));
     const int iy = (tile / tiles_x) * kPatch + (lane / kPatch);
     const int iz0 = (zblk * kWaves + wave) * ZB;
@@ -181,83 +228,28 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         z[j] = zds[min(iz, nz - 1)];
         acc[j] = 0.f;
     }
-    if (!any) {
+    if (any) {
+        float zlo = z[0], zhi = z[0];
 #pragma unroll
-        for (int j = 0; j < ZB; ++j)
-            if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = 0.f;
-        return;
-    }
-    float zlo = z[0], zhi = z[0];
-#pragma unroll
-    for (int j = 1; j < ZB; ++j) {
-        zlo = fminf(zlo, z[j]);
-        zhi = fmaxf(zhi, z[j]);
-    }
-
-    const float x = xds[ix], y = yds[iy];
-    const float dsd2 = dsd_ds * dsd_ds;
-    const float nsf = (float)ns, ntf = (float)nt;
-    const uint32_t rowb = (uint32_t)ns * T::bytes;           // bytes per detector row
-    // bytes per view (+ padding reachable by the 16-bit window loads)
-    const uint32_t vbytes = (uint32_t)nt * rowb + T::pad;
-    const int64_t viewb = view_stride * (int64_t)(FMT == F_F32 ? 4 : 2) * (RP ? 2 : 1);
-
-    const char* v0 = static_cast<const char*>(q);
-    for (int ia = 0; ia < nviews; ++ia, v0 += viewb) {
-        const ViewCS v = cs[ia];
-        const float xb = x * v.c + y * v.s;
-        const float yb = dso_ds - (-x * v.s + y * v.c);
-        const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
-        float sp, w2;
-        if (ARC) {
-            sp = dsd_ds * atan2f(xb, yb);
-            w2 = dsd2 * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
-        } else {
-            sp = mag * xb;
-            w2 = mag * mag;
+        for (int j = 1; j < ZB; ++j) {
+            zlo = fminf(zlo, z[j]);
+            zhi = fmaxf(zhi, z[j]);
         }
-        if (FMT == F_F16S) w2 *= vscale[ia];
-        const float bs = sp + ws;
-        // 1 <= floor(bs) < ns  <=>  1 <= bs < ns (also false for NaN)
-        if (!(bs >= 1.f && bs < nsf)) continue;          // the column misses the detector
-        const float isf = floorf(bs);
-        const float wb = w2 * (bs - isf);
-        const float wa = w2 - wb;
-        // byte offset of (is-1, it-1) is it*rowb + (is-1)*bytes - rowb
-        const uint32_t colb = ((uint32_t)isf - 1u) * T::bytes - rowb;
-        const float mz = mag * inv_dt_ds;
-        const __amdgpu_buffer_rsrc_t rs = view_rsrc(v0, vbytes);
-
-        float4 tp[ZB];
-        float wu[ZB];
-        const float blo = fmaf(mz, zlo, wt), bhi = fmaf(mz, zhi, wt);
-        if (blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf) {
-#pragma unroll
-            for (int j = 0; j < ZB; ++j) {
-                const float bt = fmaf(mz, z[j], wt);
-                const uint32_t it = (uint32_t)bt;               // floor, bt >= 1
-                wu[j] = __builtin_amdgcn_fractf(bt);
-                tp[j] = taps<FMT, PITCH4, RP>(rs, __umul24(it, rowb) + colb, rowb);
-            }
-        } else {
-#pragma unroll
-            for (int j = 0; j < ZB; ++j) {
-                const float bt = fmaf(mz, z[j], wt);
-                const bool ok = bt >= 1.f && bt < ntf;          // 1 <= floor(bt) < nt
-                const uint32_t it = (uint32_t)(ok ? bt : 1.f);
-                wu[j] = ok ? __builtin_amdgcn_fractf(bt) : 0.f;
-                const uint32_t boff = ok ? __umul24(it, rowb) + colb : kMissOffset;
-                tp[j] = taps<FMT, PITCH4, RP>(rs, boff, rowb);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < ZB; ++j) {
-            const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
-            const float p1 = fmaf(wu[j], tp[j].w, tp[j].z);
-            acc[j] = fmaf(wb, p1, fmaf(wa, p0, acc[j]));
+        const float x = xds[ix], y = yds[iy];
+        const float nsf = (float)ns, ntf = (float)nt;
+        const uint32_t rowb = (uint32_t)pitch * Fmt<FMT>::esz;
+        const uint32_t vbytes = (uint32_t)nt * rowb + Fmt<FMT>::pad;
+        const int64_t viewb = view_stride * (int64_t)Fmt<FMT>::esz;
+        const char* v0 = static_cast<const char*>(q);
+        for (int ia = 0; ia < nviews; ++ia, v0 += viewb) {
+            const ColView cv = col_view<ARC, FMT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
+                                                  inv_dt_ds, FMT == F_F16S ? vscale[ia] : 1.f);
+            if (!cv.hit) continue;                        // the column misses the detector
+            const float blo = fmaf(cv.mz, zlo, wt), bhi = fmaf(cv.mz, zhi, wt);
+            const bool fast = blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf;
+            gather_view<ZB, FMT>(cv, z, wt, ntf, fast, view_rsrc(v0, vbytes), rowb, acc);
         }
     }
-
 #pragma unroll
     for (int j = 0; j < ZB; ++j) {
         const int iz = iz0 + j;
@@ -268,19 +260,269 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     }
 }
 
-template <int ZB, bool ARC, int FMT, bool PITCH4, bool RP>
-void launch(const CbctArgs& a, hipStream_t st)
+// ---------------------------------------------------------------------------
+// staged kernel
+// ---------------------------------------------------------------------------
+
+constexpr int kSP = 16;               // block patch: 16 x 16 columns (2 x 2 waves of 8 x 8)
+constexpr int kSZ = 16;               // voxels per lane along z
+constexpr int kWmax = 48;             // staged window: bins (12 quads of 4) ...
+constexpr int kHmax = 32;             // ... x rows (16 groups of 2)
+constexpr int kLP = 50;               // LDS pitch in entries (rows 4 banks apart)
+constexpr int kQuads = kWmax / 4;
+constexpr int kStagers = kQuads * (kHmax / 2);   // 192 threads stage
+
+// Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H.
+// clo is a multiple of 4; W = H = 0 when nothing is staged.
+__global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __restrict__ cs,
+                                   const float* __restrict__ xds, const float* __restrict__ yds,
+                                   const float* __restrict__ zds, int nx, int ny, int nz, int ns,
+                                   int nt, float dsd_ds, float dso_ds, float ws, float wt,
+                                   float inv_dt_ds, int is_arc, int4* __restrict__ win)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= (int64_t)nblocks * nviews) return;
+    const int blk = (int)(i / nviews), ia = (int)(i % nviews);
+    const int tiles_x = (nx + kSP - 1) / kSP;
+    const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
+    const int tile = blk % tiles, zblk = blk / tiles;
+    const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP, iz0 = zblk * kSZ;
+    // extent of the block's voxel centres (any axis order)
+    float x0 = xds[ix0], x1 = x0, y0 = yds[iy0], y1 = y0, z0 = zds[min(iz0, nz - 1)], z1 = z0;
+    for (int k = 1; k < kSP; ++k) {
+        const float xv = xds[min(ix0 + k, nx - 1)], yv = yds[min(iy0 + k, ny - 1)];
+        x0 = fminf(x0, xv); x1 = fmaxf(x1, xv);
+        y0 = fminf(y0, yv); y1 = fmaxf(y1, yv);
+    }
+    for (int k = 1; k < kSZ; ++k) {
+        const float zv = zds[min(iz0 + k, nz - 1)];
+        z0 = fminf(z0, zv); z1 = fmaxf(z1, zv);
+    }
+    // bs and bt are quasi-linear in (x, y) while y_beta > 0 (a ratio of
+    // linear functions), so their extremes over the box are at its corners.
+    const ViewCS v = cs[ia];
+    float smin = 3.4e38f, smax = -3.4e38f, tmin = 3.4e38f, tmax = -3.4e38f;
+    bool ok = true;
+    for (int k = 0; k < 4; ++k) {
+        const float x = (k & 1) ? x1 : x0, y = (k & 2) ? y1 : y0;
+        const float xb = x * v.c + y * v.s;
+        const float yb = dso_ds - (-x * v.s + y * v.c);
+        ok = ok && yb > 0.f;
+        const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
+        const float sp = is_arc ? dsd_ds * atan2f(xb, yb) : mag * xb;
+        const float bs = sp + ws;
+        smin = fminf(smin, bs); smax = fmaxf(smax, bs);
+        const float mz = mag * inv_dt_ds;
+        const float ta = fmaf(mz, z0, wt), tb = fmaf(mz, z1, wt);
+        tmin = fminf(tmin, fminf(ta, tb)); tmax = fmaxf(tmax, fmaxf(ta, tb));
+    }
+    int4 w = make_int4(0, 0, 0, 0);
+    // entry columns is-1, is for 1 <= is = floor(bs) < ns, one bin of margin
+    // each side; entry rows it-1 for 1 <= it = floor(bt) < nt, likewise
+    if (ok && smax >= 1.f && smin < (float)ns && tmax >= 1.f && tmin < (float)nt) {
+        const int clo = max((int)floorf(fmaxf(smin, 0.f)) - 2, 0) & ~3;
+        const int chi = min((int)floorf(fminf(smax, (float)ns)) + 1, ns - 1);
+        const int rlo = max((int)floorf(fmaxf(tmin, 0.f)) - 2, 0);
+        const int rhi = min((int)floorf(fminf(tmax, (float)nt)), nt - 2);
+        if (chi >= clo && rhi >= rlo)
+            w = make_int4(clo, rlo, min(chi - clo + 1, kWmax), min(rhi - rlo + 1, kHmax));
+    }
+    win[i] = w;
+}
+
+template <bool ARC, int FMT>
+__global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds, const int4* __restrict__ win)
+{
+    __shared__ float2 lds[2 * kHmax * kLP];
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (nx + kSP - 1) / kSP;
+    const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
+    const int blk = blockIdx.x;
+    const int tile = blk % tiles, zblk = blk / tiles;
+    const int ix = (tile % tiles_x) * kSP + (wave & 1) * 8 + (lane & 7);
+    const int iy = (tile / tiles_x) * kSP + (wave >> 1) * 8 + (lane >> 3);
+    const int iz0 = zblk * kSZ;
+    const bool inxy = ix < nx && iy < ny;
+    const int64_t nxy = (int64_t)nx * ny;
+    const int64_t col = inxy ? ix + (int64_t)nx * iy : 0;
+
+    // Every thread stays to the end: all of them stage and meet the barriers.
+    float z[kSZ], acc[kSZ];
+    bool any = false;
+#pragma unroll
+    for (int j = 0; j < kSZ; ++j) {
+        const int iz = iz0 + j;
+        const bool in = inxy && iz < nz && (mask == nullptr || mask[col + nxy * iz]);
+        any |= in;
+        z[j] = zds[min(iz, nz - 1)];
+        acc[j] = 0.f;
+    }
+    float zlo = z[0], zhi = z[0];
+#pragma unroll
+    for (int j = 1; j < kSZ; ++j) {
+        zlo = fminf(zlo, z[j]);
+        zhi = fmaxf(zhi, z[j]);
+    }
+    const float x = inxy ? xds[ix] : 0.f, y = inxy ? yds[iy] : 0.f;
+    const float nsf = (float)ns, ntf = (float)nt;
+    const uint32_t rowb = (uint32_t)pitch * Fmt<FMT>::esz;
+    const uint32_t vbytes = (uint32_t)nt * rowb + Fmt<FMT>::pad;
+    const int64_t viewb = view_stride * (int64_t)Fmt<FMT>::esz;
+    const char* qb = static_cast<const char*>(q);
+    const int4* bwin = win + (int64_t)blk * nviews;
+
+    // Staging role: thread (quad sq, group sg) moves bins clo+4sq .. +3 of
+    // entry rows 2sg, 2sg+1, i.e. data rows 2sg .. 2sg+2 of the window.
+    const bool stager = tid < kStagers;
+    const int sq = tid % kQuads, sg = tid / kQuads;
+    float sv[3][4];
+    auto stage_load = [&](int ia, int4 w) {
+        if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
+        const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
+        const uint32_t off = ((uint32_t)(w.y + 2 * sg) * (uint32_t)pitch +
+                              (uint32_t)(w.x + 4 * sq)) * Fmt<FMT>::esz;
+        const int nrow = min(3, w.w + 1 - 2 * sg);          // data rows needed (2 or 3)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= nrow) break;
+            if (FMT == F_F32) {
+                const auto u = __builtin_amdgcn_raw_buffer_load_b128(rs, off + k * rowb, 0, 0);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) sv[k][e] = __uint_as_float(u[e]);
+            } else {
+                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, off + k * rowb, 0, 0);
+                const float2 e01 = dec2<FMT>(u[0]), e23 = dec2<FMT>(u[1]);
+                sv[k][0] = e01.x; sv[k][1] = e01.y; sv[k][2] = e23.x; sv[k][3] = e23.y;
+            }
+        }
+    };
+    auto stage_store = [&](int buf, int4 w) {
+        if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
+        float2* d = lds + buf * (kHmax * kLP) + (2 * sg) * kLP + 4 * sq;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            if (2 * sg + k >= w.w) break;
+            float4 lo, hi;
+            lo = make_float4(sv[k][0], sv[k + 1][0] - sv[k][0], sv[k][1], sv[k + 1][1] - sv[k][1]);
+            hi = make_float4(sv[k][2], sv[k + 1][2] - sv[k][2], sv[k][3], sv[k + 1][3] - sv[k][3]);
+            *reinterpret_cast<float4*>(d + k * kLP) = lo;
+            *reinterpret_cast<float4*>(d + k * kLP + 2) = hi;
+        }
+    };
+
+    int4 wcur = bwin[0];
+    stage_load(0, wcur);
+    stage_store(0, wcur);
+    __syncthreads();
+    for (int ia = 0; ia < nviews; ++ia) {
+        const int buf = ia & 1;
+        const int4 wnext = ia + 1 < nviews ? bwin[ia + 1] : make_int4(0, 0, 0, 0);
+        stage_load(ia + 1, wnext);                          // in flight during this view
+        if (any) {
+            const ColView cv = col_view<ARC, FMT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
+                                                  inv_dt_ds, FMT == F_F16S ? vscale[ia] : 1.f);
+            if (cv.hit) {
+                const float blo = fmaf(cv.mz, zlo, wt), bhi = fmaf(cv.mz, zhi, wt);
+                const float tlo = fminf(blo, bhi), thi = fmaxf(blo, bhi);
+                const bool fast = tlo >= 1.f && thi < ntf;
+                // window coordinates of the lane's taps: entry columns ce, ce+1,
+                // entry rows floor(bt)-1 over the lane's z range
+                const int ce = (int)cv.isf - 1 - wcur.x;
+                const int rl = (int)tlo - 1 - wcur.y, rh = (int)thi - 1 - wcur.y;
+                if (fast && ce >= 0 && ce + 2 <= wcur.z && rl >= 0 && rh < wcur.w) {
+                    // LDS entry of (is-1, it-1) is it*kLP + e0off (it - 1 - rlo in [rl, rh])
+                    const int e0off = buf * (kHmax * kLP) + ce - (wcur.y + 1) * kLP;
+#pragma unroll
+                    for (int h = 0; h < kSZ; h += 8) {
+                        float2 e0[8], e1[8];
+                        float wu[8];
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const float bt = fmaf(cv.mz, z[h + j], wt);
+                            wu[j] = __builtin_amdgcn_fractf(bt);
+                            const float2* p = lds + ((int)__umul24((uint32_t)bt, (uint32_t)kLP) + e0off);
+                            e0[j] = p[0];
+                            e1[j] = p[1];
+                        }
+#pragma unroll
+                        for (int j = 0; j < 8; ++j) {
+                            const float p0 = fmaf(wu[j], e0[j].y, e0[j].x);
+                            const float p1 = fmaf(wu[j], e1[j].y, e1[j].x);
+                            acc[h + j] = fmaf(cv.wb, p1, fmaf(cv.wa, p0, acc[h + j]));
+                        }
+                    }
+                } else {
+                    gather_view<kSZ, FMT>(cv, z, wt, ntf, fast,
+                                          view_rsrc(qb + ia * viewb, vbytes), rowb, acc);
+                }
+            }
+        }
+        stage_store(buf ^ 1, wnext);
+        __syncthreads();
+        wcur = wnext;
+    }
+
+    if (inxy) {
+#pragma unroll
+        for (int j = 0; j < kSZ; ++j) {
+            const int iz = iz0 + j;
+            if (iz < nz) {
+                const bool in = mask == nullptr || mask[col + nxy * iz];
+                image[col + nxy * iz] = in ? acc[j] : 0.f;
+            }
+        }
+    }
+}
+
+int64_t staged_blocks(const CbctArgs& a)
+{
+    const int64_t tiles = (int64_t)((a.nx + kSP - 1) / kSP) * ((a.ny + kSP - 1) / kSP);
+    return tiles * ((a.nz + kSZ - 1) / kSZ);
+}
+
+template <int ZB, bool ARC, int FMT>
+void launch_gather(const CbctArgs& a, hipStream_t st)
 {
     const int64_t tiles = (int64_t)((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
     const int64_t zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
     if (tiles * zblocks >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
-    dim3 grid((unsigned)(tiles * zblocks)), block(64 * kWaves);
-    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, PITCH4, RP>), grid, block, 0, st, a.q, a.ns,
-                       a.nt, a.nviews, a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx,
-                       a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt,
-                       a.inv_dt_ds, a.xcd_order ? 1 : 0);
+    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT>), dim3((unsigned)(tiles * zblocks)),
+                       dim3(64 * kWaves), 0, st, a.q, a.ns, a.nt, a.pitch, a.nviews,
+                       a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz,
+                       a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds);
     SGX_HIP(hipGetLastError());
+}
+
+template <bool ARC, int FMT>
+void launch_staged(const CbctArgs& a, hipStream_t st)
+{
+    const int64_t nb = staged_blocks(a);
+    if (nb >= (int64_t(1) << 31))
+        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
+    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT>), dim3((unsigned)nb), dim3(256), 0, st,
+                       a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
+                       a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
+                       a.ws, a.wt, a.inv_dt_ds, a.win);
+    SGX_HIP(hipGetLastError());
+}
+
+template <int FMT> void launch_fmt(const CbctArgs& a, hipStream_t st)
+{
+    if (a.win) {
+        if (a.is_arc) launch_staged<true, FMT>(a, st);
+        else launch_staged<false, FMT>(a, st);
+    } else {
+        if (a.is_arc) launch_gather<8, true, FMT>(a, st);
+        else launch_gather<8, false, FMT>(a, st);
+    }
 }
 
 __global__ void view_trig_kernel(const double* __restrict__ betas, int64_t n, int64_t stride,
@@ -306,46 +548,28 @@ void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* ou
     SGX_HIP(hipGetLastError());
 }
 
-void launch_cbct_back(const CbctArgs& args, hipStream_t st)
+int64_t cbct_window_entries(const CbctArgs& a) { return staged_blocks(a) * a.nviews; }
+
+void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
 {
-    constexpr int ZB = 8;
-    // SGX_CBCT_XCD_ORDER=1 selects the XCD-grouped block order (a tuning
-    // switch; on the 256^2 x 128 FDK workload the plain order measured faster).
-    static const bool env_xcd = [] {
-        const char* e = std::getenv("SGX_CBCT_XCD_ORDER");
-        return e && e[0] == '1';
-    }();
-    CbctArgs a = args;
-    if (env_xcd) a.xcd_order = true;
+    const int64_t nb = staged_blocks(a), n = nb * a.nviews;
+    if (n <= 0) return;
+    hipLaunchKernelGGL(cbct_window_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                       (int)nb, a.nviews, a.cs, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.ns, a.nt,
+                       a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, win);
+    SGX_HIP(hipGetLastError());
+}
+
+void launch_cbct_back(const CbctArgs& a, hipStream_t st)
+{
     // Byte offsets within one view are 32-bit.
-    if ((int64_t)a.ns * a.nt * 8 >= (int64_t(1) << 31))
+    if ((int64_t)a.pitch * a.nt * 4 + 64 >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: one view must be below 2 GiB");
-    const bool pitch4 = (a.ns % 2) == 0;       // 16-bit rows a multiple of 4 bytes
-    if (a.row_pair) {
-        if (a.fmt == SGX_Q_FMT_BF16) {
-            if (a.is_arc) launch<ZB, true, F_BF16, true, true>(a, st);
-            else launch<ZB, false, F_BF16, true, true>(a, st);
-        } else if (a.fmt == SGX_Q_FMT_F16S) {
-            if (a.is_arc) launch<ZB, true, F_F16S, true, true>(a, st);
-            else launch<ZB, false, F_F16S, true, true>(a, st);
-        } else {
-            if (a.is_arc) launch<ZB, true, F_F32, true, true>(a, st);
-            else launch<ZB, false, F_F32, true, true>(a, st);
-        }
-    } else if (a.fmt == SGX_Q_FMT_BF16) {
-        if (pitch4) {
-            if (a.is_arc) launch<ZB, true, F_BF16, true, false>(a, st);
-            else launch<ZB, false, F_BF16, true, false>(a, st);
-        } else {
-            if (a.is_arc) launch<ZB, true, F_BF16, false, false>(a, st);
-            else launch<ZB, false, F_BF16, false, false>(a, st);
-        }
-    } else if (a.fmt == SGX_Q_FMT_F16S) {
-        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: scaled FP16 data come in the row-pair layout");
-    } else {
-        if (a.is_arc) launch<ZB, true, F_F32, true, false>(a, st);
-        else launch<ZB, false, F_F32, true, false>(a, st);
-    }
+    if (a.pitch < a.ns || a.pitch % 4 != 0)
+        throw Error(SGX_ERR_ARG, "cbct_back: the row pitch must be >= ns and a multiple of 4");
+    if (a.fmt == SGX_Q_FMT_BF16) launch_fmt<F_BF16>(a, st);
+    else if (a.fmt == SGX_Q_FMT_F16S) launch_fmt<F_F16S>(a, st);
+    else launch_fmt<F_F32>(a, st);
 }
 
 }  // namespace sgx

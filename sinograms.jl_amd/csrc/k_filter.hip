@@ -21,13 +21,9 @@
 //               T*B ~ Th*Bh + Th*Bl + Tl*Bh on v_mfma_f32_32x32x16_bf16 with
 //               FP32 accumulation (split-BF16x3).
 // Output rows n >= nb (the reference's `extra` rows) are written as 0.
-//
-// Row-pair difference layout (cone-beam plans).  Entry (n, t) of a view holds
-// (q[n, t], q[n, t+1] - q[n, t]) in FP32, BF16, or FP16 scaled by a per-view
-// power of two (F16S), the layout the backprojector gathers from.  Columns
-// t and t+1 of a view are neighbouring lanes of one block; so that every
-// lane's right neighbour is in its block, row-pair blocks advance by 31
-// columns and overlap by one (the 32nd column only feeds its left neighbour).
+// Outputs: FP32, BF16 (RNE), or FP16 scaled by a per-view power of two
+// (F16S: the scale bounds the view's output, from a pre-pass over its
+// weighted input), column-major (nb + extra, ncols).
 
 #include "sgx_internal.h"
 #include "sgx.h"
@@ -41,9 +37,8 @@ typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int kKC = 64;            // input rows staged per chunk
-// output layouts: column-major (nb + extra, ncols), or the row-pair
-// difference layout of the cone-beam backprojector
-enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F32_RP = 2, OUT_BF16_RP = 3, OUT_F16S_RP = 4 };
+// output element types, column-major (nb + extra, ncols)
+enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F16S = 2 };
 constexpr int kBPadF = kKC + 1;    // FP32 staging stride (conflict-free column reads)
 constexpr int kBPadH = kKC + 8;    // bf16 staging stride (16-byte aligned fragments)
 
@@ -150,8 +145,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     __bf16* bs_lo = bs_hi + 32 * kBPadH;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    constexpr bool RPOUT = OUT == OUT_F32_RP || OUT == OUT_BF16_RP || OUT == OUT_F16S_RP;
-    const int64_t c0 = (int64_t)blockIdx.x * (RPOUT ? 31 : 32);
+    const int64_t c0 = (int64_t)blockIdx.x * 32;
     const int tile0 = blockIdx.y * 4 * RT;              // first row tile of the block
     const int n_lo = tile0 * 32;
     const int kp = ((p.nb + kKC - 1) / kKC) * kKC;       // padded K
@@ -283,36 +277,13 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
     const int64_t c = c0 + col;
-    if (!RPOUT) {
-        if (c < p.ncols) {
-#pragma unroll
-            for (int r = 0; r < RT; ++r) {
-                const int nbase = (tile0 + wave * RT + r) * 32;
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const int n = nbase + 8 * (i / 4) + 4 * half + (i % 4);
-                    if (n < p.nrows_out) {
-                        const float v = n < p.nb ? acc[r][i] : 0.f;
-                        const int64_t o = n + (int64_t)p.nrows_out * c;
-                        if (OUT == OUT_BF16) static_cast<__bf16*>(p.out)[o] = bf16_rne(v);
-                        else static_cast<float*>(p.out)[o] = v;
-                    }
-                }
-            }
-        }
-    } else {
-        // Row-pair difference layout: entry (n, t) = (q[n, t], q[n, t+1] - q[n, t]).
-        // Column t+1 is the next lane (col + 1 <= 31 within the block); the
-        // second half of a view's last row is written as 0 (never gathered).
-        const int t = (int)(c % p.nt);
-        const bool own = c < p.ncols && col < 31;
-        const bool has_next = t < p.nt - 1;
+    if (c < p.ncols) {
         float inv = 1.f;
-        if (OUT == OUT_F16S_RP && c < p.ncols) {
+        if (OUT == OUT_F16S) {
             const int view = (int)(c / p.nt);
             const float sc = f16s_scale(p.vbound[view], p.g1);
             inv = 1.f / sc;                                   // exact: a power of two
-            if (own && t == 0 && wave == 0 && half == 0 && blockIdx.y == 0)
+            if (c % p.nt == 0 && wave == 0 && half == 0 && blockIdx.y == 0)
                 p.vscale[view] = sc;
         }
 #pragma unroll
@@ -321,46 +292,30 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const int n0 = nbase + 8 * g + 4 * half;
-                float v[4], d[4];
+                if (n0 >= p.nrows_out) continue;
+                float v[4];
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    v[k] = acc[r][4 * g + k] * inv;
-                    const float y = __shfl_down(v[k], 1, 32);
-                    d[k] = has_next ? y - v[k] : 0.f;
-                }
-                if (!own || n0 >= p.nb) continue;
-                const int64_t e0 = n0 + (int64_t)p.nb * c;
-                if (OUT == OUT_F32_RP) {
-                    float* o = static_cast<float*>(p.out) + 2 * e0;
-                    if (n0 + 4 <= p.nb) {
-                        const float w8[8] = {v[0], d[0], v[1], d[1], v[2], d[2], v[3], d[3]};
-                        __builtin_memcpy(o, w8, sizeof(w8));
+                for (int k = 0; k < 4; ++k) v[k] = n0 + k < p.nb ? acc[r][4 * g + k] * inv : 0.f;
+                const int64_t o = n0 + (int64_t)p.nrows_out * c;
+                const bool whole = n0 + 4 <= p.nrows_out;
+                if (OUT == OUT_F32) {
+                    float* d = static_cast<float*>(p.out) + o;
+                    if (whole) {
+                        __builtin_memcpy(d, v, sizeof(v));
                     } else {
-                        for (int k = 0; k < 4 && n0 + k < p.nb; ++k) {
-                            o[2 * k] = v[k];
-                            o[2 * k + 1] = d[k];
-                        }
+                        for (int k = 0; k < 4 && n0 + k < p.nrows_out; ++k) d[k] = v[k];
                     }
                 } else {
-                    uint16_t h8[8];
+                    uint16_t h[4];
 #pragma unroll
-                    for (int k = 0; k < 4; ++k) {
-                        if (OUT == OUT_BF16_RP) {
-                            h8[2 * k] = __builtin_bit_cast(uint16_t, bf16_rne(v[k]));
-                            h8[2 * k + 1] = __builtin_bit_cast(uint16_t, bf16_rne(d[k]));
-                        } else {
-                            h8[2 * k] = __half_as_ushort(__float2half_rn(v[k]));
-                            h8[2 * k + 1] = __half_as_ushort(__float2half_rn(d[k]));
-                        }
-                    }
-                    uint16_t* o = static_cast<uint16_t*>(p.out) + 2 * e0;
-                    if (n0 + 4 <= p.nb) {
-                        __builtin_memcpy(o, h8, sizeof(h8));
+                    for (int k = 0; k < 4; ++k)
+                        h[k] = OUT == OUT_BF16 ? __builtin_bit_cast(uint16_t, bf16_rne(v[k]))
+                                               : __half_as_ushort(__float2half_rn(v[k]));
+                    uint16_t* d = static_cast<uint16_t*>(p.out) + o;
+                    if (whole) {
+                        __builtin_memcpy(d, h, sizeof(h));
                     } else {
-                        for (int k = 0; k < 4 && n0 + k < p.nb; ++k) {
-                            o[2 * k] = h8[2 * k];
-                            o[2 * k + 1] = h8[2 * k + 1];
-                        }
+                        for (int k = 0; k < 4 && n0 + k < p.nrows_out; ++k) d[k] = h[k];
                     }
                 }
             }
@@ -373,9 +328,7 @@ void launch_rt(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
     const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
-    // row-pair blocks advance by 31 columns (one column of overlap)
-    const int64_t cstep = (OUT == OUT_F32_RP || OUT == OUT_BF16_RP || OUT == OUT_F16S_RP) ? 31 : 32;
-    dim3 grid((unsigned)((p.ncols + cstep - 1) / cstep), (unsigned)row_blocks);
+    dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
     hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT, RT>), grid, dim3(256), 0, st, p);
     SGX_HIP(hipGetLastError());
 }
@@ -484,26 +437,19 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     if (a.nb > 2048)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
     const Params p = make_params(a);
-    if (a.row_pair && a.extra != 0)
-        throw Error(SGX_ERR_ARG, "sino_filter: the row-pair layout has no extra rows");
-    if (a.out_f16s && (!a.row_pair || a.split_bf16 || !a.vbound || !a.vscale))
-        throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output is row-pair, FP32 products");
-    const int out = a.row_pair ? (a.out_f16s ? OUT_F16S_RP : a.out_bf16 ? OUT_BF16_RP : OUT_F32_RP)
-                               : (a.out_bf16 ? OUT_BF16 : OUT_F32);
+    if (a.out_f16s && (a.out_bf16 || a.split_bf16 || !a.vbound || !a.vscale || a.nt < 1))
+        throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output takes FP32 products");
+    const int out = a.out_f16s ? OUT_F16S : a.out_bf16 ? OUT_BF16 : OUT_F32;
     if (a.split_bf16) {
         switch (out) {
         case OUT_F32: launch_mode<true, OUT_F32>(p, st); break;
-        case OUT_BF16: launch_mode<true, OUT_BF16>(p, st); break;
-        case OUT_F32_RP: launch_mode<true, OUT_F32_RP>(p, st); break;
-        default: launch_mode<true, OUT_BF16_RP>(p, st); break;
+        default: launch_mode<true, OUT_BF16>(p, st); break;
         }
     } else {
         switch (out) {
         case OUT_F32: launch_mode<false, OUT_F32>(p, st); break;
         case OUT_BF16: launch_mode<false, OUT_BF16>(p, st); break;
-        case OUT_F32_RP: launch_mode<false, OUT_F32_RP>(p, st); break;
-        case OUT_BF16_RP: launch_mode<false, OUT_BF16_RP>(p, st); break;
-        default: launch_mode<false, OUT_F16S_RP>(p, st); break;
+        default: launch_mode<false, OUT_F16S>(p, st); break;
         }
     }
 }

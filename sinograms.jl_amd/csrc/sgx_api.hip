@@ -17,6 +17,7 @@ using sgx::Error;
 
 struct sgx_ctx {
     int device = 0;
+    int cbct_kernel = 0;          // SGX_OPT_CBCT_KERNEL
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
@@ -61,6 +62,7 @@ enum Slot {
     W_OUT,          // staged output
     W_MASK,         // staged mask
     W_G,            // filter kernel g
+    W_WIN,          // cone-beam staged windows
     W_EXTRA,
 };
 
@@ -251,6 +253,22 @@ int sgx_ctx_synchronize(sgx_ctx* ctx)
     return guard(ctx, [&] { SGX_HIP(hipStreamSynchronize(ctx->stream)); });
 }
 
+int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
+{
+    return guard(ctx, [&] {
+        require(option == SGX_OPT_CBCT_KERNEL, SGX_ERR_ARG, "unknown option");
+        require(value == 0 || value == 1, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0 or 1");
+        ctx->cbct_kernel = (int)value;
+    });
+}
+
+int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value)
+{
+    if (!ctx || !value || option != SGX_OPT_CBCT_KERNEL) return SGX_ERR_ARG;
+    *value = ctx->cbct_kernel;
+    return SGX_OK;
+}
+
 int sgx_back_par(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_t ia_skip,
                  const double* sinphi, const double* cosphi, double ds, double offset,
                  const double* xc, int64_t nx, const double* yc, int64_t ny,
@@ -306,16 +324,23 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         auto* cs = reinterpret_cast<sgx::ViewCS*>(d + o_cs);
         sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), nv, 1, cs, ctx->stream);
 
+        // The kernels take rows padded to a multiple of 4 bins (16-byte aligned)
+        // and, for 16-bit data, readable bytes after the last view: device FP32
+        // input that already has that shape is used in place, anything else is
+        // copied into a padded buffer (pad bins are never read as taps).
         const size_t esz = (flags & SGX_Q_BF16) ? 2 : 4;
-        const void* q = stage_in(ctx, W_IN, proj, esz * ns * nt * na, flags);
-        if (flags & SGX_Q_BF16) {
-            // the 16-bit gathers need readable padding after the last view
-            void* d = ctx->get(W_IN, esz * ns * nt * na + sgx::kQ16Pad);
-            SGX_HIP(hipMemsetAsync(static_cast<char*>(d) + esz * ns * nt * na, 0, sgx::kQ16Pad,
-                                   ctx->stream));
-            SGX_HIP(hipMemcpyAsync(d, proj, esz * ns * nt * na, hipMemcpyDeviceToDevice,
-                                   ctx->stream));
-            q = d;
+        const int64_t pitch = (ns + 3) / 4 * 4;
+        const void* q = proj;
+        if ((flags & (SGX_HOST | SGX_Q_BF16)) || pitch != ns ||
+            reinterpret_cast<uintptr_t>(proj) % 16 != 0) {
+            const size_t body = esz * pitch * nt * na;
+            void* dq = ctx->get(W_IN, body + sgx::kQ16Pad);
+            SGX_HIP(hipMemsetAsync(static_cast<char*>(dq) + body, 0, sgx::kQ16Pad, ctx->stream));
+            SGX_HIP(hipMemcpy2DAsync(dq, esz * pitch, proj, esz * ns, esz * ns, nt * na,
+                                     (flags & SGX_HOST) ? hipMemcpyHostToDevice
+                                                        : hipMemcpyDeviceToDevice,
+                                     ctx->stream));
+            q = dq;
         }
         const int64_t nvox = nx * ny * nz;
         const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nvox, flags));
@@ -327,8 +352,9 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         a.vscale = nullptr;
         a.ns = (int)ns;
         a.nt = (int)nt;
+        a.pitch = (int)pitch;
         a.nviews = (int)nv;
-        a.view_stride = ns * nt * ia_skip;
+        a.view_stride = pitch * nt * ia_skip;
         a.cs = cs;
         a.xds = reinterpret_cast<const float*>(d + o_x);
         a.yds = reinterpret_cast<const float*>(d + o_y);
@@ -344,6 +370,12 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         a.wt = (float)((nt + 1) / 2.0 + (double)(float)offset_t);
         a.inv_dt_ds = 1.0f / ((float)dt / (float)ds);
         a.is_arc = is_arc != 0;
+        if (ctx->cbct_kernel == 0) {
+            int4* win = static_cast<int4*>(
+                ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
+            sgx::launch_cbct_windows(a, win, ctx->stream);
+            a.win = win;
+        }
         sgx::launch_cbct_back(a, ctx->stream);
         finish(ctx, image, img, 4 * nvox, flags);
     });
@@ -479,7 +511,9 @@ struct sgx_plan {
     int64_t npad = 0;
     void* tables = nullptr;       // device: cs, gam, bet, axes, g, wbin, wview, s/t axes
     uint8_t* mask = nullptr;      // device (or null)
-    void* q = nullptr;            // filtered data (ns, nt, na)
+    void* q = nullptr;            // filtered data (pitch, nt, na), column-major
+    int64_t pitch = 0;            // elements per detector row of q (>= ns)
+    int4* win = nullptr;          // cone-beam: per-(block, view) staged windows
     unsigned int* vbound = nullptr; // per-view max |weighted input| (F16S form)
     float* vscale = nullptr;        // per-view FP16 scale (F16S form)
     float g1 = 0.f;                 // sum |g| (F16S output bound)
@@ -503,6 +537,7 @@ struct sgx_plan {
         if (tables) (void)hipFree(tables);
         if (mask) (void)hipFree(mask);
         if (q) (void)hipFree(q);
+        if (win) (void)hipFree(win);
         if (vbound) (void)hipFree(vbound);
         if (vscale) (void)hipFree(vscale);
         if (partial) (void)hipFree(partial);
@@ -520,10 +555,9 @@ struct sgx_plan {
         a.ncols = G.nt * G.na;
         a.g = g;
         a.npad = (int)npad;
-        a.extra = 0;
+        a.extra = (int)(pitch - G.ns);       // zero bins that pad each row to the pitch
         a.out = out;
         a.out_bf16 = prec == SGX_PREC_BF16;
-        a.row_pair = is3d() && out == q;
         a.split_bf16 = prec == SGX_PREC_BF16;
         if (prec == SGX_PREC_F16S) {
             a.out_f16s = true;
@@ -549,6 +583,39 @@ struct sgx_plan {
         return a;
     }
 };
+
+sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
+{
+    const sgx_geom& G = P.geom;
+    sgx::CbctArgs a{};
+    a.q = P.q;
+    a.fmt = P.prec == SGX_PREC_BF16   ? sgx::SGX_Q_FMT_BF16
+            : P.prec == SGX_PREC_F16S ? sgx::SGX_Q_FMT_F16S
+                                      : sgx::SGX_Q_FMT_F32;
+    a.vscale = P.vscale;
+    a.ns = (int)G.ns;
+    a.nt = (int)G.nt;
+    a.pitch = (int)P.pitch;
+    a.nviews = (int)G.na;
+    a.view_stride = P.pitch * G.nt;
+    a.cs = P.cs;
+    a.xds = P.xds;
+    a.yds = P.yds;
+    a.zds = P.zds;
+    a.nx = (int)P.nx;
+    a.ny = (int)P.ny;
+    a.nz = (int)P.nz;
+    a.mask = P.mask;
+    a.image = img;
+    a.dsd_ds = (float)G.dsd / (float)G.ds;
+    a.dso_ds = (float)G.dso / (float)G.ds;
+    a.ws = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+    a.wt = (float)((G.nt + 1) / 2.0 + (double)(float)G.offset_t);
+    a.inv_dt_ds = 1.0f / ((float)G.dt / (float)G.ds);
+    a.is_arc = P.is_arc();
+    a.win = P.win;
+    return a;
+}
 
 extern "C" {
 
@@ -661,11 +728,13 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             SGX_HIP(hipMalloc(&P->mask, nvox));
             SGX_HIP(hipMemcpy(P->mask, mask, nvox, hipMemcpyHostToDevice));
         }
-        // cone-beam plans store the filtered data in the row-pair gather layout
-        // (two values per entry); 2-D plans column-major
-        const size_t esz = (prec == SGX_PREC_FP32 ? 4 : 2) * (is3d ? 2 : 1);
-        SGX_HIP(hipMalloc(&P->q, esz * ns * nt * na + sgx::kQ16Pad));
-        SGX_HIP(hipMemset(P->q, 0, esz * ns * nt * na + sgx::kQ16Pad));
+        // filtered data: column-major; cone-beam rows padded to a multiple of
+        // 4 bins (aligned 16-byte staging loads), padding bins written as 0
+        P->pitch = is3d ? (ns + 3) / 4 * 4 : ns;
+        const size_t esz = prec == SGX_PREC_FP32 ? 4 : 2;
+        const size_t qbytes = esz * P->pitch * nt * na + sgx::kQ16Pad;
+        SGX_HIP(hipMalloc(&P->q, qbytes));
+        SGX_HIP(hipMemset(P->q, 0, qbytes));
         if (prec == SGX_PREC_F16S) {
             SGX_HIP(hipMalloc(&P->vbound, 4 * na));
             SGX_HIP(hipMalloc(&P->vscale, 4 * na));
@@ -673,6 +742,12 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         if (!is3d) {
             P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny);
             if (P->partial_cap) SGX_HIP(hipMalloc(&P->partial, 4 * P->partial_cap));
+        }
+        if (is3d) {
+            // the staged backprojector's detector windows, fixed by the geometry
+            const int64_t nw = sgx::cbct_window_entries(cbct_args_of(*P, nullptr));
+            SGX_HIP(hipMalloc(&P->win, sizeof(int4) * nw));
+            sgx::launch_cbct_windows(cbct_args_of(*P, nullptr), P->win, ctx->stream);
         }
         for (hipEvent_t& e : P->ev) SGX_HIP(hipEventCreate(&e));
         SGX_HIP(hipStreamSynchronize(ctx->stream));
@@ -728,32 +803,8 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
         sgx::launch_sino_filter(fa, st);
         SGX_HIP(hipEventRecord(P.ev[1], st));
         if (P.is3d()) {
-            sgx::CbctArgs a{};
-            a.q = P.q;
-            a.fmt = P.prec == SGX_PREC_BF16   ? sgx::SGX_Q_FMT_BF16
-                    : P.prec == SGX_PREC_F16S ? sgx::SGX_Q_FMT_F16S
-                                              : sgx::SGX_Q_FMT_F32;
-            a.vscale = P.vscale;
-            a.row_pair = true;
-            a.ns = (int)G.ns;
-            a.nt = (int)G.nt;
-            a.nviews = (int)G.na;
-            a.view_stride = G.ns * G.nt;
-            a.cs = P.cs;
-            a.xds = P.xds;
-            a.yds = P.yds;
-            a.zds = P.zds;
-            a.nx = (int)P.nx;
-            a.ny = (int)P.ny;
-            a.nz = (int)P.nz;
-            a.mask = P.mask;
-            a.image = img;
-            a.dsd_ds = (float)G.dsd / (float)G.ds;
-            a.dso_ds = (float)G.dso / (float)G.ds;
-            a.ws = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
-            a.wt = (float)((G.nt + 1) / 2.0 + (double)(float)G.offset_t);
-            a.inv_dt_ds = 1.0f / ((float)G.dt / (float)G.ds);
-            a.is_arc = P.is_arc();
+            sgx::CbctArgs a = cbct_args_of(P, img);
+            if (ctx->cbct_kernel == 1) a.win = nullptr;
             sgx::launch_cbct_back(a, st);
         } else {
             sgx::Back2dArgs a{};
@@ -785,6 +836,8 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
 
 const void* sgx_plan_filtered(const sgx_plan* plan) { return plan ? plan->q : nullptr; }
 
+int64_t sgx_plan_pitch(const sgx_plan* plan) { return plan ? plan->pitch : 0; }
+
 int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales)
 {
     if (!plan) return SGX_ERR_ARG;
@@ -793,14 +846,8 @@ int sgx_plan_read_filtered(sgx_plan* plan, void* host, float* view_scales)
         require(host, SGX_ERR_ARG, "null pointer argument");
         const sgx_geom& G = plan->geom;
         const size_t esz = plan->prec == SGX_PREC_FP32 ? 4 : 2;
-        const int64_t n = G.ns * G.nt * G.na;
-        if (plan->is3d()) {
-            // row-pair layout: the column-major value is the first half of each entry
-            SGX_HIP(hipMemcpy2DAsync(host, esz, plan->q, 2 * esz, esz, n, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-        } else {
-            SGX_HIP(hipMemcpyAsync(host, plan->q, esz * n, hipMemcpyDeviceToHost, ctx->stream));
-        }
+        SGX_HIP(hipMemcpy2DAsync(host, esz * G.ns, plan->q, esz * plan->pitch, esz * G.ns,
+                                 G.nt * G.na, hipMemcpyDeviceToHost, ctx->stream));
         if (view_scales) {
             if (plan->prec == SGX_PREC_F16S) {
                 SGX_HIP(hipMemcpyAsync(view_scales, plan->vscale, 4 * G.na,
@@ -817,8 +864,8 @@ int64_t sgx_plan_stored_bytes(const sgx_plan* plan)
 {
     if (!plan) return 0;
     const sgx_geom& G = plan->geom;
-    const int64_t esz = (plan->prec == SGX_PREC_FP32 ? 4 : 2) * (plan->is3d() ? 2 : 1);
-    return esz * G.ns * G.nt * G.na;
+    const int64_t esz = plan->prec == SGX_PREC_FP32 ? 4 : 2;
+    return esz * plan->pitch * G.nt * G.na;
 }
 
 int sgx_plan_read_stored(sgx_plan* plan, void* host, int64_t bytes)

@@ -42,12 +42,12 @@ void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* ou
 // K3: cone-beam voxel-driven backprojection.
 enum { SGX_Q_FMT_F32 = 0, SGX_Q_FMT_BF16 = 1, SGX_Q_FMT_F16S = 2 };
 struct CbctArgs {
-    const void* q;          // (ns, nt, na) float, bf16 or scaled fp16; 16-bit data must be
-                            // followed by >= 4 bytes of readable padding
+    const void* q;          // (pitch, nt, na) float, bf16 or scaled fp16, column-major;
+                            // 16-bit data must be followed by >= 4 bytes of readable padding
     int fmt;                // SGX_Q_FMT_*
-    bool row_pair;          // q in the row-pair difference layout (2 values per entry)
     const float* vscale;    // per used view, for SGX_Q_FMT_F16S (else null)
     int ns, nt;
+    int pitch;              // elements per detector row (>= ns, a multiple of 4)
     int nviews;             // views used (after ia_skip)
     int64_t view_stride;    // elements between used views
     const ViewCS* cs;       // nviews
@@ -59,9 +59,15 @@ struct CbctArgs {
     float* image;
     float dsd_ds, dso_ds, ws, wt, inv_dt_ds;
     bool is_arc;
-    bool xcd_order = false; // renumber blocks so each XCD runs a compact sub-volume
+    // per-(block, view) detector windows of the LDS-staged kernel, from
+    // launch_cbct_windows; null selects the gather kernel
+    const int4* win;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
+// Entries of the window table the staged kernel takes for these arguments.
+int64_t cbct_window_entries(const CbctArgs& a);
+// Fill the window table (needs cs, axes, geometry and sizes of `a`).
+void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st);
 // Bytes of readable padding the 16-bit forms need after the last view.
 constexpr size_t kQ16Pad = 16;
 
@@ -97,9 +103,7 @@ struct FilterArgs {
     int extra;              // zero rows appended to each output column
     void* out;              // (nb + extra, ncols), float or bf16
     bool out_bf16;
-    bool out_f16s;          // scaled FP16 (row-pair only): q / vscale[view]
-    bool row_pair;          // write the row-pair difference layout
-                            // (entry (n,t) = q[n,t], q[n,t+1] - q[n,t])
+    bool out_f16s;          // scaled FP16: q / vscale[view]
     bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
     // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
     float scale;

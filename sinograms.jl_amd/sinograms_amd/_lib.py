@@ -42,7 +42,8 @@ EXPORTS = (
     "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
-    "sgx_plan_stored_bytes", "sgx_plan_read_stored",
+    "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
+    "sgx_ctx_set_option", "sgx_ctx_get_option",
     "sgx_project_bdd", "sgx_backproject_bdd",
 )
 
@@ -119,11 +120,18 @@ def _declare(L):
     L.sgx_plan_stored_bytes.argtypes = [vp]
     L.sgx_plan_stored_bytes.restype = ctypes.c_int64
     L.sgx_plan_read_stored.argtypes = [vp, vp, ctypes.c_int64]
+    L.sgx_ctx_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
+    L.sgx_ctx_get_option.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+    L.sgx_plan_pitch.argtypes = [vp]
+    L.sgx_plan_pitch.restype = ctypes.c_int64
     L.sgx_project_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, vp, u32]
     L.sgx_backproject_bdd.argtypes = [vp, vp, i64, i64, vp, i64, f64, f64, f64, f64, i32, vp,
                                       u32]
     L.sgx_plan_last_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_float)]
+
+
+SGX_OPT_CBCT_KERNEL = 1
 
 
 class Context:
@@ -143,6 +151,19 @@ class Context:
     def check(self, rc):
         if rc != SGX_OK:
             raise SgxError(rc, load().sgx_last_error(self.h).decode())
+
+    @property
+    def cbct_kernel(self) -> str:
+        """Cone-beam backprojector of this context: "staged" (LDS-staged
+        detector windows, the default) or "gather"."""
+        v = ctypes.c_int64()
+        self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_CBCT_KERNEL, ctypes.byref(v)))
+        return ("staged", "gather")[v.value]
+
+    @cbct_kernel.setter
+    def cbct_kernel(self, name: str):
+        self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_CBCT_KERNEL,
+                                             {"staged": 0, "gather": 1}[name]))
 
     def close(self):
         if self.h:

@@ -524,25 +524,18 @@ class _Plan:
             val = raw
         return val.reshape((G.ns, G.nt, G.na), order="F"), sc
 
-    def read_pairs(self):
-        """Cone-beam plans: the stored row-pair difference data of the last
-        execute as (a, d, scale): a[is, it, ia] = q[is, it, ia] and
-        d[is, it, ia] = q[is, it+1, ia] - q[is, it, ia] as stored (float32 arrays,
-        widened exactly), scale the per-view factor that multiplies both."""
+    def read_stored(self):
+        """The stored filtered data of the last execute as-is: (pitch, nt, na)
+        column-major array of the storage type (float32, or uint16 bit patterns
+        of bf16 / fp16), pitch >= ns with zero pad bins."""
         G = self.geom
         L = _lib.load()
         nbytes = L.sgx_plan_stored_bytes(self.h)
+        pitch = L.sgx_plan_pitch(self.h)
         raw = np.empty(nbytes, np.uint8)
         self._ctx.check(L.sgx_plan_read_stored(self.h, _dptr(raw), nbytes))
-        if self.precision == "fp32":
-            v = raw.view(np.float32)
-        elif self.precision == "bf16":
-            v = (raw.view(np.uint16).astype(np.uint32) << 16).view(np.float32)
-        else:
-            v = raw.view(np.float16).astype(np.float32)
-        v = v.reshape(2, G.ns, G.nt, G.na, order="F")
-        _, sc = self.read_filtered()
-        return np.asfortranarray(v[0]), np.asfortranarray(v[1]), sc
+        v = raw.view(np.float32 if self.precision == "fp32" else np.uint16)
+        return v.reshape((pitch, G.nt, G.na), order="F")
 
     def __del__(self):
         try:

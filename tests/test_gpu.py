@@ -43,12 +43,23 @@ def _bf16_round(x):
 
 # --- K3 cone-beam backprojection -------------------------------------------------------
 
+@pytest.fixture(params=["staged", "gather"])
+def cbct_kernel(request, sg, torch_cuda):
+    """Run a test with each cone-beam backprojector of the shared context."""
+    ctx = sg._lib.context(0)
+    old = ctx.cbct_kernel
+    ctx.cbct_kernel = request.param
+    yield request.param
+    ctx.cbct_kernel = old
+
+
 @pytest.mark.parametrize("arc", [False, True])
 @pytest.mark.parametrize("dims,ia_skip,masked", [((16, 16, 8), 1, False),
                                                  ((19, 13, 11), 2, True),
-                                                 ((9, 24, 35), 1, True)])
-def test_cbct_back_fp32_vs_oracle(sg, orc, torch_cuda, arc, dims, ia_skip, masked):
-    _say("cbct_back", arc, dims, ia_skip, masked)
+                                                 ((9, 24, 35), 1, True),
+                                                 ((37, 21, 40), 1, False)])
+def test_cbct_back_fp32_vs_oracle(sg, orc, torch_cuda, cbct_kernel, arc, dims, ia_skip, masked):
+    _say("cbct_back", cbct_kernel, arc, dims, ia_skip, masked)
     rng = np.random.default_rng(10)
     ns, nt, na = 40, 30, 24
     proj = rng.standard_normal((ns, nt, na)).astype(np.float32)
@@ -73,9 +84,9 @@ def test_cbct_back_fp32_vs_oracle(sg, orc, torch_cuda, arc, dims, ia_skip, maske
     assert np.array_equal(got_d, got)
 
 
-def test_cbct_back_detector_edges(sg, orc, torch_cuda):
+def test_cbct_back_detector_edges(sg, orc, cbct_kernel):
     # voxels projecting beyond every detector edge, including NaN-free far field
-    _say("cbct_back edges")
+    _say("cbct_back edges", cbct_kernel)
     rng = np.random.default_rng(11)
     rg = sg.CtFanFlat(ns=12, nt=10, ds=2.0, na=8, dsd=60.0, dod=20.0)
     proj = rng.standard_normal(rg.dims()).astype(np.float32)
@@ -87,6 +98,24 @@ def test_cbct_back_detector_edges(sg, orc, torch_cuda):
     assert np.isfinite(got).all()
     assert rel_rmse(got, ref) < TOL32
     assert np.any(ref == 0)          # some voxels see no detector at all
+
+
+@pytest.mark.parametrize("arc", [False, True])
+def test_cbct_back_staged_window_overflow(sg, orc, cbct_kernel, arc):
+    # Fine detector pitch: a 16 x 16 x 16 block's footprint (~75 bins x ~50
+    # rows) exceeds the staged window, so some lanes read from the window and
+    # the rest gather from global memory within the same view.
+    _say("cbct_back window overflow", cbct_kernel, arc)
+    rng = np.random.default_rng(13)
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=200, nt=150, ds=0.45, dt=0.45, na=30,
+                                                 dsd=300.0, dod=100.0)
+    ig = sg.ImageGeom((40, 40, 20), (1.0, 1.0, 1.0))
+    proj = rng.standard_normal((200, 150, 30)).astype(np.float32)
+    got = sg.cbct_back(proj, rg, ig)
+    xc, yc, zc = ig.axes()
+    ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, arc,
+                        xc, yc, zc, ig.mask, dtype=np.float32)
+    assert rel_rmse(got, ref) < TOL32
 
 
 def test_cbct_back_bf16_storage(sg, orc, torch_cuda):
@@ -320,32 +349,28 @@ def test_fdk_16bit_storage_forms_error(sg, orc, torch_cuda):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "f16s", "bf16"])
-def test_fdk_stored_pairs_exact_vs_oracle(sg, orc, torch_cuda, prec):
-    # The backprojection reads exactly the stored row-pair difference data:
-    # take the plan's stored (value, difference) pairs, widen them exactly and
-    # backproject them with the oracle's pair-reading checker.
-    _say("fdk stored pairs parity", prec)
-    rg = sg.CtFanArc(ns=64, nt=48, ds=1.0, dt=1.2, na=40, dsd=300.0, dod=100.0)
+def test_fdk_stored_data_exact_vs_oracle(sg, orc, torch_cuda, prec):
+    # The backprojection reads exactly the stored filtered data: widen the
+    # plan's stored values exactly, apply the per-view scale and backproject
+    # them with the Float32 oracle.
+    _say("fdk stored-data parity", prec)
+    rg = sg.CtFanArc(ns=66, nt=48, ds=1.0, dt=1.2, na=40, dsd=300.0, dod=100.0)
     ig = sg.ImageGeom((24, 24, 16))
     rng = np.random.default_rng(7)
     proj = rng.standard_normal(rg.dims()).astype(np.float32)
     plan = sg.plan_fbp(rg, ig, precision=prec)
     got = sg.fdk(plan, proj)
-    a, d, scale = plan.read_pairs()
-    assert np.all(d[:, -1, :] == 0)
-    if prec == "fp32":
-        # layout: d is the Float32 difference of the stored neighbours
-        assert np.array_equal(d[:, :-1, :], a[:, 1:, :] - a[:, :-1, :])
-    else:
-        assert scale.dtype == np.float32 and np.all(scale > 0)
-        if prec == "f16s":      # powers of two
-            assert np.all(np.frexp(scale)[0] == 0.5)
-            assert np.abs(a).max() <= 2.0 ** 14 and np.abs(d).max() <= 2.0 ** 15
-    a = a * scale[None, None, :]
-    d = d * scale[None, None, :]
+    q, scale = plan.read_filtered()
+    raw = plan.read_stored()
+    assert raw.shape == (68, 48, 40)                  # rows padded to a multiple of 4
+    assert np.all(raw[66:] == 0)
+    if prec == "f16s":
+        assert np.all(np.frexp(scale)[0] == 0.5)      # powers of two
+        assert np.abs(q).max() <= 2.0 ** 14
+    q = q.astype(np.float64) * scale[None, None, :]
     xc, yc, zc = ig.axes()
-    ref = orc.cbct_back_pairs(a, d, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, True,
-                              xc, yc, zc, ig.mask, dtype=np.float32)
+    ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
+                        xc, yc, zc, ig.mask, dtype=np.float32)
     assert rel_rmse(got, ref) < TOL32
 
 
