@@ -74,6 +74,9 @@ def _declare(L):
             _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp]
         getattr(L, f"orc_sino_filter_{suf}").argtypes = [
             _vp, _i64, _i64, _vp, _i64, _i64, _vp]
+        getattr(L, f"orc_cbct_back_rp_{suf}").argtypes = [
+            _vp, _vp, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _int,
+            _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp]
     L.orc_filter_kernel.argtypes = [_vp, _i64, _vp]
     L.orc_project_bdd.argtypes = [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _f64, _vp]
     L.orc_backproject_bdd.argtypes = [_vp, _i64, _i64, _vp, _i64, _f64, _f64, _f64, _f64,
@@ -325,6 +328,26 @@ def cbct_back(proj, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc,
     getattr(lib(), "orc_cbct_back_" + _suffix(dtype))(
         _p(proj), ns, nt, na, ia_skip, _p(betas), dsd, dso, ds, dt, offset_s, offset_t,
         int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask), _p(img))
+    return img
+
+
+def cbct_back_pairs(a, d, betas, dsd, dso, ds, dt, is_arc, xc, yc, zc, mask, dtype=np.float64):
+    """``cbct_back_fan!`` (src/fbp3/cbct-back.jl:95-213, offsets 0) on data kept as
+    (value, difference) pairs: the bilinear taps of row it+1 are a[., it] + d[., it]
+    instead of a[., it+1] (checker for backprojectors that keep the data as
+    (value, row difference) pairs; equal to ``cbct_back(a)`` when d is the
+    exact row difference)."""
+    a = _F(a, dtype)
+    d = _F(d, dtype)
+    assert a.shape == d.shape
+    ns, nt, na = a.shape
+    xc = _F(xc, np.float64); yc = _F(yc, np.float64); zc = _F(zc, np.float64)
+    mask = _F(mask, np.uint8)
+    betas = _F(betas, np.float64)
+    img = np.zeros((len(xc), len(yc), len(zc)), dtype=dtype, order="F")
+    getattr(lib(), "orc_cbct_back_rp_" + _suffix(dtype))(
+        _p(a), _p(d), ns, nt, na, _p(betas), dsd, dso, ds, dt, int(bool(is_arc)),
+        _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask), _p(img))
     return img
 
 

@@ -131,8 +131,12 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
                const double* betas, double dsd, double dso, double ds, double dt,
                double offset_s, double offset_t, int is_arc,
                const double* xc, int64_t nx, const double* yc, int64_t ny,
-               const double* zc, int64_t nz, const uint8_t* mask, T* image)
+               const double* zc, int64_t nz, const uint8_t* mask, T* image,
+               const T* dnext = nullptr)
 {
+    // dnext (optional, same shape as proj): the stored row-pair differences;
+    // the taps of row it+1 are then proj[., it] + dnext[., it] (a checker for
+    // data kept as (value, difference) pairs), else proj[., it+1].
     const int64_t na_sub = (na + ia_skip - 1) / ia_skip;
     const T ws = (T)(float)((ns + 1) / 2.0 + (double)(float)offset_s);  // :119
     const T wt = (T)(float)((nt + 1) / 2.0 + (double)(float)offset_t);  // :120
@@ -173,9 +177,13 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
             if (1 <= is && is < ns && 1 <= it && it < nt) {               // :197
                 const T wr = bs - (T)is, wl = 1 - wr;
                 const T wu = bt - (T)it, wd = 1 - wu;
-                const T* r0 = q + (is - 1) + ns * (it - 1);
+                const int64_t o0 = (is - 1) + ns * (it - 1);
+                const T* r0 = q + o0;
+                const T* d0 = dnext ? dnext + ns * nt * (ja * ia_skip) + o0 : nullptr;
+                const T u0 = d0 ? r0[0] + d0[0] : r0[ns];
+                const T u1 = d0 ? r0[1] + d0[1] : r0[ns + 1];
                 const T p1 = wl * r0[0] + wr * r0[1];                     // :205
-                const T p2 = wl * r0[ns] + wr * r0[ns + 1];               // :206
+                const T p2 = wl * u0 + wr * u1;                           // :206
                 voxel += w2 * (wd * p1 + wu * p2);                        // :208
             }
         }
@@ -368,6 +376,20 @@ ORC_EXPORT void orc_cbct_back_f32(const float* proj, int64_t ns, int64_t nt, int
     const uint8_t* mask, float* image)
 { cbct_back<float>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
                    xc, nx, yc, ny, zc, nz, mask, image); }
+
+ORC_EXPORT void orc_cbct_back_rp_f64(const double* proj, const double* dnext, int64_t ns,
+    int64_t nt, int64_t na, const double* betas, double dsd, double dso, double ds, double dt,
+    int is_arc, const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc,
+    int64_t nz, const uint8_t* mask, double* image)
+{ cbct_back<double>(proj, ns, nt, na, 1, betas, dsd, dso, ds, dt, 0.0, 0.0, is_arc,
+                    xc, nx, yc, ny, zc, nz, mask, image, dnext); }
+
+ORC_EXPORT void orc_cbct_back_rp_f32(const float* proj, const float* dnext, int64_t ns,
+    int64_t nt, int64_t na, const double* betas, double dsd, double dso, double ds, double dt,
+    int is_arc, const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc,
+    int64_t nz, const uint8_t* mask, float* image)
+{ cbct_back<float>(proj, ns, nt, na, 1, betas, dsd, dso, ds, dt, 0.0, 0.0, is_arc,
+                   xc, nx, yc, ny, zc, nz, mask, image, dnext); }
 
 ORC_EXPORT void orc_filter_kernel(const double* Hk, int64_t npad, double* g)
 {

@@ -93,6 +93,14 @@ template <int FMT> __device__ __forceinline__ float2 dec2(uint32_t u)
     return make_float2(f16f(u & 0xffffu), f16f(u >> 16));
 }
 
+// Bin-pair difference a - b of two FP16 pairs (v_pk_add_f16 with a negated
+// operand; each half rounded once).
+__device__ __forceinline__ uint32_t half2_sub(uint32_t a, uint32_t b)
+{
+    const __half2 r = __hsub2(__builtin_bit_cast(__half2, a), __builtin_bit_cast(__half2, b));
+    return __builtin_bit_cast(uint32_t, r);
+}
+
 // The four taps of one update from global memory as (a0, d0, a1, d1):
 // a = q[is, it] and d = q[is, it+1] - q[is, it] for the columns is (0) and
 // is+1 (1); voff is the byte offset of (is, it) in the view, rowb the row
@@ -219,16 +227,16 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const int64_t col = ix + (int64_t)nx * iy;
 
     float z[ZB], acc[ZB];
-    bool any = false;
+    uint32_t inbits = 0;           // bit j: voxel iz0 + j exists and is inside the mask
 #pragma unroll
     for (int j = 0; j < ZB; ++j) {
         const int iz = iz0 + j;
         const bool in = iz < nz && (mask == nullptr || mask[col + nxy * iz]);
-        any |= in;
+        inbits |= (uint32_t)in << j;
         z[j] = zds[min(iz, nz - 1)];
         acc[j] = 0.f;
     }
-    if (any) {
+    if (inbits != 0) {
         float zlo = z[0], zhi = z[0];
 #pragma unroll
         for (int j = 1; j < ZB; ++j) {
@@ -251,24 +259,45 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         }
     }
 #pragma unroll
-    for (int j = 0; j < ZB; ++j) {
-        const int iz = iz0 + j;
-        if (iz < nz) {
-            const bool in = mask == nullptr || mask[col + nxy * iz];
-            image[col + nxy * iz] = in ? acc[j] : 0.f;
-        }
-    }
+    for (int j = 0; j < ZB; ++j)
+        if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
 }
 
 // ---------------------------------------------------------------------------
 // staged kernel
 // ---------------------------------------------------------------------------
 
+// Build-time tuning of the staged kernel (defaults measured best on the
+// 256^2 x 128 FDK workload): LDS loads in flight per batch, minimum waves
+// per SIMD (caps the VGPRs), LDS row pitch in entries.
+#ifndef SGX_STAGED_BATCH
+#define SGX_STAGED_BATCH 8
+#endif
+#ifndef SGX_STAGED_WAVES
+#define SGX_STAGED_WAVES 3
+#endif
+#ifndef SGX_STAGED_PITCH
+#define SGX_STAGED_PITCH 58
+#endif
 constexpr int kSP = 16;               // block patch: 16 x 16 columns (2 x 2 waves of 8 x 8)
 constexpr int kSZ = 16;               // voxels per lane along z
 constexpr int kWmax = 48;             // staged window: bins (12 quads of 4) ...
 constexpr int kHmax = 32;             // ... x rows (16 groups of 2)
-constexpr int kLP = 50;               // LDS pitch in entries (rows 4 banks apart)
+constexpr int kSB = SGX_STAGED_BATCH;
+
+// LDS entry of the staged window: (q[is, it], q[is, it+1] - q[is, it]) as two
+// FP32 (8 bytes), or for the scaled-FP16 form as two FP16 (4 bytes; the
+// difference of two FP16 values is exact in FP16 whenever they are within a
+// factor 2 of each other, and otherwise rounded once).  The pitch keeps rows
+// 16-byte aligned and staggers the banks of neighbouring rows.
+template <int FMT> struct Lds {
+    static constexpr bool half = FMT == F_F16S;
+    static constexpr uint32_t esz = half ? 4 : 8;
+    static constexpr int pitch = half ? 52 : SGX_STAGED_PITCH;
+    static constexpr int buf = kHmax * pitch;                 // entries per buffer
+    static_assert(pitch >= kWmax + 2 && (pitch * esz) % 16 == 0, "pitch");
+};
+static_assert(kSZ % kSB == 0, "batch divides the voxels per lane");
 constexpr int kQuads = kWmax / 4;
 constexpr int kStagers = kQuads * (kHmax / 2);   // 192 threads stage
 
@@ -331,7 +360,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
 }
 
 template <bool ARC, int FMT>
-__global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
+__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
@@ -339,7 +368,9 @@ __global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win)
 {
-    __shared__ float2 lds[2 * kHmax * kLP];
+    using L = Lds<FMT>;
+    constexpr int kLP = L::pitch;
+    __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (nx + kSP - 1) / kSP;
@@ -354,16 +385,18 @@ __global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
     const int64_t col = inxy ? ix + (int64_t)nx * iy : 0;
 
     // Every thread stays to the end: all of them stage and meet the barriers.
+    // Bit j of inbits: voxel iz0 + j exists and is inside the mask.
     float z[kSZ], acc[kSZ];
-    bool any = false;
+    uint32_t inbits = 0;
 #pragma unroll
     for (int j = 0; j < kSZ; ++j) {
         const int iz = iz0 + j;
         const bool in = inxy && iz < nz && (mask == nullptr || mask[col + nxy * iz]);
-        any |= in;
+        inbits |= (uint32_t)in << j;
         z[j] = zds[min(iz, nz - 1)];
         acc[j] = 0.f;
     }
+    const bool any = inbits != 0;
     float zlo = z[0], zhi = z[0];
 #pragma unroll
     for (int j = 1; j < kSZ; ++j) {
@@ -382,41 +415,97 @@ __global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
     // entry rows 2sg, 2sg+1, i.e. data rows 2sg .. 2sg+2 of the window.
     const bool stager = tid < kStagers;
     const int sq = tid % kQuads, sg = tid / kQuads;
-    float sv[3][4];
+    const uint32_t toff = ((uint32_t)(2 * sg) * (uint32_t)pitch + 4u * sq) * Fmt<FMT>::esz;
+    char* const sdst = lds + ((2 * sg) * kLP + 4 * sq) * L::esz;
+    float sv[3][4];          // FP32 / BF16: the window's values, widened
+    uint2 hv[3];             // F16S: the raw FP16 values (4 bins per row)
     auto stage_load = [&](int ia, int4 w) {
         if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
-        const uint32_t off = ((uint32_t)(w.y + 2 * sg) * (uint32_t)pitch +
-                              (uint32_t)(w.x + 4 * sq)) * Fmt<FMT>::esz;
-        const int nrow = min(3, w.w + 1 - 2 * sg);          // data rows needed (2 or 3)
+        const uint32_t off = toff + ((uint32_t)w.y * (uint32_t)pitch + (uint32_t)w.x) * Fmt<FMT>::esz;
+        const bool third = 2 * sg + 2 <= w.w;              // data row 2sg+2 needed
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            if (k >= nrow) break;
+            if (k == 2 && !third) break;
             if (FMT == F_F32) {
                 const auto u = __builtin_amdgcn_raw_buffer_load_b128(rs, off + k * rowb, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) sv[k][e] = __uint_as_float(u[e]);
-            } else {
+            } else if (FMT == F_BF16) {
                 const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, off + k * rowb, 0, 0);
                 const float2 e01 = dec2<FMT>(u[0]), e23 = dec2<FMT>(u[1]);
                 sv[k][0] = e01.x; sv[k][1] = e01.y; sv[k][2] = e23.x; sv[k][3] = e23.y;
+            } else {
+                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, off + k * rowb, 0, 0);
+                hv[k] = make_uint2(u[0], u[1]);
             }
         }
     };
     auto stage_store = [&](int buf, int4 w) {
         if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
-        float2* d = lds + buf * (kHmax * kLP) + (2 * sg) * kLP + 4 * sq;
+        char* d = sdst + buf * (L::buf * L::esz);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
             if (2 * sg + k >= w.w) break;
-            float4 lo, hi;
-            lo = make_float4(sv[k][0], sv[k + 1][0] - sv[k][0], sv[k][1], sv[k + 1][1] - sv[k][1]);
-            hi = make_float4(sv[k][2], sv[k + 1][2] - sv[k][2], sv[k][3], sv[k + 1][3] - sv[k][3]);
-            *reinterpret_cast<float4*>(d + k * kLP) = lo;
-            *reinterpret_cast<float4*>(d + k * kLP + 2) = hi;
+            if (L::half) {
+                // row differences of bin pairs in FP16, then entries (a, d) per bin
+                const uint32_t d01 = half2_sub(hv[k + 1].x, hv[k].x);
+                const uint32_t d23 = half2_sub(hv[k + 1].y, hv[k].y);
+                const uint4 e = make_uint4(__builtin_amdgcn_perm(d01, hv[k].x, 0x05040100u),
+                                           __builtin_amdgcn_perm(d01, hv[k].x, 0x07060302u),
+                                           __builtin_amdgcn_perm(d23, hv[k].y, 0x05040100u),
+                                           __builtin_amdgcn_perm(d23, hv[k].y, 0x07060302u));
+                *reinterpret_cast<uint4*>(d + k * kLP * L::esz) = e;
+            } else {
+                const float4 lo = make_float4(sv[k][0], sv[k + 1][0] - sv[k][0], sv[k][1],
+                                              sv[k + 1][1] - sv[k][1]);
+                const float4 hi = make_float4(sv[k][2], sv[k + 1][2] - sv[k][2], sv[k][3],
+                                              sv[k + 1][3] - sv[k][3]);
+                *reinterpret_cast<float4*>(d + k * kLP * L::esz) = lo;
+                *reinterpret_cast<float4*>(d + k * kLP * L::esz + 16) = hi;
+            }
         }
     };
+    const char* const lds_bytes = reinterpret_cast<const char*>(lds);
 
+    // Per-view lane geometry: bs, w2, mz and the lane's window coordinates.
+    struct LaneView {
+        float bs, w2, mz, blo, bhi;
+        bool inwin;
+        int ce;
+    };
+    auto lane_view = [&](int ia, int4 w) {
+        LaneView r;
+        const ViewCS v = cs[ia];
+        const float xb = x * v.c + y * v.s;
+        const float yb = dso_ds - (-x * v.s + y * v.c);
+        const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
+        float sp;
+        if (ARC) {
+            sp = dsd_ds * atan2f(xb, yb);
+            r.w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
+        } else {
+            sp = mag * xb;
+            r.w2 = mag * mag;
+        }
+        if (FMT == F_F16S) r.w2 *= vscale[ia];
+        r.bs = sp + ws;
+        r.mz = mag * inv_dt_ds;
+        r.blo = fmaf(r.mz, zlo, wt);
+        r.bhi = fmaf(r.mz, zhi, wt);
+        // Window coordinates of the lane's taps: entry columns ce, ce+1 and
+        // entry rows (int)bt - 1 - rlo for bt from blo to bhi.  The test
+        // implies bs, blo, bhi >= 1 (truncation is floor there) and the
+        // detector bounds 1 <= floor(bs) < ns, 1 <= floor(bt) < nt.
+        r.ce = (int)r.bs - 1 - w.x;
+        const int rl = (int)r.blo - 1 - w.y, rh = (int)r.bhi - 1 - w.y;
+        const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
+        r.inwin = (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
+                  (uint32_t)rh < (uint32_t)w.w;
+        return r;
+    };
+
+    bool missed = false;       // some view's taps were outside the staged window
     int4 wcur = bwin[0];
     stage_load(0, wcur);
     stage_store(0, wcur);
@@ -426,42 +515,44 @@ __global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
         const int4 wnext = ia + 1 < nviews ? bwin[ia + 1] : make_int4(0, 0, 0, 0);
         stage_load(ia + 1, wnext);                          // in flight during this view
         if (any) {
-            const ColView cv = col_view<ARC, FMT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
-                                                  inv_dt_ds, FMT == F_F16S ? vscale[ia] : 1.f);
-            if (cv.hit) {
-                const float blo = fmaf(cv.mz, zlo, wt), bhi = fmaf(cv.mz, zhi, wt);
-                const float tlo = fminf(blo, bhi), thi = fmaxf(blo, bhi);
-                const bool fast = tlo >= 1.f && thi < ntf;
-                // window coordinates of the lane's taps: entry columns ce, ce+1,
-                // entry rows floor(bt)-1 over the lane's z range
-                const int ce = (int)cv.isf - 1 - wcur.x;
-                const int rl = (int)tlo - 1 - wcur.y, rh = (int)thi - 1 - wcur.y;
-                if (fast && ce >= 0 && ce + 2 <= wcur.z && rl >= 0 && rh < wcur.w) {
-                    // LDS entry of (is-1, it-1) is it*kLP + e0off (it - 1 - rlo in [rl, rh])
-                    const int e0off = buf * (kHmax * kLP) + ce - (wcur.y + 1) * kLP;
+            const LaneView lv = lane_view(ia, wcur);
+            if (lv.inwin) {
+                const float wb = lv.w2 * __builtin_amdgcn_fractf(lv.bs);
+                const float wa = lv.w2 - wb;
+                // byte offset of the LDS entry (ce, it - 1 - rlo) is it * kLP * esz + lb
+                const uint32_t lb = ((uint32_t)lv.ce +
+                                     (uint32_t)(buf * L::buf - (wcur.y + 1) * kLP)) * L::esz;
 #pragma unroll
-                    for (int h = 0; h < kSZ; h += 8) {
-                        float2 e0[8], e1[8];
-                        float wu[8];
+                for (int h = 0; h < kSZ; h += kSB) {
+                    float4 tp[kSB];        // (a0, d0, a1, d1)
+                    float wu[kSB];
 #pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const float bt = fmaf(cv.mz, z[h + j], wt);
-                            wu[j] = __builtin_amdgcn_fractf(bt);
-                            const float2* p = lds + ((int)__umul24((uint32_t)bt, (uint32_t)kLP) + e0off);
-                            e0[j] = p[0];
-                            e1[j] = p[1];
-                        }
-#pragma unroll
-                        for (int j = 0; j < 8; ++j) {
-                            const float p0 = fmaf(wu[j], e0[j].y, e0[j].x);
-                            const float p1 = fmaf(wu[j], e1[j].y, e1[j].x);
-                            acc[h + j] = fmaf(cv.wb, p1, fmaf(cv.wa, p0, acc[h + j]));
+                    for (int j = 0; j < kSB; ++j) {
+                        const float bt = fmaf(lv.mz, z[h + j], wt);
+                        wu[j] = __builtin_amdgcn_fractf(bt);
+                        // one in-bounds offset (the 32-bit sum wraps to the entry's offset)
+                        const char* p = lds_bytes +
+                            (int)(__umul24((uint32_t)bt, (uint32_t)(kLP * L::esz)) + lb);
+                        if (L::half) {
+                            const uint32_t* u = reinterpret_cast<const uint32_t*>(p);
+                            const uint32_t u0 = u[0], u1 = u[1];
+                            tp[j] = make_float4(f16f(u0 & 0xffffu), f16f(u0 >> 16),
+                                                f16f(u1 & 0xffffu), f16f(u1 >> 16));
+                        } else {
+                            const float2* f = reinterpret_cast<const float2*>(p);
+                            const float2 e0 = f[0], e1 = f[1];
+                            tp[j] = make_float4(e0.x, e0.y, e1.x, e1.y);
                         }
                     }
-                } else {
-                    gather_view<kSZ, FMT>(cv, z, wt, ntf, fast,
-                                          view_rsrc(qb + ia * viewb, vbytes), rowb, acc);
+#pragma unroll
+                    for (int j = 0; j < kSB; ++j) {
+                        const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
+                        const float p1 = fmaf(wu[j], tp[j].w, tp[j].z);
+                        acc[h + j] = fmaf(wb, p1, fmaf(wa, p0, acc[h + j]));
+                    }
                 }
+            } else {
+                missed = true;
             }
         }
         stage_store(buf ^ 1, wnext);
@@ -469,15 +560,30 @@ __global__ __launch_bounds__(256, 4) void cbct_staged_kernel(
         wcur = wnext;
     }
 
+    // Views whose taps missed the staged window (detector edges in t, windows
+    // larger than the LDS buffer): gathered from global memory, lane by lane.
+    if (missed) {
+        for (int ia = 0; ia < nviews; ++ia) {
+            const LaneView lv = lane_view(ia, bwin[ia]);
+            if (lv.inwin || !(lv.bs >= 1.f && lv.bs < nsf)) continue;    // 1 <= floor(bs) < ns
+            ColView cv;
+            cv.isf = floorf(lv.bs);
+            cv.wb = lv.w2 * (lv.bs - cv.isf);
+            cv.wa = lv.w2 - cv.wb;
+            cv.mz = lv.mz;
+            const float tlo = fminf(lv.blo, lv.bhi), thi = fmaxf(lv.blo, lv.bhi);
+            const bool fast = tlo >= 1.f && thi < ntf;
+            gather_view<kSZ, FMT>(cv, z, wt, ntf, fast, view_rsrc(qb + ia * viewb, vbytes),
+                                  rowb, acc);
+        }
+    }
+
+    // Masked-out voxels are written as 0 (a select on the prologue's bits:
+    // no second mask read inside a branch).
     if (inxy) {
 #pragma unroll
-        for (int j = 0; j < kSZ; ++j) {
-            const int iz = iz0 + j;
-            if (iz < nz) {
-                const bool in = mask == nullptr || mask[col + nxy * iz];
-                image[col + nxy * iz] = in ? acc[j] : 0.f;
-            }
-        }
+        for (int j = 0; j < kSZ; ++j)
+            if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
     }
 }
 

@@ -349,25 +349,59 @@ void launch_mode(const Params& p, hipStream_t st)
 }
 
 // Per-view max |weighted input|: block (view, y) covers kBoundCols columns of
-// one view, threads stride over the bins (coalesced), one atomic per block.
+// one view; thread i takes the bins i, i + 256, ... (coalesced), with the
+// bin factors of the fused weight (wbin, Parker, arc cos(s/dsd)) formed once
+// per bin and the column factors once per column.  One atomic per block.
 constexpr int kBoundCols = 16;
+constexpr int kBoundBins = 8;          // bins per thread: nb <= 2048
 __global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __restrict__ vbound)
 {
     const int view = blockIdx.x;
     const int t0 = blockIdx.y * kBoundCols;
-    float m = 0.f;
+    float wm[kBoundBins], s2[kBoundBins];
+#pragma unroll
+    for (int k = 0; k < kBoundBins; ++k) {
+        const int m = threadIdx.x + 256 * k;
+        wm[k] = 0.f;
+        s2[k] = 0.f;
+        if (m < p.nb) {
+            float w = p.wbin ? p.wbin[m] : 1.f;
+            if (p.cyl == 1) s2[k] = p.s_axis[m] * p.s_axis[m];
+            if (p.cyl == 2) w *= cosf(p.s_axis[m] / p.cyl_dsd);
+            if (p.gam) w *= (float)parker_w(p.gam[m], p.bet[view], p.gamma_max);
+            wm[k] = w;
+        }
+    }
+    const float colw = p.scale * (p.wview ? p.wview[view] : 1.f);
+    float mx = 0.f;
     for (int t = t0; t < min(t0 + kBoundCols, p.nt); ++t) {
         const int64_t c = (int64_t)view * p.nt + t;
         const float* col = p.in + (int64_t)p.nb * c;
-        for (int n = threadIdx.x; n < p.nb; n += 256) m = fmaxf(m, fabsf(col[n] * weight(p, n, c)));
+        float cw = colw, tt2 = 0.f;
+        if (p.cyl == 1) {
+            tt2 = p.t_axis[t] * p.t_axis[t] + p.cyl_dsd * p.cyl_dsd;
+            cw *= p.cyl_dso;
+        } else if (p.cyl == 2) {
+            const float r = p.t_axis[t] / p.cyl_dsd;
+            cw *= (p.cyl_dso / p.cyl_dsd) / sqrtf(r * r + 1.f);
+        }
+#pragma unroll
+        for (int k = 0; k < kBoundBins; ++k) {
+            const int m = threadIdx.x + 256 * k;
+            if (m < p.nb) {
+                float w = cw * wm[k];
+                if (p.cyl == 1) w *= rsqrtf(s2[k] + tt2);
+                mx = fmaxf(mx, fabsf(col[m] * w));
+            }
+        }
     }
-    for (int o = 32; o >= 1; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     __shared__ float red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        atomicMax(vbound + view, __float_as_uint(m));      // m >= 0: bit order = value order
+        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        atomicMax(vbound + view, __float_as_uint(mx));     // mx >= 0: bit order = value order
     }
 }
 
@@ -457,6 +491,8 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
 void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t st)
 {
     if (a.ncols <= 0) return;
+    if (a.nb > 256 * kBoundBins)
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
     const int64_t na = a.ncols / a.nt;
     launch_zero_u32(vbound, na, st);
     const Params p = make_params(a);

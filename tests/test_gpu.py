@@ -367,10 +367,21 @@ def test_fdk_stored_data_exact_vs_oracle(sg, orc, torch_cuda, prec):
     if prec == "f16s":
         assert np.all(np.frexp(scale)[0] == 0.5)      # powers of two
         assert np.abs(q).max() <= 2.0 ** 14
-    q = q.astype(np.float64) * scale[None, None, :]
     xc, yc, zc = ig.axes()
-    ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
-                        xc, yc, zc, ig.mask, dtype=np.float32)
+    if prec == "f16s":
+        # the staged kernel keeps (value, row difference) pairs in FP16: the
+        # upper tap is a + fp16(b - a); check against the pair-reading oracle
+        a = q.astype(np.float16)
+        d = np.zeros_like(a)
+        d[:, :-1, :] = a[:, 1:, :] - a[:, :-1, :]          # FP16 arithmetic, rounded once
+        sc = scale[None, None, :].astype(np.float64)
+        ref = orc.cbct_back_pairs(a.astype(np.float64) * sc, d.astype(np.float64) * sc,
+                                  rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, True,
+                                  xc, yc, zc, ig.mask, dtype=np.float32)
+    else:
+        q = q.astype(np.float64) * scale[None, None, :]
+        ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
+                            xc, yc, zc, ig.mask, dtype=np.float32)
     assert rel_rmse(got, ref) < TOL32
 
 

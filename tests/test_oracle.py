@@ -387,3 +387,24 @@ def test_bdd_matches_brute_force_overlap(orc):
         L[k] = abs(p / ((detm[k + 1] - detm[k]) * math.cos(th)))
     assert np.allclose(sino[0], total * L, rtol=2e-4, atol=1e-6 * np.abs(total * L).max())
 
+
+def test_cbct_back_pairs_matches_plain(orc):
+    # The pair-reading checker equals the plain restatement when the stored
+    # differences are the exact row differences (Float64), for flat and arc.
+    rng = np.random.default_rng(12)
+    ns, nt, na = 20, 14, 9
+    q = rng.standard_normal((ns, nt, na))
+    d = np.zeros_like(q)
+    d[:, :-1, :] = q[:, 1:, :] - q[:, :-1, :]
+    d[:, -1, :] = rng.standard_normal((ns, na))      # never read: it = nt is out of range
+    betas = np.linspace(0, 2 * np.pi, na, endpoint=False)
+    xc = (np.arange(10) - 4.5) * 1.1
+    yc = (np.arange(9) - 4.0) * 1.1
+    zc = (np.arange(7) - 3.0) * 1.3
+    mask = np.ones((10, 9, 7), dtype=bool)
+    for arc in (False, True):
+        args = (betas, 200.0, 120.0, 1.0, 1.1)
+        ref = orc.cbct_back(q, *args, 0.0, 0.0, arc, xc, yc, zc, mask)
+        got = orc.cbct_back_pairs(q, d, *args, arc, xc, yc, zc, mask)
+        assert np.abs(ref).max() > 0
+        assert np.allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
