@@ -47,6 +47,26 @@ def make_problem(device):
     return sg, rg, ig, ell, proj
 
 
+TRAFFIC_FILE = "profiles/fdk256_counters.md"
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` as recorded in the committed rocprofv3
+    counter summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy
+    kernel of known size), or None when the summary does not list it."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
+    try:
+        with open(path) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and \
+                        parts[1] == kernel:
+                    return float(parts[2])
+    except OSError:
+        return None
+    return None
+
+
 def time_steps(fn, steps, warmup):
     import torch
     for _ in range(warmup):
@@ -173,14 +193,26 @@ def main():
             f"back {kt[f][1]:.3f} ms)")
     vols = {f: outs[f].permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
             for f in forms}
+    # PCIe-inclusive rate: host projections in, host volume out (the Julia
+    # binding's ccall path), FP32 plan.
+    log("timing host-buffer executes ...")
+    proj_host = np.asfortranarray(proj.permute(2, 1, 0).contiguous().cpu().numpy()
+                                  .transpose(2, 1, 0))
+    t_host = time_steps(lambda: plans["fp32"].execute(proj_host), 3, 1)
 
     t32 = step["fp32"]
     f32ms, b32ms = kt["fp32"]
     achieved = 16.0 * updates / (b32ms * 1e-3) / 1e9
+    traffic = measured_traffic("cbct_staged_kernel<false,0>") \
+        if args.cbct_kernel == "staged" else None
+    compulsory = 4 * rg.ns * rg.nt * rg.na + 4 * updates // rg.na
     roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
             "kernel": f"cbct_back FP32, {args.cbct_kernel} (gather-equivalent 16 B/update)",
-            "kernel_ms": b32ms, "filter_ms": f32ms}
+            "kernel_ms": b32ms, "filter_ms": f32ms,
+            "traffic_source": TRAFFIC_FILE if traffic else None,
+            "traffic_GBs_at_kernel_ms": traffic / (b32ms * 1e-3) / 1e9 if traffic else None,
+            "compulsory_bytes": compulsory}
     filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
 
     def form(f, bpu):
@@ -209,6 +241,10 @@ def main():
                    "updates_per_step": updates, "projections_per_s": rg.na / t32,
                    "filter_TFLOPs_algorithmic_fp32": filt_flops / (f32ms * 1e-3) / 1e12},
         "roofline": roof,
+        "pcie_inclusive": {"value": updates / t_host / 1e9, "unit": "Gupdates/s",
+                           "ms_per_step": t_host * 1e3,
+                           "note": "FP32 plan, host projections in and host volume out "
+                                   "(pageable numpy arrays)"},
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
     }

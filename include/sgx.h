@@ -82,7 +82,9 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
 /* Kernel choices of a context (no counterpart in the reference; the results
  * agree to FP32 rounding whatever the choice).
  *   SGX_OPT_CBCT_KERNEL: 0 = LDS-staged cone-beam backprojector (default),
- *                        1 = gather backprojector. */
+ *                        1 = gather backprojector.
+ * Plans take the context's choice when they are created (and an autotuned
+ * choice after sgx_plan_autotune). */
 enum { SGX_OPT_CBCT_KERNEL = 1 };
 int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value);
 int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value);
@@ -237,6 +239,43 @@ int sgx_plan_view_weight(sgx_plan* plan, float* out);
 /* Device time (ms) of the backprojection and filter launches of the last
  * execute, measured with HIP events on the plan's stream. */
 int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms);
+
+/* ---- autotuning and batches (fbp(plan, aa) serial slice loop,
+ *      src/fbp2/fbp.jl:46-68, as one graph) --------------------------------- */
+
+/* Time the plan's candidate kernel shapes on (sino, image) (device pointers;
+ * `reps` timed executes each) and keep the fastest in the plan.  The choice
+ * is cached in the context per size bucket (kind, precision and the
+ * power-of-two ceilings of nx, ny, nz, ns, nt, na); with `use_cache` a plan
+ * whose bucket is cached takes it without timing.  `image` holds the chosen
+ * shape's result on return.  Candidates: cone-beam, the staged and gather
+ * backprojectors; 2-D, view-grouping targets {1024, 2048, 4096} blocks x LDS
+ * view chunks {16, 32, 64}. */
+int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
+                      int use_cache);
+/* The plan's kernel shape: values[0] = cone-beam kernel (0 staged, 1
+ * gather), values[1] = 2-D view-grouping target, values[2] = 2-D view chunk,
+ * values[3] = 1 when autotuned. */
+int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);
+
+/* A batch: many reconstructions (plan, input, output; device pointers) run as
+ * one HIP graph.  The first execute after items were added autotunes the
+ * plans not yet tuned (through the context cache) and captures every item
+ * into the graph, with independent plans on parallel branches (items of one
+ * plan stay in order on one branch); later executes replay the graph with
+ * the same pointers.  No memset nodes are captured. */
+typedef struct sgx_batch sgx_batch;
+int sgx_batch_create(sgx_ctx* ctx, sgx_batch** out);
+void sgx_batch_destroy(sgx_batch* batch);
+int sgx_batch_add(sgx_batch* batch, sgx_plan* plan, const float* sino, float* image);
+/* flags: SGX_ASYNC returns without waiting for the graph;
+ * SGX_BATCH_NO_TUNE keeps the plans' current shapes. */
+#define SGX_BATCH_NO_TUNE 0x100u
+int sgx_batch_execute(sgx_batch* batch, uint32_t flags);
+/* Milliseconds of the last execute's graph launch (HIP events). */
+int sgx_batch_last_ms(sgx_batch* batch, float* ms);
+/* Number of items and of parallel branches of the captured graph. */
+int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches);
 
 #ifdef __cplusplus
 }

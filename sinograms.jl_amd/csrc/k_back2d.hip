@@ -29,7 +29,10 @@ constexpr int kTile = 16;
 constexpr int kThreads = kTile * kTile;
 constexpr int kLdsBytes = 48 * 1024;
 
-int chunk_views(int nb) { return std::max(1, std::min(64, kLdsBytes / (4 * nb))); }
+int chunk_views(int nb, int vc_max)
+{
+    return std::max(1, std::min(vc_max, kLdsBytes / (4 * nb)));
+}
 
 template <int GEOM>  // 0 parallel, 1 fan arc, 2 fan flat
 __global__ __launch_bounds__(kThreads) void back2d_kernel(
@@ -112,20 +115,20 @@ __global__ void sum_partials_kernel(const float* __restrict__ part, int groups, 
     image[i] = (mask == nullptr || mask[i]) ? s : 0.f;
 }
 
-int groups_for(int nb, int nviews, int nx, int ny)
+int groups_for(int nb, int nviews, int nx, int ny, int want_blocks, int vc_max)
 {
     const int tiles = ((nx + kTile - 1) / kTile) * ((ny + kTile - 1) / kTile);
-    const int vc = chunk_views(nb);
+    const int vc = chunk_views(nb, vc_max);
     const int max_groups = (nviews + vc - 1) / vc;
-    const int want = (2048 + tiles - 1) / tiles;
+    const int want = (want_blocks + tiles - 1) / tiles;
     return std::max(1, std::min(want, max_groups));
 }
 
 }  // namespace
 
-int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny)
+int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_blocks, int vc_max)
 {
-    const int g = groups_for(nb, nviews, nx, ny);
+    const int g = groups_for(nb, nviews, nx, ny, want_blocks, vc_max);
     return g > 1 ? (int64_t)g * nx * ny : 0;
 }
 
@@ -133,8 +136,9 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
 {
     if ((int64_t)a.nb * 4 > kLdsBytes)
         throw Error(SGX_ERR_UNSUPPORTED, "back2d: nb > 12288 bins is not supported");
-    const int vc = chunk_views(a.nb);
-    int groups = groups_for(a.nb, a.nviews, a.nx, a.ny);
+    if (a.want_blocks < 1 || a.vc_max < 1) throw Error(SGX_ERR_ARG, "back2d: bad shape");
+    const int vc = chunk_views(a.nb, a.vc_max);
+    int groups = groups_for(a.nb, a.nviews, a.nx, a.ny, a.want_blocks, a.vc_max);
     const int64_t npix = (int64_t)a.nx * a.ny;
     if (groups > 1 && (a.partial == nullptr || a.partial_cap < groups * npix)) groups = 1;
     // views per group: a whole number of LDS chunks

@@ -8,15 +8,32 @@
 
 #include "sgx_internal.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <sstream>
+#include <string>
+#include <vector>
 
 using sgx::Error;
 
+// Kernel shapes of one plan: the cone-beam backprojector (0 staged, 1
+// gather); the 2-D backprojector's view grouping target and LDS view chunk.
+struct PlanTune {
+    int cbct = 0;
+    int want_blocks = 2048;
+    int vc_max = 64;
+};
+
+// Largest view-grouping target and smallest LDS chunk the autotuner tries.
+constexpr int kTuneMaxBlocks = 4096;
+constexpr int kTuneMinChunk = 16;
+
 struct sgx_ctx {
     int device = 0;
+    std::map<std::string, PlanTune> tune_cache;   // autotuned shapes per size bucket
     int cbct_kernel = 0;          // SGX_OPT_CBCT_KERNEL
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -197,7 +214,8 @@ void back2d_common(sgx_ctx* ctx, int geom, const float* sino, int64_t nb, int64_
     a.dsd_ds = (float)dsd / (float)ds;
     a.dso_ds = (float)dso / (float)ds;
     a.so_ds = (float)source_offset / (float)ds;
-    const int64_t pf = sgx::back2d_partial_floats(a.nb, a.nviews, a.nx, a.ny);
+    const int64_t pf = sgx::back2d_partial_floats(a.nb, a.nviews, a.nx, a.ny, a.want_blocks,
+                                                  a.vc_max);
     a.partial = static_cast<float*>(ctx->get(W_PARTIAL, 4 * pf));
     a.partial_cap = pf;
     sgx::launch_back2d(a, ctx->stream);
@@ -505,6 +523,8 @@ int sgx_backproject_bdd(sgx_ctx* ctx, const double* sino, int64_t nPix, int64_t 
 
 struct sgx_plan {
     sgx_ctx* ctx = nullptr;
+    PlanTune tune;
+    bool tuned = false;
     sgx_geom geom{};
     int prec = SGX_PREC_FP32;
     int64_t nx = 0, ny = 0, nz = 1;
@@ -617,6 +637,132 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     return a;
 }
 
+// One plan execute on stream `st` (device pointers): weights + filter, then
+// backprojection, with the plan's kernel shapes.  `timed` brackets the two
+// phases with the plan's events (left out under graph capture).
+void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool timed)
+{
+    const sgx_geom& G = P.geom;
+    if (timed) SGX_HIP(hipEventRecord(P.ev[0], st));
+    const sgx::FilterArgs fa = P.filter_args(in, P.q);
+    if (P.prec == SGX_PREC_F16S) sgx::launch_filter_bound(fa, P.vbound, st);
+    sgx::launch_sino_filter(fa, st);
+    if (timed) SGX_HIP(hipEventRecord(P.ev[1], st));
+    if (P.is3d()) {
+        sgx::CbctArgs a = cbct_args_of(P, img);
+        if (P.tune.cbct == 1) a.win = nullptr;
+        sgx::launch_cbct_back(a, st);
+    } else {
+        sgx::Back2dArgs a{};
+        a.q = static_cast<const float*>(P.q);
+        a.nb = (int)G.ns;
+        a.nviews = (int)G.na;
+        a.view_stride = G.ns;
+        a.cs = P.cs;
+        a.xds = P.xds;
+        a.yds = P.yds;
+        a.nx = (int)P.nx;
+        a.ny = (int)P.ny;
+        a.mask = P.mask;
+        a.image = img;
+        a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+        a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+        a.dsd_ds = (float)G.dsd / (float)G.ds;
+        a.dso_ds = (float)G.dso / (float)G.ds;
+        a.so_ds = (float)G.source_offset / (float)G.ds;
+        a.partial = P.partial;
+        a.partial_cap = P.partial_cap;
+        a.want_blocks = P.tune.want_blocks;
+        a.vc_max = P.tune.vc_max;
+        sgx::launch_back2d(a, st);
+    }
+    if (timed) {
+        SGX_HIP(hipEventRecord(P.ev[2], st));
+        P.timed = true;
+    }
+}
+
+// Candidate shapes of the autotuner.
+std::vector<PlanTune> tune_candidates(const sgx_plan& P)
+{
+    std::vector<PlanTune> c;
+    if (P.is3d()) {
+        for (int k : {0, 1}) c.push_back(PlanTune{k, 2048, 64});
+    } else {
+        for (int want : {1024, 2048, kTuneMaxBlocks})
+            for (int vc : {kTuneMinChunk, 32, 64}) c.push_back(PlanTune{0, want, vc});
+    }
+    return c;
+}
+
+uint64_t ceil_log2(int64_t v)
+{
+    uint64_t k = 0;
+    while ((int64_t(1) << k) < v) ++k;
+    return k;
+}
+
+// Size bucket of a plan: kind, precision and the power-of-two ceilings of
+// the image and detector sizes and the view count.
+std::string tune_bucket(const sgx_plan& P)
+{
+    const sgx_geom& G = P.geom;
+    std::ostringstream os;
+    os << G.kind << '/' << P.prec << '/' << ceil_log2(P.nx) << '/' << ceil_log2(P.ny) << '/'
+       << ceil_log2(P.nz) << '/' << ceil_log2(G.ns) << '/' << ceil_log2(G.nt) << '/'
+       << ceil_log2(G.na);
+    return os.str();
+}
+
+// Time every candidate shape on (in, img) and keep the fastest in the plan
+// and in the context's cache for the plan's size bucket.
+void autotune(sgx_plan& P, const float* in, float* img, int reps, bool use_cache)
+{
+    sgx_ctx* ctx = P.ctx;
+    const std::string key = tune_bucket(P);
+    if (use_cache) {
+        auto it = ctx->tune_cache.find(key);
+        if (it != ctx->tune_cache.end()) {
+            P.tune = it->second;
+            P.tuned = true;
+            plan_run(P, in, img, ctx->stream, false);
+            return;
+        }
+    }
+    hipStream_t st = ctx->stream;
+    hipEvent_t e0, e1;
+    SGX_HIP(hipEventCreate(&e0));
+    SGX_HIP(hipEventCreate(&e1));
+    float best = 3.4e38f;
+    PlanTune best_t = P.tune;
+    try {
+        for (const PlanTune& t : tune_candidates(P)) {
+            P.tune = t;
+            plan_run(P, in, img, st, false);                      // warm
+            SGX_HIP(hipEventRecord(e0, st));
+            for (int r = 0; r < reps; ++r) plan_run(P, in, img, st, false);
+            SGX_HIP(hipEventRecord(e1, st));
+            SGX_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            SGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (ms < best) {
+                best = ms;
+                best_t = t;
+            }
+        }
+    } catch (...) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    P.tune = best_t;
+    P.tuned = true;
+    ctx->tune_cache[key] = best_t;
+    plan_run(P, in, img, st, false);          // leave the chosen shape's result in img
+}
+
 extern "C" {
 
 int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
@@ -651,6 +797,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->ny = ny;
         P->nz = is3d ? nz : 1;
         P->npad = npad;
+        P->tune.cbct = ctx->cbct_kernel;
 
         const int64_t ns = G.ns, nt = G.nt, na = G.na;
         TablePack pack;
@@ -740,7 +887,9 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             SGX_HIP(hipMalloc(&P->vscale, 4 * na));
         }
         if (!is3d) {
-            P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny);
+            // sized for the largest shape the autotuner tries
+            P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny,
+                                                        kTuneMaxBlocks, kTuneMinChunk);
             if (P->partial_cap) SGX_HIP(hipMalloc(&P->partial, 4 * P->partial_cap));
         }
         if (is3d) {
@@ -795,41 +944,7 @@ int sgx_plan_execute(sgx_plan* plan, const float* sino, float* image, uint32_t f
         const int64_t nin = G.ns * G.nt * G.na, nvox = P.nx * P.ny * P.nz;
         const float* in = static_cast<const float*>(stage_in(ctx, W_IN, sino, 4 * nin, flags));
         float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
-        hipStream_t st = ctx->stream;
-
-        SGX_HIP(hipEventRecord(P.ev[0], st));
-        const sgx::FilterArgs fa = P.filter_args(in, P.q);
-        if (P.prec == SGX_PREC_F16S) sgx::launch_filter_bound(fa, P.vbound, st);
-        sgx::launch_sino_filter(fa, st);
-        SGX_HIP(hipEventRecord(P.ev[1], st));
-        if (P.is3d()) {
-            sgx::CbctArgs a = cbct_args_of(P, img);
-            if (ctx->cbct_kernel == 1) a.win = nullptr;
-            sgx::launch_cbct_back(a, st);
-        } else {
-            sgx::Back2dArgs a{};
-            a.q = static_cast<const float*>(P.q);
-            a.nb = (int)G.ns;
-            a.nviews = (int)G.na;
-            a.view_stride = G.ns;
-            a.cs = P.cs;
-            a.xds = P.xds;
-            a.yds = P.yds;
-            a.nx = (int)P.nx;
-            a.ny = (int)P.ny;
-            a.mask = P.mask;
-            a.image = img;
-            a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
-            a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
-            a.dsd_ds = (float)G.dsd / (float)G.ds;
-            a.dso_ds = (float)G.dso / (float)G.ds;
-            a.so_ds = (float)G.source_offset / (float)G.ds;
-            a.partial = P.partial;
-            a.partial_cap = P.partial_cap;
-            sgx::launch_back2d(a, st);
-        }
-        SGX_HIP(hipEventRecord(P.ev[2], st));
-        P.timed = true;
+        plan_run(P, in, img, ctx->stream, true);
         finish(ctx, image, img, 4 * nvox, flags);
     });
 }
@@ -907,6 +1022,200 @@ int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms)
         if (filter_ms) *filter_ms = a;
         if (back_ms) *back_ms = b;
     });
+}
+
+int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps, int use_cache)
+{
+    if (!plan) return SGX_ERR_ARG;
+    return guard(plan->ctx, [&] {
+        require(sino && image, SGX_ERR_ARG, "null pointer argument");
+        require(reps >= 1, SGX_ERR_ARG, "reps must be >= 1");
+        autotune(*plan, sino, image, reps, use_cache != 0);
+        SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
+    });
+}
+
+int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4])
+{
+    if (!plan || !values) return SGX_ERR_ARG;
+    values[0] = plan->tune.cbct;
+    values[1] = plan->tune.want_blocks;
+    values[2] = plan->tune.vc_max;
+    values[3] = plan->tuned ? 1 : 0;
+    return SGX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+
+struct sgx_batch {
+    struct Item {
+        sgx_plan* plan;
+        const float* in;
+        float* out;
+    };
+    sgx_ctx* ctx = nullptr;
+    std::vector<Item> items;
+    std::vector<hipStream_t> branches;       // capture streams (default priority)
+    std::vector<hipEvent_t> joins;
+    hipStream_t origin = nullptr;            // capture origin
+    hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool dirty = true;
+    bool launched = false;
+    int64_t nbranch = 0;
+
+    void drop_graph()
+    {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~sgx_batch()
+    {
+        drop_graph();
+        for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        for (hipStream_t s : branches) (void)hipStreamDestroy(s);
+        for (hipEvent_t e : {fork, t0, t1})
+            if (e) (void)hipEventDestroy(e);
+        if (origin) (void)hipStreamDestroy(origin);
+    }
+
+    // Capture every item: plans are spread over at most kBranches parallel
+    // branches, largest work first onto the least loaded branch; the items of
+    // one plan stay on one branch, in the order they were added.
+    void capture()
+    {
+        constexpr int kBranches = 4;
+        std::vector<sgx_plan*> plans;
+        std::map<sgx_plan*, double> work;
+        for (const Item& it : items) {
+            if (!work.count(it.plan)) plans.push_back(it.plan);
+            const sgx_plan& P = *it.plan;
+            work[it.plan] += (double)P.nx * P.ny * P.nz * P.geom.na;
+        }
+        std::sort(plans.begin(), plans.end(),
+                  [&](sgx_plan* a, sgx_plan* b) { return work[a] > work[b]; });
+        const int nb = (int)std::min<size_t>(kBranches, plans.size());
+        std::vector<double> load(nb, 0.0);
+        std::map<sgx_plan*, int> branch_of;
+        for (sgx_plan* p : plans) {
+            const int b = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+            branch_of[p] = b;
+            load[b] += work[p];
+        }
+        while ((int)branches.size() < nb) {
+            hipStream_t s;
+            SGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            branches.push_back(s);
+            hipEvent_t e;
+            SGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            joins.push_back(e);
+        }
+        drop_graph();
+        SGX_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
+        try {
+            SGX_HIP(hipEventRecord(fork, origin));
+            for (int b = 0; b < nb; ++b) SGX_HIP(hipStreamWaitEvent(branches[b], fork, 0));
+            for (const Item& it : items)
+                plan_run(*it.plan, it.in, it.out, branches[branch_of[it.plan]], false);
+            for (int b = 0; b < nb; ++b) {
+                SGX_HIP(hipEventRecord(joins[b], branches[b]));
+                SGX_HIP(hipStreamWaitEvent(origin, joins[b], 0));
+            }
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(origin, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        SGX_HIP(hipStreamEndCapture(origin, &graph));
+        SGX_HIP(hipGraphInstantiateWithFlags(&exec, graph, 0));
+        nbranch = nb;
+        dirty = false;
+    }
+};
+
+extern "C" {
+
+int sgx_batch_create(sgx_ctx* ctx, sgx_batch** out)
+{
+    return guard(ctx, [&] {
+        require(out, SGX_ERR_ARG, "null pointer argument");
+        *out = nullptr;
+        auto B = std::make_unique<sgx_batch>();
+        B->ctx = ctx;
+        SGX_HIP(hipStreamCreateWithFlags(&B->origin, hipStreamNonBlocking));
+        SGX_HIP(hipEventCreateWithFlags(&B->fork, hipEventDisableTiming));
+        SGX_HIP(hipEventCreate(&B->t0));
+        SGX_HIP(hipEventCreate(&B->t1));
+        *out = B.release();
+    });
+}
+
+void sgx_batch_destroy(sgx_batch* batch)
+{
+    if (!batch) return;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipStreamSynchronize(batch->ctx->stream);
+    delete batch;
+}
+
+int sgx_batch_add(sgx_batch* batch, sgx_plan* plan, const float* sino, float* image)
+{
+    if (!batch) return SGX_ERR_ARG;
+    return guard(batch->ctx, [&] {
+        require(plan && sino && image, SGX_ERR_ARG, "null pointer argument");
+        require(plan->ctx == batch->ctx, SGX_ERR_ARG, "the plan belongs to another context");
+        batch->items.push_back({plan, sino, image});
+        batch->dirty = true;
+    });
+}
+
+int sgx_batch_execute(sgx_batch* batch, uint32_t flags)
+{
+    if (!batch) return SGX_ERR_ARG;
+    sgx_ctx* ctx = batch->ctx;
+    return guard(ctx, [&] {
+        require(!batch->items.empty(), SGX_ERR_ARG, "empty batch");
+        require(!(flags & SGX_HOST), SGX_ERR_ARG, "batches take device pointers");
+        if (batch->dirty) {
+            if (!(flags & SGX_BATCH_NO_TUNE)) {
+                for (auto& it : batch->items)
+                    if (!it.plan->tuned) autotune(*it.plan, it.in, it.out, 3, true);
+            }
+            SGX_HIP(hipStreamSynchronize(ctx->stream));
+            batch->capture();
+        }
+        SGX_HIP(hipEventRecord(batch->t0, ctx->stream));
+        SGX_HIP(hipGraphLaunch(batch->exec, ctx->stream));
+        SGX_HIP(hipEventRecord(batch->t1, ctx->stream));
+        batch->launched = true;
+        if (!(flags & SGX_ASYNC)) SGX_HIP(hipStreamSynchronize(ctx->stream));
+    });
+}
+
+int sgx_batch_last_ms(sgx_batch* batch, float* ms)
+{
+    if (!batch || !ms) return SGX_ERR_ARG;
+    return guard(batch->ctx, [&] {
+        require(batch->launched, SGX_ERR_ARG, "no execute recorded yet");
+        SGX_HIP(hipEventSynchronize(batch->t1));
+        SGX_HIP(hipEventElapsedTime(ms, batch->t0, batch->t1));
+    });
+}
+
+int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches)
+{
+    if (!batch) return SGX_ERR_ARG;
+    if (items) *items = (int64_t)batch->items.size();
+    if (branches) *branches = batch->nbranch;
+    return SGX_OK;
 }
 
 }  // extern "C"

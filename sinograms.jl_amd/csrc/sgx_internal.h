@@ -88,9 +88,13 @@ struct Back2dArgs {
     float dsd_ds, dso_ds, so_ds;
     float* partial;         // workspace for view-group partial images (may be null)
     int64_t partial_cap;    // floats available in partial
+    // shape: views are split into groups until pixel tiles x groups reaches
+    // want_blocks; at most vc_max views per LDS chunk (and 48 KB)
+    int want_blocks = 2048;
+    int vc_max = 64;
 };
 void launch_back2d(const Back2dArgs& a, hipStream_t st);
-int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny);
+int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_blocks, int vc_max);
 
 // K4 (+K5): Toeplitz row filter with fused weights on MFMA.
 struct FilterArgs {

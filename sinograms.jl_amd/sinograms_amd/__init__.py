@@ -14,6 +14,7 @@ from .recon import (Boxcar, FBPNormalPlan, FDKplan, Hamming, Hann, NoWindow, Siz
                     fbp_window, fdk, fdk_weight_cyl, parker_weight, parker_weight_fan_short,
                     parker_weight_par, plan_fbp, ramp_arc, ramp_flat)
 from . import phantom
+from .batch import Batch, autotune, tuning
 from .sys2 import backproject_bdd, project_bdd
 from ._lib import LIB_PATH, SgxError
 
@@ -24,5 +25,5 @@ __all__ = [
     "Window", "NoWindow", "Boxcar", "Hamming", "Hann", "WindowVect", "parker_weight",
     "parker_weight_par", "parker_weight_fan_short", "fbp_sino_weight", "fdk_weight_cyl",
     "FBPNormalPlan", "FDKplan", "SizeMismatch", "SgxError", "phantom", "project_bdd",
-    "backproject_bdd",
+    "backproject_bdd", "Batch", "autotune", "tuning",
 ]

@@ -43,7 +43,9 @@ EXPORTS = (
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
     "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
-    "sgx_ctx_set_option", "sgx_ctx_get_option",
+    "sgx_ctx_set_option", "sgx_ctx_get_option", "sgx_plan_autotune", "sgx_plan_tuning",
+    "sgx_batch_create", "sgx_batch_destroy", "sgx_batch_add", "sgx_batch_execute",
+    "sgx_batch_last_ms", "sgx_batch_info",
     "sgx_project_bdd", "sgx_backproject_bdd",
 )
 
@@ -129,9 +131,20 @@ def _declare(L):
                                       u32]
     L.sgx_plan_last_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                       ctypes.POINTER(ctypes.c_float)]
+    L.sgx_plan_autotune.argtypes = [vp, vp, vp, i32, i32]
+    L.sgx_plan_tuning.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.sgx_batch_create.argtypes = [vp, ctypes.POINTER(vp)]
+    L.sgx_batch_destroy.argtypes = [vp]
+    L.sgx_batch_destroy.restype = None
+    L.sgx_batch_add.argtypes = [vp, vp, vp, vp]
+    L.sgx_batch_execute.argtypes = [vp, u32]
+    L.sgx_batch_last_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
+    L.sgx_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                 ctypes.POINTER(ctypes.c_int64)]
 
 
 SGX_OPT_CBCT_KERNEL = 1
+SGX_BATCH_NO_TUNE = 0x100
 
 
 class Context:

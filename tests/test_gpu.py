@@ -423,3 +423,90 @@ def test_backproject_bdd_vs_oracle_and_adjoint(sg, orc, torch_cuda, nPix, nDet, 
     lhs = np.dot(sg.project_bdd(x, geo).ravel(), y.ravel())
     rhs = np.dot(x.ravel(), sg.backproject_bdd(y, geo, ref_alias=False).ravel())
     assert lhs == pytest.approx(rhs, rel=1e-12)
+
+
+# --- K7 batches (one HIP graph) and autotuning ----------------------------------------
+
+def _colmajor(torch_cuda, a):
+    a = np.asarray(a, dtype=np.float32)
+    return torch_cuda.from_numpy(np.ascontiguousarray(a.T)).cuda().permute(
+        *range(a.ndim - 1, -1, -1))
+
+
+def _batch_specs(sg):
+    """Uneven reconstructions: 2-D parallel / fan (arc, flat) slices and FDK
+    volumes (flat, arc; FP32 and scaled FP16)."""
+    specs = []
+    for kind, n, na in [("arc", 64, 90), ("flat", 96, 120), ("par", 80, 60), ("arc", 128, 180),
+                        ("arc", 64, 90)]:
+        if kind == "par":
+            rg = sg.SinoPar(nb=int(1.45 * n), na=na, d=1.0)
+        else:
+            rg = (sg.SinoFanArc if kind == "arc" else sg.SinoFanFlat)(
+                nb=int(1.6 * n), na=na, d=1.0, dsd=4.0 * n, dod=1.5 * n)
+        specs.append((rg, sg.ImageGeom((n, n), (1.0, 1.0)), "fp32"))
+    for kind, n, na, prec in [("flat", 32, 60, "fp32"), ("arc", 48, 72, "f16s"),
+                              ("flat", 40, 48, "fp32")]:
+        rg = (sg.CtFanArc if kind == "arc" else sg.CtFanFlat)(
+            ns=2 * n, nt=int(1.5 * n), ds=1.0, dt=1.0, na=na, dsd=6.0 * n, dod=2.0 * n)
+        specs.append((rg, sg.ImageGeom((n, n, n // 2), (1.0, 1.0, 1.0)), prec))
+    return specs
+
+
+def test_batch_graph_matches_unbatched(sg, torch_cuda):
+    _say("batch graph vs unbatched")
+    rng = np.random.default_rng(21)
+    specs = _batch_specs(sg)
+    plans = [sg.plan_fbp(rg, ig, precision=prec) for rg, ig, prec in specs]
+    # the last plan gets a second item (items of one plan share its buffers)
+    items = [(p, spec) for p, spec in zip(plans, specs)] + [(plans[-1], specs[-1])]
+    sinos = [_colmajor(torch_cuda, rng.standard_normal(spec[0].dims())) for _, spec in items]
+    outs = [sg.recon._empty_colmajor(spec[1].dims, torch_cuda.device("cuda:0"))
+            for _, spec in items]
+    batch = sg.Batch()
+    for (p, _), x, y in zip(items, sinos, outs):
+        batch.add(p, x, y)
+    batch.execute()                       # autotune + capture + first replay
+    assert batch.info() == {"items": len(items), "branches": 4}
+    assert all(sg.tuning(p)["tuned"] for p in plans)
+    refs = []
+    for (p, spec), x in zip(items, sinos):
+        r = sg.recon._empty_colmajor(spec[1].dims, torch_cuda.device("cuda:0"))
+        p.execute(x, out=r)
+        refs.append(r.clone())
+    for rep in range(3):
+        if rep:
+            for y in outs:                # eager work between replays
+                y.zero_()
+            plans[0].execute(sinos[0])
+            torch_cuda.cuda.synchronize()
+            batch.execute()
+        for y, r in zip(outs, refs):
+            e = float((y - r).norm() / r.norm())
+            assert e <= 1e-6, (rep, e)
+    assert batch.last_ms() > 0
+
+
+def test_autotune_choice_and_cache(sg, orc, torch_cuda):
+    _say("autotune")
+    rng = np.random.default_rng(22)
+    rg = sg.CtFanFlat(ns=96, nt=72, ds=1.0, dt=1.0, na=40, dsd=300.0, dod=100.0)
+    ig = sg.ImageGeom((40, 40, 24))
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    x = _colmajor(torch_cuda, proj)
+    dev = torch_cuda.device("cuda:0")
+    p1 = sg.plan_fbp(rg, ig)
+    y1 = sg.recon._empty_colmajor(ig.dims, dev)
+    t = sg.autotune(p1, x, y1)
+    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather")
+    ref = orc.fdk(proj.astype(np.float64), p1.filter,
+                  orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] *
+                  orc.angle_weight(rg.ar(f32=True).astype(np.float32)),
+                  rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False,
+                  *ig.axes(), ig.mask)
+    assert rel_rmse(y1.cpu().numpy(), ref) < 1e-5       # the tuned result
+    # a plan of the same size bucket takes the cached choice
+    p2 = sg.plan_fbp(rg, ig)
+    y2 = sg.recon._empty_colmajor(ig.dims, dev)
+    assert sg.autotune(p2, x, y2) == t
+    assert float((y2 - y1).abs().max()) == 0.0
