@@ -91,6 +91,62 @@ def kernel_times(plan, proj, out, n):
     return f / n, b / n
 
 
+def batch_leg(sg, device, steps, seed=3):
+    """The batching form: 48 independent reconstructions of uneven size drawn
+    from a seed -- 32 fan-beam arc slices (image side in {128, 256, 384, 512},
+    views in {180, 360, 720}; the 888-bin arc detector at dso 541 / dsd 949)
+    and 16 FDK volumes (side in {64, 128, 192}, 120-360 views; the 512 x 384
+    flat panel at dso 800 / dsd 1200) -- run as one autotuned HIP graph, and
+    timed against the same executes issued one after another."""
+    import numpy as np
+    import torch
+    rng = np.random.default_rng(seed)
+    items = []
+    for i in range(32):
+        n = int(rng.choice([128, 256, 384, 512]))
+        na = int(rng.choice([180, 360, 720]))
+        rg = sg.SinoFanArc(nb=888, na=na, d=1.0239, dsd=949.0, dod=949.0 - 541.0)
+        ig = sg.ImageGeom((n, n), (0.98 * 512 / n, 0.98 * 512 / n))
+        ell = sg.phantom.random_ellipses(seed * 1000 + i, 15, 0.98 * 512)
+        x = sg.phantom.sino_ellipses(ell, rg, device=device).float().T.contiguous().T
+        items.append((rg, ig, x))
+    for i in range(16):
+        n = int(rng.choice([64, 128, 192]))
+        na = int(rng.integers(120, 361))
+        rg = sg.CtFanFlat(ns=512, nt=384, ds=1.2, dt=1.2, na=na, dsd=1200.0, dod=400.0)
+        ig = sg.ImageGeom((n, n, n), (256.0 / n,) * 3)
+        ell = sg.phantom.random_ellipsoids(seed * 1000 + 100 + i, 12, 256.0)
+        items.append((rg, ig, sg.phantom.proj_ellipsoids(ell, rg, device=device).float()))
+    updates = sum(int(np.prod(ig.dims)) * rg.na for rg, ig, _ in items)
+    plans = [sg.plan_fbp(rg, ig) for rg, ig, _ in items]
+    outs = [sg.recon._empty_colmajor(ig.dims, device) for _, ig, _ in items]
+    batch = sg.Batch()
+    for p, (_, _, x), y in zip(plans, items, outs):
+        batch.add(p, x, y)
+    batch.execute()                                  # autotune + capture + first replay
+    refs = [sg.recon._empty_colmajor(ig.dims, device) for _, ig, _ in items]
+
+    def serial():
+        for p, (_, _, x), r in zip(plans, items, refs):
+            p.execute(x, out=r)
+    t_serial = time_steps(serial, steps, 1)
+    t_graph = time_steps(batch.execute, steps, 1)
+    err = max(float((y - r).norm() / r.norm()) for y, r in zip(outs, refs))
+    tunes = [sg.tuning(p) for p in plans]
+    return {"workload": "48 reconstructions of uneven size (seed 3): 32 fan-beam arc slices "
+                        "128-512 px x 180-720 views, 16 FDK volumes 64-192 voxels x 120-360 views",
+            "items": len(items), "updates": updates, "branches": batch.info()["branches"],
+            "graph_ms": t_graph * 1e3, "serial_ms": t_serial * 1e3,
+            "speedup_vs_serial": t_serial / t_graph,
+            "value": updates / t_graph / 1e9, "unit": "Gupdates/s",
+            "max_rel_rmse_vs_unbatched": err,
+            "cbct_kernels": sorted({t["cbct_kernel"] for t, (rg, _, _) in zip(tunes, items)
+                                    if isinstance(rg, sg.CtFan)}),
+            "b2d_shapes": sorted({(t["want_blocks"], t["view_chunk"])
+                                  for t, (rg, _, _) in zip(tunes, items)
+                                  if not isinstance(rg, sg.CtFan)})}
+
+
 def cpu_leg(sg, rg, ig, proj, Hk, vols, nslab):
     """Oracle timing on a bounded sample and the accuracy of every GPU form on
     it.  Sample: the nslab central z-slices of the volume (all 256 x 256
@@ -161,6 +217,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the 48-reconstruction batch (one HIP graph) leg")
     ap.add_argument("--cpu-slices", type=int, default=128)
     ap.add_argument("--cbct-kernel", choices=("staged", "gather"), default="staged",
                     help="cone-beam backprojector (LDS-staged windows or global gathers)")
@@ -248,6 +306,9 @@ def main():
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
     }
+    if not args.no_batch:
+        log("batch leg ...")
+        result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))
     if not args.no_cpu:
         log("CPU oracle leg ...")
         cpu, acc = cpu_leg(sg, rg, ig, proj, plans["fp32"].filter, vols, args.cpu_slices)
