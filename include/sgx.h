@@ -277,6 +277,44 @@ int sgx_batch_last_ms(sgx_batch* batch, float* ms);
 /* Number of items and of parallel branches of the captured graph. */
 int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches);
 
+/* ---- learned sparse-view correction (no counterpart in the reference) -----
+ * A 1-D convolutional network applied along detector rows of the filtered
+ * sinogram of a 2-D plan, before backprojection: q' = q + f(q), f = L layers
+ * of width-9 convolutions (1 -> C -> ... -> C -> 1 channels) with GELU
+ * between them; weights kept in FP32, used rounded to BF16, FP32
+ * accumulation.  Callers: training loops and sparse-view reconstruction;
+ * plain FBP (f = 0, the initial state) is the fallback.  Device pointers. */
+
+/* Weights + ramp filter of a 2-D FP32 plan into q (ns, na) (no backprojection). */
+int sgx_plan_filter(sgx_plan* plan, const float* sino, float* q, uint32_t flags);
+/* Backprojection of a given filtered sinogram q (ns, na) with the plan's geometry. */
+int sgx_plan_back(sgx_plan* plan, const float* q, float* image, uint32_t flags);
+/* The exact transpose of sgx_plan_back: q (ns, na) from an image. */
+int sgx_plan_back_t(sgx_plan* plan, const float* image, float* q, uint32_t flags);
+
+typedef struct sgx_net sgx_net;
+/* layers 2..16, channels 1..32; hidden weights uniform(+-sqrt(6/fan_in)) from
+ * `seed`, last layer and biases 0 (the initial correction is zero). */
+int sgx_net_create(sgx_ctx* ctx, int layers, int channels, uint64_t seed, sgx_net** out);
+void sgx_net_destroy(sgx_net* net);
+int64_t sgx_net_num_params(const sgx_net* net);
+/* Copy the FP32 parameters to (set = 0) or from (set = 1) host memory; per
+ * layer the weights (cout, cin, 9) row-major, then the biases (cout). */
+int sgx_net_params(sgx_net* net, float* host, int set);
+/* The accumulated parameter gradients (host), same layout. */
+int sgx_net_grads(sgx_net* net, float* host);
+int sgx_net_zero_grad(sgx_net* net);
+/* out (rows, n) = q + f(q) for q (rows, n); keep = 1 keeps the activations
+ * (and the q pointer) for sgx_net_backward. */
+int sgx_net_forward(sgx_net* net, const float* q, int64_t rows, int64_t n, float* out, int keep);
+/* Add dL/dparams to the gradients, given gout = dL/dout of the kept forward. */
+int sgx_net_backward(sgx_net* net, const float* gout);
+/* One Adam step on the FP32 parameters (bias-corrected, step counted inside). */
+int sgx_net_adam(sgx_net* net, float lr, float beta1, float beta2, float eps);
+/* grad = scale * 2 (x - t) (grad may be NULL); *sum_sq = sum (x - t)^2. */
+int sgx_mse(sgx_ctx* ctx, const float* x, const float* t, int64_t n, float scale, float* grad,
+            double* sum_sq);
+
 #ifdef __cplusplus
 }
 #endif

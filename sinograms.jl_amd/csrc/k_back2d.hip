@@ -29,6 +29,29 @@ constexpr int kTile = 16;
 constexpr int kThreads = kTile * kTile;
 constexpr int kLdsBytes = 48 * 1024;
 
+// Detector position bb (1-based bins) and weight w2 of pixel (x, y) (/ ds) in
+// one view; shared by the backprojector and its transpose.
+template <int GEOM>
+__device__ __forceinline__ void pixel_bin(float x, float y, ViewCS t, float wb, float dsd_ds,
+                                          float dso_ds, float so_ds, float& bb, float& w2)
+{
+    w2 = 1.f;
+    if (GEOM == 0) {
+        bb = x * t.c + y * t.s + wb;
+    } else {
+        const float d = dso_ds + x * t.s - y * t.c;
+        const float r = x * t.c + y * t.s - so_ds;
+        if (GEOM == 1) {
+            bb = dsd_ds * atan2f(r, d) + wb;
+            w2 = dsd_ds * dsd_ds / (d * d + r * r);
+        } else {
+            const float m = dsd_ds / d;
+            bb = m * r + wb;
+            w2 = m * m;
+        }
+    }
+}
+
 int chunk_views(int nb, int vc_max)
 {
     return std::max(1, std::min(vc_max, kLdsBytes / (4 * nb)));
@@ -61,7 +84,6 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     const float x = inside ? xds[ix] : 0.f;
     const float y = inside ? yds[iy] : 0.f;
     const float nb1 = (float)nb;               // valid: 1 <= bb < nb
-    const float dsd2 = dsd_ds * dsd_ds;
     float acc = 0.f;
 
     for (int c0 = v_begin; c0 < v_end; c0 += vc) {
@@ -74,22 +96,8 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
         __syncthreads();
         if (active) {
             for (int v = 0; v < nv; ++v) {
-                const ViewCS t = cs[c0 + v];
-                float bb, w2 = 1.f;
-                if (GEOM == 0) {
-                    bb = x * t.c + y * t.s + wb;
-                } else {
-                    const float d = dso_ds + x * t.s - y * t.c;
-                    const float r = x * t.c + y * t.s - so_ds;
-                    if (GEOM == 1) {
-                        bb = dsd_ds * atan2f(r, d) + wb;
-                        w2 = dsd2 / (d * d + r * r);
-                    } else {
-                        const float m = dsd_ds / d;
-                        bb = m * r + wb;
-                        w2 = m * m;
-                    }
-                }
+                float bb, w2;
+                pixel_bin<GEOM>(x, y, cs[c0 + v], wb, dsd_ds, dso_ds, so_ds, bb, w2);
                 // 1 <= floor(bb) <= nb - 1; out of range: clamped address, weight 0
                 const bool ok = bb >= 1.f && bb < nb1;
                 const float w = ok ? w2 : 0.f;
@@ -103,6 +111,55 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
         }
     }
     if (inside) dst[pix] = active ? acc : 0.f;
+}
+
+// Exact transpose of back2d_kernel:
+//   qt[b, v] = sum over active pixels p of img[p] w2 ((1 - wr) [b = il - 1] + wr [b = il])
+// Block (view v, pixel group g): bins accumulate in LDS (atomics), then are
+// added into qt[:, v] (global atomics when a view has several groups; qt is
+// zeroed first).  Float atomics: the sum order varies from run to run.
+template <int GEOM>
+__global__ __launch_bounds__(kThreads) void back2d_t_kernel(
+    const float* __restrict__ img, int nb, int nviews, const ViewCS* __restrict__ cs,
+    const float* __restrict__ xds, const float* __restrict__ yds, int nx, int ny,
+    const uint8_t* __restrict__ mask, float* __restrict__ qt, float wb, float dsd_ds,
+    float dso_ds, float so_ds)
+{
+    extern __shared__ float bins[];
+    const int v = blockIdx.x;
+    const int64_t npix = (int64_t)nx * ny;
+    for (int b = threadIdx.x; b < nb; b += kThreads) bins[b] = 0.f;
+    __syncthreads();
+    const ViewCS t = cs[v];
+    const float nb1 = (float)nb;
+    for (int64_t p = blockIdx.y * (int64_t)kThreads + threadIdx.x; p < npix;
+         p += (int64_t)gridDim.y * kThreads) {
+        if (mask != nullptr && !mask[p]) continue;
+        const float g = img[p];
+        if (g == 0.f) continue;
+        const int ix = (int)(p % nx), iy = (int)(p / nx);
+        float bb, w2;
+        pixel_bin<GEOM>(xds[ix], yds[iy], t, wb, dsd_ds, dso_ds, so_ds, bb, w2);
+        if (!(bb >= 1.f && bb < nb1)) continue;        // 1 <= floor(bb) <= nb - 1
+        const float ilf = floorf(bb);
+        const float wr = bb - ilf;
+        const int il = (int)ilf;
+        const float gw = g * w2;
+        atomicAdd(&bins[il - 1], gw * (1.f - wr));
+        atomicAdd(&bins[il], gw * wr);
+    }
+    __syncthreads();
+    float* dst = qt + (int64_t)nb * v;
+    for (int b = threadIdx.x; b < nb; b += kThreads) {
+        if (gridDim.y == 1) dst[b] = bins[b];
+        else if (bins[b] != 0.f) atomicAdd(&dst[b], bins[b]);
+    }
+}
+
+__global__ void zero_f32_kernel(float* p, int64_t n)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
 }
 
 __global__ void sum_partials_kernel(const float* __restrict__ part, int groups, int64_t n,
@@ -172,6 +229,39 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
                            0, st, a.partial, groups, npix, a.mask, a.image);
         SGX_HIP(hipGetLastError());
     }
+}
+
+void launch_back2d_t(const Back2dArgs& a, const float* img, float* qt, hipStream_t st)
+{
+    if ((int64_t)a.nb * 4 > kLdsBytes)
+        throw Error(SGX_ERR_UNSUPPORTED, "back2d_t: nb > 12288 bins is not supported");
+    const int64_t npix = (int64_t)a.nx * a.ny;
+    // pixel groups per view so that views x groups fill the GPU
+    const int groups = (int)std::max<int64_t>(1, std::min<int64_t>((2048 + a.nviews - 1) / a.nviews,
+                                                                   (npix + 4095) / 4096));
+    if (groups > 1) {
+        const int64_t n = (int64_t)a.nb * a.nviews;
+        hipLaunchKernelGGL(zero_f32_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           qt, n);
+        SGX_HIP(hipGetLastError());
+    }
+    dim3 grid((unsigned)a.nviews, (unsigned)groups), block(kThreads);
+    const size_t lds = (size_t)a.nb * sizeof(float);
+    switch (a.geom) {
+    case 0:
+        hipLaunchKernelGGL(back2d_t_kernel<0>, grid, block, lds, st, img, a.nb, a.nviews, a.cs,
+                           a.xds, a.yds, a.nx, a.ny, a.mask, qt, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    case 1:
+        hipLaunchKernelGGL(back2d_t_kernel<1>, grid, block, lds, st, img, a.nb, a.nviews, a.cs,
+                           a.xds, a.yds, a.nx, a.ny, a.mask, qt, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    default:
+        hipLaunchKernelGGL(back2d_t_kernel<2>, grid, block, lds, st, img, a.nb, a.nviews, a.cs,
+                           a.xds, a.yds, a.nx, a.ny, a.mask, qt, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        break;
+    }
+    SGX_HIP(hipGetLastError());
 }
 
 }  // namespace sgx

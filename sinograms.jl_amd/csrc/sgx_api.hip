@@ -1219,3 +1219,280 @@ int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// K8: plan pieces for training (filter only, backprojection of a given q and
+// its exact transpose) and the learned correction network
+// ---------------------------------------------------------------------------
+
+namespace {
+
+sgx::Back2dArgs back2d_args_of(const sgx_plan& P, const float* q, float* img)
+{
+    const sgx_geom& G = P.geom;
+    sgx::Back2dArgs a{};
+    a.q = q;
+    a.nb = (int)G.ns;
+    a.nviews = (int)G.na;
+    a.view_stride = G.ns;
+    a.cs = P.cs;
+    a.xds = P.xds;
+    a.yds = P.yds;
+    a.nx = (int)P.nx;
+    a.ny = (int)P.ny;
+    a.mask = P.mask;
+    a.image = img;
+    a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+    a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+    a.dsd_ds = (float)G.dsd / (float)G.ds;
+    a.dso_ds = (float)G.dso / (float)G.ds;
+    a.so_ds = (float)G.source_offset / (float)G.ds;
+    a.partial = P.partial;
+    a.partial_cap = P.partial_cap;
+    a.want_blocks = P.tune.want_blocks;
+    a.vc_max = P.tune.vc_max;
+    return a;
+}
+
+}  // namespace
+
+struct sgx_net {
+    sgx_ctx* ctx = nullptr;
+    int layers = 0, channels = 0;
+    std::vector<int> cin, cout;
+    std::vector<int64_t> woff, boff;       // offsets of each layer's weights / biases
+    int64_t nparams = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    int step = 0;
+    // training state of the last kept forward
+    const float* q = nullptr;
+    int64_t rows = 0, n = 0;
+    float* z = nullptr;                    // pre-activations of layers 1 .. L-1
+    int64_t z_cap = 0;
+    float* d = nullptr;                    // two gradient buffers
+    int64_t d_cap = 0;
+    float* part = nullptr;
+    int64_t part_cap = 0;
+    bool kept = false;
+
+    ~sgx_net()
+    {
+        for (float* p : {w, g, m, v, z, d, part})
+            if (p) (void)hipFree(p);
+    }
+    void grow(float*& p, int64_t& cap, int64_t need)
+    {
+        if (cap >= need) return;
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        if (p) SGX_HIP(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        SGX_HIP(hipMalloc(&p, sizeof(float) * need));
+        cap = need;
+    }
+};
+
+extern "C" {
+
+int sgx_plan_filter(sgx_plan* plan, const float* sino, float* q, uint32_t flags)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(sino && q, SGX_ERR_ARG, "null pointer argument");
+        require(!(flags & SGX_HOST), SGX_ERR_ARG, "device pointers only");
+        require(!plan->is3d() && plan->prec == SGX_PREC_FP32, SGX_ERR_UNSUPPORTED,
+                "sgx_plan_filter is provided for 2-D FP32 plans");
+        sgx::launch_sino_filter(plan->filter_args(sino, q), ctx->stream);
+        finish(ctx, nullptr, nullptr, 0, flags);
+    });
+}
+
+int sgx_plan_back(sgx_plan* plan, const float* q, float* image, uint32_t flags)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(q && image, SGX_ERR_ARG, "null pointer argument");
+        require(!(flags & SGX_HOST), SGX_ERR_ARG, "device pointers only");
+        require(!plan->is3d(), SGX_ERR_UNSUPPORTED, "sgx_plan_back is provided for 2-D plans");
+        sgx::launch_back2d(back2d_args_of(*plan, q, image), ctx->stream);
+        finish(ctx, nullptr, nullptr, 0, flags);
+    });
+}
+
+int sgx_plan_back_t(sgx_plan* plan, const float* image, float* q, uint32_t flags)
+{
+    if (!plan) return SGX_ERR_ARG;
+    sgx_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        require(q && image, SGX_ERR_ARG, "null pointer argument");
+        require(!(flags & SGX_HOST), SGX_ERR_ARG, "device pointers only");
+        require(!plan->is3d(), SGX_ERR_UNSUPPORTED, "sgx_plan_back_t is provided for 2-D plans");
+        sgx::launch_back2d_t(back2d_args_of(*plan, nullptr, nullptr), image, q, ctx->stream);
+        finish(ctx, nullptr, nullptr, 0, flags);
+    });
+}
+
+int sgx_net_create(sgx_ctx* ctx, int layers, int channels, uint64_t seed, sgx_net** out)
+{
+    return guard(ctx, [&] {
+        require(out, SGX_ERR_ARG, "null pointer argument");
+        require(layers >= 2 && layers <= 16 && channels >= 1 && channels <= 32, SGX_ERR_ARG,
+                "layers 2..16, channels 1..32");
+        *out = nullptr;
+        auto N = std::make_unique<sgx_net>();
+        N->ctx = ctx;
+        N->layers = layers;
+        N->channels = channels;
+        for (int l = 0; l < layers; ++l) {
+            N->cin.push_back(l == 0 ? 1 : channels);
+            N->cout.push_back(l == layers - 1 ? 1 : channels);
+            N->woff.push_back(N->nparams);
+            N->nparams += (int64_t)N->cout[l] * N->cin[l] * 9;
+            N->boff.push_back(N->nparams);
+            N->nparams += N->cout[l];
+        }
+        // uniform(-a, a) with a = sqrt(6 / fan_in) (GELU layers), the last
+        // layer and every bias 0: the initial correction is exactly zero
+        std::vector<float> h(N->nparams, 0.f);
+        uint64_t st = seed * 6364136223846793005ull + 1442695040888963407ull;
+        auto uni = [&]() {
+            st = st * 6364136223846793005ull + 1442695040888963407ull;
+            return (double)(st >> 11) * (1.0 / 9007199254740992.0);
+        };
+        for (int l = 0; l + 1 < layers; ++l) {
+            const double a = std::sqrt(6.0 / (N->cin[l] * 9.0));
+            for (int64_t e = 0; e < (int64_t)N->cout[l] * N->cin[l] * 9; ++e)
+                h[N->woff[l] + e] = (float)((2.0 * uni() - 1.0) * a);
+        }
+        for (float** p : {&N->w, &N->g, &N->m, &N->v}) SGX_HIP(hipMalloc(p, 4 * N->nparams));
+        SGX_HIP(hipMemcpy(N->w, h.data(), 4 * N->nparams, hipMemcpyHostToDevice));
+        for (float* p : {N->g, N->m, N->v}) sgx::launch_fill(p, N->nparams, 0.f, ctx->stream);
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        *out = N.release();
+    });
+}
+
+void sgx_net_destroy(sgx_net* net)
+{
+    if (!net) return;
+    (void)hipSetDevice(net->ctx->device);
+    (void)hipStreamSynchronize(net->ctx->stream);
+    delete net;
+}
+
+int64_t sgx_net_num_params(const sgx_net* net) { return net ? net->nparams : 0; }
+
+int sgx_net_params(sgx_net* net, float* host, int set)
+{
+    if (!net) return SGX_ERR_ARG;
+    return guard(net->ctx, [&] {
+        require(host, SGX_ERR_ARG, "null pointer argument");
+        SGX_HIP(hipStreamSynchronize(net->ctx->stream));
+        if (set) SGX_HIP(hipMemcpy(net->w, host, 4 * net->nparams, hipMemcpyHostToDevice));
+        else SGX_HIP(hipMemcpy(host, net->w, 4 * net->nparams, hipMemcpyDeviceToHost));
+    });
+}
+
+int sgx_net_grads(sgx_net* net, float* host)
+{
+    if (!net) return SGX_ERR_ARG;
+    return guard(net->ctx, [&] {
+        require(host, SGX_ERR_ARG, "null pointer argument");
+        SGX_HIP(hipStreamSynchronize(net->ctx->stream));
+        SGX_HIP(hipMemcpy(host, net->g, 4 * net->nparams, hipMemcpyDeviceToHost));
+    });
+}
+
+int sgx_net_zero_grad(sgx_net* net)
+{
+    if (!net) return SGX_ERR_ARG;
+    return guard(net->ctx, [&] { sgx::launch_fill(net->g, net->nparams, 0.f, net->ctx->stream); });
+}
+
+int sgx_net_forward(sgx_net* net, const float* q, int64_t rows, int64_t n, float* out, int keep)
+{
+    if (!net) return SGX_ERR_ARG;
+    sgx_ctx* ctx = net->ctx;
+    return guard(ctx, [&] {
+        require(q && out, SGX_ERR_ARG, "null pointer argument");
+        require(rows > 0 && n > 0 && rows < 65536, SGX_ERR_ARG, "rows must be 1..65535");
+        sgx_net& N = *net;
+        const int L = N.layers;
+        const int64_t per = rows * N.channels * n;          // one hidden activation
+        // hidden pre-activations: all L-1 kept for training, else two in turn
+        N.grow(N.z, N.z_cap, per * (keep ? (L - 1) : std::min(L - 1, 2)));
+        const float* in = q;
+        for (int l = 0; l < L; ++l) {
+            const bool last = l == L - 1;
+            float* dst = last ? out : N.z + per * (keep ? l : (l % 2));
+            sgx::launch_conv(in, (int)rows, N.cin[l], N.cout[l], (int)n, N.w + N.woff[l],
+                             N.w + N.boff[l], last ? q : nullptr, l > 0, dst, ctx->stream);
+            in = dst;
+        }
+        N.q = q;
+        N.rows = rows;
+        N.n = n;
+        N.kept = keep != 0;
+    });
+}
+
+int sgx_net_backward(sgx_net* net, const float* gout)
+{
+    if (!net) return SGX_ERR_ARG;
+    sgx_ctx* ctx = net->ctx;
+    return guard(ctx, [&] {
+        require(gout, SGX_ERR_ARG, "null pointer argument");
+        sgx_net& N = *net;
+        require(N.kept, SGX_ERR_ARG, "sgx_net_backward needs a kept forward");
+        const int L = N.layers;
+        const int64_t per = N.rows * N.channels * N.n;
+        N.grow(N.d, N.d_cap, 2 * per);
+        const int blocks = sgx::nn_wgrad_blocks((int)N.rows, (int)N.n);
+        N.grow(N.part, N.part_cap, (int64_t)blocks * (N.channels * N.channels * 9 + N.channels));
+        const float* delta = gout;               // dL / dz_L (the output is q + z_L)
+        for (int l = L - 1; l >= 0; --l) {
+            const float* x = l == 0 ? N.q : N.z + per * (l - 1);
+            sgx::launch_conv_wgrad(delta, x, l > 0, (int)N.rows, N.cin[l], N.cout[l], (int)N.n,
+                                   N.part, N.g + N.woff[l], N.g + N.boff[l], ctx->stream);
+            if (l == 0) break;
+            float* din = N.d + per * (l % 2);
+            sgx::launch_conv_dgrad(delta, (int)N.rows, N.cin[l], N.cout[l], (int)N.n,
+                                   N.w + N.woff[l], N.z + per * (l - 1), din, ctx->stream);
+            delta = din;
+        }
+    });
+}
+
+int sgx_net_adam(sgx_net* net, float lr, float beta1, float beta2, float eps)
+{
+    if (!net) return SGX_ERR_ARG;
+    return guard(net->ctx, [&] {
+        net->step += 1;
+        sgx::launch_adam(net->w, net->g, net->m, net->v, net->nparams, lr, beta1, beta2, eps,
+                         net->step, net->ctx->stream);
+    });
+}
+
+int sgx_mse(sgx_ctx* ctx, const float* x, const float* t, int64_t n, float scale, float* grad,
+            double* sum_sq)
+{
+    return guard(ctx, [&] {
+        require(x && t && n > 0, SGX_ERR_ARG, "bad argument");
+        const int blocks = sgx::nn_mse_blocks(n);
+        float* part = static_cast<float*>(ctx->get(W_EXTRA, 4 * blocks));
+        sgx::launch_mse(x, t, n, scale, grad, part, ctx->stream);
+        if (sum_sq) {
+            std::vector<float> h(blocks);
+            SGX_HIP(hipMemcpyAsync(h.data(), part, 4 * blocks, hipMemcpyDeviceToHost,
+                                   ctx->stream));
+            SGX_HIP(hipStreamSynchronize(ctx->stream));
+            double s = 0;
+            for (float v : h) s += v;
+            *sum_sq = s;
+        }
+    });
+}
+
+}  // extern "C"

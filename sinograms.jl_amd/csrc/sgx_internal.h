@@ -95,6 +95,26 @@ struct Back2dArgs {
 };
 void launch_back2d(const Back2dArgs& a, hipStream_t st);
 int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_blocks, int vc_max);
+// Exact transpose of launch_back2d (view_stride = nb): qt (nb, nviews) from an
+// image; uses the geometry fields of `a` (q, image and partial are unused).
+void launch_back2d_t(const Back2dArgs& a, const float* img, float* qt, hipStream_t st);
+
+// K8: the learned correction's kernels (csrc/k_nn.hip).  Activations are
+// (rows, channels, n); weights (cout, cin, 9) FP32, rounded to BF16 in use.
+void launch_conv(const float* in, int rows, int cin, int cout, int n, const float* w,
+                 const float* bias, const float* resid, bool gelu_in, float* out,
+                 hipStream_t st);
+void launch_conv_dgrad(const float* dout, int rows, int cin, int cout, int n, const float* w,
+                       const float* zprev, float* din, hipStream_t st);
+int nn_wgrad_blocks(int rows, int n);
+void launch_conv_wgrad(const float* dout, const float* x, bool gelu_in, int rows, int cin,
+                       int cout, int n, float* part, float* gw, float* gb, hipStream_t st);
+void launch_adam(float* w, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                 float b2, float eps, int step, hipStream_t st);
+void launch_fill(float* p, int64_t n, float v, hipStream_t st);
+int nn_mse_blocks(int64_t n);
+void launch_mse(const float* x, const float* t, int64_t n, float scale, float* grad, float* part,
+                hipStream_t st);
 
 // K4 (+K5): Toeplitz row filter with fused weights on MFMA.
 struct FilterArgs {

@@ -45,7 +45,10 @@ EXPORTS = (
     "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
     "sgx_ctx_set_option", "sgx_ctx_get_option", "sgx_plan_autotune", "sgx_plan_tuning",
     "sgx_batch_create", "sgx_batch_destroy", "sgx_batch_add", "sgx_batch_execute",
-    "sgx_batch_last_ms", "sgx_batch_info",
+    "sgx_batch_last_ms", "sgx_batch_info", "sgx_plan_filter", "sgx_plan_back",
+    "sgx_plan_back_t", "sgx_net_create", "sgx_net_destroy", "sgx_net_num_params",
+    "sgx_net_params", "sgx_net_grads", "sgx_net_zero_grad", "sgx_net_forward",
+    "sgx_net_backward", "sgx_net_adam", "sgx_mse",
     "sgx_project_bdd", "sgx_backproject_bdd",
 )
 
@@ -141,6 +144,22 @@ def _declare(L):
     L.sgx_batch_last_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sgx_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_int64)]
+    L.sgx_plan_filter.argtypes = [vp, vp, vp, u32]
+    L.sgx_plan_back.argtypes = [vp, vp, vp, u32]
+    L.sgx_plan_back_t.argtypes = [vp, vp, vp, u32]
+    L.sgx_net_create.argtypes = [vp, i32, i32, ctypes.c_uint64, ctypes.POINTER(vp)]
+    L.sgx_net_destroy.argtypes = [vp]
+    L.sgx_net_destroy.restype = None
+    L.sgx_net_num_params.argtypes = [vp]
+    L.sgx_net_num_params.restype = ctypes.c_int64
+    L.sgx_net_params.argtypes = [vp, vp, i32]
+    L.sgx_net_grads.argtypes = [vp, vp]
+    L.sgx_net_zero_grad.argtypes = [vp]
+    L.sgx_net_forward.argtypes = [vp, vp, i64, i64, vp, i32]
+    L.sgx_net_backward.argtypes = [vp, vp]
+    L.sgx_net_adam.argtypes = [vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                               ctypes.c_float]
+    L.sgx_mse.argtypes = [vp, vp, vp, i64, ctypes.c_float, vp, ctypes.POINTER(ctypes.c_double)]
 
 
 SGX_OPT_CBCT_KERNEL = 1

@@ -408,3 +408,38 @@ def test_cbct_back_pairs_matches_plain(orc):
         got = orc.cbct_back_pairs(q, d, *args, arc, xc, yc, zc, mask)
         assert np.abs(ref).max() > 0
         assert np.allclose(got, ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+
+
+def test_nn_oracle_backward_matches_finite_differences():
+    # The float64 network restatement's hand-written backward pass against
+    # central differences of sum(gout * out) (weights unrounded, so the loss
+    # is smooth in them).
+    from oracle import nn as onn
+    rng = np.random.default_rng(31)
+    layers, ch, R, N = 3, 3, 2, 13
+    sizes = [ch * 1 * 9 + ch, ch * ch * 9 + ch, 1 * ch * 9 + 1]
+    flat = rng.standard_normal(sum(sizes)) * 0.4
+    q = rng.standard_normal((R, N))
+    gout = rng.standard_normal((R, N))
+
+    def loss(f):
+        out, _ = onn.forward(onn.split_params(f, layers, ch), q, round_bf16=False)
+        return float((gout * out).sum())
+
+    params = onn.split_params(flat, layers, ch)
+    _, cache = onn.forward(params, q, round_bf16=False)
+    g = onn.flatten(onn.backward(params, cache, gout, round_bf16=False))
+    h = 1e-6
+    fd = np.array([(loss(flat + h * e) - loss(flat - h * e)) / (2 * h)
+                   for e in np.eye(len(flat))])
+    assert np.abs(g - fd).max() <= 1e-6 * max(1.0, np.abs(fd).max())
+    # the residual form: with a zero last layer the output is q itself
+    f0 = flat.copy()
+    f0[sizes[0] + sizes[1]:] = 0
+    out, _ = onn.forward(onn.split_params(f0, layers, ch), q)
+    assert np.array_equal(out, q)
+    # the exact-erf GELU and its derivative
+    x = np.linspace(-4, 4, 81)
+    assert np.allclose(onn.gelu(x), x * 0.5 * (1 + np.vectorize(math.erf)(x / math.sqrt(2))))
+    assert np.allclose(onn.gelu_grad(x), (onn.gelu(x + 1e-6) - onn.gelu(x - 1e-6)) / 2e-6,
+                       atol=1e-6)
