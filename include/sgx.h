@@ -64,8 +64,9 @@ enum {
 enum {
     SGX_PREC_FP32 = 0,        /* FP32 filter (FP32 MFMA), FP32 filtered storage  */
     SGX_PREC_BF16 = 1,        /* split-BF16x3 MFMA filter, BF16 filtered storage  */
-    SGX_PREC_F16S = 2         /* FP32 MFMA filter, FP16 filtered storage with a
-                                 per-view scale (view max -> 2^15)                */
+    SGX_PREC_F16S = 2         /* split-FP16x3 MFMA filter (power-of-two scaled
+                                 operands), FP16 filtered storage scaled per view
+                                 by a power of two (cone-beam plans)             */
 };
 
 /* ---- context ------------------------------------------------------------ */

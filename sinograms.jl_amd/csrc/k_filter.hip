@@ -63,6 +63,7 @@ struct Params {
     double gamma_max;
     const unsigned int* vbound;   // F16S: per-view max |weighted input| (float bits)
     float g1;                     // F16S: sum |g|
+    float gscale;                 // split-FP16: power of two applied to g
     float* vscale;                // F16S: per-view output scale (written)
     int row_tiles_per_block;
 };
@@ -134,15 +135,54 @@ __device__ __forceinline__ float bf16_to_f(__bf16 h)
     return __uint_as_float((uint32_t)__builtin_bit_cast(uint16_t, h) << 16);
 }
 
-template <bool SPLIT, int OUT, int RT>
+// Products: FP32 MFMA, split-BF16x3 or split-FP16x3 (x = hi + lo per operand,
+// hi*hi + hi*lo + lo*hi accumulated in FP32).  FP16 operands are scaled by
+// powers of two first (g by p.gscale, each view's input by 2^14 over the
+// power of two above its bound) so that both parts stay in FP16's normal
+// range; the epilogue divides the scales out exactly.
+enum { M_F32 = 0, M_BF16X3 = 1, M_F16X3 = 2 };
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+constexpr int kBandMax = 4 * 8 * 32 + 2048;       // longest band of g (RT = 8, nb <= 2048)
+
+__device__ __forceinline__ float in_scale(const unsigned int* vbound, int view)
+{
+    const float b = __uint_as_float(vbound[view]);
+    if (!(b > 0.f)) return 1.f;
+    int e;
+    (void)frexpf(b, &e);                 // b <= 2^e
+    return ldexpf(1.f, 14 - e);
+}
+
+template <int MODE> __device__ __forceinline__ void split2(float x, uint16_t& hi, uint16_t& lo)
+{
+    if (MODE == M_BF16X3) {
+        const __bf16 h = bf16_trunc(x);
+        hi = __builtin_bit_cast(uint16_t, h);
+        lo = __builtin_bit_cast(uint16_t, bf16_rne(x - bf16_to_f(h)));
+    } else {
+        const _Float16 h = (_Float16)x;
+        hi = __builtin_bit_cast(uint16_t, h);
+        lo = __builtin_bit_cast(uint16_t, (_Float16)(x - (float)h));
+    }
+}
+
+template <int MODE, int OUT, int RT>
 __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 {
-    __shared__ float gl[4 * 8 * 32 + 2048 + 64];
+    constexpr bool SPLIT = MODE != M_F32;
+    // FP32: the band of g as floats.  Split modes: hi / lo parts of the band,
+    // reversed (R[i] = band[glen-1-i], so a fragment's 8 taps ascend), each
+    // in two copies shifted by one element so every fragment starts on a
+    // dword: band16[2*part + copy][i] = R[i + copy].
+    __shared__ __attribute__((aligned(16))) unsigned char gbuf[SPLIT ? 4 * kBandMax * 2
+                                                                     : (kBandMax + 64) * 4];
     __shared__ __attribute__((aligned(16))) unsigned char bstage[SPLIT ? 2 * 32 * kBPadH * 2
                                                                         : 32 * kBPadF * 4];
+    float* gl = reinterpret_cast<float*>(gbuf);
+    uint16_t* band16 = reinterpret_cast<uint16_t*>(gbuf);
     float* bs_f = reinterpret_cast<float*>(bstage);
-    __bf16* bs_hi = reinterpret_cast<__bf16*>(bstage);
-    __bf16* bs_lo = bs_hi + 32 * kBPadH;
+    uint16_t* bs_hi = reinterpret_cast<uint16_t*>(bstage);
+    uint16_t* bs_lo = bs_hi + 32 * kBPadH;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * 32;
@@ -151,11 +191,23 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     const int kp = ((p.nb + kKC - 1) / kKC) * kKC;       // padded K
     const int glen = 4 * RT * 32 + kp - 1;
 
-    // g band: gl[j] = g[(n_lo - (kp - 1) + j) mod npad];  T[n, m] = gl[n - n_lo - m + kp - 1]
+    // band: band[j] = g[(n_lo - (kp - 1) + j) mod npad];  T[n, m] = band[n - n_lo - m + kp - 1]
     for (int j = tid; j < glen; j += 256) {
         int k = (n_lo - (kp - 1) + j) % p.npad;
         if (k < 0) k += p.npad;
-        gl[j] = p.g[k];
+        if (SPLIT) {
+            uint16_t hi, lo;
+            split2<MODE>(MODE == M_F16X3 ? p.g[k] * p.gscale : p.g[k], hi, lo);
+            const int i = glen - 1 - j;
+            band16[0 * kBandMax + i] = hi;
+            band16[2 * kBandMax + i] = lo;
+            if (i >= 1) {
+                band16[1 * kBandMax + i - 1] = hi;
+                band16[3 * kBandMax + i - 1] = lo;
+            }
+        } else {
+            gl[j] = p.g[k];
+        }
     }
 
     floatx16 acc[RT];
@@ -178,6 +230,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     }
     float colw = p.scale;
     if (cvalid && p.wview) colw *= p.wview[sview];
+    if (MODE == M_F16X3 && cvalid) colw *= in_scale(p.vbound, sview);
     float cyl_c = 0.f, cyl_k = 0.f;
     if (cvalid && p.cyl == 1) {            // dso / sqrt(s^2 + t^2 + dsd^2)
         const float tt = p.t_axis[st];
@@ -211,6 +264,13 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         return w;
     };
 
+    // Split modes: per lane, the start of its A fragments in the reversed
+    // band and the copy that puts it on a dword (its parity is the same for
+    // every row tile and K step).
+    const int par = (glen - col - kp) & 1;
+    const uint16_t* ahi = band16 + (0 + par) * kBandMax;
+    const uint16_t* alo = band16 + (2 + par) * kBandMax;
+
     float vnext[8];
     load8(0, vnext);
     for (int m0 = 0; m0 < p.nb; m0 += kKC) {
@@ -227,9 +287,10 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         if (SPLIT) {
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
-                const __bf16 h = bf16_trunc(v[e]);
-                bs_hi[scol * kBPadH + skk + e] = h;
-                bs_lo[scol * kBPadH + skk + e] = bf16_rne(v[e] - bf16_to_f(h));
+                uint16_t hi, lo;
+                split2<MODE>(v[e], hi, lo);
+                bs_hi[scol * kBPadH + skk + e] = hi;
+                bs_lo[scol * kBPadH + skk + e] = lo;
             }
         } else {
 #pragma unroll
@@ -241,23 +302,34 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 #pragma unroll
             for (int ks = 0; ks < kKC / 16; ++ks) {
                 const int kb = ks * 16 + half * 8;
-                const bf16x8 bh = *reinterpret_cast<const bf16x8*>(bs_hi + col * kBPadH + kb);
-                const bf16x8 bl = *reinterpret_cast<const bf16x8*>(bs_lo + col * kBPadH + kb);
+                const uint4 bh4 = *reinterpret_cast<const uint4*>(bs_hi + col * kBPadH + kb);
+                const uint4 bl4 = *reinterpret_cast<const uint4*>(bs_lo + col * kBPadH + kb);
 #pragma unroll
                 for (int r = 0; r < RT; ++r) {
                     const int n = (tile0 + wave * RT + r) * 32 + col;
-                    const float* ga = gl + (n - n_lo) - (m0 + kb) + kp - 1;
-                    bf16x8 ah, al;
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) {
-                        const float x = ga[-j];
-                        const __bf16 h = bf16_trunc(x);
-                        ah[j] = h;
-                        al[j] = bf16_rne(x - bf16_to_f(h));
+                    // A[n][m0 + kb + j] = R[s + j], s = glen - 1 - (n - n_lo - m0 - kb + kp - 1)
+                    const int s = glen - (n - n_lo) + m0 + kb - kp;
+                    const uint32_t* a32 = reinterpret_cast<const uint32_t*>(ahi + (s - par));
+                    const uint32_t* l32 = reinterpret_cast<const uint32_t*>(alo + (s - par));
+                    const uint4 ah4 = make_uint4(a32[0], a32[1], a32[2], a32[3]);
+                    const uint4 al4 = make_uint4(l32[0], l32[1], l32[2], l32[3]);
+                    if (MODE == M_BF16X3) {
+                        const bf16x8 ah = __builtin_bit_cast(bf16x8, ah4);
+                        const bf16x8 al = __builtin_bit_cast(bf16x8, al4);
+                        const bf16x8 bh = __builtin_bit_cast(bf16x8, bh4);
+                        const bf16x8 bl = __builtin_bit_cast(bf16x8, bl4);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[r], 0, 0, 0);
+                    } else {
+                        const halfx8 ah = __builtin_bit_cast(halfx8, ah4);
+                        const halfx8 al = __builtin_bit_cast(halfx8, al4);
+                        const halfx8 bh = __builtin_bit_cast(halfx8, bh4);
+                        const halfx8 bl = __builtin_bit_cast(halfx8, bl4);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[r], 0, 0, 0);
                     }
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[r], 0, 0, 0);
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[r], 0, 0, 0);
-                    acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[r], 0, 0, 0);
                 }
             }
         } else {
@@ -278,11 +350,13 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
     const int64_t c = c0 + col;
     if (c < p.ncols) {
+        // every factor below is a power of two: the rescaling is exact
         float inv = 1.f;
+        const int view = (int)(c / p.nt);
+        if (MODE == M_F16X3) inv = 1.f / (p.gscale * in_scale(p.vbound, view));
         if (OUT == OUT_F16S) {
-            const int view = (int)(c / p.nt);
             const float sc = f16s_scale(p.vbound[view], p.g1);
-            inv = 1.f / sc;                                   // exact: a power of two
+            inv /= sc;
             if (c % p.nt == 0 && wave == 0 && half == 0 && blockIdx.y == 0)
                 p.vscale[view] = sc;
         }
@@ -323,17 +397,17 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     }
 }
 
-template <bool SPLIT, int OUT, int RT>
+template <int MODE, int OUT, int RT>
 void launch_rt(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
     const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
     dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
-    hipLaunchKernelGGL((toeplitz_kernel<SPLIT, OUT, RT>), grid, dim3(256), 0, st, p);
+    hipLaunchKernelGGL((toeplitz_kernel<MODE, OUT, RT>), grid, dim3(256), 0, st, p);
     SGX_HIP(hipGetLastError());
 }
 
-template <bool SPLIT, int OUT>
+template <int MODE, int OUT>
 void launch_mode(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
@@ -342,10 +416,10 @@ void launch_mode(const Params& p, hipStream_t st)
     // split the rows when the columns alone cannot fill the GPU.
     int rt = (row_tiles + 3) / 4;
     if (col_blocks < 1024) rt = 1;
-    if (rt <= 1) launch_rt<SPLIT, OUT, 1>(p, st);
-    else if (rt <= 2) launch_rt<SPLIT, OUT, 2>(p, st);
-    else if (rt <= 4) launch_rt<SPLIT, OUT, 4>(p, st);
-    else launch_rt<SPLIT, OUT, 8>(p, st);
+    if (rt <= 1) launch_rt<MODE, OUT, 1>(p, st);
+    else if (rt <= 2) launch_rt<MODE, OUT, 2>(p, st);
+    else if (rt <= 4) launch_rt<MODE, OUT, 4>(p, st);
+    else launch_rt<MODE, OUT, 8>(p, st);
 }
 
 // Per-view max |weighted input|: block (view, y) covers kBoundCols columns of
@@ -448,6 +522,7 @@ Params make_params(const FilterArgs& a)
     p.gamma_max = a.gamma_max;
     p.vbound = a.vbound;
     p.g1 = a.g1;
+    p.gscale = a.gscale;
     p.vscale = a.vscale;
     p.row_tiles_per_block = 0;
     return p;
@@ -472,18 +547,25 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
     const Params p = make_params(a);
     if (a.out_f16s && (a.out_bf16 || a.split_bf16 || !a.vbound || !a.vscale || a.nt < 1))
-        throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output takes FP32 products");
+        throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output takes FP32 or FP16 products");
+    if (a.split_f16 && (a.split_bf16 || !a.vbound || !(a.gscale > 0.f)))
+        throw Error(SGX_ERR_ARG, "sino_filter: split-FP16 products need the input bound and g scale");
     const int out = a.out_f16s ? OUT_F16S : a.out_bf16 ? OUT_BF16 : OUT_F32;
     if (a.split_bf16) {
         switch (out) {
-        case OUT_F32: launch_mode<true, OUT_F32>(p, st); break;
-        default: launch_mode<true, OUT_BF16>(p, st); break;
+        case OUT_F32: launch_mode<M_BF16X3, OUT_F32>(p, st); break;
+        default: launch_mode<M_BF16X3, OUT_BF16>(p, st); break;
+        }
+    } else if (a.split_f16) {
+        switch (out) {
+        case OUT_F32: launch_mode<M_F16X3, OUT_F32>(p, st); break;
+        default: launch_mode<M_F16X3, OUT_F16S>(p, st); break;
         }
     } else {
         switch (out) {
-        case OUT_F32: launch_mode<false, OUT_F32>(p, st); break;
-        case OUT_BF16: launch_mode<false, OUT_BF16>(p, st); break;
-        default: launch_mode<false, OUT_F16S>(p, st); break;
+        case OUT_F32: launch_mode<M_F32, OUT_F32>(p, st); break;
+        case OUT_BF16: launch_mode<M_F32, OUT_BF16>(p, st); break;
+        default: launch_mode<M_F32, OUT_F16S>(p, st); break;
         }
     }
 }

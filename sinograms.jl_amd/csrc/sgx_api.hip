@@ -537,6 +537,7 @@ struct sgx_plan {
     unsigned int* vbound = nullptr; // per-view max |weighted input| (F16S form)
     float* vscale = nullptr;        // per-view FP16 scale (F16S form)
     float g1 = 0.f;                 // sum |g| (F16S output bound)
+    float gscale = 1.f;             // power of two with max|g| * gscale <= 2^14 (split FP16)
     float* partial = nullptr;
     int64_t partial_cap = 0;
     // table offsets
@@ -581,6 +582,8 @@ struct sgx_plan {
         a.split_bf16 = prec == SGX_PREC_BF16;
         if (prec == SGX_PREC_F16S) {
             a.out_f16s = true;
+            a.split_f16 = true;
+            a.gscale = gscale;
             a.vbound = vbound;
             a.g1 = g1;
             a.vscale = vscale;
@@ -818,6 +821,13 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         double g1 = 0.0;
         for (float v : gf) g1 += std::fabs((double)v);
         P->g1 = (float)g1;   // |q| <= g1 max|input|; FP16 keeps 2x headroom above the bound
+        {
+            float gmax = 0.f;
+            for (float v : gf) gmax = std::max(gmax, std::fabs(v));
+            int e = 0;
+            if (gmax > 0.f) (void)std::frexp(gmax, &e);       // gmax <= 2^e
+            P->gscale = std::ldexp(1.f, 14 - e);
+        }
         // detector axis s (and t), fan angles gamma
         std::vector<double> s(ns), gamv(ns);
         for (int64_t i = 0; i < ns; ++i)

@@ -129,6 +129,8 @@ struct FilterArgs {
     bool out_bf16;
     bool out_f16s;          // scaled FP16: q / vscale[view]
     bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
+    bool split_f16;         // split-FP16x3 products (needs vbound and gscale)
+    float gscale;           // split-FP16: the power of two applied to g (g * gscale <= 2^14)
     // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
     float scale;
     const float* wbin;      // nb or null

@@ -580,8 +580,9 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
 
     precision: "fp32" (FP32 filter and storage), or for cone-beam plans the
     16-bit storage forms "bf16" (split-BF16x3 MFMA filter, BF16 filtered
-    projections) and "f16s" (FP32 MFMA filter, FP16 filtered projections
-    with a per-view scale); backprojection always accumulates in FP32.
+    projections) and "f16s" (split-FP16x3 MFMA filter with power-of-two
+    scaled operands, FP16 filtered projections scaled per view by a power of
+    two); backprojection always accumulates in FP32.
     """
     nb = rg.nb if isinstance(rg, SinoGeom) else rg.ns
     npad = npad or _nextpow2(nb + 1)
