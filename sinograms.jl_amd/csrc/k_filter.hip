@@ -178,6 +178,9 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                                                                      : (kBandMax + 64) * 4];
     __shared__ __attribute__((aligned(16))) unsigned char bstage[SPLIT ? 2 * 32 * kBPadH * 2
                                                                         : 32 * kBPadF * 4];
+    // per-bin factors of the fused weight: wbin (and the arc cylinder weight's
+    // cos(s/dsd)), and s^2 for the flat cylinder weight
+    __shared__ float bin_a[2048], bin_s2[2048];
     float* gl = reinterpret_cast<float*>(gbuf);
     uint16_t* band16 = reinterpret_cast<uint16_t*>(gbuf);
     float* bs_f = reinterpret_cast<float*>(bstage);
@@ -208,6 +211,13 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         } else {
             gl[j] = p.g[k];
         }
+    }
+
+    for (int m = tid; m < p.nb; m += 256) {
+        float a = p.wbin ? p.wbin[m] : 1.f;
+        if (p.cyl == 2) a *= cosf(p.s_axis[m] / p.cyl_dsd);
+        bin_a[m] = a;
+        bin_s2[m] = p.cyl == 1 ? p.s_axis[m] * p.s_axis[m] : 0.f;
     }
 
     floatx16 acc[RT];
@@ -252,14 +262,9 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         }
     };
     auto weight_of = [&](int m) -> float {
-        float w = colw;
-        if (p.wbin) w *= p.wbin[m];
-        if (p.cyl == 1) {
-            const float s = p.s_axis[m];
-            w *= p.cyl_dso * rsqrtf(s * s + cyl_c);
-        } else if (p.cyl == 2) {
-            w *= cyl_k * cosf(p.s_axis[m] / p.cyl_dsd);
-        }
+        float w = colw * bin_a[m];
+        if (p.cyl == 1) w *= p.cyl_dso * rsqrtf(bin_s2[m] + cyl_c);
+        else if (p.cyl == 2) w *= cyl_k;
         if (p.gam) w *= (float)parker_w(p.gam[m], sbet, p.gamma_max);
         return w;
     };
@@ -285,13 +290,11 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         }
         __syncthreads();
         if (SPLIT) {
+            uint16_t hi[8], lo[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                uint16_t hi, lo;
-                split2<MODE>(v[e], hi, lo);
-                bs_hi[scol * kBPadH + skk + e] = hi;
-                bs_lo[scol * kBPadH + skk + e] = lo;
-            }
+            for (int e = 0; e < 8; ++e) split2<MODE>(v[e], hi[e], lo[e]);
+            __builtin_memcpy(bs_hi + scol * kBPadH + skk, hi, sizeof(hi));   // 16-byte stores
+            __builtin_memcpy(bs_lo + scol * kBPadH + skk, lo, sizeof(lo));
         } else {
 #pragma unroll
             for (int e = 0; e < 8; ++e) bs_f[scol * kBPadF + skk + e] = v[e];
