@@ -425,58 +425,104 @@ void launch_mode(const Params& p, hipStream_t st)
     else launch_rt<MODE, OUT, 8>(p, st);
 }
 
-// Per-view max |weighted input|: block (view, y) covers kBoundCols columns of
-// one view; thread i takes the bins i, i + 256, ... (coalesced), with the
-// bin factors of the fused weight (wbin, Parker, arc cos(s/dsd)) formed once
-// per bin and the column factors once per column.  One atomic per block.
-constexpr int kBoundCols = 16;
-constexpr int kBoundBins = 8;          // bins per thread: nb <= 2048
-__global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __restrict__ vbound)
+// Per-view max |weighted input|.  Block (view, y) covers kBoundRows detector
+// rows of one view, a contiguous run of rows x nb samples that it reads as
+// one flat range: 16-byte loads when nb is a multiple of 4 and the input is
+// 16-byte aligned (the 4 samples of a load then share a row), else scalar
+// loads; kBoundLoads loads in flight per thread.  The bin factors of the
+// fused weight (wbin, Parker, arc cos(s/dsd), s^2 of the flat cylinder
+// weight) and the row factors sit in LDS.  One atomic per block.
+constexpr int kBoundRows = 16;
+constexpr int kBoundLoads = 8;
+constexpr int kBoundMaxBins = 2048;
+__global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __restrict__ vbound,
+                                                    int vec)
 {
+    __shared__ __attribute__((aligned(16))) float bin_a[kBoundMaxBins];
+    __shared__ __attribute__((aligned(16))) float bin_s2[kBoundMaxBins];
+    __shared__ float row_c[kBoundRows], row_t2[kBoundRows];
+    const int tid = threadIdx.x;
     const int view = blockIdx.x;
-    const int t0 = blockIdx.y * kBoundCols;
-    float wm[kBoundBins], s2[kBoundBins];
-#pragma unroll
-    for (int k = 0; k < kBoundBins; ++k) {
-        const int m = threadIdx.x + 256 * k;
-        wm[k] = 0.f;
-        s2[k] = 0.f;
-        if (m < p.nb) {
-            float w = p.wbin ? p.wbin[m] : 1.f;
-            if (p.cyl == 1) s2[k] = p.s_axis[m] * p.s_axis[m];
-            if (p.cyl == 2) w *= cosf(p.s_axis[m] / p.cyl_dsd);
-            if (p.gam) w *= (float)parker_w(p.gam[m], p.bet[view], p.gamma_max);
-            wm[k] = w;
-        }
+    const int t0 = blockIdx.y * kBoundRows;
+    const int nrows = min(kBoundRows, p.nt - t0);
+    const double sbet = p.gam ? p.bet[view] : 0.0;
+    for (int m = tid; m < p.nb; m += 256) {
+        float a = p.wbin ? p.wbin[m] : 1.f;
+        if (p.cyl == 2) a *= cosf(p.s_axis[m] / p.cyl_dsd);
+        if (p.gam) a *= (float)parker_w(p.gam[m], sbet, p.gamma_max);
+        bin_a[m] = fabsf(a);
+        bin_s2[m] = p.cyl == 1 ? p.s_axis[m] * p.s_axis[m] : 0.f;
     }
-    const float colw = p.scale * (p.wview ? p.wview[view] : 1.f);
-    float mx = 0.f;
-    for (int t = t0; t < min(t0 + kBoundCols, p.nt); ++t) {
-        const int64_t c = (int64_t)view * p.nt + t;
-        const float* col = p.in + (int64_t)p.nb * c;
-        float cw = colw, tt2 = 0.f;
-        if (p.cyl == 1) {
+    if (tid < nrows) {
+        const int t = t0 + tid;
+        float cw = p.scale * (p.wview ? p.wview[view] : 1.f), tt2 = 0.f;
+        if (p.cyl == 1) {                  // dso / sqrt(s^2 + t^2 + dsd^2)
             tt2 = p.t_axis[t] * p.t_axis[t] + p.cyl_dsd * p.cyl_dsd;
             cw *= p.cyl_dso;
-        } else if (p.cyl == 2) {
+        } else if (p.cyl == 2) {           // (dso/dsd) cos(s/dsd) / sqrt((t/dsd)^2 + 1)
             const float r = p.t_axis[t] / p.cyl_dsd;
             cw *= (p.cyl_dso / p.cyl_dsd) / sqrtf(r * r + 1.f);
         }
+        row_c[tid] = fabsf(cw);
+        row_t2[tid] = tt2;
+    }
+    __syncthreads();
+
+    // flat index i = r * nb + m; r = floor((i + 0.5) / nb) is exact in FP32
+    // for i < kBoundRows * kBoundMaxBins
+    const float inv_nb = 1.f / (float)p.nb;
+    const float* base = p.in + ((int64_t)view * p.nt + t0) * p.nb;
+    const int n = nrows * p.nb;
+    auto weight = [&](int r, int m) -> float {
+        float w = row_c[r] * bin_a[m];
+        if (p.cyl == 1) w *= rsqrtf(bin_s2[m] + row_t2[r]);
+        return w;
+    };
+    float mx = 0.f;
+    if (vec) {
+        const float4* b4 = reinterpret_cast<const float4*>(base);
+        const int n4 = n >> 2;
+        for (int i0 = tid; i0 < n4; i0 += 256 * kBoundLoads) {
+            float4 v[kBoundLoads];
 #pragma unroll
-        for (int k = 0; k < kBoundBins; ++k) {
-            const int m = threadIdx.x + 256 * k;
-            if (m < p.nb) {
-                float w = cw * wm[k];
-                if (p.cyl == 1) w *= rsqrtf(s2[k] + tt2);
-                mx = fmaxf(mx, fabsf(col[m] * w));
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i4 = i0 + 256 * k;
+                v[k] = i4 < n4 ? b4[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i = 4 * (i0 + 256 * k);
+                if (i >= n) break;
+                const int r = (int)(((float)i + 0.5f) * inv_nb);
+                const int m = i - r * p.nb;
+                mx = fmaxf(mx, fabsf(v[k].x) * weight(r, m));
+                mx = fmaxf(mx, fabsf(v[k].y) * weight(r, m + 1));
+                mx = fmaxf(mx, fabsf(v[k].z) * weight(r, m + 2));
+                mx = fmaxf(mx, fabsf(v[k].w) * weight(r, m + 3));
+            }
+        }
+    } else {
+        for (int i0 = tid; i0 < n; i0 += 256 * kBoundLoads) {
+            float v[kBoundLoads];
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i = i0 + 256 * k;
+                v[k] = i < n ? base[i] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i = i0 + 256 * k;
+                if (i >= n) break;
+                const int r = (int)(((float)i + 0.5f) * inv_nb);
+                mx = fmaxf(mx, fabsf(v[k]) * weight(r, i - r * p.nb));
             }
         }
     }
     for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
     __shared__ float red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
         mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
         atomicMax(vbound + view, __float_as_uint(mx));     // mx >= 0: bit order = value order
     }
@@ -576,16 +622,17 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
 void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t st)
 {
     if (a.ncols <= 0) return;
-    if (a.nb > 256 * kBoundBins)
+    if (a.nb > kBoundMaxBins)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
     const int64_t na = a.ncols / a.nt;
     launch_zero_u32(vbound, na, st);
     const Params p = make_params(a);
-    if ((a.nt + kBoundCols - 1) / kBoundCols > 65535)
+    if ((a.nt + kBoundRows - 1) / kBoundRows > 65535)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many rows per view");
+    const int vec = a.nb % 4 == 0 && (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
     hipLaunchKernelGGL(bound_kernel, dim3((unsigned)na,
-                                          (unsigned)((a.nt + kBoundCols - 1) / kBoundCols)),
-                       dim3(256), 0, st, p, vbound);
+                                          (unsigned)((a.nt + kBoundRows - 1) / kBoundRows)),
+                       dim3(256), 0, st, p, vbound, vec);
     SGX_HIP(hipGetLastError());
 }
 
