@@ -301,8 +301,13 @@ static_assert(kSZ % kSB == 0, "batch divides the voxels per lane");
 constexpr int kQuads = kWmax / 4;
 constexpr int kStagers = kQuads * (kHmax / 2);   // 192 threads stage
 
-// Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H.
-// clo is a multiple of 4; W = H = 0 when nothing is staged.
+// Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
+// stored as int4 {clo | interior, rlo, W, H}.  clo is a multiple of 4;
+// W = H = 0 when nothing is staged.  `interior` (bit 0) marks a window that
+// was not clamped and whose corner bound keeps one bin and one row of margin
+// inside the detector on every side: every lane of the block then has
+// 1 <= floor(bs) < ns - 1, 1 <= floor(bt) < nt - 1 and its taps inside the
+// window, so the kernel skips the per-lane window tests.
 __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __restrict__ cs,
                                    const float* __restrict__ xds, const float* __restrict__ yds,
                                    const float* __restrict__ zds, int nx, int ny, int nz, int ns,
@@ -353,8 +358,12 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         const int chi = min((int)floorf(fminf(smax, (float)ns)) + 1, ns - 1);
         const int rlo = max((int)floorf(fmaxf(tmin, 0.f)) - 2, 0);
         const int rhi = min((int)floorf(fminf(tmax, (float)nt)), nt - 2);
-        if (chi >= clo && rhi >= rlo)
-            w = make_int4(clo, rlo, min(chi - clo + 1, kWmax), min(rhi - rlo + 1, kHmax));
+        if (chi >= clo && rhi >= rlo) {
+            const int W = chi - clo + 1, H = rhi - rlo + 1;
+            const bool interior = smin >= 2.f && smax < (float)(ns - 1) && tmin >= 2.f &&
+                                  tmax < (float)(nt - 2) && W <= kWmax && H <= kHmax;
+            w = make_int4(clo | (interior ? 1 : 0), rlo, min(W, kWmax), min(H, kHmax));
+        }
     }
     win[i] = w;
 }
@@ -422,7 +431,8 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
     auto stage_load = [&](int ia, int4 w) {
         if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
-        const uint32_t off = toff + ((uint32_t)w.y * (uint32_t)pitch + (uint32_t)w.x) * Fmt<FMT>::esz;
+        const uint32_t off = toff + ((uint32_t)w.y * (uint32_t)pitch + (uint32_t)(w.x & ~3)) *
+                                        Fmt<FMT>::esz;
         const bool third = 2 * sg + 2 <= w.w;              // data row 2sg+2 needed
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -474,7 +484,8 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         bool inwin;
         int ce;
     };
-    auto lane_view = [&](int ia, int4 w) {
+    // check = false: the window is interior (all lanes inside, see above).
+    auto lane_view = [&](int ia, int4 w, bool check) {
         LaneView r;
         const ViewCS v = cs[ia];
         const float xb = x * v.c + y * v.s;
@@ -491,17 +502,21 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         if (FMT == F_F16S) r.w2 *= vscale[ia];
         r.bs = sp + ws;
         r.mz = mag * inv_dt_ds;
-        r.blo = fmaf(r.mz, zlo, wt);
-        r.bhi = fmaf(r.mz, zhi, wt);
         // Window coordinates of the lane's taps: entry columns ce, ce+1 and
         // entry rows (int)bt - 1 - rlo for bt from blo to bhi.  The test
         // implies bs, blo, bhi >= 1 (truncation is floor there) and the
         // detector bounds 1 <= floor(bs) < ns, 1 <= floor(bt) < nt.
-        r.ce = (int)r.bs - 1 - w.x;
-        const int rl = (int)r.blo - 1 - w.y, rh = (int)r.bhi - 1 - w.y;
-        const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
-        r.inwin = (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
-                  (uint32_t)rh < (uint32_t)w.w;
+        r.ce = (int)r.bs - 1 - (w.x & ~3);
+        r.inwin = true;
+        r.blo = r.bhi = 0.f;
+        if (check) {
+            r.blo = fmaf(r.mz, zlo, wt);
+            r.bhi = fmaf(r.mz, zhi, wt);
+            const int rl = (int)r.blo - 1 - w.y, rh = (int)r.bhi - 1 - w.y;
+            const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
+            r.inwin = (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
+                      (uint32_t)rh < (uint32_t)w.w;
+        }
         return r;
     };
 
@@ -515,7 +530,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         const int4 wnext = ia + 1 < nviews ? bwin[ia + 1] : make_int4(0, 0, 0, 0);
         stage_load(ia + 1, wnext);                          // in flight during this view
         if (any) {
-            const LaneView lv = lane_view(ia, wcur);
+            const LaneView lv = lane_view(ia, wcur, (wcur.x & 1) == 0);
             if (lv.inwin) {
                 const float wb = lv.w2 * __builtin_amdgcn_fractf(lv.bs);
                 const float wa = lv.w2 - wb;
@@ -564,7 +579,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
     // larger than the LDS buffer): gathered from global memory, lane by lane.
     if (missed) {
         for (int ia = 0; ia < nviews; ++ia) {
-            const LaneView lv = lane_view(ia, bwin[ia]);
+            const LaneView lv = lane_view(ia, bwin[ia], true);
             if (lv.inwin || !(lv.bs >= 1.f && lv.bs < nsf)) continue;    // 1 <= floor(bs) < ns
             ColView cv;
             cv.isf = floorf(lv.bs);
