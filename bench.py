@@ -12,7 +12,7 @@ the BF16-storage form is reported beside it.  The roofline object describes
 the dominant kernel (the backprojector) under the gather-equivalent model:
 16 B per voxel-view update (4 taps x 4 B) for FP32 storage, 8 B for BF16.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-host]
 """
 
 import argparse
@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the 48-reconstruction batch (one HIP graph) leg")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--cpu-slices", type=int, default=128)
     ap.add_argument("--cbct-kernel", choices=("staged", "gather"), default="staged",
                     help="cone-beam backprojector (LDS-staged windows or global gathers)")
@@ -253,10 +255,12 @@ def main():
             for f in forms}
     # PCIe-inclusive rate: host projections in, host volume out (the Julia
     # binding's ccall path), FP32 plan.
-    log("timing host-buffer executes ...")
-    proj_host = np.asfortranarray(proj.permute(2, 1, 0).contiguous().cpu().numpy()
-                                  .transpose(2, 1, 0))
-    t_host = time_steps(lambda: plans["fp32"].execute(proj_host), 3, 1)
+    t_host = None
+    if not args.no_host:
+        log("timing host-buffer executes ...")
+        proj_host = np.asfortranarray(proj.permute(2, 1, 0).contiguous().cpu().numpy()
+                                      .transpose(2, 1, 0))
+        t_host = time_steps(lambda: plans["fp32"].execute(proj_host), 3, 1)
 
     t32 = step["fp32"]
     f32ms, b32ms = kt["fp32"]
@@ -299,10 +303,9 @@ def main():
                    "updates_per_step": updates, "projections_per_s": rg.na / t32,
                    "filter_TFLOPs_algorithmic_fp32": filt_flops / (f32ms * 1e-3) / 1e12},
         "roofline": roof,
-        "pcie_inclusive": {"value": updates / t_host / 1e9, "unit": "Gupdates/s",
-                           "ms_per_step": t_host * 1e3,
-                           "note": "FP32 plan, host projections in and host volume out "
-                                   "(pageable numpy arrays)"},
+        "pcie_inclusive": None if t_host is None else {
+            "value": updates / t_host / 1e9, "unit": "Gupdates/s", "ms_per_step": t_host * 1e3,
+            "note": "FP32 plan, host projections in and host volume out (pageable numpy arrays)"},
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
     }
