@@ -12,7 +12,11 @@ the BF16-storage form is reported beside it.  The roofline object describes
 the dominant kernel (the backprojector) under the gather-equivalent model:
 16 B per voxel-view update (4 taps x 4 B) for FP32 storage, 8 B for BF16.
 
+The 2-D configurations (parallel beam; arc fan beam, full and short scan)
+are reported beside it under "slices".
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-host]
+                       [--no-slices]
 """
 
 import argparse
@@ -30,6 +34,7 @@ for p in (ROOT, os.path.join(ROOT, "sinograms.jl_amd")):
 METRIC = ("FDK backproj Gupdates/s & % HBM peak; mixed vs plain FP32 at "
           "rel-RMSE≤1e-4")
 HBM_PEAK_GBS = 8000.0
+MU_UNIT = 0.005           # 1 / mm: attenuation of a unit phantom density (C1 noise)
 
 
 def log(*a):
@@ -201,6 +206,117 @@ def cpu_leg(sg, rg, ig, proj, Hk, vols, nslab):
     return cpu, acc
 
 
+def slice_legs(sg, device, steps, warmup, with_cpu):
+    """The 2-D configurations, each one plan_fbp + fbp with the sinogram
+    resident in HBM: C0, parallel beam (367 bins at 1 mm, 360 views over 180
+    deg, Hann window) onto 256 x 256 at 1 mm, 10 random ellipses (seed 0); C1,
+    the 888-bin arc fan (1.0239 mm, dso 541, dsd 949) onto 512 x 512 at
+    0.98 mm inside the circle of radius rfov, 15 random ellipses (seed 1) with
+    1e5-photon Poisson noise (seed 2; densities taken in units of 0.005 / mm,
+    so the longest ray attenuates by about e^-4.3; counts of 0 clamped to 1),
+    as a full scan (984 views over 360 deg) and a short scan (642 views over
+    180 deg + fan angle, Parker weights).  Analytic Float64 sinograms cast to Float32.
+    The backprojector's roofline uses the gather-equivalent 8 B per update
+    (2 taps x 4 B).  With `with_cpu`, the oracle's Float32 backprojection is
+    timed on the host and every result is compared with the Float64 and
+    Float32 oracles."""
+    import numpy as np
+    import torch
+    cases = []
+    rg0 = sg.SinoPar(nb=367, d=1.0, na=360)
+    ig0 = sg.ImageGeom((256, 256), (1.0, 1.0))
+    s0 = sg.phantom.sino_ellipses(sg.phantom.random_ellipses(0, 10, 256.0), rg0).numpy()
+    cases.append(("C0_par", "SinoPar 367 bins x 360 views over 180 deg, Hann window -> "
+                  "256x256 at 1 mm, full mask; 10 random ellipses (seed 0)",
+                  rg0, ig0, s0.astype(np.float32), sg.Window(sg.Hann())))
+    kw = dict(nb=888, d=1.0239, dsd=949.0, dod=949.0 - 541.0)
+    full = sg.SinoFanArc(na=984, **kw)
+    short = sg.SinoFanArc(na=642, orbit=full.orbit_short(), **kw)
+    ig1 = sg.ImageGeom((512, 512), (0.98, 0.98))
+    ig1 = ig1.with_mask(ig1.circle(full.rfov()))
+    ell1 = sg.phantom.random_ellipses(1, 15, 512 * 0.98)
+    for name, rg in (("C1_fan_arc_full", full), ("C1_fan_arc_short", short)):
+        p = sg.phantom.sino_ellipses(ell1, rg).numpy()
+        # densities read in units of MU_UNIT for the photon statistics (the
+        # longest ray then attenuates by about e^-4.3); the noisy line
+        # integrals return to phantom units
+        noisy = (sg.phantom.poisson_log(MU_UNIT * p, 1e5, 2) / MU_UNIT).astype(np.float32)
+        cases.append((name, f"SinoFanArc 888 bins x {rg.na} views over {rg.orbit:.2f} deg "
+                      "(dso 541, dsd 949) -> 512x512 at 0.98 mm in the rfov circle "
+                      f"({int(ig1.mask.sum())} pixels); 15 random ellipses (seed 1), "
+                      "1e5-photon Poisson noise (seed 2, densities in units of 0.005/mm, "
+                      "zero counts clamped to 1)",
+                      rg, ig1, noisy, None))
+    out = {}
+    for name, desc, rg, ig, sino, win in cases:
+        log(f"timing {name} ...")
+        plan = sg.plan_fbp(rg, ig, window=win) if win is not None else sg.plan_fbp(rg, ig)
+        x = torch.from_numpy(np.ascontiguousarray(sino.T)).to(device).T      # (nb, na) col-major
+        y = sg.recon._empty_colmajor(ig.dims, device)
+        t = time_steps(lambda: plan.execute(x, out=y), steps, warmup)
+        fm, bm = kernel_times(plan, x, y, max(3, min(steps, 10)))
+        updates = int(ig.mask.sum()) * rg.na
+        r = {"workload": desc, "updates_per_step": updates, "ms_per_step": t * 1e3,
+             "value": updates / t / 1e9, "unit": "Gupdates/s", "projections_per_s": rg.na / t,
+             "back_kernel_ms": bm, "filter_ms": fm,
+             "roofline": {"bound": "hbm", "achieved": 8.0 * updates / (bm * 1e-3) / 1e9,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": 8.0 * updates / (bm * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "model": "gather-equivalent 8 B per pixel-view update"},
+             "filter_TFLOPs_algorithmic": 2.0 * rg.nb ** 2 * rg.na / (fm * 1e-3) / 1e12}
+        log(f"  {name}: step {t * 1e3:.3f} ms (filter {fm:.3f} ms, back {bm:.3f} ms)")
+        if with_cpu:
+            r.update(_slice_cpu(sg, rg, ig, sino, plan, y.cpu().numpy()))
+        out[name] = r
+    return out
+
+
+def _slice_cpu(sg, rg, ig, sino, plan, got):
+    """Oracle timing (Float32 backprojection of the filtered sinogram) and the
+    GPU result's rel-RMSE against the Float32 and Float64 oracles."""
+    import numpy as np
+    import oracle
+    oracle.lib()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    ar = rg.ar(f32=True).astype(np.float32)
+    xc, yc = ig.axes()
+    vw = oracle.angle_weight(ar)
+    if isinstance(rg, sg.SinoPar):
+        args = (plan.filter, vw, ar, rg.d, 0.0, xc, yc, ig.mask)
+        run = oracle.fbp_par
+    else:
+        if rg.orbit < 360:
+            vw = vw * oracle.parker_fan_short(rg.gamma(), ar, rg.gamma_max())
+        sw = oracle.fan_sino_weight(rg.gamma(), rg.dso, rg.dsd)
+        args = (plan.filter, vw, sw, ar, rg.dsd, rg.dso, 0.0, True, rg.d, 0.0, xc, yc, ig.mask)
+        run = oracle.fbp_fan
+    ref64 = run(sino.astype(np.float64), *args)
+    ref32 = run(sino, *args, dtype=np.float32)
+    # timed baseline: the Float32 backprojection alone, on its own filtered input
+    s = sino.astype(np.float32) * np.asarray(vw, np.float32)
+    if not isinstance(rg, sg.SinoPar):
+        s = s * np.asarray(sw, np.float32)[:, None]
+    q = oracle.sino_filter(s, plan.filter, dtype=np.float32)
+    a = ar.astype(np.float32)
+    t0 = time.perf_counter()
+    if isinstance(rg, sg.SinoPar):
+        oracle.back_par(q, np.sin(a), np.cos(a), rg.d, 0.0, xc, yc, ig.mask, dtype=np.float32)
+    else:
+        oracle.back_fan(q, np.sin(a), np.cos(a), rg.dsd, rg.dso, 0.0, True, rg.d, 0.0, xc, yc,
+                        ig.mask, dtype=np.float32)
+    tb = time.perf_counter() - t0
+    updates = int(ig.mask.sum()) * rg.na
+
+    def err(x, r):
+        return float(np.sqrt(((x - r) ** 2).sum() / (r ** 2).sum()))
+    return {"rel_rmse_vs_f64_oracle": err(got, ref64), "rel_rmse_vs_f32_oracle": err(got, ref32),
+            "cpu_baseline": {"value": updates / tb / 1e9, "unit": "Gupdates/s", "cores": threads,
+                             "kind": "port",
+                             "sample": f"oracle Float32 backprojection (C++/OpenMP restatement, "
+                                       f"not the reference), whole image: {updates} updates in "
+                                       f"{tb:.3f} s; CPU {cpu_model()}"}}
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -221,6 +337,8 @@ def main():
                     help="skip the 48-reconstruction batch (one HIP graph) leg")
     ap.add_argument("--no-host", action="store_true",
                     help="skip the host-buffer (PCIe-inclusive) leg")
+    ap.add_argument("--no-slices", action="store_true",
+                    help="skip the 2-D legs (parallel-beam and arc fan-beam FBP)")
     ap.add_argument("--cpu-slices", type=int, default=128)
     ap.add_argument("--cbct-kernel", choices=("staged", "gather"), default="staged",
                     help="cone-beam backprojector (LDS-staged windows or global gathers)")
@@ -309,6 +427,8 @@ def main():
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
     }
+    if not args.no_slices:
+        result["slices"] = slice_legs(sg, dev, args.steps, args.warmup, not args.no_cpu)
     if not args.no_batch:
         log("batch leg ...")
         result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))

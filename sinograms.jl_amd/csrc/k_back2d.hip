@@ -7,14 +7,20 @@
 //   parallel: bb = x cos + y sin + wb,                     w2 = 1
 //   fan:      d = dso + x sin - y cos,  r = x cos + y sin - source_offset
 //             arc:  bb = dsd atan(r, d) + wb,  w2 = dsd^2 / (d^2 + r^2)
+//                   (atan2 by a minimax polynomial, w2 by a reciprocal)
 //             flat: m = dsd / d, bb = m r + wb, w2 = m^2
 //   il = floor(bb); if 1 <= il and il + 1 <= nb: add w2 * lerp(q[il], q[il+1])
 //
 // Layout.  A block owns a 16 x 16 pixel tile (one pixel per lane) and a
-// group of views.  The views of the group are streamed through LDS in chunks
-// (every view row of the chunk is read from HBM once per block, coalesced),
-// the two taps of each update are LDS reads, per-view cos/sin are
-// wave-uniform scalar loads and the sum is kept in a register.  When there
+// group of views, processed in chunks of views.  For each view of a chunk one
+// thread computes the tile's detector window from the tile's corners (bb is
+// monotone along each axis of the pixel box while the source is outside it),
+// with a bin of margin; only the windows are staged in LDS, several loads in
+// flight per thread.  The two taps of each update are then LDS reads; a lane
+// whose taps fall outside its view's window (windows wider than the LDS
+// slot, pixels at or behind the source) reads them from global memory, so
+// the result never depends on the window.  Per-view cos/sin are wave-uniform
+// scalar loads and the sum is kept in a register, views in order.  When there
 // are too few pixel tiles to fill the GPU the views are split over G groups;
 // each group writes a partial image and a second kernel adds the G partials
 // in a fixed order (deterministic, no atomics).
@@ -28,6 +34,37 @@ namespace {
 constexpr int kTile = 16;
 constexpr int kThreads = kTile * kTile;
 constexpr int kLdsBytes = 48 * 1024;
+constexpr int kU = 8;           // views per unrolled step of the backprojector
+constexpr int kWinMax = 128;    // staged window slot per view (bins), a power of two
+constexpr int kStage = 8;       // staging loads in flight per thread
+
+// atan2(y, x) in FP32 for the arc fan's gamma: the usual octant reduction
+// onto atan(u), u = min(|x|, |y|) / max(|x|, |y|) in [0, 1] (one reciprocal),
+// then u * P(u^2) with P the degree-9 minimax (relative error) polynomial
+// of csrc/gen_atan_coeffs.py: 1.3 ulp from float64 atan in FP32 Horner form,
+// about 20 VALU instructions against 45 for the library's atan2f.
+// (0, 0) gives NaN, which fails every bin test.
+__device__ __forceinline__ float atan2_fast(float y, float x)
+{
+    const float ay = fabsf(y), ax = fabsf(x);
+    const float mx = fmaxf(ay, ax), mn = fminf(ay, ax);
+    const float u = mn * __builtin_amdgcn_rcpf(mx);
+    const float u2 = u * u;
+    float p = -1.753933378e-03f;
+    p = fmaf(p, u2, 1.072695851e-02f);
+    p = fmaf(p, u2, -3.080462478e-02f);
+    p = fmaf(p, u2, 5.755212903e-02f);
+    p = fmaf(p, u2, -8.377317339e-02f);
+    p = fmaf(p, u2, 1.094203144e-01f);
+    p = fmaf(p, u2, -1.426193267e-01f);
+    p = fmaf(p, u2, 1.999826580e-01f);
+    p = fmaf(p, u2, -3.333328366e-01f);
+    p = fmaf(p, u2, 1.0f);
+    float a = u * p;
+    if (ay > ax) a = 1.57079637f - a;            // pi/2 - atan(1/t)
+    if (x < 0.f) a = 3.14159274f - a;            // pi - atan(t)
+    return copysignf(a, y);
+}
 
 // Detector position bb (1-based bins) and weight w2 of pixel (x, y) (/ ds) in
 // one view; shared by the backprojector and its transpose.
@@ -42,8 +79,8 @@ __device__ __forceinline__ void pixel_bin(float x, float y, ViewCS t, float wb, 
         const float d = dso_ds + x * t.s - y * t.c;
         const float r = x * t.c + y * t.s - so_ds;
         if (GEOM == 1) {
-            bb = dsd_ds * atan2f(r, d) + wb;
-            w2 = dsd_ds * dsd_ds / (d * d + r * r);
+            bb = dsd_ds * atan2_fast(r, d) + wb;
+            w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(d * d + r * r);
         } else {
             const float m = dsd_ds / d;
             bb = m * r + wb;
@@ -52,9 +89,17 @@ __device__ __forceinline__ void pixel_bin(float x, float y, ViewCS t, float wb, 
     }
 }
 
+// Window slot per view: the smallest power of two >= nb, at most kWinMax.
+int window_bins(int nb)
+{
+    int w = 4;
+    while (w < nb && w < kWinMax) w *= 2;
+    return w;
+}
+
 int chunk_views(int nb, int vc_max)
 {
-    return std::max(1, std::min(vc_max, kLdsBytes / (4 * nb)));
+    return std::max(1, std::min(vc_max, kLdsBytes / (4 * window_bins(nb) + 8)));
 }
 
 template <int GEOM>  // 0 parallel, 1 fan arc, 2 fan flat
@@ -62,13 +107,19 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ xds,
     const float* __restrict__ yds, int nx, int ny, const uint8_t* __restrict__ mask,
-    float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, float wb,
-    float dsd_ds, float dso_ds, float so_ds)
+    float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, int wshift,
+    float wb, float dsd_ds, float dso_ds, float so_ds)
 {
+    // vc window slots of wm bins, then per view int2 {lo + 1, n - 1}: the
+    // window holds 0-based bins lo .. lo + n - 1 (n = 0: nothing staged)
     extern __shared__ float lds[];
+    const int wm = 1 << wshift;
+    int2* win = reinterpret_cast<int2*>(lds + vc * wm);
+    const int tid = threadIdx.x;
     const int tiles_x = (nx + kTile - 1) / kTile;
-    const int ix = (blockIdx.x % tiles_x) * kTile + (threadIdx.x % kTile);
-    const int iy = (blockIdx.x / tiles_x) * kTile + (threadIdx.x / kTile);
+    const int tx0 = (blockIdx.x % tiles_x) * kTile, ty0 = (blockIdx.x / tiles_x) * kTile;
+    const int ix = tx0 + (tid % kTile);
+    const int iy = ty0 + (tid / kTile);
     const int64_t pix = ix + (int64_t)nx * iy;
     const bool inside = ix < nx && iy < ny;
     const bool active = inside && (mask == nullptr || mask[pix]);
@@ -84,28 +135,97 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     const float x = inside ? xds[ix] : 0.f;
     const float y = inside ? yds[iy] : 0.f;
     const float nb1 = (float)nb;               // valid: 1 <= bb < nb
+    // the tile's box of pixel centres
+    float bx0 = xds[tx0], bx1 = bx0, by0 = yds[ty0], by1 = by0;
+    for (int k = 1; k < kTile; ++k) {
+        const float xv = xds[min(tx0 + k, nx - 1)], yv = yds[min(ty0 + k, ny - 1)];
+        bx0 = fminf(bx0, xv); bx1 = fmaxf(bx1, xv);
+        by0 = fminf(by0, yv); by1 = fmaxf(by1, yv);
+    }
     float acc = 0.f;
 
     for (int c0 = v_begin; c0 < v_end; c0 += vc) {
         const int nv = min(vc, v_end - c0);
+        __syncthreads();                       // the previous chunk is consumed
+        if (tid < nv) {
+            const ViewCS t = cs[c0 + tid];
+            float bmin = 3.4e38f, bmax = -3.4e38f;
+            bool ok = true;
+            for (int k = 0; k < 4; ++k) {
+                const float cx = (k & 1) ? bx1 : bx0, cy = (k & 2) ? by1 : by0;
+                if (GEOM != 0) ok = ok && dso_ds + cx * t.s - cy * t.c > 0.f;   // d > 0
+                float bb, w2;
+                pixel_bin<GEOM>(cx, cy, t, wb, dsd_ds, dso_ds, so_ds, bb, w2);
+                bmin = fminf(bmin, bb);
+                bmax = fmaxf(bmax, bb);
+            }
+            int lo = 0, n = 0;
+            if (ok && bmax >= 1.f && bmin < nb1) {      // false for NaN
+                lo = max((int)floorf(fmaxf(bmin, 0.f)) - 2, 0);
+                const int hi = min((int)floorf(fminf(bmax, nb1)) + 1, nb - 1);
+                n = hi - lo + 1;
+                if (n > wm) n = 0;                       // too wide: taps from global memory
+            }
+            win[tid] = make_int2(lo + 1, n - 1);
+        }
         __syncthreads();
-        for (int v = 0; v < nv; ++v) {
-            const float* src = q + view_stride * (int64_t)(c0 + v);
-            for (int b = threadIdx.x; b < nb; b += kThreads) lds[v * nb + b] = src[b];
+        const int total = nv << wshift;
+        for (int i0 = tid; i0 < total; i0 += kThreads * kStage) {
+            float t[kStage];
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int i = i0 + u * kThreads;
+                const int v = i >> wshift, k = i & (wm - 1);
+                t[u] = 0.f;
+                if (i < total) {
+                    const int2 w = win[v];
+                    if (k <= w.y) t[u] = q[view_stride * (int64_t)(c0 + v) + (w.x - 1) + k];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kStage; ++u) {
+                const int i = i0 + u * kThreads;
+                if (i < total) lds[i] = t[u];
+            }
         }
         __syncthreads();
         if (active) {
-            for (int v = 0; v < nv; ++v) {
+            // one view's update; out of range (1 <= floor(bb) <= nb - 1
+            // fails): bin 1, weight 0
+            auto tap = [&](int v, float& w, float& wr, float& a, float& b) {
                 float bb, w2;
                 pixel_bin<GEOM>(x, y, cs[c0 + v], wb, dsd_ds, dso_ds, so_ds, bb, w2);
-                // 1 <= floor(bb) <= nb - 1; out of range: clamped address, weight 0
                 const bool ok = bb >= 1.f && bb < nb1;
-                const float w = ok ? w2 : 0.f;
+                w = ok ? w2 : 0.f;
                 const float bbc = ok ? bb : 1.f;
                 const float ilf = floorf(bbc);
-                const float wr = bbc - ilf;
-                const float* p = lds + v * nb + ((int)ilf - 1);
-                const float a = p[0], b = p[1];
+                wr = bbc - ilf;
+                const int il = (int)ilf;                  // taps il - 1 and il (0-based)
+                const int2 wv = win[v];
+                const int local = il - wv.x;             // il - 1 - lo
+                if (local >= 0 && local < wv.y) {
+                    const float* p = lds + (v << wshift) + local;
+                    a = p[0];
+                    b = p[1];
+                } else {
+                    const float* p = q + view_stride * (int64_t)(c0 + v) + (il - 1);
+                    a = p[0];
+                    b = p[1];
+                }
+            };
+            // kU views at a time (their loads in flight together); the sum
+            // keeps the view order
+            int v = 0;
+            for (; v + kU <= nv; v += kU) {
+                float w[kU], wr[kU], a[kU], b[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) tap(v + u, w[u], wr[u], a[u], b[u]);
+#pragma unroll
+                for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
+            }
+            for (; v < nv; ++v) {
+                float w, wr, a, b;
+                tap(v, w, wr, a, b);
                 acc = fmaf(w, fmaf(wr, b - a, a), acc);
             }
         }
@@ -191,8 +311,6 @@ int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_block
 
 void launch_back2d(const Back2dArgs& a, hipStream_t st)
 {
-    if ((int64_t)a.nb * 4 > kLdsBytes)
-        throw Error(SGX_ERR_UNSUPPORTED, "back2d: nb > 12288 bins is not supported");
     if (a.want_blocks < 1 || a.vc_max < 1) throw Error(SGX_ERR_ARG, "back2d: bad shape");
     const int vc = chunk_views(a.nb, a.vc_max);
     int groups = groups_for(a.nb, a.nviews, a.nx, a.ny, a.want_blocks, a.vc_max);
@@ -205,22 +323,25 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
     const int tiles = ((a.nx + kTile - 1) / kTile) * ((a.ny + kTile - 1) / kTile);
     float* out = groups > 1 ? a.partial : a.image;
     dim3 grid(tiles, groups), block(kThreads);
-    const size_t lds = (size_t)vc * a.nb * sizeof(float);
+    const int wm = window_bins(a.nb);
+    int wshift = 0;
+    while ((1 << wshift) < wm) ++wshift;
+    const size_t lds = (size_t)vc * wm * sizeof(float) + (size_t)vc * sizeof(int2);
     switch (a.geom) {
     case 0:
         hipLaunchKernelGGL(back2d_kernel<0>, grid, block, lds, st, a.q, a.nb, a.nviews,
                            a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
         break;
     case 1:
         hipLaunchKernelGGL(back2d_kernel<1>, grid, block, lds, st, a.q, a.nb, a.nviews,
                            a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
         break;
     default:
         hipLaunchKernelGGL(back2d_kernel<2>, grid, block, lds, st, a.q, a.nb, a.nviews,
                            a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
         break;
     }
     SGX_HIP(hipGetLastError());
