@@ -38,34 +38,6 @@ constexpr int kU = 8;           // views per unrolled step of the backprojector
 constexpr int kWinMax = 128;    // staged window slot per view (bins), a power of two
 constexpr int kStage = 8;       // staging loads in flight per thread
 
-// atan2(y, x) in FP32 for the arc fan's gamma: the usual octant reduction
-// onto atan(u), u = min(|x|, |y|) / max(|x|, |y|) in [0, 1] (one reciprocal),
-// then u * P(u^2) with P the degree-9 minimax (relative error) polynomial
-// of csrc/gen_atan_coeffs.py: 1.3 ulp from float64 atan in FP32 Horner form,
-// about 20 VALU instructions against 45 for the library's atan2f.
-// (0, 0) gives NaN, which fails every bin test.
-__device__ __forceinline__ float atan2_fast(float y, float x)
-{
-    const float ay = fabsf(y), ax = fabsf(x);
-    const float mx = fmaxf(ay, ax), mn = fminf(ay, ax);
-    const float u = mn * __builtin_amdgcn_rcpf(mx);
-    const float u2 = u * u;
-    float p = -1.753933378e-03f;
-    p = fmaf(p, u2, 1.072695851e-02f);
-    p = fmaf(p, u2, -3.080462478e-02f);
-    p = fmaf(p, u2, 5.755212903e-02f);
-    p = fmaf(p, u2, -8.377317339e-02f);
-    p = fmaf(p, u2, 1.094203144e-01f);
-    p = fmaf(p, u2, -1.426193267e-01f);
-    p = fmaf(p, u2, 1.999826580e-01f);
-    p = fmaf(p, u2, -3.333328366e-01f);
-    p = fmaf(p, u2, 1.0f);
-    float a = u * p;
-    if (ay > ax) a = 1.57079637f - a;            // pi/2 - atan(1/t)
-    if (x < 0.f) a = 3.14159274f - a;            // pi - atan(t)
-    return copysignf(a, y);
-}
-
 // Detector position bb (1-based bins) and weight w2 of pixel (x, y) (/ ds) in
 // one view; shared by the backprojector and its transpose.
 template <int GEOM>

@@ -140,7 +140,7 @@ __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float ds
     const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
     float sp, w2;
     if (ARC) {
-        sp = dsd_ds * atan2f(xb, yb);
+        sp = dsd_ds * atan2_fast(xb, yb);
         w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
     } else {
         sp = mag * xb;
@@ -343,7 +343,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         const float yb = dso_ds - (-x * v.s + y * v.c);
         ok = ok && yb > 0.f;
         const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
-        const float sp = is_arc ? dsd_ds * atan2f(xb, yb) : mag * xb;
+        const float sp = is_arc ? dsd_ds * atan2_fast(xb, yb) : mag * xb;
         const float bs = sp + ws;
         smin = fminf(smin, bs); smax = fmaxf(smax, bs);
         const float mz = mag * inv_dt_ds;
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
         float sp;
         if (ARC) {
-            sp = dsd_ds * atan2f(xb, yb);
+            sp = dsd_ds * atan2_fast(xb, yb);
             r.w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
         } else {
             sp = mag * xb;
