@@ -179,6 +179,49 @@ def test_back2d_fp32_vs_oracle(sg, orc, torch_cuda, kind, nx, ny, na, ia_skip):
     assert rel_rmse(got, ref) < TOL32
 
 
+@pytest.mark.parametrize("kind", ["par", "arc", "flat"])
+def test_back2d_window_fallbacks(sg, orc, torch_cuda, kind):
+    # The windowed kernel's two fallbacks to global reads: a 16-pixel tile
+    # that spans more bins than a window slot holds (fine detector pitch),
+    # and, for the fan beams, pixels at or behind the source (dso smaller
+    # than the image), where the tile's corner bound does not hold.
+    _say("back2d window fallbacks", kind)
+    rng = np.random.default_rng(21)
+    if kind == "par":
+        rg = sg.SinoPar(nb=1500, d=0.1, na=40, offset=0.5)
+        ig = sg.ImageGeom((96, 80), (1.0, 1.1))
+    else:
+        rg = (sg.SinoFanArc if kind == "arc" else sg.SinoFanFlat)(
+            nb=700, d=0.25, na=48, dsd=60.0, dod=20.0, offset=0.5)
+        ig = sg.ImageGeom((112, 96), (1.0, 1.0))
+        # leave out a ring around the source orbit, where w2 = dsd^2 / (d^2 + r^2)
+        # amplifies FP32 rounding of d and r without bound
+        xc, yc = ig.axes()
+        rad = np.hypot(xc[:, None], yc[None, :])
+        ig = ig.with_mask(np.abs(rad - rg.dso) > 3.0)
+    sino = rng.standard_normal(rg.dims()).astype(np.float32)
+    got = sg.fbp_back(rg, ig, sino)
+    xc, yc = ig.axes()
+    ar32 = rg.ar(f32=True).astype(np.float32)
+    sn, cs = np.sin(ar32), np.cos(ar32)
+    refs = {}
+    for dt in (np.float32, np.float64):
+        q, a, b = sino.astype(dt), sn.astype(dt), cs.astype(dt)
+        if kind == "par":
+            refs[dt] = orc.back_par(q, a, b, rg.d, rg.offset, xc, yc, ig.mask, dtype=dt)
+        else:
+            refs[dt] = orc.back_fan(q, a, b, rg.dsd, rg.dso, 0.0, kind == "arc", rg.d,
+                                    rg.offset, xc, yc, ig.mask, dtype=dt)
+    # random data on a fine detector: the reference-precision (Float32) path
+    # is itself far from Float64 here, so the bar is relative to its error
+    floor = rel_rmse(refs[np.float32], refs[np.float64])
+    e64 = rel_rmse(got, refs[np.float64])
+    _say(f"  rel-RMSE vs Float64 {e64:.3g} (Float32 oracle {floor:.3g})")
+    assert np.abs(refs[np.float64]).max() > 0
+    assert e64 < 2 * floor + 1e-6
+    assert rel_rmse(got, refs[np.float32]) < max(TOL32, 3 * floor)
+
+
 # --- K4 filter and K5 Parker ------------------------------------------------------------
 
 @pytest.mark.parametrize("nb,ncols,npad,extra", [(37, 5, 64, 3), (367, 360, 512, 0),
