@@ -162,43 +162,48 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
         }
         __syncthreads();
         if (active) {
-            // one view's update; out of range (1 <= floor(bb) <= nb - 1
-            // fails): bin 1, weight 0
-            auto tap = [&](int v, float& w, float& wr, float& a, float& b) {
-                float bb, w2;
-                pixel_bin<GEOM>(x, y, cs[c0 + v], wb, dsd_ds, dso_ds, so_ds, bb, w2);
-                const bool ok = bb >= 1.f && bb < nb1;
-                w = ok ? w2 : 0.f;
-                const float bbc = ok ? bb : 1.f;
-                const float ilf = floorf(bbc);
-                wr = bbc - ilf;
-                const int il = (int)ilf;                  // taps il - 1 and il (0-based)
-                const int2 wv = win[v];
-                const int local = il - wv.x;             // il - 1 - lo
-                if (local >= 0 && local < wv.y) {
-                    const float* p = lds + (v << wshift) + local;
-                    a = p[0];
-                    b = p[1];
-                } else {
-                    const float* p = q + view_stride * (int64_t)(c0 + v) + (il - 1);
-                    a = p[0];
-                    b = p[1];
-                }
-            };
-            // kU views at a time (their loads in flight together); the sum
-            // keeps the view order
-            int v = 0;
-            for (; v + kU <= nv; v += kU) {
+            // kU views at a time, their LDS reads in flight together; the
+            // last batch of a chunk repeats view nv - 1 with weight 0.  Out of
+            // range (1 <= floor(bb) <= nb - 1 fails): bin 1, weight 0.  Taps
+            // outside a staged window are reloaded from global memory in a
+            // branch taken only when some lane of the wave needs it.  The sum
+            // keeps the view order.
+            for (int v = 0; v < nv; v += kU) {
                 float w[kU], wr[kU], a[kU], b[kU];
+                int il[kU];
+                uint32_t miss = 0;
 #pragma unroll
-                for (int u = 0; u < kU; ++u) tap(v + u, w[u], wr[u], a[u], b[u]);
+                for (int u = 0; u < kU; ++u) {
+                    const int vu = min(v + u, nv - 1);
+                    float bb, w2;
+                    pixel_bin<GEOM>(x, y, cs[c0 + vu], wb, dsd_ds, dso_ds, so_ds, bb, w2);
+                    const bool ok = bb >= 1.f && bb < nb1;
+                    w[u] = (ok && v + u < nv) ? w2 : 0.f;
+                    const float bbc = ok ? bb : 1.f;
+                    const float ilf = floorf(bbc);
+                    wr[u] = bbc - ilf;
+                    il[u] = (int)ilf;                     // taps il - 1 and il (0-based)
+                    const int2 wv = win[vu];
+                    const int local = il[u] - wv.x;      // il - 1 - lo
+                    const bool in = local >= 0 && local < wv.y;
+                    miss |= (uint32_t)(!in) << u;
+                    const float* p = lds + (vu << wshift) + (in ? local : 0);
+                    a[u] = p[0];
+                    b[u] = p[1];
+                }
+                if (__any(miss != 0u)) {
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if ((miss >> u) & 1u) {
+                            const float* qv = q + view_stride * (int64_t)(c0 + min(v + u, nv - 1));
+                            const uint32_t o = (uint32_t)(il[u] - 1);
+                            a[u] = qv[o];
+                            b[u] = qv[o + 1u];
+                        }
+                    }
+                }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
-            }
-            for (; v < nv; ++v) {
-                float w, wr, a, b;
-                tap(v, w, wr, a, b);
-                acc = fmaf(w, fmaf(wr, b - a, a), acc);
             }
         }
     }

@@ -219,6 +219,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         bin_a[m] = a;
         bin_s2[m] = p.cyl == 1 ? p.s_axis[m] * p.s_axis[m] : 0.f;
     }
+    __syncthreads();          // the first chunk's weights read other threads' factors
 
     floatx16 acc[RT];
 #pragma unroll
