@@ -13,7 +13,8 @@ the dominant kernel (the backprojector) under the gather-equivalent model:
 16 B per voxel-view update (4 taps x 4 B) for FP32 storage, 8 B for BF16.
 
 The 2-D configurations (parallel beam; arc fan beam, full and short scan)
-are reported beside it under "slices".
+are reported beside it under "slices", the distance-driven projector pair
+under "dd_projector".
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-batch] [--no-host]
                        [--no-slices]
@@ -271,6 +272,34 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
     return out
 
 
+def dd_leg(sg, device, steps):
+    """The distance-driven projector pair (FP64, as the reference computes
+    it) at the learned form's size: a 512 x 512 image of 0.98 mm onto 984
+    views x 888 bins of 1.0239 mm (dso 541, dsd 949), and at the reference
+    test's size (128 x 128 onto 90 x 226).  Device arrays in and out."""
+    import numpy as np
+    import torch
+    out = {}
+    for nPix, nDet, na in [(512, 888, 984), (128, 226, 90)]:
+        geo = dict(DSD=949.0, DS0=541.0, pSize=512.0 * 0.98 / nPix,
+                   dSize=1.0239 * 888 / nDet, nPix=nPix, nDet=nDet,
+                   angle=np.arange(na) * 2 * np.pi / na)
+        g = torch.Generator(device=device).manual_seed(0)
+        img = torch.rand((nPix, nPix), device=device, dtype=torch.float64,
+                         generator=g).t().contiguous().t()
+        sino = torch.rand((na, nDet), device=device, dtype=torch.float64,
+                          generator=g).t().contiguous().t()
+        tp = time_steps(lambda: sg.project_bdd(img, geo), steps, 2)
+        tb = time_steps(lambda: sg.backproject_bdd(sino, geo), steps, 2)
+        out[f"{nPix}x{nPix}_to_{na}x{nDet}"] = {
+            "project_ms": tp * 1e3, "backproject_ms": tb * 1e3,
+            "rays_per_s": na * nDet / tp, "pixel_views_per_s": nPix * nPix * na / tp,
+            "dtype": "f64"}
+        log(f"  project_bdd {nPix}^2 -> {na} x {nDet}: {tp * 1e3:.3f} ms, "
+            f"adjoint {tb * 1e3:.3f} ms")
+    return out
+
+
 def _slice_cpu(sg, rg, ig, sino, plan, got):
     """Oracle timing (Float32 backprojection of the filtered sinogram) and the
     GPU result's rel-RMSE against the Float32 and Float64 oracles."""
@@ -429,6 +458,8 @@ def main():
     }
     if not args.no_slices:
         result["slices"] = slice_legs(sg, dev, args.steps, args.warmup, not args.no_cpu)
+        log("timing the distance-driven projector ...")
+        result["dd_projector"] = dd_leg(sg, dev, max(3, min(args.steps, 10)))
     if not args.no_batch:
         log("batch leg ...")
         result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))
