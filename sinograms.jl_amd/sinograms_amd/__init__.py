@@ -16,6 +16,8 @@ from .recon import (Boxcar, FBPNormalPlan, FDKplan, Hamming, Hann, NoWindow, Siz
 from . import phantom
 from .batch import Batch, autotune, tuning
 from .sys2 import backproject_bdd, project_bdd
+from . import raygeom
+from .raygeom import footprint_size, rays
 from ._lib import LIB_PATH, SgxError
 
 __all__ = [
@@ -25,5 +27,5 @@ __all__ = [
     "Window", "NoWindow", "Boxcar", "Hamming", "Hann", "WindowVect", "parker_weight",
     "parker_weight_par", "parker_weight_fan_short", "fbp_sino_weight", "fdk_weight_cyl",
     "FBPNormalPlan", "FDKplan", "SizeMismatch", "SgxError", "phantom", "project_bdd",
-    "backproject_bdd", "Batch", "autotune", "tuning",
+    "backproject_bdd", "Batch", "autotune", "tuning", "rays", "footprint_size", "raygeom",
 ]

@@ -369,6 +369,25 @@ def test_fdk_plan_vs_oracle_and_volume(sg, orc, torch_cuda, arc, short):
     assert np.allclose(w, np.broadcast_to(vw, w.shape), rtol=2e-6, atol=1e-9)
 
 
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("short", [False, True])
+def test_fdk_volume_from_ray_projections(sg, torch_cuda, arc, short):
+    # test/fbp3/fdk.jl:14-34 as written there: projections are line integrals
+    # along rays(rg), in the flat case with the reference's polar-angle sign
+    _say("fdk volume from rays(rg)", arc, short)
+    cls = sg.CtFanArc if arc else sg.CtFanFlat
+    kw = dict(ns=64, nt=40, ds=1.0, dt=1.2)
+    rg = cls.short(na=59, **kw) if short else cls(na=32, **kw)
+    ig = sg.ImageGeom((30, 28, 26), (1.0, 1.0, 1.0))
+    ell = np.array([[2.0, 3.0, 4.0, 8.0, 7.0, 6.0, 1.0]])
+    proj = sg.raygeom.radon_ellipsoids(*sg.rays(rg), ell).numpy().astype(np.float32)
+    got = sg.fdk(sg.plan_fbp(rg, ig), proj)
+    vt = 4 / 3 * math.pi * 8 * 7 * 6
+    err = abs(got.sum() - vt) / vt
+    _say(f"  volume error {err:.3g}")
+    assert err < (3e-3 if not short else 2e-2)
+
+
 def test_fdk_16bit_storage_forms_error(sg, orc, torch_cuda):
     _say("fdk 16-bit storage plans")
     rg = sg.CtFanFlat(ns=128, nt=96, ds=2.4, dt=2.4, na=90, dsd=1200.0, dod=400.0)

@@ -1,0 +1,188 @@
+"""CPU tests of the ray conventions (rays, cb_*_to_par, tau, footprint_size).
+
+The reference's own tests of these functions check only their types
+(test/geom/sino-geom.jl:60-80, test/geom/ct-geom.jl:120-140), so the pins
+here are geometric: line integrals along the returned rays must equal the
+source-to-detector projector's, points placed on a ray must project to that
+ray's bin, and the footprint must bound the measured spread of pixel corners.
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+ELL3 = np.array([[10.0, -5.0, 3.0, 40.0, 30.0, 25.0, 1.0],
+                 [-20.0, 15.0, -8.0, 15.0, 20.0, 12.0, 0.5]])
+
+
+def _cone(sg, cls, na=12):
+    return cls(ns=24, nt=16, ds=4.0, dt=3.5, offset_s=0.25, offset_t=-0.5, na=na,
+               orbit_start=10.0, dsd=600.0, dod=200.0)
+
+
+def test_arc_cone_rays_give_the_geometric_projection(sg):
+    from sinograms_amd import raygeom as R
+    rg = _cone(sg, sg.CtFanArc)
+    u, v, phi, theta = R.rays(rg)
+    assert u.shape == rg.dims()
+    p_geo = sg.phantom.proj_ellipsoids(ELL3, rg).numpy()
+    p_ray = R.radon_ellipsoids(u, v, phi, theta, ELL3).numpy()
+    assert np.abs(p_ray - p_geo).max() < 1e-9 * p_geo.max()
+
+
+def test_flat_cone_rays_have_the_reference_polar_sign(sg):
+    # cb_flat_to_par.jl:28 "note sign!": its rays are the true rays with the
+    # polar angle negated, in the frame that cb_arc_to_par's rays satisfy
+    from sinograms_amd import raygeom as R
+    rg = _cone(sg, sg.CtFanFlat)
+    u, v, phi, theta = R.rays(rg)
+    p_geo = sg.phantom.proj_ellipsoids(ELL3, rg).numpy()
+    p_neg = R.radon_ellipsoids(u, v, phi, -theta, ELL3).numpy()
+    assert np.abs(p_neg - p_geo).max() < 1e-9 * p_geo.max()
+    p_ray = R.radon_ellipsoids(u, v, phi, theta, ELL3).numpy()
+    assert np.abs(p_ray - p_geo).max() > 1e-2 * p_geo.max()
+    # the polar angle takes the sign of t, as the arc form's does
+    assert np.array_equal(np.sign(theta), np.sign(rg.t())[None, :, None] * np.ones(theta.shape))
+    assert np.array_equal(np.sign(R.rays(_cone(sg, sg.CtFanArc))[3]), np.sign(theta))
+
+
+def test_flat_and_arc_rays_of_one_line_agree(sg):
+    # a flat detector point (s, t) and the arc point (dsd atan(s/dsd), t cos(gamma))
+    # see the same source-to-detector direction
+    from sinograms_amd import raygeom as R
+    dso, dod = 400.0, 200.0
+    dsd = dso + dod
+    s = np.linspace(-150, 150, 9)[:, None]
+    t = np.linspace(-90, 90, 7)[None, :]
+    g = np.arctan(s / dsd)
+    uf, vf, pf, tf = R.cb_flat_to_par(s, t, 0.3, dso, dod)
+    ua, va, pa, ta = R.cb_arc_to_par(dsd * g, t * np.cos(g), 0.3, dso, dod)
+    assert np.allclose(uf, -ua, atol=1e-12)
+    assert np.allclose(vf, va, atol=1e-12)
+    assert np.allclose(tf, ta, atol=1e-14)
+    assert np.allclose(np.cos(pf - pa), -1.0, atol=1e-14)
+    # closed forms: |u| = dso sin(gamma) and the cone angle of the ray
+    assert np.allclose(uf, dso * np.sin(g), atol=1e-12)
+    assert np.allclose(np.tan(tf), t / np.sqrt(s * s + dsd * dsd), atol=1e-14)
+
+
+@pytest.mark.parametrize("kind", ["arc", "flat"])
+def test_fan_rays_give_the_central_cone_row(sg, kind):
+    # the 2-D fan rays against the geometric cone projector's t = 0 row, for
+    # ellipsoids whose z = 0 sections are the ellipses
+    from sinograms_amd import raygeom as R
+    fan, cone = ((sg.SinoFanArc, sg.CtFanArc) if kind == "arc"
+                 else (sg.SinoFanFlat, sg.CtFanFlat))
+    kw = dict(na=20, orbit_start=7.0, dsd=500.0, dod=150.0)
+    rg = fan(nb=40, d=2.5, offset=0.25, **kw)
+    ct = cone(ns=40, nt=1, ds=2.5, offset_s=0.25, **kw)
+    ell2 = np.array([[r[0], r[1], r[3], r[4], 0.0, r[6]] for r in ELL3])
+    ell3 = ELL3.copy()
+    ell3[:, 2] = 0.0
+    r, phi = R.rays(rg)
+    assert r.shape == (rg.nb, rg.na)
+    got = R.radon_ellipses(r, phi, ell2).numpy()
+    ref = sg.phantom.proj_ellipsoids(ell3, ct).numpy()[:, 0, :]
+    assert np.abs(got - ref).max() < 1e-9 * ref.max()
+
+
+def test_par_rays_closed_form_disk(sg):
+    from sinograms_amd import raygeom as R
+    rg = sg.SinoPar(nb=50, d=1.5, offset=-0.25, na=30, orbit_start=5.0)
+    r, phi = R.rays(rg)
+    assert np.array_equal(r[:, 3], rg.s()) and np.array_equal(phi[7], rg.ar())
+    got = R.radon_ellipses(r, phi, np.array([[7.0, -4.0, 20.0, 20.0, 0.3, 2.0]])).numpy()
+    tau = r - 7.0 * np.cos(phi) + 4.0 * np.sin(phi)
+    ref = 2.0 * 2 * np.sqrt(np.clip(400.0 - tau ** 2, 0, None))
+    assert np.abs(got - ref).max() < 1e-9
+
+
+def _points_on_rays(r, phi, ell):
+    """Points at distance ell along the 2-D parallel rays (r, phi)."""
+    return (r * np.cos(phi) - ell * np.sin(phi), r * np.sin(phi) + ell * np.cos(phi))
+
+
+@pytest.mark.parametrize("kind", ["par", "arc", "flat", "cone_arc", "cone_flat"])
+def test_tau_of_points_on_a_ray_is_its_bin(sg, kind):
+    from sinograms_amd import raygeom as R
+    kw = dict(na=24, orbit_start=-20.0)
+    fan = dict(dsd=400.0, dod=120.0, **kw)
+    rg = {"par": lambda: sg.SinoPar(nb=60, d=1.25, offset=0.5, **kw),
+          "arc": lambda: sg.SinoFanArc(nb=60, d=1.25, offset=0.5, **fan),
+          "flat": lambda: sg.SinoFanFlat(nb=60, d=1.25, offset=0.5, **fan),
+          "cone_arc": lambda: sg.CtFanArc(ns=60, nt=4, ds=1.25, offset_s=0.5, **fan),
+          "cone_flat": lambda: sg.CtFanFlat(ns=60, nt=4, ds=1.25, offset_s=0.5, **fan)}[kind]()
+    twin = rg
+    if kind.startswith("cone"):      # in-plane rays of the cone geometry
+        twin = (sg.SinoFanArc if rg.is_arc else sg.SinoFanFlat)(
+            nb=60, d=1.25, offset=0.5, **fan)
+    r, phi = R.rays(twin)
+    ws = (r.shape[0] - 1) / 2 + 0.5
+    rng = np.random.default_rng(3)
+    for ib in (0, 17, 30, 59):
+        ell = rng.uniform(-60, 60, size=5)
+        for ia in range(r.shape[1]):
+            x, y = _points_on_rays(r[ib, ia], phi[ib, ia], ell)
+            got = R.tau(rg, x, y)[:, ia]
+            assert np.allclose(got + ws, ib, atol=1e-9), (ib, ia, got + ws)
+    x = np.arange(1.0, 5.0)
+    assert R.tau(rg)(x, 2 * x).shape == (4, rg.na)
+    with pytest.raises(ValueError):
+        R.tau(rg, x, x[:3])
+
+
+def test_tau_with_a_source_offset(sg):
+    # the source sits at (source_offset, dso) in the frame rotated by beta,
+    # the central ray along -y; a point on the ray at fan angle gamma
+    from sinograms_amd import raygeom as R
+    for cls in (sg.SinoFanArc, sg.SinoFanFlat):
+        rg = cls(nb=64, d=1.0, na=9, dsd=400.0, dod=100.0, source_offset=6.0)
+        b = rg.ar()
+        for gam in (-0.1, 0.03, 0.12):
+            for ell in (120.0, 300.0, 380.0):
+                xb = rg.source_offset + ell * math.sin(gam)
+                yb = rg.dso - ell * math.cos(gam)
+                x = xb * np.cos(b) - yb * np.sin(b)
+                y = xb * np.sin(b) + yb * np.cos(b)
+                got = np.diag(R.tau(rg, x, y)) * rg.d
+                want = rg.dsd * (gam if rg.is_arc else math.tan(gam))
+                assert np.allclose(got, want, atol=1e-9)
+
+
+def _corner_spread(rg, ig, rad=None):
+    from sinograms_amd import raygeom as R
+    xc, yc = ig.axes()[:2]
+    X, Y = np.meshgrid(xc, yc, indexing="ij")
+    if rad is not None:
+        m = np.hypot(X, Y) <= rad
+        X, Y = X[m], Y[m]
+    dx, dy = ig.deltas[:2]
+    ts = np.stack([R.tau(rg, X + sx * dx / 2, Y + sy * dy / 2)
+                   for sx in (-1, 1) for sy in (-1, 1)])
+    return float((ts.max(0) - ts.min(0)).max())
+
+
+def test_footprint_size(sg):
+    from sinograms_amd import raygeom as R
+    ig = sg.ImageGeom((48, 40), (1.0, 1.0))
+    # parallel beam: the diagonal, reached in the 45 degree view
+    rg = sg.SinoPar(nb=80, d=0.8, na=360)
+    fp = R.footprint_size(rg, ig)
+    assert isinstance(fp, np.float32)
+    assert fp == np.float32(math.sqrt(2) / 0.8)
+    assert abs(_corner_spread(rg, ig) - fp) < 1e-5 * fp
+    # fan beams: bounds the spread of pixels inside the field-of-view circle
+    rfov = 48 / 2
+    for cls in (sg.SinoFanArc, sg.SinoFanFlat, sg.CtFanArc, sg.CtFanFlat):
+        if cls in (sg.CtFanArc, sg.CtFanFlat):
+            rg = cls(ns=80, nt=4, ds=0.8, na=360, dsd=240.0, dod=80.0)
+            ig3 = sg.ImageGeom((48, 40, 4), (1.0, 1.0, 1.0))
+        else:
+            rg = cls(nb=80, d=0.8, na=360, dsd=240.0, dod=80.0)
+            ig3 = ig
+        fp = R.footprint_size(rg, ig3)
+        spread = _corner_spread(rg, ig3, rfov - math.sqrt(2) / 2)
+        assert 0.9 * fp < spread <= fp, (cls.__name__, spread, fp)
+    with pytest.raises(ValueError, match="bad dso"):
+        R.footprint_size(sg.SinoFanArc(nb=80, d=0.8, dsd=60.0, dod=40.0), ig)
