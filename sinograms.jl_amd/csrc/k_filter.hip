@@ -41,6 +41,8 @@ constexpr int kKC = 64;            // input rows staged per chunk
 enum { OUT_F32 = 0, OUT_BF16 = 1, OUT_F16S = 2 };
 constexpr int kBPadF = kKC + 1;    // FP32 staging stride (conflict-free column reads)
 constexpr int kBPadH = kKC + 8;    // bf16 staging stride (16-byte aligned fragments)
+constexpr int kStageHalves = 2 * 32 * kBPadH;   // one staging buffer: hi and lo parts
+constexpr int kStageFloats = 32 * kBPadF;       // one FP32 staging buffer
 
 struct Params {
     const float* in;
@@ -58,9 +60,7 @@ struct Params {
     float cyl_dsd, cyl_dso;
     const float* s_axis;
     const float* t_axis;
-    const double* gam;
-    const double* bet;
-    double gamma_max;
+    const float* ptab;            // (nb, na) short-scan Parker weights, or null
     const unsigned int* vbound;   // F16S: per-view max |weighted input| (float bits)
     float g1;                     // F16S: sum |g|
     float gscale;                 // split-FP16: power of two applied to g
@@ -114,7 +114,7 @@ __device__ __forceinline__ float weight(const Params& p, int m, int64_t c)
         }
     }
     if (p.wview) w *= p.wview[c / p.nt];
-    if (p.gam) w *= (float)parker_w(p.gam[m], p.bet[c / p.nt], p.gamma_max);
+    if (p.ptab) w *= p.ptab[(c / p.nt) * p.nb + m];
     return w;
 }
 
@@ -142,6 +142,7 @@ __device__ __forceinline__ float bf16_to_f(__bf16 h)
 // range; the epilogue divides the scales out exactly.
 enum { M_F32 = 0, M_BF16X3 = 1, M_F16X3 = 2 };
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
+
 constexpr int kBandMax = 4 * 8 * 32 + 2048;       // longest band of g (RT = 8, nb <= 2048)
 
 __device__ __forceinline__ float in_scale(const unsigned int* vbound, int view)
@@ -173,11 +174,12 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     // FP32: the band of g as floats.  Split modes: hi / lo parts of the band,
     // reversed (R[i] = band[glen-1-i], so a fragment's 8 taps ascend), each
     // in two copies shifted by one element so every fragment starts on a
-    // dword: band16[2*part + copy][i] = R[i + copy].
+    // dword: band16[(2*part + copy) * kBandMax + i] = R[i + copy].
     __shared__ __attribute__((aligned(16))) unsigned char gbuf[SPLIT ? 4 * kBandMax * 2
                                                                      : (kBandMax + 64) * 4];
-    __shared__ __attribute__((aligned(16))) unsigned char bstage[SPLIT ? 2 * 32 * kBPadH * 2
-                                                                        : 32 * kBPadF * 4];
+    // two staging buffers (chunk c in buffer c & 1)
+    __shared__ __attribute__((aligned(16))) unsigned char bstage[2 * (SPLIT ? 2 * 32 * kBPadH * 2
+                                                                             : 32 * kBPadF * 4)];
     // per-bin factors of the fused weight: wbin (and the arc cylinder weight's
     // cos(s/dsd)), and s^2 for the flat cylinder weight
     __shared__ float bin_a[2048], bin_s2[2048];
@@ -185,7 +187,6 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     uint16_t* band16 = reinterpret_cast<uint16_t*>(gbuf);
     float* bs_f = reinterpret_cast<float*>(bstage);
     uint16_t* bs_hi = reinterpret_cast<uint16_t*>(bstage);
-    uint16_t* bs_lo = bs_hi + 32 * kBPadH;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t c0 = (int64_t)blockIdx.x * 32;
@@ -242,18 +243,25 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     float colw = p.scale;
     if (cvalid && p.wview) colw *= p.wview[sview];
     if (MODE == M_F16X3 && cvalid) colw *= in_scale(p.vbound, sview);
-    float cyl_c = 0.f, cyl_k = 0.f;
-    if (cvalid && p.cyl == 1) {            // dso / sqrt(s^2 + t^2 + dsd^2)
+    // Cylinder weight, branch-free: flat dso / sqrt(s^2 + t^2 + dsd^2) is
+    // cyl_k * rsqrt(bin_s2[m] + cyl_c) with cyl_k = dso; arc (dso/dsd)
+    // cos(s/dsd) / sqrt((t/dsd)^2 + 1) is bin_a[m] (the cosine) times cyl_k;
+    // without one, cyl_k = 1.  Multiplying by 1 is exact.
+    const bool cyl_flat = p.cyl == 1;
+    float cyl_c = 1.f, cyl_k = 1.f;
+    if (cvalid && p.cyl == 1) {
         const float tt = p.t_axis[st];
         cyl_c = tt * tt + p.cyl_dsd * p.cyl_dsd;
-    } else if (cvalid && p.cyl == 2) {     // (dso/dsd) cos(s/dsd) / sqrt((t/dsd)^2 + 1)
+        cyl_k = p.cyl_dso;
+    } else if (cvalid && p.cyl == 2) {
         const float r = p.t_axis[st] / p.cyl_dsd;
         cyl_k = (p.cyl_dso / p.cyl_dsd) / sqrtf(r * r + 1.f);
     }
-    const double sbet = (cvalid && p.gam) ? p.bet[sview] : 0.0;
     const float* src = p.in + (int64_t)p.nb * (cvalid ? sc : 0);
+    const float* prow = (p.ptab && cvalid) ? p.ptab + (int64_t)sview * p.nb : nullptr;
 
-    auto load8 = [&](int m0, float (&v)[8]) {
+    // rows m0 + skk .. + 7 of this thread's column, and their Parker factors
+    auto load8 = [&](int m0, float (&v)[8], float (&pw)[8]) {
         const int m = m0 + skk;
         if (cvalid && m + 8 <= p.nb) {
             __builtin_memcpy(v, src + m, sizeof(float) * 8);
@@ -261,13 +269,38 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 #pragma unroll
             for (int e = 0; e < 8; ++e) v[e] = (cvalid && m + e < p.nb) ? src[m + e] : 0.f;
         }
+        if (prow && m + 8 <= p.nb) {
+            __builtin_memcpy(pw, prow + m, sizeof(float) * 8);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) pw[e] = (prow && m + e < p.nb) ? prow[m + e] : 1.f;
+        }
     };
-    auto weight_of = [&](int m) -> float {
-        float w = colw * bin_a[m];
-        if (p.cyl == 1) w *= p.cyl_dso * rsqrtf(bin_s2[m] + cyl_c);
-        else if (p.cyl == 2) w *= cyl_k;
-        if (p.gam) w *= (float)parker_w(p.gam[m], sbet, p.gamma_max);
-        return w;
+    // the fused weight, applied without branches (rows past nb become 0)
+    auto weigh = [&](int m0, float (&v)[8], const float (&pw)[8]) {
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int m = m0 + skk + e;
+            const int mm = min(m, p.nb - 1);
+            float w = colw * bin_a[mm];
+            w *= cyl_flat ? cyl_k * rsqrtf(bin_s2[mm] + cyl_c) : cyl_k;
+            w *= pw[e];
+            v[e] = (cvalid && m < p.nb) ? v[e] * w : 0.f;
+        }
+    };
+    auto put = [&](int buf, const float (&v)[8]) {
+        if (SPLIT) {
+            uint16_t hi[8], lo[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) split2<MODE>(v[e], hi[e], lo[e]);
+            uint16_t* bh = bs_hi + buf * kStageHalves;
+            __builtin_memcpy(bh + scol * kBPadH + skk, hi, sizeof(hi));   // 16-byte stores
+            __builtin_memcpy(bh + 32 * kBPadH + scol * kBPadH + skk, lo, sizeof(lo));
+        } else {
+            float* bf = bs_f + buf * kStageFloats;
+#pragma unroll
+            for (int e = 0; e < 8; ++e) bf[scol * kBPadF + skk + e] = v[e];
+        }
     };
 
     // Split modes: per lane, the start of its A fragments in the reversed
@@ -277,37 +310,15 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     const uint16_t* ahi = band16 + (0 + par) * kBandMax;
     const uint16_t* alo = band16 + (2 + par) * kBandMax;
 
-    float vnext[8];
-    load8(0, vnext);
-    for (int m0 = 0; m0 < p.nb; m0 += kKC) {
-        float v[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = vnext[e];
-        if (m0 + kKC < p.nb) load8(m0 + kKC, vnext);     // next chunk in flight meanwhile
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int m = m0 + skk + e;
-            v[e] = (cvalid && m < p.nb) ? v[e] * weight_of(m) : 0.f;
-        }
-        __syncthreads();
+    auto mma = [&](int buf, int m0) {
         if (SPLIT) {
-            uint16_t hi[8], lo[8];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) split2<MODE>(v[e], hi[e], lo[e]);
-            __builtin_memcpy(bs_hi + scol * kBPadH + skk, hi, sizeof(hi));   // 16-byte stores
-            __builtin_memcpy(bs_lo + scol * kBPadH + skk, lo, sizeof(lo));
-        } else {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) bs_f[scol * kBPadF + skk + e] = v[e];
-        }
-        __syncthreads();
-
-        if (SPLIT) {
+            const uint16_t* bh = bs_hi + buf * kStageHalves;
+            const uint16_t* bl = bh + 32 * kBPadH;
 #pragma unroll
             for (int ks = 0; ks < kKC / 16; ++ks) {
                 const int kb = ks * 16 + half * 8;
-                const uint4 bh4 = *reinterpret_cast<const uint4*>(bs_hi + col * kBPadH + kb);
-                const uint4 bl4 = *reinterpret_cast<const uint4*>(bs_lo + col * kBPadH + kb);
+                const uint4 bh4 = *reinterpret_cast<const uint4*>(bh + col * kBPadH + kb);
+                const uint4 bl4 = *reinterpret_cast<const uint4*>(bl + col * kBPadH + kb);
 #pragma unroll
                 for (int r = 0; r < RT; ++r) {
                     const int n = (tile0 + wave * RT + r) * 32 + col;
@@ -320,27 +331,28 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                     if (MODE == M_BF16X3) {
                         const bf16x8 ah = __builtin_bit_cast(bf16x8, ah4);
                         const bf16x8 al = __builtin_bit_cast(bf16x8, al4);
-                        const bf16x8 bh = __builtin_bit_cast(bf16x8, bh4);
-                        const bf16x8 bl = __builtin_bit_cast(bf16x8, bl4);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc[r], 0, 0, 0);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc[r], 0, 0, 0);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc[r], 0, 0, 0);
+                        const bf16x8 b_h = __builtin_bit_cast(bf16x8, bh4);
+                        const bf16x8 b_l = __builtin_bit_cast(bf16x8, bl4);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b_h, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, b_l, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, b_h, acc[r], 0, 0, 0);
                     } else {
                         const halfx8 ah = __builtin_bit_cast(halfx8, ah4);
                         const halfx8 al = __builtin_bit_cast(halfx8, al4);
-                        const halfx8 bh = __builtin_bit_cast(halfx8, bh4);
-                        const halfx8 bl = __builtin_bit_cast(halfx8, bl4);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[r], 0, 0, 0);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[r], 0, 0, 0);
-                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[r], 0, 0, 0);
+                        const halfx8 b_h = __builtin_bit_cast(halfx8, bh4);
+                        const halfx8 b_l = __builtin_bit_cast(halfx8, bl4);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b_h, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, b_l, acc[r], 0, 0, 0);
+                        acc[r] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, b_h, acc[r], 0, 0, 0);
                     }
                 }
             }
         } else {
+            const float* bf = bs_f + buf * kStageFloats;
 #pragma unroll 4
             for (int ks = 0; ks < kKC / 2; ++ks) {
                 const int k = ks * 2 + half;
-                const float b = bs_f[col * kBPadF + k];
+                const float b = bf[col * kBPadF + k];
 #pragma unroll
                 for (int r = 0; r < RT; ++r) {
                     const int n = (tile0 + wave * RT + r) * 32 + col;
@@ -349,6 +361,34 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                 }
             }
         }
+    };
+
+    // Double-buffered chunks: chunk c is multiplied out of buffer c & 1 while
+    // chunk c + 1 is weighted, split and written to the other buffer, one
+    // barrier per chunk; the global loads run a chunk ahead of the weighting.
+    float vq[8], pq[8];
+    {
+        float v0[8], p0[8];
+        load8(0, v0, p0);
+        load8(kKC, vq, pq);                 // rows past nb load as 0
+        weigh(0, v0, p0);
+        put(0, v0);
+    }
+    __syncthreads();
+    int buf = 0;
+    for (int m0 = 0; m0 < p.nb; m0 += kKC) {
+        float v[8], pw[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            v[e] = vq[e];
+            pw[e] = pq[e];
+        }
+        load8(m0 + 2 * kKC, vq, pq);
+        weigh(m0 + kKC, v, pw);
+        mma(buf, m0);
+        put(buf ^ 1, v);                    // past the last chunk: written, never read
+        __syncthreads();
+        buf ^= 1;
     }
 
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
@@ -446,11 +486,10 @@ __global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __re
     const int view = blockIdx.x;
     const int t0 = blockIdx.y * kBoundRows;
     const int nrows = min(kBoundRows, p.nt - t0);
-    const double sbet = p.gam ? p.bet[view] : 0.0;
     for (int m = tid; m < p.nb; m += 256) {
         float a = p.wbin ? p.wbin[m] : 1.f;
         if (p.cyl == 2) a *= cosf(p.s_axis[m] / p.cyl_dsd);
-        if (p.gam) a *= (float)parker_w(p.gam[m], sbet, p.gamma_max);
+        if (p.ptab) a *= p.ptab[(int64_t)view * p.nb + m];
         bin_a[m] = fabsf(a);
         bin_s2[m] = p.cyl == 1 ? p.s_axis[m] * p.s_axis[m] : 0.f;
     }
@@ -567,9 +606,7 @@ Params make_params(const FilterArgs& a)
     p.cyl_dso = a.cyl_dso;
     p.s_axis = a.s_axis;
     p.t_axis = a.t_axis;
-    p.gam = a.gam;
-    p.bet = a.bet;
-    p.gamma_max = a.gamma_max;
+    p.ptab = a.ptab;
     p.vbound = a.vbound;
     p.g1 = a.g1;
     p.gscale = a.gscale;

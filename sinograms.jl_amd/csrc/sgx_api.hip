@@ -547,6 +547,7 @@ struct sgx_plan {
     float *xds = nullptr, *yds = nullptr, *zds = nullptr, *g = nullptr, *wbin = nullptr,
           *wview = nullptr, *s_axis = nullptr, *t_axis = nullptr;
     float* wview_owned = nullptr;
+    float* ptab = nullptr;          // short-scan Parker weights (ns, na), formed once in FP64
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
 
@@ -555,6 +556,7 @@ struct sgx_plan {
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
         if (wview_owned) (void)hipFree(wview_owned);
+        if (ptab) (void)hipFree(ptab);
         if (tables) (void)hipFree(tables);
         if (mask) (void)hipFree(mask);
         if (q) (void)hipFree(q);
@@ -598,11 +600,7 @@ struct sgx_plan {
             a.s_axis = s_axis;
             a.t_axis = t_axis;
         }
-        if (G.parker) {
-            a.gam = gam;
-            a.bet = bet;
-            a.gamma_max = G.gamma_max;
-        }
+        a.ptab = ptab;
         return a;
     }
 };
@@ -880,6 +878,13 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         }
         sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), na, 1, P->cs,
                               ctx->stream);
+        if (G.parker) {
+            // parker_weight_fan_short (parker.jl:96-106) in FP64, cast to Float32
+            // once per plan; the filter multiplies it into its operand load
+            SGX_HIP(hipMalloc(&P->ptab, 4 * ns * na));
+            sgx::launch_parker(P->gam, P->bet, (int)ns, (int)na, G.gamma_max, P->ptab,
+                               ctx->stream);
+        }
         const int64_t nvox = nx * ny * P->nz;
         if (mask) {
             SGX_HIP(hipMalloc(&P->mask, nvox));

@@ -159,7 +159,7 @@ struct FilterArgs {
     bool split_bf16;        // split-BF16x3 products (else FP32 MFMA)
     bool split_f16;         // split-FP16x3 products (needs vbound and gscale)
     float gscale;           // split-FP16: the power of two applied to g (g * gscale <= 2^14)
-    // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * parker(m, view)
+    // fused weights (all optional): w = scale * wbin[m] * cyl(m, t) * ptab[m, view]
     float scale;
     const float* wbin;      // nb or null
     const float* wview;     // na or null (per-view weight, e.g. parallel-beam Parker)
@@ -167,9 +167,7 @@ struct FilterArgs {
     float cyl_dsd, cyl_dso;
     const float* s_axis;    // nb (cyl)
     const float* t_axis;    // nt (cyl)
-    const double* gam;      // nb (parker) or null
-    const double* bet;      // na, beta - min(beta), clamped >= 0 (parker)
-    double gamma_max;
+    const float* ptab;      // (nb, na) short-scan Parker weights, or null
     // scaled FP16: vbound[view] = max |weighted input| of the view (float
     // bits); the output scale is the power of two vscale[view] >= that bound
     // times g1 / 2^14, so |q| / vscale <= 2^14 and the differences fit FP16
