@@ -124,6 +124,31 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
                   const double* zc, int64_t nz, const uint8_t* mask, float* image,
                   uint32_t flags);
 
+/* Cone-beam options that the reference leaves commented out in cbct_back
+ * (src/fbp3/cbct-back.jl:36-59 and 188: source_zs; :180: the arc detector's
+ * t magnification).  All zero (or a NULL pointer to this struct) is the
+ * reference's geometry.
+ *   source_zs    na source z positions (host, units of xc / ds, e.g. the
+ *                helical _source_zs of src/geom/ct-geom.jl:134-140), or NULL
+ *                for z = 0: t' = mag (z - source_zs[ia]);
+ *   arc_t_exact  1: for the arc detector the t magnification of the ray that
+ *                meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2),
+ *                instead of the flat detector's dsd / y_beta. */
+typedef struct {
+    const double* source_zs;
+    int32_t arc_t_exact;
+} sgx_cbct_opts;
+
+/* sgx_cbct_back with the options above; no counterpart in the reference (it
+ * extends cbct_back, src/fbp3/cbct-back.jl:28-45, with what that file keeps
+ * commented out).  source_zs is indexed like betas (ia_skip applies). */
+int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                       int64_t ia_skip, const double* betas, double dsd, double dso,
+                       double ds, double dt, double offset_s, double offset_t, int is_arc,
+                       const double* xc, int64_t nx, const double* yc, int64_t ny,
+                       const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                       uint32_t flags, const sgx_cbct_opts* opts);
+
 /* ---- row filter ------------------------------------------------------------ */
 
 /* fbp_sino_filter(sino, filter; extra)
@@ -197,6 +222,11 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
                     const double* zc, int64_t nz, const uint8_t* mask,
                     const float* Hk, int64_t npad, int prec, sgx_plan** out);
 void sgx_plan_destroy(sgx_plan* plan);
+
+/* The cone-beam options above for a cone-beam plan (the plan's backprojection
+ * of every later execute); NULL restores the reference's geometry.  No
+ * counterpart in the reference (see sgx_cbct_back_opts). */
+int sgx_plan_set_cbct_opts(sgx_plan* plan, const sgx_cbct_opts* opts);
 
 /* Per-view weight (na Float32, host) multiplied into the filter's operand,
  * e.g. parker_weight_par for a parallel-beam orbit that is not a multiple of

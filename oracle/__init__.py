@@ -74,6 +74,9 @@ def _declare(L):
             _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp]
         getattr(L, f"orc_sino_filter_{suf}").argtypes = [
             _vp, _i64, _i64, _vp, _i64, _i64, _vp]
+        getattr(L, f"orc_cbct_back_opts_{suf}").argtypes = [
+            _vp, _i64, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _f64, _f64, _int,
+            _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _int, _vp]
         getattr(L, f"orc_cbct_back_rp_{suf}").argtypes = [
             _vp, _vp, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _int,
             _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp]
@@ -317,17 +320,30 @@ def back_fan(sino, sinb, cosb, dsd, dso, source_offset, is_arc, ds, offset, xc, 
 
 
 def cbct_back(proj, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc, zc, mask,
-              ia_skip=1, dtype=np.float64):
-    """``cbct_back_fan!`` (src/fbp3/cbct-back.jl:95-144)."""
+              ia_skip=1, dtype=np.float64, source_zs=None, arc_t_exact=False):
+    """``cbct_back_fan!`` (src/fbp3/cbct-back.jl:95-144).  ``source_zs`` (na
+    source z positions) and ``arc_t_exact`` are the options the reference
+    leaves commented out (:36-59, :180, :188): t' = mag (z - source_zs[ia]), and
+    for the arc detector mag = dsd / sqrt(x_beta^2 + y_beta^2) in t."""
     proj = _F(proj, dtype)
     ns, nt, na = proj.shape
     xc = _F(xc, np.float64); yc = _F(yc, np.float64); zc = _F(zc, np.float64)
     mask = _F(mask, np.uint8)
     betas = _F(betas, np.float64)
     img = np.zeros((len(xc), len(yc), len(zc)), dtype=dtype, order="F")
-    getattr(lib(), "orc_cbct_back_" + _suffix(dtype))(
+    if source_zs is None and not arc_t_exact:
+        getattr(lib(), "orc_cbct_back_" + _suffix(dtype))(
+            _p(proj), ns, nt, na, ia_skip, _p(betas), dsd, dso, ds, dt, offset_s, offset_t,
+            int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask),
+            _p(img))
+        return img
+    zs = None if source_zs is None else _F(source_zs, np.float64)
+    if zs is not None and zs.shape != (na,):
+        raise ValueError("source_zs needs one z per view")
+    getattr(lib(), "orc_cbct_back_opts_" + _suffix(dtype))(
         _p(proj), ns, nt, na, ia_skip, _p(betas), dsd, dso, ds, dt, offset_s, offset_t,
-        int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask), _p(img))
+        int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask),
+        None if zs is None else _p(zs), int(bool(arc_t_exact)), _p(img))
     return img
 
 
@@ -410,9 +426,10 @@ def fbp_fan(sino, Hk, view_weight, sino_weight, ar, dsd, dso, source_offset, is_
 
 
 def fdk(proj, Hk, view_weight, ar, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
-        xc, yc, zc, mask, dtype=np.float64):
-    """``fdk(plan, proj)`` (src/fbp3/fdk.jl:65-81)."""
+        xc, yc, zc, mask, dtype=np.float64, **back_options):
+    """``fdk(plan, proj)`` (src/fbp3/fdk.jl:65-81); ``back_options`` are the
+    backprojector's (``source_zs``, ``arc_t_exact``)."""
     p = _F(proj, dtype) * np.asarray(view_weight, dtype=dtype)
     f = sino_filter(p, np.asarray(Hk, dtype=dtype), dtype=dtype)
     return cbct_back(f, ar, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc, zc, mask,
-                     dtype=dtype)
+                     dtype=dtype, **back_options)

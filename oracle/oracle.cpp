@@ -124,7 +124,12 @@ void back_fan(const T* sino, int64_t nb, int64_t na, int64_t ia_skip,
 
 // ---------------------------------------------------------------------------
 // cbct_back_fan! / cbct_back_fan_voxel   (src/fbp3/cbct-back.jl:95-213)
-// source_offset and source_zs are ignored, as in the reference (:36,:59,:188).
+// source_offset is ignored, as in the reference (:36,:53).  Two options that
+// the reference leaves commented out, off by default:
+//   zs (na source z positions, or null): t' = mag (z - zs[ia]) (:188);
+//   arc_t_exact: for the arc detector, the t magnification of the ray that
+//   meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2), instead of
+//   the flat form dsd / y_beta (:180).
 // ---------------------------------------------------------------------------
 template <class T>
 void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_skip,
@@ -132,7 +137,7 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
                double offset_s, double offset_t, int is_arc,
                const double* xc, int64_t nx, const double* yc, int64_t ny,
                const double* zc, int64_t nz, const uint8_t* mask, T* image,
-               const T* dnext = nullptr)
+               const T* dnext = nullptr, const double* zs = nullptr, int arc_t_exact = 0)
 {
     // dnext (optional, same shape as proj): the stored row-pair differences;
     // the taps of row it+1 are then proj[., it] + dnext[., it] (a checker for
@@ -141,10 +146,11 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
     const T ws = (T)(float)((ns + 1) / 2.0 + (double)(float)offset_s);  // :119
     const T wt = (T)(float)((nt + 1) / 2.0 + (double)(float)offset_t);  // :120
     const T dt_ds = (T)dt / (T)ds, dsd_ds = (T)dsd / (T)ds, dso_ds = (T)dso / (T)ds;
-    std::vector<T> s(na_sub), c(na_sub), xd(nx), yd(ny), zd(nz);
+    std::vector<T> s(na_sub), c(na_sub), xd(nx), yd(ny), zd(nz), zsd(na_sub, (T)0);
     for (int64_t i = 0; i < na_sub; ++i) {
         const T b = (T)betas[i * ia_skip];
         s[i] = std::sin(b); c[i] = std::cos(b);                           // :126-127
+        if (zs) zsd[i] = (T)zs[i * ia_skip] / (T)ds;
     }
     for (int64_t i = 0; i < nx; ++i) xd[i] = (T)xc[i] / (T)ds;            // :129-131
     for (int64_t i = 0; i < ny; ++i) yd[i] = (T)yc[i] / (T)ds;
@@ -170,7 +176,10 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
                 sprime = mag * x_beta;
                 w2 = mag * mag;
             }
-            const T tprime = mag * (z - 0);                               // :189
+            const T magt = (is_arc && arc_t_exact)
+                               ? dsd_ds / std::sqrt(x_beta * x_beta + y_beta * y_beta)
+                               : mag;
+            const T tprime = magt * (z - zsd[ja]);                        // :188-189
             const T bs = sprime + ws;
             const T bt = tprime / dt_ds + wt;                             // :192
             const int64_t is = jfloor(bs), it = jfloor(bt);
@@ -376,6 +385,22 @@ ORC_EXPORT void orc_cbct_back_f32(const float* proj, int64_t ns, int64_t nt, int
     const uint8_t* mask, float* image)
 { cbct_back<float>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
                    xc, nx, yc, ny, zc, nz, mask, image); }
+
+ORC_EXPORT void orc_cbct_back_opts_f64(const double* proj, int64_t ns, int64_t nt, int64_t na,
+    int64_t ia_skip, const double* betas, double dsd, double dso, double ds, double dt,
+    double offset_s, double offset_t, int is_arc,
+    const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc, int64_t nz,
+    const uint8_t* mask, const double* zs, int arc_t_exact, double* image)
+{ cbct_back<double>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
+                    xc, nx, yc, ny, zc, nz, mask, image, nullptr, zs, arc_t_exact); }
+
+ORC_EXPORT void orc_cbct_back_opts_f32(const float* proj, int64_t ns, int64_t nt, int64_t na,
+    int64_t ia_skip, const double* betas, double dsd, double dso, double ds, double dt,
+    double offset_s, double offset_t, int is_arc,
+    const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc, int64_t nz,
+    const uint8_t* mask, const double* zs, int arc_t_exact, float* image)
+{ cbct_back<float>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
+                   xc, nx, yc, ny, zc, nz, mask, image, nullptr, zs, arc_t_exact); }
 
 ORC_EXPORT void orc_cbct_back_rp_f64(const double* proj, const double* dnext, int64_t ns,
     int64_t nt, int64_t na, const double* betas, double dsd, double dso, double ds, double dt,

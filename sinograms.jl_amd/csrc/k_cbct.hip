@@ -8,6 +8,12 @@
 //         w2 = dsd^2 / (x_beta^2 + y_beta^2)
 //   t' = mag z,  bs = s' + ws,  bt = t' / (dt/ds) + wt
 //   if 1 <= floor(bs) < ns and 1 <= floor(bt) < nt: add w2 * bilinear(q)
+// Options (off by default, commented out in the reference at :36-59, :180,
+// :188): t' = mag_t (z - zs[view]) with a per-view source z, and for the arc
+// detector mag_t = dsd / sqrt(x_beta^2 + y_beta^2) (the ray meets the
+// detector cylinder) instead of dsd / y_beta.  Both enter as per-view,
+// z-independent terms: bt = mz z + wtv with mz = mag_t / (dt/ds) and
+// wtv = wt - mz zs (= wt exactly when zs = 0).
 //
 // Storage.  q is column-major (ns, nt, na) with a row pitch of `pitch`
 // elements (>= ns), in FP32, bfloat16, or FP16 with a per-view scale folded
@@ -127,12 +133,28 @@ struct ColView {
     float isf;       // floor(bs)
     float wa, wb;    // w2 (1 - wr), w2 wr
     float mz;        // dbt / dz
+    float wt;        // bt at z = 0: wt - mz zs
 };
 
-template <bool ARC, int FMT>
+// mz and the t offset of one column and view (see the options above); OPT
+// false compiles the options out (the reference's geometry).
+template <bool ARC, bool OPT>
+__device__ __forceinline__ void t_map(float xb, float yb, float mag, float dsd_ds,
+                                      float inv_dt_ds, float wt, const float* zs, int ia,
+                                      int arc_t_exact, float& mz, float& wtv)
+{
+    float m = mag;
+    if (ARC && OPT && arc_t_exact) m = dsd_ds * __builtin_amdgcn_rsqf(xb * xb + yb * yb);
+    mz = m * inv_dt_ds;
+    wtv = wt;
+    if (OPT && zs) wtv = fmaf(-mz, zs[ia], wt);
+}
+
+template <bool ARC, int FMT, bool OPT>
 __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float dsd_ds,
                                             float dso_ds, float ws, float nsf, float inv_dt_ds,
-                                            float vs)
+                                            float vs, float wt, const float* zs, int ia,
+                                            int arc_t_exact)
 {
     ColView r;
     const float xb = x * v.c + y * v.s;
@@ -152,7 +174,7 @@ __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float ds
     r.isf = floorf(bs);
     r.wb = w2 * (bs - r.isf);
     r.wa = w2 - r.wb;
-    r.mz = mag * inv_dt_ds;
+    t_map<ARC, OPT>(xb, yb, mag, dsd_ds, inv_dt_ds, wt, zs, ia, arc_t_exact, r.mz, r.wt);
     return r;
 }
 
@@ -204,14 +226,14 @@ __device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[
 constexpr int kWaves = 4;      // waves per block, each on its own z-slab
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
-template <int ZB, bool ARC, int FMT>
+template <int ZB, bool ARC, int FMT, bool OPT>
 __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds)
+    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -250,12 +272,14 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         const int64_t viewb = view_stride * (int64_t)Fmt<FMT>::esz;
         const char* v0 = static_cast<const char*>(q);
         for (int ia = 0; ia < nviews; ++ia, v0 += viewb) {
-            const ColView cv = col_view<ARC, FMT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
-                                                  inv_dt_ds, FMT == F_F16S ? vscale[ia] : 1.f);
+            const ColView cv = col_view<ARC, FMT, OPT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
+                                                       inv_dt_ds,
+                                                       FMT == F_F16S ? vscale[ia] : 1.f, wt, zs,
+                                                       ia, arc_t_exact);
             if (!cv.hit) continue;                        // the column misses the detector
-            const float blo = fmaf(cv.mz, zlo, wt), bhi = fmaf(cv.mz, zhi, wt);
+            const float blo = fmaf(cv.mz, zlo, cv.wt), bhi = fmaf(cv.mz, zhi, cv.wt);
             const bool fast = blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf;
-            gather_view<ZB, FMT>(cv, z, wt, ntf, fast, view_rsrc(v0, vbytes), rowb, acc);
+            gather_view<ZB, FMT>(cv, z, cv.wt, ntf, fast, view_rsrc(v0, vbytes), rowb, acc);
         }
     }
 #pragma unroll
@@ -312,7 +336,8 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
                                    const float* __restrict__ xds, const float* __restrict__ yds,
                                    const float* __restrict__ zds, int nx, int ny, int nz, int ns,
                                    int nt, float dsd_ds, float dso_ds, float ws, float wt,
-                                   float inv_dt_ds, int is_arc, int4* __restrict__ win)
+                                   float inv_dt_ds, int is_arc, const float* __restrict__ zs,
+                                   int arc_t_exact, int4* __restrict__ win)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nblocks * nviews) return;
@@ -334,8 +359,15 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     }
     // bs and bt are quasi-linear in (x, y) while y_beta > 0 (a ratio of
     // linear functions), so their extremes over the box are at its corners.
+    // With the exact arc t magnification dsd / |(x_beta, y_beta)| that does
+    // not hold for bt; it is bounded by the magnifications dsd / max |.| and
+    // dsd / min y_beta over the corners (|.| >= y_beta; |.| is convex and
+    // y_beta linear, so both extremes are at corners).
     const ViewCS v = cs[ia];
+    const float zsv = zs ? zs[ia] : 0.f;
+    const bool texact = is_arc && arc_t_exact;
     float smin = 3.4e38f, smax = -3.4e38f, tmin = 3.4e38f, tmax = -3.4e38f;
+    float ybmin = 3.4e38f, dmax = 0.f;
     bool ok = true;
     for (int k = 0; k < 4; ++k) {
         const float x = (k & 1) ? x1 : x0, y = (k & 2) ? y1 : y0;
@@ -346,9 +378,20 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         const float sp = is_arc ? dsd_ds * atan2_fast(xb, yb) : mag * xb;
         const float bs = sp + ws;
         smin = fminf(smin, bs); smax = fmaxf(smax, bs);
-        const float mz = mag * inv_dt_ds;
-        const float ta = fmaf(mz, z0, wt), tb = fmaf(mz, z1, wt);
-        tmin = fminf(tmin, fminf(ta, tb)); tmax = fmaxf(tmax, fmaxf(ta, tb));
+        ybmin = fminf(ybmin, yb);
+        dmax = fmaxf(dmax, sqrtf(xb * xb + yb * yb));
+        if (!texact) {
+            const float mz = mag * inv_dt_ds;
+            const float ta = fmaf(mz, z0 - zsv, wt), tb = fmaf(mz, z1 - zsv, wt);
+            tmin = fminf(tmin, fminf(ta, tb)); tmax = fmaxf(tmax, fmaxf(ta, tb));
+        }
+    }
+    if (texact) {
+        const float mzs[2] = {dsd_ds / dmax * inv_dt_ds, dsd_ds / ybmin * inv_dt_ds};
+        for (int k = 0; k < 2; ++k) {
+            const float ta = fmaf(mzs[k], z0 - zsv, wt), tb = fmaf(mzs[k], z1 - zsv, wt);
+            tmin = fminf(tmin, fminf(ta, tb)); tmax = fmaxf(tmax, fmaxf(ta, tb));
+        }
     }
     int4 w = make_int4(0, 0, 0, 0);
     // entry columns is-1, is for 1 <= is = floor(bs) < ns, one bin of margin
@@ -368,14 +411,15 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     win[i] = w;
 }
 
-template <bool ARC, int FMT>
+template <bool ARC, int FMT, bool OPT>
 __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds, const int4* __restrict__ win)
+    float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
+    int arc_t_exact)
 {
     using L = Lds<FMT>;
     constexpr int kLP = L::pitch;
@@ -480,7 +524,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
 
     // Per-view lane geometry: bs, w2, mz and the lane's window coordinates.
     struct LaneView {
-        float bs, w2, mz, blo, bhi;
+        float bs, w2, mz, wt, blo, bhi;
         bool inwin;
         int ce;
     };
@@ -501,7 +545,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         }
         if (FMT == F_F16S) r.w2 *= vscale[ia];
         r.bs = sp + ws;
-        r.mz = mag * inv_dt_ds;
+        t_map<ARC, OPT>(xb, yb, mag, dsd_ds, inv_dt_ds, wt, zs, ia, arc_t_exact, r.mz, r.wt);
         // Window coordinates of the lane's taps: entry columns ce, ce+1 and
         // entry rows (int)bt - 1 - rlo for bt from blo to bhi.  The test
         // implies bs, blo, bhi >= 1 (truncation is floor there) and the
@@ -510,8 +554,8 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         r.inwin = true;
         r.blo = r.bhi = 0.f;
         if (check) {
-            r.blo = fmaf(r.mz, zlo, wt);
-            r.bhi = fmaf(r.mz, zhi, wt);
+            r.blo = fmaf(r.mz, zlo, r.wt);
+            r.bhi = fmaf(r.mz, zhi, r.wt);
             const int rl = (int)r.blo - 1 - w.y, rh = (int)r.bhi - 1 - w.y;
             const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
             r.inwin = (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
@@ -543,7 +587,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
                     float wu[kSB];
 #pragma unroll
                     for (int j = 0; j < kSB; ++j) {
-                        const float bt = fmaf(lv.mz, z[h + j], wt);
+                        const float bt = fmaf(lv.mz, z[h + j], lv.wt);
                         wu[j] = __builtin_amdgcn_fractf(bt);
                         // one in-bounds offset (the 32-bit sum wraps to the entry's offset)
                         const char* p = lds_bytes +
@@ -586,9 +630,10 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
             cv.wb = lv.w2 * (lv.bs - cv.isf);
             cv.wa = lv.w2 - cv.wb;
             cv.mz = lv.mz;
+            cv.wt = lv.wt;
             const float tlo = fminf(lv.blo, lv.bhi), thi = fmaxf(lv.blo, lv.bhi);
             const bool fast = tlo >= 1.f && thi < ntf;
-            gather_view<kSZ, FMT>(cv, z, wt, ntf, fast, view_rsrc(qb + ia * viewb, vbytes),
+            gather_view<kSZ, FMT>(cv, z, lv.wt, ntf, fast, view_rsrc(qb + ia * viewb, vbytes),
                                   rowb, acc);
         }
     }
@@ -608,42 +653,49 @@ int64_t staged_blocks(const CbctArgs& a)
     return tiles * ((a.nz + kSZ - 1) / kSZ);
 }
 
-template <int ZB, bool ARC, int FMT>
+template <int ZB, bool ARC, int FMT, bool OPT>
 void launch_gather(const CbctArgs& a, hipStream_t st)
 {
     const int64_t tiles = (int64_t)((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
     const int64_t zblocks = (a.nz + ZB * kWaves - 1) / (ZB * kWaves);
     if (tiles * zblocks >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
-    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT>), dim3((unsigned)(tiles * zblocks)),
+    hipLaunchKernelGGL((cbct_back_kernel<ZB, ARC, FMT, OPT>), dim3((unsigned)(tiles * zblocks)),
                        dim3(64 * kWaves), 0, st, a.q, a.ns, a.nt, a.pitch, a.nviews,
                        a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz,
-                       a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds);
+                       a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.zs,
+                       a.arc_t_exact ? 1 : 0);
     SGX_HIP(hipGetLastError());
 }
 
-template <bool ARC, int FMT>
+template <bool ARC, int FMT, bool OPT>
 void launch_staged(const CbctArgs& a, hipStream_t st)
 {
     const int64_t nb = staged_blocks(a);
     if (nb >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
-    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT>), dim3((unsigned)nb), dim3(256), 0, st,
+    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT>), dim3((unsigned)nb), dim3(256), 0, st,
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
-                       a.ws, a.wt, a.inv_dt_ds, a.win);
+                       a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0);
     SGX_HIP(hipGetLastError());
+}
+
+template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
+{
+    if (a.win) {
+        if (a.is_arc) launch_staged<true, FMT, OPT>(a, st);
+        else launch_staged<false, FMT, OPT>(a, st);
+    } else {
+        if (a.is_arc) launch_gather<8, true, FMT, OPT>(a, st);
+        else launch_gather<8, false, FMT, OPT>(a, st);
+    }
 }
 
 template <int FMT> void launch_fmt(const CbctArgs& a, hipStream_t st)
 {
-    if (a.win) {
-        if (a.is_arc) launch_staged<true, FMT>(a, st);
-        else launch_staged<false, FMT>(a, st);
-    } else {
-        if (a.is_arc) launch_gather<8, true, FMT>(a, st);
-        else launch_gather<8, false, FMT>(a, st);
-    }
+    if (a.zs || a.arc_t_exact) launch_opt<FMT, true>(a, st);
+    else launch_opt<FMT, false>(a, st);
 }
 
 __global__ void view_trig_kernel(const double* __restrict__ betas, int64_t n, int64_t stride,
@@ -677,7 +729,8 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
     if (n <= 0) return;
     hipLaunchKernelGGL(cbct_window_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        (int)nb, a.nviews, a.cs, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.ns, a.nt,
-                       a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, win);
+                       a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, a.zs,
+                       a.arc_t_exact ? 1 : 0, win);
     SGX_HIP(hipGetLastError());
 }
 

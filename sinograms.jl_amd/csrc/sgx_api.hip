@@ -310,14 +310,15 @@ int sgx_back_fan(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_
     });
 }
 
-int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
-                  int64_t ia_skip, const double* betas, double dsd, double dso,
-                  double ds, double dt, double offset_s, double offset_t, int is_arc,
-                  const double* xc, int64_t nx, const double* yc, int64_t ny,
-                  const double* zc, int64_t nz, const uint8_t* mask, float* image,
-                  uint32_t flags)
+namespace {
+void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                    int64_t ia_skip, const double* betas, double dsd, double dso,
+                    double ds, double dt, double offset_s, double offset_t, int is_arc,
+                    const double* xc, int64_t nx, const double* yc, int64_t ny,
+                    const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                    uint32_t flags, const sgx_cbct_opts* opts)
 {
-    return guard(ctx, [&] {
+    {
         require(proj && betas && xc && yc && zc && image, SGX_ERR_ARG, "null pointer argument");
         require(ns > 1 && nt > 1 && na > 0 && nx > 0 && ny > 0 && nz > 0 && ia_skip >= 1,
                 SGX_ERR_ARG, "bad size");
@@ -338,6 +339,14 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         const size_t o_y = pack.add(yv.data(), yv.size() * 4);
         const size_t o_z = pack.add(zv.data(), zv.size() * 4);
         const size_t o_cs = pack.add(b.data(), nv * sizeof(sgx::ViewCS));   // space only
+        const bool has_zs = opts && opts->source_zs;
+        std::vector<float> zsv;
+        if (has_zs) {
+            zsv.resize(nv);
+            for (int64_t i = 0; i < nv; ++i)
+                zsv[i] = (float)((float)opts->source_zs[i * ia_skip] / (float)ds);
+        }
+        const size_t o_zs = has_zs ? pack.add(zsv.data(), nv * 4) : 0;
         unsigned char* d = pack.upload(ctx, W_TABLES);
         auto* cs = reinterpret_cast<sgx::ViewCS*>(d + o_cs);
         sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), nv, 1, cs, ctx->stream);
@@ -388,6 +397,8 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         a.wt = (float)((nt + 1) / 2.0 + (double)(float)offset_t);
         a.inv_dt_ds = 1.0f / ((float)dt / (float)ds);
         a.is_arc = is_arc != 0;
+        a.zs = has_zs ? reinterpret_cast<const float*>(d + o_zs) : nullptr;
+        a.arc_t_exact = opts && opts->arc_t_exact;
         if (ctx->cbct_kernel == 0) {
             int4* win = static_cast<int4*>(
                 ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
@@ -396,6 +407,33 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
         }
         sgx::launch_cbct_back(a, ctx->stream);
         finish(ctx, image, img, 4 * nvox, flags);
+    }
+}
+}  // namespace
+
+int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                  int64_t ia_skip, const double* betas, double dsd, double dso,
+                  double ds, double dt, double offset_s, double offset_t, int is_arc,
+                  const double* xc, int64_t nx, const double* yc, int64_t ny,
+                  const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                  uint32_t flags)
+{
+    return guard(ctx, [&] {
+        cbct_back_impl(ctx, proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s,
+                       offset_t, is_arc, xc, nx, yc, ny, zc, nz, mask, image, flags, nullptr);
+    });
+}
+
+int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
+                       int64_t ia_skip, const double* betas, double dsd, double dso,
+                       double ds, double dt, double offset_s, double offset_t, int is_arc,
+                       const double* xc, int64_t nx, const double* yc, int64_t ny,
+                       const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                       uint32_t flags, const sgx_cbct_opts* opts)
+{
+    return guard(ctx, [&] {
+        cbct_back_impl(ctx, proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s,
+                       offset_t, is_arc, xc, nx, yc, ny, zc, nz, mask, image, flags, opts);
     });
 }
 
@@ -548,6 +586,8 @@ struct sgx_plan {
           *wview = nullptr, *s_axis = nullptr, *t_axis = nullptr;
     float* wview_owned = nullptr;
     float* ptab = nullptr;          // short-scan Parker weights (ns, na), formed once in FP64
+    float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
+    bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
 
@@ -557,6 +597,7 @@ struct sgx_plan {
             if (e) (void)hipEventDestroy(e);
         if (wview_owned) (void)hipFree(wview_owned);
         if (ptab) (void)hipFree(ptab);
+        if (zs) (void)hipFree(zs);
         if (tables) (void)hipFree(tables);
         if (mask) (void)hipFree(mask);
         if (q) (void)hipFree(q);
@@ -635,6 +676,8 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     a.inv_dt_ds = 1.0f / ((float)G.dt / (float)G.ds);
     a.is_arc = P.is_arc();
     a.win = P.win;
+    a.zs = P.zs;
+    a.arc_t_exact = P.arc_t_exact;
     return a;
 }
 
@@ -945,6 +988,35 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview)
         SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
         SGX_HIP(hipMemcpy(P.wview_owned, wview, 4 * na, hipMemcpyHostToDevice));
         P.wview = P.wview_owned;
+    });
+}
+
+int sgx_plan_set_cbct_opts(sgx_plan* plan, const sgx_cbct_opts* opts)
+{
+    if (!plan) return SGX_ERR_ARG;
+    return guard(plan->ctx, [&] {
+        sgx_plan& P = *plan;
+        require(P.is3d(), SGX_ERR_ARG, "cone-beam options need a cone-beam plan");
+        const int64_t na = P.geom.na;
+        SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
+        if (opts && opts->source_zs) {
+            std::vector<float> z(na);
+            for (int64_t i = 0; i < na; ++i)
+                z[i] = (float)((float)opts->source_zs[i] / (float)P.geom.ds);
+            if (!P.zs) {
+                void* p = nullptr;
+                SGX_HIP(hipMalloc(&p, 4 * na));
+                P.zs = static_cast<float*>(p);
+            }
+            SGX_HIP(hipMemcpy(P.zs, z.data(), 4 * na, hipMemcpyHostToDevice));
+        } else if (P.zs) {
+            SGX_HIP(hipFree(P.zs));
+            P.zs = nullptr;
+        }
+        P.arc_t_exact = opts && opts->arc_t_exact;
+        // the staged backprojector's windows follow the t mapping
+        sgx::launch_cbct_windows(cbct_args_of(P, nullptr), P.win, plan->ctx->stream);
+        SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
     });
 }
 

@@ -90,6 +90,12 @@ struct CbctArgs {
     // per-(block, view) detector windows of the LDS-staged kernel, from
     // launch_cbct_windows; null selects the gather kernel
     const int4* win;
+    // options the reference leaves commented out (cbct-back.jl:36-59, 180,
+    // 188): per used view the source z / ds (t' = mag (z - zs)), or null for
+    // zs = 0; and for the arc detector the t magnification of the ray that
+    // meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2)
+    const float* zs;
+    bool arc_t_exact;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 // Entries of the window table the staged kernel takes for these arguments.

@@ -6,8 +6,8 @@ the HIP library whose C ABI is include/sgx.h.  The device work always runs
 in libsgx; there is no CPU fallback.
 """
 
-from .geom import (CtFan, CtFanArc, CtFanFlat, ImageGeom, SinoFan, SinoFanArc, SinoFanFlat,
-                   SinoGeom, SinoPar)
+from .geom import (CtFan, CtFanArc, CtFanFlat, CtSourceCircle, CtSourceHelix, ImageGeom,
+                   SinoFan, SinoFanArc, SinoFanFlat, SinoGeom, SinoPar)
 from .recon import (Boxcar, FBPNormalPlan, FDKplan, Hamming, Hann, NoWindow, SizeMismatch,
                     Window, WindowVect, _angle_weights, _fbp_weights, _fdk_weights, cbct_back,
                     fbp, fbp_back, fbp_filter, fbp_ramp, fbp_sino_filter, fbp_sino_weight,
@@ -22,7 +22,7 @@ from ._lib import LIB_PATH, SgxError
 
 __all__ = [
     "SinoGeom", "SinoPar", "SinoFan", "SinoFanArc", "SinoFanFlat", "CtFan", "CtFanArc",
-    "CtFanFlat", "ImageGeom", "plan_fbp", "fbp", "fdk", "fbp_back", "cbct_back",
+    "CtFanFlat", "CtSourceCircle", "CtSourceHelix", "ImageGeom", "plan_fbp", "fbp", "fdk", "fbp_back", "cbct_back",
     "fbp_sino_filter", "fbp_filter", "fbp_ramp", "ramp_flat", "ramp_arc", "fbp_window",
     "Window", "NoWindow", "Boxcar", "Hamming", "Hann", "WindowVect", "parker_weight",
     "parker_weight_par", "parker_weight_fan_short", "fbp_sino_weight", "fdk_weight_cyl",

@@ -39,7 +39,8 @@ SGX_PREC_F16S = 2
 EXPORTS = (
     "sgx_version", "sgx_ctx_create", "sgx_ctx_destroy", "sgx_last_error",
     "sgx_ctx_set_stream", "sgx_ctx_synchronize", "sgx_back_par", "sgx_back_fan",
-    "sgx_cbct_back", "sgx_sino_filter", "sgx_parker_fan_short", "sgx_plan_create",
+    "sgx_cbct_back", "sgx_cbct_back_opts", "sgx_plan_set_cbct_opts", "sgx_sino_filter",
+    "sgx_parker_fan_short", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
     "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
@@ -63,6 +64,10 @@ class SgxGeom(ctypes.Structure):
         ("source_offset", ctypes.c_double), ("gamma_max", ctypes.c_double),
         ("angle_weight", ctypes.c_double),
     ]
+
+
+class SgxCbctOpts(ctypes.Structure):
+    _fields_ = [("source_zs", ctypes.c_void_p), ("arc_t_exact", ctypes.c_int32)]
 
 
 class SgxError(RuntimeError):
@@ -110,6 +115,8 @@ def _declare(L):
                                vp, i64, vp, i64, vp, vp, u32]
     L.sgx_cbct_back.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64, f64,
                                 i32, vp, i64, vp, i64, vp, i64, vp, vp, u32]
+    L.sgx_cbct_back_opts.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64,
+                                     f64, i32, vp, i64, vp, i64, vp, i64, vp, vp, u32, vp]
     L.sgx_sino_filter.argtypes = [vp, vp, i64, i64, vp, i64, i64, i32, vp, u32]
     L.sgx_parker_fan_short.argtypes = [vp, i64, i64, vp, vp, f64, vp, u32]
     L.sgx_plan_create.argtypes = [vp, ctypes.POINTER(SgxGeom), vp, vp, i64, vp, i64, vp, i64,
@@ -117,6 +124,7 @@ def _declare(L):
     L.sgx_plan_destroy.argtypes = [vp]
     L.sgx_plan_destroy.restype = None
     L.sgx_plan_set_view_weight.argtypes = [vp, vp]
+    L.sgx_plan_set_cbct_opts.argtypes = [vp, vp]
     L.sgx_plan_execute.argtypes = [vp, vp, vp, u32]
     L.sgx_plan_filtered.argtypes = [vp]
     L.sgx_plan_filtered.restype = vp

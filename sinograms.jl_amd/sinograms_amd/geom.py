@@ -7,6 +7,9 @@ path consumes:
   angles, _ar, _dso, _gamma, _rfov ... src/geom/common.jl:27-99
   _s, _w                               src/geom/sino-geom.jl:18-20
   _s, _t, _ws, _wt                     src/geom/ct-geom.jl:19-23
+  CtSourceCircle, CtSourceHelix        src/geom/ct-source.jl:22-57
+  _cone_angle, _zfov, _source_dz_per_view, _source_zs
+                                       src/geom/ct-geom.jl:115-140
 The `:short` constructors are spelled ``SinoFanArc.short(...)``.
 
 Scalars are Python floats.  The reference's Float32 geometry records are
@@ -68,6 +71,28 @@ class SinoGeom:
 
     def ones(self, dtype=np.float32):
         return np.ones(self.dims(), dtype=dtype, order="F")
+
+    def cone_angle(self) -> float:
+        """``_cone_angle`` = atan(nt dt / 2 / dsd) (ct-geom.jl:115)."""
+        return math.atan(self.nt * self.dt / 2 / self.dsd)
+
+    def zfov(self) -> float:
+        """``_zfov`` = dso / dsd * nt * dt (ct-geom.jl:119)."""
+        return self.dso / self.dsd * self.nt * self.dt
+
+    def source_dz_per_view(self) -> float:
+        """``_source_dz_per_view`` (ct-geom.jl:122-131): 0 for a circle; for a
+        helix pitch * zfov / (na * 360 / orbit), and 0 for one view."""
+        if not isinstance(self.src, CtSourceHelix) or self.na == 1:
+            return 0.0
+        na_per_360 = self.na * (360 / self.orbit)
+        return self.src.pitch * self.zfov() / na_per_360
+
+    def source_zs(self) -> np.ndarray:
+        """``_source_zs`` (ct-geom.jl:134-140): the source z of every view."""
+        if not isinstance(self.src, CtSourceHelix):
+            return np.zeros(self.na)
+        return np.arange(self.na) * self.source_dz_per_view() + self.src.source_z0
 
     def zeros(self, dtype=np.float32):
         return np.zeros(self.dims(), dtype=dtype, order="F")
@@ -153,8 +178,27 @@ class SinoFanFlat(SinoFan):
 
 
 @dataclasses.dataclass(frozen=True)
+class CtSourceCircle:
+    """Circular source trajectory (ct-source.jl:22-26)."""
+
+
+@dataclasses.dataclass(frozen=True)
+class CtSourceHelix:
+    """Helical source trajectory of constant pitch (ct-source.jl:29-57):
+    ``pitch`` is a unitless fraction of the z field of view per 360 degrees
+    (stored as Float32, as the reference does), ``source_z0`` the source z of
+    the first view, in the units of the detector."""
+    pitch: float = 0.0
+    source_z0: float = 0.0
+
+    def __post_init__(self):
+        object.__setattr__(self, "pitch", _f32(self.pitch))
+
+
+@dataclasses.dataclass(frozen=True)
 class CtFan:
-    """3-D cone-beam geometry with a circular source (type3.jl:145-184)."""
+    """3-D cone-beam geometry (type3.jl:145-184); ``src`` is its source
+    trajectory, circular by default."""
     ns: int = 128
     nt: int = 64
     ds: float = 1.0
@@ -167,10 +211,13 @@ class CtFan:
     source_offset: float = 0.0
     dsd: float = 0.0
     dod: float = 0.0
+    src: object = CtSourceCircle()
 
     is_arc = True
 
     def __post_init__(self):
+        if not isinstance(self.src, (CtSourceCircle, CtSourceHelix)):
+            raise TypeError("src must be a CtSourceCircle or a CtSourceHelix")
         if self.dt == 0.0:
             object.__setattr__(self, "dt", self.ds)
         if self.dsd == 0.0:
@@ -232,6 +279,28 @@ class CtFan:
 
     def ones(self, dtype=np.float32):
         return np.ones(self.dims(), dtype=dtype, order="F")
+
+    def cone_angle(self) -> float:
+        """``_cone_angle`` = atan(nt dt / 2 / dsd) (ct-geom.jl:115)."""
+        return math.atan(self.nt * self.dt / 2 / self.dsd)
+
+    def zfov(self) -> float:
+        """``_zfov`` = dso / dsd * nt * dt (ct-geom.jl:119)."""
+        return self.dso / self.dsd * self.nt * self.dt
+
+    def source_dz_per_view(self) -> float:
+        """``_source_dz_per_view`` (ct-geom.jl:122-131): 0 for a circle; for a
+        helix pitch * zfov / (na * 360 / orbit), and 0 for one view."""
+        if not isinstance(self.src, CtSourceHelix) or self.na == 1:
+            return 0.0
+        na_per_360 = self.na * (360 / self.orbit)
+        return self.src.pitch * self.zfov() / na_per_360
+
+    def source_zs(self) -> np.ndarray:
+        """``_source_zs`` (ct-geom.jl:134-140): the source z of every view."""
+        if not isinstance(self.src, CtSourceHelix):
+            return np.zeros(self.na)
+        return np.arange(self.na) * self.source_dz_per_view() + self.src.source_z0
 
 
 @dataclasses.dataclass(frozen=True)

@@ -33,7 +33,7 @@ import math
 
 import numpy as np
 
-from .geom import CtFan, ImageGeom, SinoFan, SinoGeom, SinoPar
+from .geom import CtFan, CtSourceCircle, ImageGeom, SinoFan, SinoGeom, SinoPar
 
 
 def cb_flat_to_par(s, t, beta, dso: float, dod: float):
@@ -77,6 +77,8 @@ def rays(rg):
         g = rg.gamma(s)
         return rg.dso * np.sin(g), g + b
     if isinstance(rg, CtFan):
+        if not isinstance(rg.src, CtSourceCircle):
+            raise ValueError("non-circular not done")          # ct-geom.jl:99
         s, t, b = np.meshgrid(rg.s(), rg.t(), rg.ar(), indexing="ij")
         f = cb_arc_to_par if rg.is_arc else cb_flat_to_par
         return f(s, t, b, rg.dso, rg.dod)

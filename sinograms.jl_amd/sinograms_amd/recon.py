@@ -390,8 +390,25 @@ def fbp_back(rg, ig: ImageGeom, sino, ia_skip: int = 1):
     return img
 
 
-def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1):
-    """Voxel-driven cone-beam backprojection (cbct-back.jl:25-45)."""
+def _cbct_opts(rg: CtFan, use_source_zs: bool, arc_t_exact: bool):
+    """The cone-beam options (None: the reference's geometry) and the array
+    they point to, which must stay alive for the call."""
+    if not use_source_zs and not arc_t_exact:
+        return None, None
+    zs = _f64(rg.source_zs()) if use_source_zs else None
+    return _lib.SgxCbctOpts(_dptr(zs), int(bool(arc_t_exact))), zs
+
+
+def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1, use_source_zs: bool = False,
+              arc_t_exact: bool = False):
+    """Voxel-driven cone-beam backprojection (cbct-back.jl:25-45).
+
+    Options the reference keeps commented out (:36-59, :180, :188), off by
+    default: ``use_source_zs`` backprojects with the source z of every view,
+    ``rg.source_zs()`` (a helical ``CtSourceHelix`` trajectory), t' = mag
+    (z - zs); ``arc_t_exact`` uses, for the arc detector, the t magnification
+    of the ray that meets the detector cylinder, dsd / sqrt(x_b^2 + y_b^2).
+    """
     a = _Arr(proj)
     if a.shape != rg.dims() or len(ig.dims) != 3:
         raise SizeMismatch("size mismatch")
@@ -401,10 +418,14 @@ def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1):
     mask = _mask_arr(ig, a)
     img, iptr = a.out(ig.dims)
     ctx = _ctx_for(a)
-    ctx.check(_lib.load().sgx_cbct_back(
-        ctx.h, a.ptr, ns, nt, na, ia_skip, _dptr(betas), rg.dsd, rg.dso, rg.ds, rg.dt,
-        rg.offset_s, rg.offset_t, int(rg.is_arc), _dptr(xc), len(xc), _dptr(yc), len(yc),
-        _dptr(zc), len(zc), _mask_ptr(mask), iptr, a.flags))
+    opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact)
+    args = (ctx.h, a.ptr, ns, nt, na, ia_skip, _dptr(betas), rg.dsd, rg.dso, rg.ds, rg.dt,
+            rg.offset_s, rg.offset_t, int(rg.is_arc), _dptr(xc), len(xc), _dptr(yc), len(yc),
+            _dptr(zc), len(zc), _mask_ptr(mask), iptr, a.flags)
+    if opts is None:
+        ctx.check(_lib.load().sgx_cbct_back(*args))
+    else:
+        ctx.check(_lib.load().sgx_cbct_back_opts(*args, ctypes.byref(opts)))
     return img
 
 
@@ -575,7 +596,8 @@ def _fdk_weights(rg, device=0):
 
 
 def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1: bool = True,
-             filter=None, precision: str = "fp32", device: int = 0, **_ignored):
+             filter=None, precision: str = "fp32", device: int = 0,
+             use_source_zs: bool = False, arc_t_exact: bool = False, **_ignored):
     """``plan_fbp`` (plan2.jl:106-138 for SinoGeom, plan3.jl:60-72 for CtGeom).
 
     precision: "fp32" (FP32 filter and storage), or for cone-beam plans the
@@ -583,6 +605,9 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
     projections) and "f16s" (split-FP16x3 MFMA filter with power-of-two
     scaled operands, FP16 filtered projections scaled per view by a power of
     two); backprojection always accumulates in FP32.
+
+    use_source_zs, arc_t_exact (cone-beam plans): the backprojection options
+    of ``cbct_back``, which the reference keeps commented out; off by default.
     """
     nb = rg.nb if isinstance(rg, SinoGeom) else rg.ns
     npad = npad or _nextpow2(nb + 1)
@@ -590,7 +615,13 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
         filter = fbp_filter(rg, npad=npad, window=window or Window(), decon1=decon1)
     if isinstance(rg, CtFan):
         vw = np.float32(_angle_weights(rg.ar(f32=True)))   # the full array is on the device
-        return FDKplan(rg, ig, filter, vw, precision, device)
+        plan = FDKplan(rg, ig, filter, vw, precision, device)
+        opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact)
+        if opts is not None:
+            plan._ctx.check(_lib.load().sgx_plan_set_cbct_opts(plan.h, ctypes.byref(opts)))
+        return plan
+    if use_source_zs or arc_t_exact:
+        raise ValueError("use_source_zs and arc_t_exact apply to cone-beam geometries")
     vw = parker_weight_par(rg.orbit, rg.angles() - rg.orbit_start) if isinstance(rg, SinoPar) \
         else np.ones((1, 1), dtype=np.float32)
     return FBPNormalPlan(rg, ig, filter, vw, precision, device)

@@ -102,6 +102,66 @@ def test_cbct_back_detector_edges(sg, orc, cbct_kernel):
 
 
 @pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("helix,exact", [(True, False), (False, True), (True, True)])
+def test_cbct_back_options_vs_oracle(sg, orc, torch_cuda, cbct_kernel, arc, helix, exact):
+    # the source z of every view (a helix) and the exact arc-detector t
+    # magnification, which the reference keeps commented out
+    # (cbct-back.jl:36-59, 180, 188); many voxels leave the detector in t
+    _say("cbct_back options", cbct_kernel, arc, helix, exact)
+    rng = np.random.default_rng(12)
+    src = sg.CtSourceHelix(pitch=0.8, source_z0=-6.0) if helix else sg.CtSourceCircle()
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=48, nt=36, ds=1.1, dt=1.3, na=30,
+                                                 offset_s=0.25, offset_t=-0.5, dsd=120.0,
+                                                 dod=50.0, src=src)
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    ig = sg.ImageGeom((29, 23, 26), (1.0, 0.9, 1.1), (0.3, 0.0, -0.2))
+    xc, yc, zc = ig.axes()
+    zs = rg.source_zs() if helix else None
+    for ia_skip in (1, 2):
+        got = sg.cbct_back(proj, rg, ig, ia_skip=ia_skip, use_source_zs=helix,
+                           arc_t_exact=exact)
+        ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, rg.offset_s,
+                            rg.offset_t, arc, xc, yc, zc, ig.mask, ia_skip=ia_skip,
+                            dtype=np.float32, source_zs=zs, arc_t_exact=exact)
+        assert rel_rmse(got, ref) < TOL32, ia_skip
+    base = sg.cbct_back(proj, rg, ig, ia_skip=2)
+    if helix or arc:
+        assert rel_rmse(base, got) > 1e-3           # the options change the geometry
+    else:
+        assert np.array_equal(base, got)             # arc_t_exact leaves a flat detector
+
+
+@pytest.mark.parametrize("prec", ["fp32", "f16s"])
+def test_fdk_plan_options_vs_oracle(sg, orc, torch_cuda, prec):
+    _say("fdk plan options", prec)
+    rng = np.random.default_rng(13)
+    rg = sg.CtFanArc(ns=64, nt=40, ds=1.0, dt=1.2, na=32,
+                     src=sg.CtSourceHelix(pitch=0.5, source_z0=-3.0))
+    ig = sg.ImageGeom((30, 28, 26), (1.0, 1.0, 1.0))
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    opts = dict(use_source_zs=True, arc_t_exact=True)
+    plan = sg.plan_fbp(rg, ig, precision=prec, **opts)
+    got = sg.fdk(plan, proj)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, True)[:, :, None] * \
+        orc.angle_weight(ar)
+    xc, yc, zc = ig.axes()
+    args = (plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True, xc, yc, zc,
+            ig.mask)
+    back = dict(source_zs=rg.source_zs(), arc_t_exact=True)
+    if prec == "fp32":
+        e32 = rel_rmse(got, orc.fdk(proj, *args, dtype=np.float32, **back))
+        _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}")
+        assert e32 < TOL32
+    else:
+        e = rel_rmse(got, sg.fdk(sg.plan_fbp(rg, ig, **opts), proj))
+        _say(f"  F16S vs FP32 with the options {e:.3g}")
+        assert e < 1e-3
+    # the plan's default is the reference's geometry
+    assert rel_rmse(sg.fdk(sg.plan_fbp(rg, ig, precision=prec), proj), got) > 1e-3
+
+
+@pytest.mark.parametrize("arc", [False, True])
 def test_cbct_back_staged_window_overflow(sg, orc, cbct_kernel, arc):
     # Fine detector pitch: a 16 x 16 x 16 block's footprint (~75 bins x ~50
     # rows) exceeds the staged window, so some lanes read from the window and

@@ -320,6 +320,37 @@ def test_fdk_ellipsoid_volume(orc, arc):
     assert abs(vol - vt) / vt < 3e-3
 
 
+
+@pytest.mark.parametrize("arc", [True, False])
+@pytest.mark.parametrize("exact", [False, True])
+def test_cbct_back_t_mapping_closed_form(orc, arc, exact):
+    # One view of projections that are linear in t (and constant in s):
+    # bilinear interpolation is exact, so backprojecting them and dividing by
+    # the backprojection of ones gives the detector t each voxel maps to.
+    # Checked against world geometry: the source at (-dso sin b, dso cos b, zs);
+    # the reference form scales z - zs by dsd over the depth along the
+    # central ray, the exact arc form by dsd over the in-plane distance from
+    # the source (where the ray meets the detector cylinder).
+    ns, nt, ds, dt, dsd, dso = 80, 64, 1.0, 1.25, 300.0, 200.0
+    beta, zsrc = 0.7, 3.5
+    t_axis = dt * (np.arange(nt) - (nt - 1) / 2)
+    lin = np.broadcast_to(t_axis[None, :, None], (ns, nt, 1)).copy()
+    ones = np.ones((ns, nt, 1))
+    xc, yc, zc = _grid(9, 4.0), _grid(7, 5.0), _grid(5, 6.0)
+    mask = np.ones((9, 7, 5), bool)
+    kw = dict(source_zs=np.array([zsrc]), arc_t_exact=exact)
+    args = ([beta], dsd, dso, ds, dt, 0.0, 0.0, arc, xc, yc, zc, mask)
+    t_got = orc.cbct_back(lin, *args, **kw) / orc.cbct_back(ones, *args, **kw)
+    X, Y, Z = np.meshgrid(xc, yc, zc, indexing="ij")
+    sx, sy = -dso * math.sin(beta), dso * math.cos(beta)
+    depth = ((sx - X) * sx + (sy - Y) * sy) / dso         # along the central ray
+    dist = np.hypot(X - sx, Y - sy)                        # in-plane, source to voxel
+    scale = dist if (arc and exact) else depth
+    assert np.allclose(t_got, dsd * (Z - zsrc) / scale, rtol=0, atol=1e-9)
+    # without the options: zs = 0 and the reference form
+    t_ref = orc.cbct_back(lin, *args) / orc.cbct_back(ones, *args)
+    assert np.allclose(t_ref, dsd * Z / depth, rtol=0, atol=1e-9)
+
 # --- distance-driven pair (src/sys2/bdd_2d.jl) ----------------------------------------
 
 def _bdd_geo(nPix=32, nDet=57, na=24):
