@@ -272,6 +272,38 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
     return out
 
 
+def phantom_leg(sg, device, steps):
+    """The analytic projector (FP64, one thread per ray): the FDK workload's
+    projections (12 ellipsoids onto 512 x 384 x 360 flat-detector rays) and
+    the 888 x 984 arc fan sinogram of 15 ellipses, by libsgx's kernels and, for
+    comparison, by the torch elementwise form on the same GPU."""
+    import numpy as np
+    from sinograms_amd import phantom as ph
+    rg3 = sg.CtFanFlat(ns=512, nt=384, ds=1.2, dt=1.2, na=360, dsd=1200.0, dod=400.0)
+    ell3 = ph.random_ellipsoids(0, 12, 256.0)
+    rg2 = sg.SinoFanArc(nb=888, na=984, d=1.0239, dsd=949.0, dod=949.0 - 541.0)
+    ell2 = ph.random_ellipses(1, 15, 500.0)
+    idx = np.arange(rg3.na)
+    cases = {
+        "cone_512x384x360_12_ellipsoids": (
+            rg3.ns * rg3.nt * rg3.na, len(ell3),
+            lambda: ph.proj_ellipsoids(ell3, rg3, device),
+            lambda: ph._proj_ellipsoids_torch(ell3, rg3, device, idx, None, 64)),
+        "fan_888x984_15_ellipses": (
+            rg2.nb * rg2.na, len(ell2),
+            lambda: ph.sino_ellipses(ell2, rg2, device),
+            lambda: ph._sino_ellipses_torch(ell2, rg2, device)),
+    }
+    out = {}
+    for name, (rays, nc, hip, tch) in cases.items():
+        th = time_steps(hip, steps, 1)
+        tt = time_steps(tch, max(1, steps // 3), 1)
+        out[name] = {"hip_ms": th * 1e3, "torch_ms": tt * 1e3, "rays_per_s": rays / th,
+                     "ray_components_per_s": rays * nc / th, "dtype": "f64"}
+        log(f"  analytic projector {name}: {th * 1e3:.3f} ms (torch form {tt * 1e3:.3f} ms)")
+    return out
+
+
 def dd_leg(sg, device, steps):
     """The distance-driven projector pair (FP64, as the reference computes
     it) at the learned form's size: a 512 x 512 image of 0.98 mm onto 984
@@ -460,6 +492,8 @@ def main():
         result["slices"] = slice_legs(sg, dev, args.steps, args.warmup, not args.no_cpu)
         log("timing the distance-driven projector ...")
         result["dd_projector"] = dd_leg(sg, dev, max(3, min(args.steps, 10)))
+        log("timing the analytic projector ...")
+        result["analytic_projector"] = phantom_leg(sg, dev, max(3, min(args.steps, 10)))
     if not args.no_batch:
         log("batch leg ...")
         result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))

@@ -192,6 +192,34 @@ int sgx_backproject_bdd(sgx_ctx* ctx, const double* sino, int64_t nPix, int64_t 
                         double pSize, double dSize, int ref_alias, double* image,
                         uint32_t flags);
 
+/* ---- analytic phantom projections (test and training data) -------------- */
+
+/* Line integrals in Float64 of ellipses, rows (cx, cy, a, b, theta, rho)
+ * (nell <= 512), along the rays of a 2-D geometry: kind SGX_SINO_PAR,
+ * (r, phi) = (s[ib], betas[ia]) (src/geom/sino-geom.jl:99-104), or
+ * SGX_SINO_FAN_ARC / SGX_SINO_FAN_FLAT, (r, phi) = (dso sin g, g + beta) with
+ * g = s / dsd or atan(s / dsd) (sino-geom.jl:118-132).  s (nb), betas (na),
+ * ell: host arrays; out: (nb, na) column-major, host memory with SGX_HOST,
+ * else device memory.  No counterpart in the reference, whose tests make
+ * these with ImagePhantoms' radon(rays(rg), ob) on the CPU. */
+int sgx_radon_ellipses(sgx_ctx* ctx, int kind, const double* s, int64_t nb,
+                       const double* betas, int64_t na, double dsd, double dso,
+                       const double* ell, int64_t nell, double* out, uint32_t flags);
+
+/* Cone-beam line integrals in Float64 of axis-aligned ellipsoids, rows
+ * (cx, cy, cz, a, b, c, rho) (nell <= 512), along the lines from the source
+ * (-dso sin b, dso cos b, zs) to the detector point of (s, t) (kind
+ * SGX_CT_FAN_ARC: (dsd sin g, dso - dsd cos g, t), g = s / dsd; SGX_CT_FAN_FLAT:
+ * (s, dso - dsd, t)), rotated by b and raised by zs; the arc case is the
+ * line of cb_arc_to_par (src/fbp3/cb_arc_to_par.jl:23-43).  s (ns), t (nt),
+ * betas (na), source_zs (na, or NULL for 0), ell: host arrays; out: (ns, nt, na)
+ * column-major, host or device memory as above.  No counterpart in the
+ * reference (see sgx_radon_ellipses). */
+int sgx_radon_ellipsoids(sgx_ctx* ctx, int kind, const double* s, int64_t ns,
+                         const double* t, int64_t nt, const double* betas,
+                         const double* source_zs, int64_t na, double dsd, double dso,
+                         const double* ell, int64_t nell, double* out, uint32_t flags);
+
 /* ---- plans: the fused whole path ----------------------------------------- */
 
 /* Geometry and weighting of a plan (plan_fbp, src/fbp2/plan2.jl:106-138 and

@@ -437,6 +437,73 @@ int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, i
     });
 }
 
+int sgx_radon_ellipses(sgx_ctx* ctx, int kind, const double* s, int64_t nb,
+                       const double* betas, int64_t na, double dsd, double dso,
+                       const double* ell, int64_t nell, double* out, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(s && betas && out && (ell || nell == 0), SGX_ERR_ARG, "null pointer argument");
+        require(kind == SGX_SINO_PAR || kind == SGX_SINO_FAN_ARC || kind == SGX_SINO_FAN_FLAT,
+                SGX_ERR_ARG, "radon_ellipses: 2-D geometry kind");
+        require(nb > 0 && na > 0 && nell >= 0 && nb * na < (int64_t(1) << 40), SGX_ERR_ARG,
+                "bad size");
+        require(kind == SGX_SINO_PAR || dsd != 0.0, SGX_ERR_ARG, "dsd must be nonzero");
+        TablePack pack;
+        const size_t o_s = pack.add(s, 8 * nb), o_b = pack.add(betas, 8 * na);
+        const size_t o_e = nell ? pack.add(ell, 8 * 6 * nell) : 0;
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        double* dout = (flags & SGX_HOST)
+                           ? static_cast<double*>(ctx->get(W_OUT, 8 * nb * na))
+                           : out;
+        sgx::launch_radon_ellipses(kind, reinterpret_cast<const double*>(d + o_s), (int)nb,
+                                   reinterpret_cast<const double*>(d + o_b), (int)na, dsd, dso,
+                                   reinterpret_cast<const double*>(d + o_e), (int)nell, dout,
+                                   ctx->stream);
+        finish(ctx, out, dout, 8 * nb * na, flags);
+    });
+}
+
+int sgx_radon_ellipsoids(sgx_ctx* ctx, int kind, const double* s, int64_t ns,
+                         const double* t, int64_t nt, const double* betas,
+                         const double* source_zs, int64_t na, double dsd, double dso,
+                         const double* ell, int64_t nell, double* out, uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require(s && t && betas && out && (ell || nell == 0), SGX_ERR_ARG,
+                "null pointer argument");
+        require(kind == SGX_CT_FAN_ARC || kind == SGX_CT_FAN_FLAT, SGX_ERR_ARG,
+                "radon_ellipsoids: cone-beam geometry kind");
+        require(ns > 0 && nt > 0 && na > 0 && nell >= 0 && ns * nt * na < (int64_t(1) << 40),
+                SGX_ERR_ARG, "bad size");
+        require(dsd != 0.0, SGX_ERR_ARG, "dsd must be nonzero");
+        // the unrotated detector point of each bin (the cone rays' geometry)
+        std::vector<double> lx(ns), ly(ns);
+        for (int64_t i = 0; i < ns; ++i) {
+            if (kind == SGX_CT_FAN_ARC) {
+                const double g = s[i] / dsd;
+                lx[i] = dsd * std::sin(g);
+                ly[i] = dso - dsd * std::cos(g);
+            } else {
+                lx[i] = s[i];
+                ly[i] = dso - dsd;
+            }
+        }
+        TablePack pack;
+        const size_t o_x = pack.add(lx.data(), 8 * ns), o_y = pack.add(ly.data(), 8 * ns);
+        const size_t o_t = pack.add(t, 8 * nt), o_b = pack.add(betas, 8 * na);
+        const size_t o_z = source_zs ? pack.add(source_zs, 8 * na) : 0;
+        const size_t o_e = nell ? pack.add(ell, 8 * 7 * nell) : 0;
+        unsigned char* d = pack.upload(ctx, W_TABLES);
+        const int64_t n = ns * nt * na;
+        double* dout = (flags & SGX_HOST) ? static_cast<double*>(ctx->get(W_OUT, 8 * n)) : out;
+        auto P = [&](size_t o) { return reinterpret_cast<const double*>(d + o); };
+        sgx::launch_radon_ellipsoids(P(o_x), P(o_y), (int)ns, P(o_t), (int)nt, P(o_b),
+                                     source_zs ? P(o_z) : nullptr, (int)na, dso, P(o_e),
+                                     (int)nell, dout, ctx->stream);
+        finish(ctx, out, dout, 8 * n, flags);
+    });
+}
+
 int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
                     const float* Hk, int64_t npad, int64_t extra, int prec,
                     float* out, uint32_t flags)

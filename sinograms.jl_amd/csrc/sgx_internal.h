@@ -105,6 +105,18 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st);
 // Bytes of readable padding the 16-bit forms need after the last view.
 constexpr size_t kQ16Pad = 16;
 
+// Analytic phantom projections in FP64 (k_phantom.hip): ellipse rows
+// (cx, cy, a, b, theta, rho) along 2-D rays of kind SGX_SINO_*; axis-aligned
+// ellipsoid rows (cx, cy, cz, a, b, c, rho) along cone-beam rays from the
+// source to the detector point (lx[is], ly[is], t[it]) of the unrotated view,
+// rotated by beta, with an optional per-view source z (zs, or null).
+void launch_radon_ellipses(int kind, const double* s, int nb, const double* betas, int na,
+                           double dsd, double dso, const double* comp, int nc, double* out,
+                           hipStream_t st);
+void launch_radon_ellipsoids(const double* lx, const double* ly, int ns, const double* t,
+                             int nt, const double* betas, const double* zs, int na, double dso,
+                             const double* comp, int nc, double* out, hipStream_t st);
+
 // K1/K2: 2-D pixel-driven backprojection with view chunks in LDS.
 struct Back2dArgs {
     const float* q;         // (nb, na) float

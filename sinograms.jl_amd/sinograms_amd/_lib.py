@@ -40,7 +40,7 @@ EXPORTS = (
     "sgx_version", "sgx_ctx_create", "sgx_ctx_destroy", "sgx_last_error",
     "sgx_ctx_set_stream", "sgx_ctx_synchronize", "sgx_back_par", "sgx_back_fan",
     "sgx_cbct_back", "sgx_cbct_back_opts", "sgx_plan_set_cbct_opts", "sgx_sino_filter",
-    "sgx_parker_fan_short", "sgx_plan_create",
+    "sgx_parker_fan_short", "sgx_radon_ellipses", "sgx_radon_ellipsoids", "sgx_plan_create",
     "sgx_plan_destroy", "sgx_plan_set_view_weight", "sgx_plan_execute", "sgx_plan_filtered",
     "sgx_plan_view_weight", "sgx_plan_last_times", "sgx_plan_read_filtered",
     "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
@@ -118,6 +118,9 @@ def _declare(L):
     L.sgx_cbct_back_opts.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64,
                                      f64, i32, vp, i64, vp, i64, vp, i64, vp, vp, u32, vp]
     L.sgx_sino_filter.argtypes = [vp, vp, i64, i64, vp, i64, i64, i32, vp, u32]
+    L.sgx_radon_ellipses.argtypes = [vp, i32, vp, i64, vp, i64, f64, f64, vp, i64, vp, u32]
+    L.sgx_radon_ellipsoids.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, i64, f64, f64, vp,
+                                       i64, vp, u32]
     L.sgx_parker_fan_short.argtypes = [vp, i64, i64, vp, vp, f64, vp, u32]
     L.sgx_plan_create.argtypes = [vp, ctypes.POINTER(SgxGeom), vp, vp, i64, vp, i64, vp, i64,
                                   vp, vp, i64, i32, ctypes.POINTER(vp)]

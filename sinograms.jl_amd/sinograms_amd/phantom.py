@@ -7,8 +7,10 @@ following the reference's ray conventions:
   cone rays      source (-dso sin b, dso cos b, 0) to the rotated detector
                  pixel: flat (s, -dod, t); arc (dsd sin g, dso - dsd cos g, t),
                  g = s / dsd               (src/fbp3/cb_flat_to_par.jl, cb_arc_to_par.jl)
-Everything is computed with torch in float64 on the requested device
-(CPU or GPU) and returned as column-major (Julia-order) tensors.
+Everything is computed in float64 and returned as column-major (Julia-order)
+torch tensors on the requested device: on a GPU the projections run in
+libsgx's analytic projector (sgx_radon_ellipses / sgx_radon_ellipsoids), on
+the CPU in torch.
 """
 
 from __future__ import annotations
@@ -23,6 +25,31 @@ from .geom import CtFan, SinoFan, SinoPar
 def _torch():
     import torch
     return torch
+
+
+def _gpu(device):
+    """The CUDA device `device` names, or None for the CPU."""
+    torch = _torch()
+    d = torch.device(device)
+    return d if d.type == "cuda" else None
+
+
+def _colmajor_f64(shape, dev):
+    torch = _torch()
+    return torch.empty(tuple(reversed(shape)), dtype=torch.float64, device=dev).permute(
+        tuple(range(len(shape) - 1, -1, -1)))
+
+
+def _radon_hip(fn, dev, *args):
+    """Call an sgx_radon_* entry point on the device's shared context."""
+    from . import _lib
+    ctx = _lib.context(dev.index if dev.index is not None else 0)
+    ctx.check(getattr(_lib.load(), fn)(ctx.h, *args))
+
+
+def _p(a):
+    import ctypes
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
 
 
 def random_ellipses(seed: int, n: int, fov: float, center: float = 0.4,
@@ -74,6 +101,27 @@ def _ellipse_rays(ell, r, phi):
 def sino_ellipses(ell, rg, device="cpu"):
     """Exact sinogram (nb, na) of ellipses for SinoPar or SinoFan (source_offset 0)."""
     torch = _torch()
+    dev = _gpu(device)
+    if dev is not None:
+        from . import _lib
+        if isinstance(rg, SinoFan) and rg.source_offset != 0:
+            raise ValueError("sino_ellipses: source_offset must be 0")
+        kind = (_lib.SGX_SINO_PAR if isinstance(rg, SinoPar) else
+                _lib.SGX_SINO_FAN_ARC if rg.is_arc else _lib.SGX_SINO_FAN_FLAT)
+        e = np.ascontiguousarray(np.asarray(ell, np.float64).reshape(-1, 6))
+        sv, bv = np.ascontiguousarray(rg.s()), np.ascontiguousarray(rg.ar())
+        out = _colmajor_f64((rg.nb, rg.na), dev)
+        fan = isinstance(rg, SinoFan)
+        _radon_hip("sgx_radon_ellipses", dev, kind, _p(sv), rg.nb, _p(bv), rg.na,
+                   rg.dsd if fan else 0.0, rg.dso if fan else 0.0, _p(e), len(e),
+                   out.data_ptr(), 0)
+        return out
+    return _sino_ellipses_torch(ell, rg, device)
+
+
+def _sino_ellipses_torch(ell, rg, device):
+    """sino_ellipses as torch elementwise operations on any device."""
+    torch = _torch()
     s = torch.as_tensor(rg.s(), device=device)[:, None]
     beta = torch.as_tensor(rg.ar(), device=device)[None, :]
     if isinstance(rg, SinoPar):
@@ -97,15 +145,39 @@ def image_ellipses(ell, x, y, device="cpu"):
     return out
 
 
-def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None, chunk: int = 64):
+def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None, chunk: int = 64,
+                    source_zs=None):
     """Exact cone-beam projections (ns, nt, na) of axis-aligned ellipsoids,
-    as a column-major float64 tensor.  ``views`` selects a subset of views.
-    Views are processed in chunks (one batched pass per ellipsoid)."""
+    as a column-major float64 tensor.  ``views`` selects a subset of views;
+    ``source_zs`` (na) raises the source and the detector of every view (a
+    helical trajectory, ``rg.source_zs()``).  On the CPU, views are processed
+    in chunks (one batched pass per ellipsoid)."""
+    torch = _torch()
+    idx = np.arange(rg.na) if views is None else np.asarray(views)
+    zs_all = None if source_zs is None else np.asarray(source_zs, np.float64)
+    if zs_all is not None and zs_all.shape != (rg.na,):
+        raise ValueError("source_zs needs one z per view")
+    dev = _gpu(device)
+    if dev is not None:
+        from . import _lib
+        kind = _lib.SGX_CT_FAN_ARC if rg.is_arc else _lib.SGX_CT_FAN_FLAT
+        e = np.ascontiguousarray(np.asarray(ell, np.float64).reshape(-1, 7))
+        sv, tv = np.ascontiguousarray(rg.s()), np.ascontiguousarray(rg.t())
+        bv = np.ascontiguousarray(np.asarray(rg.ar(), np.float64)[idx])
+        zv = None if zs_all is None else np.ascontiguousarray(zs_all[idx])
+        out = _colmajor_f64((rg.ns, rg.nt, len(idx)), dev)
+        _radon_hip("sgx_radon_ellipsoids", dev, kind, _p(sv), rg.ns, _p(tv), rg.nt, _p(bv),
+                   _p(zv), len(idx), rg.dsd, rg.dso, _p(e), len(e), out.data_ptr(), 0)
+        return out
+    return _proj_ellipsoids_torch(ell, rg, device, idx, zs_all, chunk)
+
+
+def _proj_ellipsoids_torch(ell, rg, device, idx, zs_all, chunk):
+    """proj_ellipsoids as batched torch operations on any device."""
     torch = _torch()
     s = torch.as_tensor(rg.s(), device=device)
     t = torch.as_tensor(rg.t(), device=device)
     beta_all = np.asarray(rg.ar(), np.float64)
-    idx = np.arange(rg.na) if views is None else np.asarray(views)
     out = torch.zeros((len(idx), rg.nt, rg.ns), dtype=torch.float64, device=device)
     S, T = s[None, None, :], t[None, :, None]          # (1, 1, ns), (1, nt, 1)
     if rg.is_arc:
@@ -118,6 +190,8 @@ def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None, chunk: int = 64):
         b = torch.as_tensor(beta_all[idx[k0:k0 + chunk]], device=device)[:, None, None]
         cb, sb = torch.cos(b), torch.sin(b)
         px, py = -rg.dso * sb, rg.dso * cb             # source, (v, 1, 1)
+        pz = 0.0 if zs_all is None else \
+            torch.as_tensor(zs_all[idx[k0:k0 + chunk]], device=device)[:, None, None]
         dx = (lx * cb - ly * sb) - px                  # (v, 1, ns)
         dy = (lx * sb + ly * cb) - py
         dz = T                                         # (1, nt, 1)
@@ -125,7 +199,7 @@ def proj_ellipsoids(ell, rg: CtFan, device="cpu", views=None, chunk: int = 64):
         ux, uy, uz = dx / norm, dy / norm, dz / norm   # (v, nt, ns)
         acc = out[k0:k0 + chunk]
         for cx, cy, cz, a, bb, c, rho in ell:
-            qx, qy, qz = (px - cx) / a, (py - cy) / bb, (0.0 - cz) / c
+            qx, qy, qz = (px - cx) / a, (py - cy) / bb, (pz - cz) / c
             vx, vy, vz = ux / a, uy / bb, uz / c
             A = vx * vx + vy * vy + vz * vz
             B = 2 * (qx * vx + qy * vy + qz * vz)
