@@ -177,9 +177,11 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     // dword: band16[(2*part + copy) * kBandMax + i] = R[i + copy].
     __shared__ __attribute__((aligned(16))) unsigned char gbuf[SPLIT ? 4 * kBandMax * 2
                                                                      : (kBandMax + 64) * 4];
-    // two staging buffers (chunk c in buffer c & 1)
-    __shared__ __attribute__((aligned(16))) unsigned char bstage[2 * (SPLIT ? 2 * 32 * kBPadH * 2
-                                                                             : 32 * kBPadF * 4)];
+    // staging: two buffers (chunk c in buffer c & 1), one for RT = 1 (a
+    // larger static LDS block measurably slows those small launches)
+    __shared__ __attribute__((aligned(16))) unsigned char bstage[(RT == 1 ? 1 : 2) *
+                                                                 (SPLIT ? 2 * 32 * kBPadH * 2
+                                                                        : 32 * kBPadF * 4)];
     // per-bin factors of the fused weight: wbin (and the arc cylinder weight's
     // cos(s/dsd)), and s^2 for the flat cylinder weight
     __shared__ float bin_a[2048], bin_s2[2048];
@@ -243,10 +245,10 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     float colw = p.scale;
     if (cvalid && p.wview) colw *= p.wview[sview];
     if (MODE == M_F16X3 && cvalid) colw *= in_scale(p.vbound, sview);
-    // Cylinder weight, branch-free: flat dso / sqrt(s^2 + t^2 + dsd^2) is
+    // Cylinder weight: flat dso / sqrt(s^2 + t^2 + dsd^2) is
     // cyl_k * rsqrt(bin_s2[m] + cyl_c) with cyl_k = dso; arc (dso/dsd)
-    // cos(s/dsd) / sqrt((t/dsd)^2 + 1) is bin_a[m] (the cosine) times cyl_k;
-    // without one, cyl_k = 1.  Multiplying by 1 is exact.
+    // cos(s/dsd) / sqrt((t/dsd)^2 + 1) is bin_a[m] (the cosine) times cyl_k.
+    // The weight's branches are uniform (launch parameters).
     const bool cyl_flat = p.cyl == 1;
     float cyl_c = 1.f, cyl_k = 1.f;
     if (cvalid && p.cyl == 1) {
@@ -269,6 +271,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 #pragma unroll
             for (int e = 0; e < 8; ++e) v[e] = (cvalid && m + e < p.nb) ? src[m + e] : 0.f;
         }
+        if (!p.ptab) return;                // pw is read only with a table
         if (prow && m + 8 <= p.nb) {
             __builtin_memcpy(pw, prow + m, sizeof(float) * 8);
         } else {
@@ -276,15 +279,16 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
             for (int e = 0; e < 8; ++e) pw[e] = (prow && m + e < p.nb) ? prow[m + e] : 1.f;
         }
     };
-    // the fused weight, applied without branches (rows past nb become 0)
+    // the fused weight (rows past nb become 0)
     auto weigh = [&](int m0, float (&v)[8], const float (&pw)[8]) {
 #pragma unroll
         for (int e = 0; e < 8; ++e) {
             const int m = m0 + skk + e;
             const int mm = min(m, p.nb - 1);
             float w = colw * bin_a[mm];
-            w *= cyl_flat ? cyl_k * rsqrtf(bin_s2[mm] + cyl_c) : cyl_k;
-            w *= pw[e];
+            if (cyl_flat) w *= cyl_k * rsqrtf(bin_s2[mm] + cyl_c);
+            else if (p.cyl) w *= cyl_k;
+            if (p.ptab) w *= pw[e];
             v[e] = (cvalid && m < p.nb) ? v[e] * w : 0.f;
         }
     };
@@ -363,32 +367,55 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
         }
     };
 
-    // Double-buffered chunks: chunk c is multiplied out of buffer c & 1 while
-    // chunk c + 1 is weighted, split and written to the other buffer, one
-    // barrier per chunk; the global loads run a chunk ahead of the weighting.
-    float vq[8], pq[8];
-    {
-        float v0[8], p0[8];
-        load8(0, v0, p0);
-        load8(kKC, vq, pq);                 // rows past nb load as 0
-        weigh(0, v0, p0);
-        put(0, v0);
-    }
-    __syncthreads();
-    int buf = 0;
-    for (int m0 = 0; m0 < p.nb; m0 += kKC) {
-        float v[8], pw[8];
+    if (RT == 1) {
+        // Small launches (one block per CU): single-buffered chunks, the
+        // next chunk's loads in flight while the current one is weighted,
+        // staged and multiplied out (the shortest latency per chunk).
+        float vn[8], pn[8];
+        load8(0, vn, pn);
+        for (int m0 = 0; m0 < p.nb; m0 += kKC) {
+            float v[8], pw[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            v[e] = vq[e];
-            pw[e] = pq[e];
+            for (int e = 0; e < 8; ++e) {
+                v[e] = vn[e];
+                pw[e] = pn[e];
+            }
+            if (m0 + kKC < p.nb) load8(m0 + kKC, vn, pn);
+            weigh(m0, v, pw);
+            __syncthreads();
+            put(0, v);
+            __syncthreads();
+            mma(0, m0);
         }
-        load8(m0 + 2 * kKC, vq, pq);
-        weigh(m0 + kKC, v, pw);
-        mma(buf, m0);
-        put(buf ^ 1, v);                    // past the last chunk: written, never read
+    } else {
+        // Double-buffered chunks: chunk c is multiplied out of buffer c & 1
+        // while chunk c + 1 is weighted, split and written to the other
+        // buffer, one barrier per chunk; the global loads run a chunk ahead
+        // of the weighting.
+        float vq[8], pq[8];
+        {
+            float v0[8], p0[8];
+            load8(0, v0, p0);
+            load8(kKC, vq, pq);                 // rows past nb load as 0
+            weigh(0, v0, p0);
+            put(0, v0);
+        }
         __syncthreads();
-        buf ^= 1;
+        int buf = 0;
+        for (int m0 = 0; m0 < p.nb; m0 += kKC) {
+            float v[8], pw[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                v[e] = vq[e];
+                pw[e] = pq[e];
+            }
+            load8(m0 + 2 * kKC, vq, pq);
+            weigh(m0 + kKC, v, pw);
+            mma(buf, m0);
+            put(buf ^ 1, v);                    // past the last chunk: written, never read
+            __syncthreads();
+            buf ^= 1;
+        }
     }
 
     // store: acc[r][i] holds row 8*(i/4) + 4*half + i%4 of tile r, column `col`
