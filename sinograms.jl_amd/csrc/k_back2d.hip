@@ -50,8 +50,8 @@ __device__ __forceinline__ void pixel_bin(float x, float y, ViewCS t, float wb, 
     } else {
         const float d = dso_ds + x * t.s - y * t.c;
         const float r = x * t.c + y * t.s - so_ds;
-        if (GEOM == 1) {
-            bb = dsd_ds * atan2_fast(r, d) + wb;
+        if (GEOM == 1 || GEOM == 3) {
+            bb = dsd_ds * (GEOM == 3 ? atan2_narrow(r, d) : atan2_fast(r, d)) + wb;
             w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(d * d + r * r);
         } else {
             const float m = dsd_ds / d;
@@ -74,7 +74,9 @@ int chunk_views(int nb, int vc_max)
     return std::max(1, std::min(vc_max, kLdsBytes / (4 * window_bins(nb) + 8)));
 }
 
-template <int GEOM>  // 0 parallel, 1 fan arc, 2 fan flat
+// GEOM: 0 parallel, 1 fan arc, 2 fan flat, 3 fan arc whose detector spans
+// less than +-45 degrees (atan2_narrow)
+template <int GEOM>
 __global__ __launch_bounds__(kThreads) void back2d_kernel(
     const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ xds,
@@ -311,9 +313,16 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
                            vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
         break;
     case 1:
-        hipLaunchKernelGGL(back2d_kernel<1>, grid, block, lds, st, a.q, a.nb, a.nviews,
-                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        // bins 1 <= bb < nb are at |gamma| = |bb - wb| / dsd_ds; below 0.75 rad
+        // (43 degrees) on both sides the narrow arctangent serves
+        if (std::max(std::fabs(1.f - a.wb), std::fabs((float)a.nb - a.wb)) < 0.75f * a.dsd_ds)
+            hipLaunchKernelGGL(back2d_kernel<3>, grid, block, lds, st, a.q, a.nb, a.nviews,
+                               a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix,
+                               vpg, vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        else
+            hipLaunchKernelGGL(back2d_kernel<1>, grid, block, lds, st, a.q, a.nb, a.nviews,
+                               a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix,
+                               vpg, vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
         break;
     default:
         hipLaunchKernelGGL(back2d_kernel<2>, grid, block, lds, st, a.q, a.nb, a.nviews,

@@ -33,11 +33,9 @@ struct Error : std::runtime_error {
 // of csrc/gen_atan_coeffs.py: 1.3 ulp from float64 atan in FP32 Horner form,
 // about 20 VALU instructions against 45 for the library's atan2f.
 // (0, 0) gives NaN, which fails every bin test.
-__device__ __forceinline__ float atan2_fast(float y, float x)
+__device__ __forceinline__ float atan_unit(float u)
 {
-    const float ay = fabsf(y), ax = fabsf(x);
-    const float mx = fmaxf(ay, ax), mn = fminf(ay, ax);
-    const float u = mn * __builtin_amdgcn_rcpf(mx);
+    // u * P(u^2) = atan(u) for |u| <= 1, odd in u
     const float u2 = u * u;
     float p = -1.753933378e-03f;
     p = fmaf(p, u2, 1.072695851e-02f);
@@ -49,10 +47,28 @@ __device__ __forceinline__ float atan2_fast(float y, float x)
     p = fmaf(p, u2, 1.999826580e-01f);
     p = fmaf(p, u2, -3.333328366e-01f);
     p = fmaf(p, u2, 1.0f);
-    float a = u * p;
+    return u * p;
+}
+
+__device__ __forceinline__ float atan2_fast(float y, float x)
+{
+    const float ay = fabsf(y), ax = fabsf(x);
+    const float mx = fmaxf(ay, ax), mn = fminf(ay, ax);
+    float a = atan_unit(mn * __builtin_amdgcn_rcpf(mx));
     if (ay > ax) a = 1.57079637f - a;            // pi/2 - atan(1/t)
     if (x < 0.f) a = 3.14159274f - a;            // pi - atan(t)
     return copysignf(a, y);
+}
+
+// atan2(y, x) for |y| < x, bit for bit what atan2_fast gives there (the
+// same reciprocal and polynomial; the polynomial is odd); anywhere else it
+// returns +-pi/2 with the sign of y.  Those points are at least 45 degrees
+// off the x axis: for a detector that spans less than +-45 degrees both
+// answers map off the detector, on the same side.
+__device__ __forceinline__ float atan2_narrow(float y, float x)
+{
+    const float a = atan_unit(y * __builtin_amdgcn_rcpf(x));
+    return fabsf(y) < x ? a : copysignf(1.57079637f, y);
 }
 
 // Per-view geometry of the backprojectors, formed on the device in FP64 and
