@@ -154,7 +154,7 @@ template <bool ARC, int FMT, bool OPT>
 __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float dsd_ds,
                                             float dso_ds, float ws, float nsf, float inv_dt_ds,
                                             float vs, float wt, const float* zs, int ia,
-                                            int arc_t_exact)
+                                            int arc_t_exact, int narrow)
 {
     ColView r;
     const float xb = x * v.c + y * v.s;
@@ -162,7 +162,7 @@ __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float ds
     const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
     float sp, w2;
     if (ARC) {
-        sp = dsd_ds * atan2_fast(xb, yb);
+        sp = dsd_ds * (narrow ? atan2_narrow(xb, yb) : atan2_fast(xb, yb));
         w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
     } else {
         sp = mag * xb;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact)
+    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -275,7 +275,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
             const ColView cv = col_view<ARC, FMT, OPT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
                                                        inv_dt_ds,
                                                        FMT == F_F16S ? vscale[ia] : 1.f, wt, zs,
-                                                       ia, arc_t_exact);
+                                                       ia, arc_t_exact, narrow);
             if (!cv.hit) continue;                        // the column misses the detector
             const float blo = fmaf(cv.mz, zlo, cv.wt), bhi = fmaf(cv.mz, zhi, cv.wt);
             const bool fast = blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf;
@@ -293,7 +293,10 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 
 // Build-time tuning of the staged kernel (defaults measured best on the
 // 256^2 x 128 FDK workload): LDS loads in flight per batch, minimum waves
-// per SIMD (caps the VGPRs), LDS row pitch in entries.
+// per SIMD (caps the VGPRs), LDS row pitch in entries.  The arc detector's
+// 16-bit forms ask for 4 waves: uncapped they take 131 VGPRs, one step past
+// 128, and run at 3 waves per SIMD 26 % slower (3 VGPRs spill, outside the
+// view loop).
 #ifndef SGX_STAGED_BATCH
 #define SGX_STAGED_BATCH 8
 #endif
@@ -412,14 +415,15 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
 }
 
 template <bool ARC, int FMT, bool OPT>
-__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
+__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
+cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
-    int arc_t_exact)
+    int arc_t_exact, int narrow)
 {
     using L = Lds<FMT>;
     constexpr int kLP = L::pitch;
@@ -537,7 +541,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
         const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
         float sp;
         if (ARC) {
-            sp = dsd_ds * atan2_fast(xb, yb);
+            sp = dsd_ds * (narrow ? atan2_narrow(xb, yb) : atan2_fast(xb, yb));
             r.w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
         } else {
             sp = mag * xb;
@@ -647,6 +651,14 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_staged_kernel(
     }
 }
 
+// An arc detector whose bins (1 <= bs < ns, |gamma| = |bs - ws| / dsd_ds)
+// span less than 0.75 rad (43 degrees) on both sides takes atan2_narrow.
+int arc_narrow(const CbctArgs& a)
+{
+    return a.is_arc && std::max(std::fabs(1.f - a.ws), std::fabs((float)a.ns - a.ws)) <
+                           0.75f * a.dsd_ds;
+}
+
 int64_t staged_blocks(const CbctArgs& a)
 {
     const int64_t tiles = (int64_t)((a.nx + kSP - 1) / kSP) * ((a.ny + kSP - 1) / kSP);
@@ -664,7 +676,7 @@ void launch_gather(const CbctArgs& a, hipStream_t st)
                        dim3(64 * kWaves), 0, st, a.q, a.ns, a.nt, a.pitch, a.nviews,
                        a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz,
                        a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.zs,
-                       a.arc_t_exact ? 1 : 0);
+                       a.arc_t_exact ? 1 : 0, arc_narrow(a));
     SGX_HIP(hipGetLastError());
 }
 
@@ -677,7 +689,8 @@ void launch_staged(const CbctArgs& a, hipStream_t st)
     hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT>), dim3((unsigned)nb), dim3(256), 0, st,
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
-                       a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0);
+                       a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0,
+                       arc_narrow(a));
     SGX_HIP(hipGetLastError());
 }
 
