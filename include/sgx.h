@@ -157,7 +157,9 @@ int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, i
  * Computed as the Toeplitz contraction out[n,c] = sum_m g[(n-m) mod npad] in[m,c]
  * with g = real(ifft(Hk)), on MFMA.  prec selects SGX_PREC_FP32 (FP32 output)
  * or SGX_PREC_BF16 (split-BF16x3 products, FP32 accumulation, FP32 output).
- * Returns SGX_ERR_VALUE for the reference's error("nb + extra > npad"). */
+ * Returns SGX_ERR_VALUE for the reference's error("nb + extra > npad"), and
+ * SGX_ERR_UNSUPPORTED for nb > 8192 (the reference has no such limit; the
+ * band of g lives in one workgroup's LDS).  Plans share this limit. */
 int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
                     const float* Hk, int64_t npad, int64_t extra, int prec,
                     float* out, uint32_t flags);

@@ -143,7 +143,12 @@ __device__ __forceinline__ float bf16_to_f(__bf16 h)
 enum { M_F32 = 0, M_BF16X3 = 1, M_F16X3 = 2 };
 typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
-constexpr int kBandMax = 4 * 8 * 32 + 2048;       // longest band of g (RT = 8, nb <= 2048)
+// Detector widths: the common instance takes nb <= 2048, the wide one nb <=
+// 8192 (its band of g and bin factors use about 154 KB of LDS, one block per
+// CU).  The band of a block is 4 RT 32 + kp - 1 elements long.
+constexpr int kNbSmall = 2048;
+constexpr int kNbWide = 8192;
+__host__ __device__ constexpr int band_max(int nbmax) { return 4 * 8 * 32 + nbmax; }
 
 __device__ __forceinline__ float in_scale(const unsigned int* vbound, int view)
 {
@@ -167,10 +172,11 @@ template <int MODE> __device__ __forceinline__ void split2(float x, uint16_t& hi
     }
 }
 
-template <int MODE, int OUT, int RT>
+template <int MODE, int OUT, int RT, int NBMAX>
 __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
 {
     constexpr bool SPLIT = MODE != M_F32;
+    constexpr int kBandMax = band_max(NBMAX);            // longest band of g
     // FP32: the band of g as floats.  Split modes: hi / lo parts of the band,
     // reversed (R[i] = band[glen-1-i], so a fragment's 8 taps ascend), each
     // in two copies shifted by one element so every fragment starts on a
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
                                                                         : 32 * kBPadF * 4)];
     // per-bin factors of the fused weight: wbin (and the arc cylinder weight's
     // cos(s/dsd)), and s^2 for the flat cylinder weight
-    __shared__ float bin_a[2048], bin_s2[2048];
+    __shared__ float bin_a[NBMAX], bin_s2[NBMAX];
     float* gl = reinterpret_cast<float*>(gbuf);
     uint16_t* band16 = reinterpret_cast<uint16_t*>(gbuf);
     float* bs_f = reinterpret_cast<float*>(bstage);
@@ -468,13 +474,13 @@ __global__ __launch_bounds__(256) void toeplitz_kernel(Params p)
     }
 }
 
-template <int MODE, int OUT, int RT>
+template <int MODE, int OUT, int RT, int NBMAX>
 void launch_rt(const Params& p, hipStream_t st)
 {
     const int row_tiles = (p.nrows_out + 31) / 32;
     const int row_blocks = (row_tiles + 4 * RT - 1) / (4 * RT);
     dim3 grid((unsigned)((p.ncols + 31) / 32), (unsigned)row_blocks);
-    hipLaunchKernelGGL((toeplitz_kernel<MODE, OUT, RT>), grid, dim3(256), 0, st, p);
+    hipLaunchKernelGGL((toeplitz_kernel<MODE, OUT, RT, NBMAX>), grid, dim3(256), 0, st, p);
     SGX_HIP(hipGetLastError());
 }
 
@@ -487,10 +493,18 @@ void launch_mode(const Params& p, hipStream_t st)
     // split the rows when the columns alone cannot fill the GPU.
     int rt = (row_tiles + 3) / 4;
     if (col_blocks < 1024) rt = 1;
-    if (rt <= 1) launch_rt<MODE, OUT, 1>(p, st);
-    else if (rt <= 2) launch_rt<MODE, OUT, 2>(p, st);
-    else if (rt <= 4) launch_rt<MODE, OUT, 4>(p, st);
-    else launch_rt<MODE, OUT, 8>(p, st);
+    if (p.nb > kNbSmall) {            // wide detectors: 64+ row tiles, RT 1 or 8
+        if (rt <= 1) launch_rt<MODE, OUT, 1, kNbWide>(p, st);
+        else launch_rt<MODE, OUT, 8, kNbWide>(p, st);
+    } else if (rt <= 1) {
+        launch_rt<MODE, OUT, 1, kNbSmall>(p, st);
+    } else if (rt <= 2) {
+        launch_rt<MODE, OUT, 2, kNbSmall>(p, st);
+    } else if (rt <= 4) {
+        launch_rt<MODE, OUT, 4, kNbSmall>(p, st);
+    } else {
+        launch_rt<MODE, OUT, 8, kNbSmall>(p, st);
+    }
 }
 
 // Per-view max |weighted input|.  Block (view, y) covers kBoundRows detector
@@ -502,7 +516,7 @@ void launch_mode(const Params& p, hipStream_t st)
 // weight) and the row factors sit in LDS.  One atomic per block.
 constexpr int kBoundRows = 16;
 constexpr int kBoundLoads = 8;
-constexpr int kBoundMaxBins = 2048;
+template <int kBoundMaxBins>
 __global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __restrict__ vbound,
                                                     int vec)
 {
@@ -669,8 +683,8 @@ void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st)
 void launch_sino_filter(const FilterArgs& a, hipStream_t st)
 {
     if (a.ncols <= 0) return;
-    if (a.nb > 2048)
-        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
+    if (a.nb > kNbWide)
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 8192 is not supported");
     const Params p = make_params(a);
     if (a.out_f16s && (a.out_bf16 || a.split_bf16 || !a.vbound || !a.vscale || a.nt < 1))
         throw Error(SGX_ERR_ARG, "sino_filter: scaled FP16 output takes FP32 or FP16 products");
@@ -699,17 +713,19 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
 void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t st)
 {
     if (a.ncols <= 0) return;
-    if (a.nb > kBoundMaxBins)
-        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 2048 is not supported");
+    if (a.nb > kNbWide)
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: nb > 8192 is not supported");
     const int64_t na = a.ncols / a.nt;
     launch_zero_u32(vbound, na, st);
     const Params p = make_params(a);
     if ((a.nt + kBoundRows - 1) / kBoundRows > 65535)
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many rows per view");
     const int vec = a.nb % 4 == 0 && (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
-    hipLaunchKernelGGL(bound_kernel, dim3((unsigned)na,
-                                          (unsigned)((a.nt + kBoundRows - 1) / kBoundRows)),
-                       dim3(256), 0, st, p, vbound, vec);
+    const dim3 grid((unsigned)na, (unsigned)((a.nt + kBoundRows - 1) / kBoundRows));
+    if (a.nb > kNbSmall)
+        hipLaunchKernelGGL(bound_kernel<kNbWide>, grid, dim3(256), 0, st, p, vbound, vec);
+    else
+        hipLaunchKernelGGL(bound_kernel<kNbSmall>, grid, dim3(256), 0, st, p, vbound, vec);
     SGX_HIP(hipGetLastError());
 }
 

@@ -285,12 +285,13 @@ def test_back2d_window_fallbacks(sg, orc, torch_cuda, kind):
 # --- K4 filter and K5 Parker ------------------------------------------------------------
 
 @pytest.mark.parametrize("nb,ncols,npad,extra", [(37, 5, 64, 3), (367, 360, 512, 0),
-                                                 (888, 70, 1024, 0), (512, 1000, 1024, 2)])
+                                                 (888, 70, 1024, 0), (512, 1000, 1024, 2),
+                                                 (3000, 40, 4096, 5), (8192, 6, 8192, 0)])
 def test_sino_filter_fp32_vs_oracle(sg, orc, torch_cuda, nb, ncols, npad, extra):
     _say("filter fp32", nb, ncols, npad, extra)
     rng = np.random.default_rng(30)
     x = rng.standard_normal((nb, ncols)).astype(np.float32)
-    h, _ = orc.ramp_arc(npad, 1.0, 900.0)
+    h, _ = orc.ramp_arc(npad, 1.0, max(900.0, float(npad)))
     Hk = orc.fbp_filter(h, npad, 1.0, window=orc.fbp_window("hann", npad)).astype(np.float32)
     got = sg.fbp_sino_filter(x, Hk, extra=extra)
     ref = orc.sino_filter(x, Hk, extra=extra, dtype=np.float32)
@@ -299,7 +300,8 @@ def test_sino_filter_fp32_vs_oracle(sg, orc, torch_cuda, nb, ncols, npad, extra)
     assert rel_rmse(got, ref) < 2e-6
 
 
-@pytest.mark.parametrize("nb,ncols,npad", [(37, 5, 64), (367, 360, 512), (512, 1000, 1024)])
+@pytest.mark.parametrize("nb,ncols,npad", [(37, 5, 64), (367, 360, 512), (512, 1000, 1024),
+                                          (3000, 40, 4096)])
 def test_sino_filter_split_bf16x3(sg, orc, torch_cuda, nb, ncols, npad):
     _say("filter bf16x3", nb, ncols, npad)
     rng = np.random.default_rng(31)
@@ -311,6 +313,72 @@ def test_sino_filter_split_bf16x3(sg, orc, torch_cuda, nb, ncols, npad):
     err = rel_rmse(got, ref)
     _say("  split-bf16x3 rel-RMSE vs Float64:", err)
     assert err < 3e-5
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_sino_filter_wide_many_columns(sg, orc, torch_cuda, precision):
+    # 2500 bins x 33000 columns: the wide-detector instance with 8 row tiles
+    # per wave (more than 1024 column blocks); the columns are independent,
+    # so a sample of them is checked against the oracle.
+    _say("filter wide, many columns", precision)
+    nb, ncols, npad = 2500, 33000, 4096
+    rng = np.random.default_rng(32)
+    x = rng.standard_normal((nb, ncols)).astype(np.float32)
+    h, _ = orc.ramp_arc(npad, 1.0, 1500.0)
+    Hk = orc.fbp_filter(h, npad, 1.0).astype(np.float32)
+    got = sg.fbp_sino_filter(x, Hk, extra=3, precision=precision)
+    assert got.shape == (nb + 3, ncols)
+    cols = np.unique(np.r_[0, 1, 31, 32, ncols - 1, rng.integers(0, ncols, 40)])
+    ref = orc.sino_filter(x[:, cols].astype(np.float64), Hk.astype(np.float64), extra=3)
+    err = rel_rmse(got[:, cols], ref)
+    _say(f"  rel-RMSE vs Float64 over {len(cols)} columns: {err:.3g}")
+    assert np.all(got[nb:] == 0)
+    assert err < (2e-6 if precision == "fp32" else 3e-5)
+
+
+def test_wide_detector_plans_vs_oracle(sg, orc, torch_cuda):
+    # Fan and cone plans whose detectors are wider than 2048 bins.  With
+    # thousands of fine bins the Float32 bin coordinate itself limits FP32
+    # accuracy (the Float32 oracle is ~1e-4 from Float64 on a noise
+    # sinogram here), so the bar is: as close to the Float64 oracle as the
+    # Float32 oracle is, within a factor 1.5.
+    _say("wide-detector fan and cone plans")
+    rng = np.random.default_rng(33)
+    rg = sg.SinoFanArc(nb=2600, na=24, d=0.25, dsd=1100.0, dod=400.0)
+    ig = sg.ImageGeom((48, 40), (4.0, 4.0))
+    sino = rng.standard_normal(rg.dims()).astype(np.float32)
+    plan = sg.plan_fbp(rg, ig)
+    got = sg.fbp(plan, sino)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.angle_weight(ar)
+    sw = orc.fan_sino_weight(rg.gamma(), rg.dso, rg.dsd)
+    xc, yc = ig.axes()
+    args = (plan.filter, vw, sw, ar, rg.dsd, rg.dso, 0.0, True, rg.d, 0.0, xc, yc, ig.mask)
+    ref64 = orc.fbp_fan(sino.astype(np.float64), *args)
+    e, e_orc = rel_rmse(got, ref64), rel_rmse(orc.fbp_fan(sino, *args, dtype=np.float32), ref64)
+    _say(f"  fan nb=2600 rel-RMSE vs Float64 oracle {e:.3g} (Float32 oracle {e_orc:.3g})")
+    assert e < 1.5 * e_orc + 1e-6
+
+    cg = sg.CtFanFlat(ns=2304, nt=6, ds=0.2, dt=0.2, na=12, dsd=1200.0, dod=400.0)
+    cig = sg.ImageGeom((40, 36, 4), (4.0, 4.0, 0.15))
+    proj = rng.standard_normal(cg.dims()).astype(np.float32)
+    p32 = sg.plan_fbp(cg, cig)
+    r32 = sg.fdk(p32, proj)
+    car = cg.ar(f32=True).astype(np.float32)
+    cvw = orc.fdk_weight_cyl(cg.s(), cg.t(), cg.dsd, cg.dso, False)[:, :, None] * \
+        orc.angle_weight(car)
+    cx, cy, cz = cig.axes()
+    cargs = (p32.filter, cvw, car, cg.dsd, cg.dso, cg.ds, cg.dt, 0.0, 0.0, False, cx, cy, cz,
+             cig.mask)
+    cref64 = orc.fdk(proj.astype(np.float64), *cargs)
+    e = rel_rmse(r32, cref64)
+    e_orc = rel_rmse(orc.fdk(proj, *cargs, dtype=np.float32), cref64)
+    _say(f"  cone ns=2304 FP32 rel-RMSE vs Float64 oracle {e:.3g} (Float32 oracle {e_orc:.3g})")
+    assert e < 1.5 * e_orc + 1e-6
+    for prec, tol in (("f16s", 1e-3), ("bf16", 1e-2)):
+        e = rel_rmse(sg.fdk(sg.plan_fbp(cg, cig, precision=prec), proj), r32)
+        _say(f"  cone ns=2304 {prec} vs FP32 {e:.3g}")
+        assert e < tol
 
 
 def test_parker_device_vs_oracle(sg, orc, torch_cuda):
