@@ -85,6 +85,36 @@ def time_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+def filter_form(plan):
+    """Which ramp-filter kernel a plan runs: the FFT form at npad = 1024,
+    else the Toeplitz form on MFMA (also when SGX_FILTER_TOEPLITZ=1 was set
+    while the plan was made)."""
+    return "toeplitz" if len(plan.filter) != 1024 or getattr(plan, "_toeplitz", False) \
+        else "fft"
+
+
+def filter_figures(plan, fm, nb, ncols, out_bytes):
+    """Filter figures: HBM bytes moved per launch (input read once, output
+    written once) over the launch time, and for the Toeplitz form the
+    algorithmic 2 nb^2 ncols flops over it."""
+    byts = 4.0 * nb * ncols + out_bytes * nb * ncols
+    r = {"filter_form": filter_form(plan), "filter_hbm_GBs": byts / (fm * 1e-3) / 1e9}
+    if r["filter_form"] == "toeplitz":
+        r["filter_TFLOPs_algorithmic"] = 2.0 * nb * nb * ncols / (fm * 1e-3) / 1e12
+    return r
+
+
+def plan_toeplitz(sg, rg, ig, **kw):
+    """A plan whose filter is the Toeplitz form (the A/B against the FFT form)."""
+    os.environ["SGX_FILTER_TOEPLITZ"] = "1"
+    try:
+        plan = sg.plan_fbp(rg, ig, **kw)
+    finally:
+        del os.environ["SGX_FILTER_TOEPLITZ"]
+    plan._toeplitz = True
+    return plan
+
+
 def kernel_times(plan, proj, out, n):
     """Average (filter_ms, back_ms) over n executes, from HIP events recorded
     on the stream the kernels run on."""
@@ -263,8 +293,8 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
              "roofline": {"bound": "hbm", "achieved": 8.0 * updates / (bm * 1e-3) / 1e9,
                           "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": 8.0 * updates / (bm * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "model": "gather-equivalent 8 B per pixel-view update"},
-             "filter_TFLOPs_algorithmic": 2.0 * rg.nb ** 2 * rg.na / (fm * 1e-3) / 1e12}
+                          "model": "gather-equivalent 8 B per pixel-view update"}}
+        r.update(filter_figures(plan, fm, rg.nb, rg.na, 4))
         log(f"  {name}: step {t * 1e3:.3f} ms (filter {fm:.3f} ms, back {bm:.3f} ms)")
         if with_cpu:
             r.update(_slice_cpu(sg, rg, ig, sino, plan, y.cpu().numpy()))
@@ -454,16 +484,33 @@ def main():
             "traffic_source": TRAFFIC_FILE if traffic else None,
             "traffic_GBs_at_kernel_ms": traffic / (b32ms * 1e-3) / 1e9 if traffic else None,
             "compulsory_bytes": compulsory}
-    filt_flops = 2.0 * rg.ns * rg.ns * rg.nt * rg.na
+    ncols = rg.nt * rg.na
 
     def form(f, bpu):
         fm, bm = kt[f]
-        return {"value": updates / step[f] / 1e9, "unit": "Gupdates/s",
-                "ms_per_step": step[f] * 1e3, "speedup_vs_fp32": t32 / step[f],
-                "back_kernel_ms": bm, "filter_ms": fm,
-                "back_achieved_GBs_model": bpu * updates / (bm * 1e-3) / 1e9,
-                "back_speedup_vs_fp32": b32ms / bm,
-                "filter_TFLOPs_algorithmic": filt_flops / (fm * 1e-3) / 1e12}
+        r = {"value": updates / step[f] / 1e9, "unit": "Gupdates/s",
+             "ms_per_step": step[f] * 1e3, "speedup_vs_fp32": t32 / step[f],
+             "back_kernel_ms": bm, "filter_ms": fm,
+             "back_achieved_GBs_model": bpu * updates / (bm * 1e-3) / 1e9,
+             "back_speedup_vs_fp32": b32ms / bm}
+        r.update(filter_figures(plans[f], fm, rg.ns, ncols, 2))
+        return r
+
+    # A/B: the same plans with the Toeplitz (MFMA) filter in place of the FFT form
+    toep = {}
+    for f in ("fp32", "f16s"):
+        log(f"timing {f} with the Toeplitz filter ...")
+        pt = plan_toeplitz(sg, rg, ig, precision=f)
+        t = time_steps(lambda: pt.execute(proj, out=outs[f]), args.steps, args.warmup)
+        fm, bm = kernel_times(pt, proj, outs[f], max(3, min(args.steps, 10)))
+        toep[f] = {"ms_per_step": t * 1e3, "filter_ms": fm, "back_kernel_ms": bm,
+                   "speedup_vs_fp32_toeplitz": None}
+        toep[f].update(filter_figures(pt, fm, rg.ns, ncols, 4 if f == "fp32" else 2))
+        log(f"  {f} (Toeplitz filter): step {t * 1e3:.3f} ms (filter {fm:.3f} ms)")
+        del pt
+    toep["f16s"]["speedup_vs_fp32_toeplitz"] = toep["fp32"]["ms_per_step"] / \
+        toep["f16s"]["ms_per_step"]
+    del toep["fp32"]["speedup_vs_fp32_toeplitz"]
     result = {
         "metric": METRIC,
         "value": updates / t32 / 1e9,
@@ -480,13 +527,14 @@ def main():
                 "projections cast to Float32",
         "config": {"workload": "FDK CtFanFlat 256x256x128 volume, 512x384 detector, 360 views",
                    "updates_per_step": updates, "projections_per_s": rg.na / t32,
-                   "filter_TFLOPs_algorithmic_fp32": filt_flops / (f32ms * 1e-3) / 1e12},
+                   **filter_figures(plans["fp32"], f32ms, rg.ns, ncols, 4)},
         "roofline": roof,
         "pcie_inclusive": None if t_host is None else {
             "value": updates / t_host / 1e9, "unit": "Gupdates/s", "ms_per_step": t_host * 1e3,
             "note": "FP32 plan, host projections in and host volume out (pageable numpy arrays)"},
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
+        "toeplitz_filter_ab": toep,
     }
     if not args.no_slices:
         result["slices"] = slice_legs(sg, dev, args.steps, args.warmup, not args.no_cpu)

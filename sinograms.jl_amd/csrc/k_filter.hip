@@ -30,6 +30,9 @@
 
 #include <hip/hip_fp16.h>
 
+#include <cmath>
+#include <cstdlib>
+
 namespace sgx {
 namespace {
 
@@ -66,6 +69,7 @@ struct Params {
     float gscale;                 // split-FP16: power of two applied to g
     float* vscale;                // F16S: per-view output scale (written)
     int row_tiles_per_block;
+    const float* fft;             // FFT form (npad = 1024): twiddles and spectrum, or null
 };
 
 // F16S output scale of a view: the power of two s >= bound * g1 / 2^14, so
@@ -507,6 +511,326 @@ void launch_mode(const Params& p, hipStream_t st)
     }
 }
 
+// ---------------------------------------------------------------------------
+// FFT form, npad = 1024: the reference's own algorithm (zero-pad, fft, .* Hk,
+// ifft, real part), on two columns at once.  z = x_a + i x_b (the weighted
+// columns) is transformed; the response enters as He = (Hk[k] + Hk[-k]) / 2,
+// the even part of Hk (real(ifft(fft(x) .* Hk)) = x circularly convolved
+// with real(ifft(Hk)) = ifft(He)), and since He is real and even,
+// ifft(fft(z) .* He) = y_a + i y_b exactly: the real part is column a's
+// output and the imaginary part column b's.
+//
+// A half-wave (32 lanes) owns one transform of N = 32 x 32 points, held in
+// registers (32 complex per lane), by the four-step method:
+//   n = n1 + 32 n2, k = 32 k1 + k2;   lane n1 holds z[n1 + 32 n2], n2 = 0..31
+//   A: 32-point DFT over n2 in registers        Y[n1, k2]
+//   B: Y[n1, k2] *= W_N^(n1 k2)                  (table in LDS)
+//   C: transpose through LDS (wave-local)         lane k2 holds Y[., k2]
+//   D: 32-point DFT over n1 in registers        X[32 k1 + k2] at lane k2
+// The inverse is the same forward transform applied to the swapped spectrum:
+// ifft(Y) = swap(fft(swap(Y))) / N with swap(a + ib) = b + ia, and 1/N is in
+// the spectrum table.  Lane k2 holding k1 is exactly the layout step A takes
+// (n1 = k2, n2 = k1), so the inverse needs no extra data movement, and its
+// output R[32 j1 + j2] lands at lane j2: rows 32 j1 + j2, stored coalesced.
+// The 32-point DFTs are radix-2 decimation in frequency on registers, their
+// output in bit-reversed register order (a renaming, resolved at compile time).
+//
+// Bound: VALU (about 500 wave instructions per column for both transforms)
+// and HBM (each column read once and written once).
+
+constexpr int kFftN = 1024;
+constexpr int kFftCols = 16;          // columns per block: 8 transforms of 2 columns
+constexpr int kFftTP = 33;            // transpose pitch (floats): conflict-free rows and columns
+
+__host__ __device__ constexpr int bitrev5(int i)
+{
+    return ((i & 1) << 4) | ((i & 2) << 2) | (i & 4) | ((i & 8) >> 2) | ((i & 16) >> 4);
+}
+
+// cos and sin of 2 pi t / 32, t = 0..15
+__host__ __device__ constexpr float tw32_cos(int t)
+{
+    constexpr float c[16] = {1.0f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f,
+                             7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f,
+                             1.950903220e-01f, 0.0f, -1.950903220e-01f, -3.826834324e-01f,
+                             -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f,
+                             -9.238795325e-01f, -9.807852804e-01f};
+    return c[t];
+}
+__host__ __device__ constexpr float tw32_sin(int t)
+{
+    constexpr float s[16] = {0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f,
+                             7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f,
+                             9.807852804e-01f, 1.0f, 9.807852804e-01f, 9.238795325e-01f,
+                             8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f,
+                             3.826834324e-01f, 1.950903220e-01f};
+    return s[t];
+}
+
+// One radix-2 DIF stage of a 32-point forward DFT (W = e^(-2 pi i / 32)).
+template <int SPAN>
+__device__ __forceinline__ void dif_stage(float (&re)[32], float (&im)[32])
+{
+#pragma unroll
+    for (int g = 0; g < 32; g += 2 * SPAN) {
+#pragma unroll
+        for (int j = 0; j < SPAN; ++j) {
+            const int a = g + j, b = a + SPAN, t = j * (16 / SPAN);
+            const float dr = re[a] - re[b], di = im[a] - im[b];
+            re[a] += re[b];
+            im[a] += im[b];
+            if (t == 0) {
+                re[b] = dr;
+                im[b] = di;
+            } else if (t == 8) {                 // W^8 = -i
+                re[b] = di;
+                im[b] = -dr;
+            } else {                             // (dr + i di)(c - i s)
+                const float c = tw32_cos(t), sn = tw32_sin(t);
+                re[b] = fmaf(dr, c, di * sn);
+                im[b] = fmaf(di, c, -(dr * sn));
+            }
+        }
+    }
+}
+
+// The first stage when inputs 16..31 are zero (the zero-padded half).
+__device__ __forceinline__ void dif_stage16_half(float (&re)[32], float (&im)[32])
+{
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int b = j + 16;
+        const float dr = re[j], di = im[j];
+        if (j == 0) {
+            re[b] = dr;
+            im[b] = di;
+        } else if (j == 8) {
+            re[b] = di;
+            im[b] = -dr;
+        } else {
+            const float c = tw32_cos(j), sn = tw32_sin(j);
+            re[b] = fmaf(dr, c, di * sn);
+            im[b] = fmaf(di, c, -(dr * sn));
+        }
+    }
+}
+
+// In-place 32-point forward DFT; X[k] ends in register bitrev5(k).  HALF:
+// inputs 16..31 are zero (not read).
+template <bool HALF = false>
+__device__ __forceinline__ void dft32(float (&re)[32], float (&im)[32])
+{
+    if (HALF) dif_stage16_half(re, im);
+    else dif_stage<16>(re, im);
+    dif_stage<8>(re, im);
+    dif_stage<4>(re, im);
+    dif_stage<2>(re, im);
+    dif_stage<1>(re, im);
+}
+
+// Steps B and C: twiddle the DFT output (natural index k2 in register
+// bitrev5(k2), lane n1) and transpose it through this half-wave's LDS rows,
+// so that lane k2 holds element n1 in register n1.
+__device__ __forceinline__ void twiddle_transpose(float (&re)[32], float (&im)[32],
+                                                  const float* tw_re, const float* tw_im,
+                                                  float* tb, int l)
+{
+    float tr[32], ti[32];
+#pragma unroll
+    for (int k2 = 0; k2 < 32; ++k2) {
+        const int r = bitrev5(k2);
+        if (k2 == 0) {
+            tr[0] = re[r];
+            ti[0] = im[r];
+        } else {
+            const float wr = tw_re[k2 * 32 + l], wi = tw_im[k2 * 32 + l];
+            tr[k2] = fmaf(re[r], wr, -(im[r] * wi));
+            ti[k2] = fmaf(re[r], wi, im[r] * wr);
+        }
+    }
+    // The half-wave's own LDS rows: LDS operations of a wave complete in
+    // order; the wave barrier keeps the compiler from moving them across.
+#pragma unroll
+    for (int k2 = 0; k2 < 32; ++k2) tb[k2 * kFftTP + l] = tr[k2];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1) re[n1] = tb[l * kFftTP + n1];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int k2 = 0; k2 < 32; ++k2) tb[k2 * kFftTP + l] = ti[k2];
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int n1 = 0; n1 < 32; ++n1) im[n1] = tb[l * kFftTP + n1];
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Buffer resource over one column: loads past `bytes` return 0 and stores
+// past it are dropped, without touching memory.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes,
+                                             0x00020000);
+}
+
+// HALF: nb <= 512 and nb + extra <= 512 (npad = 1024 from 2 nb - 1): the
+// upper half of every input is zero and of every output column unused.
+template <int OUT, bool HALF>
+__global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
+{
+    // the table: twiddles (re, im), spectrum, bin factors (a, s^2), 0 past nb
+    __shared__ __attribute__((aligned(16))) float tab[5 * kFftN];
+    __shared__ float tbuf[8 * 32 * kFftTP];                       // a region per transform
+    const float* tw_re = tab;                                     // tw[k2 * 32 + n1]
+    const float* tw_im = tab + kFftN;
+    const float* spec = tab + 2 * kFftN;
+    const float* bin_a = tab + 3 * kFftN;
+    const float* bin_s2 = tab + 4 * kFftN;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    {
+        float4 v[5];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) v[r] = reinterpret_cast<const float4*>(p.fft)[tid + 256 * r];
+#pragma unroll
+        for (int r = 0; r < 5; ++r) reinterpret_cast<float4*>(tab)[tid + 256 * r] = v[r];
+    }
+    __syncthreads();
+
+    const int l = lane & 31, half = lane >> 5;
+    const int f = wave * 2 + half;                        // transform within the block
+    float* tb = tbuf + f * 32 * kFftTP;
+    const int64_t ca = (int64_t)blockIdx.x * kFftCols + 2 * f, cb = ca + 1;
+    const bool va = ca < p.ncols, vb = cb < p.ncols;
+    // The wave's 4 columns (its two transforms) under one buffer resource,
+    // which must be wave-uniform: reads past them return 0.
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const int64_t cw = (int64_t)blockIdx.x * kFftCols + 4 * wv;
+    const int64_t rem = p.ncols - cw;
+    const int nw = rem <= 0 ? 0 : rem >= 4 ? 4 : (int)rem;
+
+    // Step A input: lane n1 = l holds z[l + 32 j] in register j.  Every
+    // load is issued before the first is used (no branches between them);
+    // loads past a column's nb samples return 0.
+    constexpr int JN = HALF ? 16 : 32;
+    const __amdgpu_buffer_rsrc_t rin =
+        col_rsrc(p.in + (int64_t)p.nb * (nw ? cw : 0), 4u * (uint32_t)p.nb * (uint32_t)nw);
+    const uint32_t oa = 4u * (uint32_t)(2 * half * p.nb), ob = oa + 4u * (uint32_t)p.nb;
+    float re[32], im[32];
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+        const uint32_t off = 4u * (uint32_t)(l + 32 * j);
+        re[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, oa + off, 0, 0));
+        im[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, ob + off, 0, 0));
+    }
+    // Column factors of the fused weight (see the Toeplitz form): colw, the
+    // cylinder constants and the Parker row.
+    float colw[2] = {0.f, 0.f}, cyl_c[2] = {1.f, 1.f}, cyl_k[2] = {1.f, 1.f};
+    const float* prow[2] = {nullptr, nullptr};
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // a column past ncols takes the last column's factors (its input reads as 0)
+        const int64_t c = std::min(h ? cb : ca, p.ncols - 1);
+        const int view = (int)(c / p.nt), t = (int)(c - (int64_t)view * p.nt);
+        colw[h] = p.scale * (p.wview ? p.wview[view] : 1.f);
+        if (p.cyl == 1) {
+            const float tt = p.t_axis[t];
+            cyl_c[h] = tt * tt + p.cyl_dsd * p.cyl_dsd;
+            cyl_k[h] = p.cyl_dso;
+        } else if (p.cyl == 2) {
+            const float r = p.t_axis[t] / p.cyl_dsd;
+            cyl_k[h] = (p.cyl_dso / p.cyl_dsd) / sqrtf(r * r + 1.f);
+        }
+        if (p.ptab) prow[h] = p.ptab + (int64_t)view * p.nb;
+    }
+
+    if (p.ptab) {
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int mm = min(l + 32 * j, p.nb - 1);
+            re[j] *= prow[0][mm];
+            im[j] *= prow[1][mm];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < JN; ++j) {
+        const int m = l + 32 * j;                       // < 1024: bin tables are 0 past nb
+        const float a = bin_a[m];
+        float wa = colw[0] * a, wb = colw[1] * a;
+        if (p.cyl == 1) {
+            wa *= cyl_k[0] * rsqrtf(bin_s2[m] + cyl_c[0]);
+            wb *= cyl_k[1] * rsqrtf(bin_s2[m] + cyl_c[1]);
+        } else if (p.cyl) {
+            wa *= cyl_k[0];
+            wb *= cyl_k[1];
+        }
+        // samples past nb belong to the next column: weight 0, value 0
+        re[j] = m < p.nb ? re[j] * wa : 0.f;
+        im[j] = m < p.nb ? im[j] * wb : 0.f;
+    }
+
+    dft32<HALF>(re, im);                                        // A
+    twiddle_transpose(re, im, tw_re, tw_im, tb, l);             // B, C
+    dft32(re, im);                                              // D: X[32 k1 + l] in bitrev5(k1)
+    // Spectrum (He / N) and swap: the inverse's step A input, natural order.
+    float ar[32], ai[32];
+#pragma unroll
+    for (int k1 = 0; k1 < 32; ++k1) {
+        const float h = spec[32 * k1 + l];
+        ar[k1] = im[bitrev5(k1)] * h;
+        ai[k1] = re[bitrev5(k1)] * h;
+    }
+    dft32(ar, ai);
+    twiddle_transpose(ar, ai, tw_re, tw_im, tb, l);
+    dft32(ar, ai);                                              // R[32 j1 + l] in bitrev5(j1)
+
+    // Output: column a is Im R, column b is Re R; rows nb .. nrows_out - 1 are 0.
+    constexpr uint32_t esz = OUT == OUT_F32 ? 4u : 2u;
+    const __amdgpu_buffer_rsrc_t ro =
+        col_rsrc(static_cast<char*>(p.out) + (int64_t)p.nrows_out * (nw ? cw : 0) * esz,
+                 esz * (uint32_t)p.nrows_out * (uint32_t)nw);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int64_t c = h ? cb : ca;
+        if (!(h ? vb : va)) continue;
+        float inv = 1.f;
+        if (OUT == OUT_F16S) {
+            const int view = (int)(c / p.nt);
+            const float sc = f16s_scale(p.vbound[view], p.g1);
+            inv = 1.f / sc;                      // a power of two: exact
+            if (c % p.nt == 0 && l == 0) p.vscale[view] = sc;
+        }
+        const uint32_t oc = esz * (uint32_t)((2 * half + h) * p.nrows_out);
+#pragma unroll
+        for (int j1 = 0; j1 < JN; ++j1) {
+            const int n = 32 * j1 + l;
+            if (n >= p.nrows_out) continue;              // the next column's rows
+            const float v = n < p.nb ? (h ? ar[bitrev5(j1)] : ai[bitrev5(j1)]) * inv : 0.f;
+            if (OUT == OUT_F32) {
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), ro, oc + esz * n, 0,
+                                                      0);
+            } else {
+                const uint16_t u = OUT == OUT_BF16 ? __builtin_bit_cast(uint16_t, bf16_rne(v))
+                                                   : __half_as_ushort(__float2half_rn(v));
+                __builtin_amdgcn_raw_buffer_store_b16(u, ro, oc + esz * n, 0, 0);
+            }
+        }
+    }
+}
+
+template <int OUT>
+void launch_fft(const Params& p, hipStream_t st)
+{
+    const int64_t blocks = (p.ncols + kFftCols - 1) / kFftCols;
+    if (blocks >= (int64_t(1) << 31))
+        throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many columns for one launch");
+    if (p.nrows_out <= kFftN / 2)
+        hipLaunchKernelGGL((fft_filter_kernel<OUT, true>), dim3((unsigned)blocks), dim3(256),
+                           0, st, p);
+    else
+        hipLaunchKernelGGL((fft_filter_kernel<OUT, false>), dim3((unsigned)blocks), dim3(256),
+                           0, st, p);
+    SGX_HIP(hipGetLastError());
+}
+
 // Per-view max |weighted input|.  Block (view, y) covers kBoundRows detector
 // rows of one view, a contiguous run of rows x nb samples that it reads as
 // one flat range: 16-byte loads when nb is a multiple of 4 and the input is
@@ -665,10 +989,37 @@ Params make_params(const FilterArgs& a)
     p.gscale = a.gscale;
     p.vscale = a.vscale;
     p.row_tiles_per_block = 0;
+    p.fft = a.fft;
     return p;
 }
 
 }  // namespace
+
+std::vector<float> fft_filter_tables(const float* Hk, int64_t npad, int64_t nb,
+                                     const float* wbin, const double* s, int cyl, double dsd)
+{
+    // SGX_FILTER_TOEPLITZ=1 keeps every filter on the Toeplitz form (A/B runs)
+    const char* env = std::getenv("SGX_FILTER_TOEPLITZ");
+    if (npad != kFftN || nb > kFftN || (env && env[0] == '1')) return {};
+    std::vector<float> t(5 * kFftN, 0.f);
+    for (int64_t m = 0; m < nb; ++m) {
+        float a = wbin ? wbin[m] : 1.f;
+        if (cyl == 2) a *= (float)std::cos(s[m] / dsd);
+        t[3 * kFftN + m] = a;
+        t[4 * kFftN + m] = cyl == 1 ? (float)(s[m] * s[m]) : 0.f;
+    }
+    const double w = 2.0 * M_PI / (double)kFftN;
+    for (int k2 = 0; k2 < 32; ++k2)
+        for (int n1 = 0; n1 < 32; ++n1) {
+            const double ph = w * (double)(n1 * k2);      // W^(n1 k2) = e^(-i ph)
+            t[k2 * 32 + n1] = (float)std::cos(ph);
+            t[kFftN + k2 * 32 + n1] = (float)-std::sin(ph);
+        }
+    for (int k = 0; k < kFftN; ++k)
+        t[2 * kFftN + k] = (float)(0.5 * ((double)Hk[k] + (double)Hk[(kFftN - k) % kFftN]) /
+                                   (double)kFftN);
+    return t;
+}
 
 void launch_filter_weights(const FilterArgs& a, float* out, hipStream_t st)
 {
@@ -691,6 +1042,17 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     if (a.split_f16 && (a.split_bf16 || !a.vbound || !(a.gscale > 0.f)))
         throw Error(SGX_ERR_ARG, "sino_filter: split-FP16 products need the input bound and g scale");
     const int out = a.out_f16s ? OUT_F16S : a.out_bf16 ? OUT_BF16 : OUT_F32;
+    if (a.fft) {
+        // FP32 transforms; the split-product flags do not apply
+        if (a.npad != kFftN || a.nb > kFftN || p.nrows_out > kFftN)
+            throw Error(SGX_ERR_ARG, "sino_filter: the FFT form takes npad = 1024");
+        switch (out) {
+        case OUT_F32: launch_fft<OUT_F32>(p, st); break;
+        case OUT_BF16: launch_fft<OUT_BF16>(p, st); break;
+        default: launch_fft<OUT_F16S>(p, st); break;
+        }
+        return;
+    }
     if (a.split_bf16) {
         switch (out) {
         case OUT_F32: launch_mode<M_BF16X3, OUT_F32>(p, st); break;

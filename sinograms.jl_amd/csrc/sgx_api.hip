@@ -519,8 +519,14 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
         }
         const auto gd = filter_kernel_g(Hk, npad);
         std::vector<float> gf(gd.begin(), gd.end());
-        float* g = static_cast<float*>(ctx->get(W_G, 4 * npad));
-        SGX_HIP(hipMemcpyAsync(g, gf.data(), 4 * npad, hipMemcpyHostToDevice, ctx->stream));
+        // FP32 at npad = 1024: the FFT form (its tables after g)
+        const auto ft = prec == SGX_PREC_FP32
+                            ? sgx::fft_filter_tables(Hk, npad, nb, nullptr, nullptr, 0, 0.0)
+                            : std::vector<float>();
+        gf.insert(gf.end(), ft.begin(), ft.end());
+        float* g = static_cast<float*>(ctx->get(W_G, 4 * gf.size()));
+        SGX_HIP(hipMemcpyAsync(g, gf.data(), 4 * gf.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
         SGX_HIP(hipStreamSynchronize(ctx->stream));
         const int64_t nout = (nb + extra) * ncols;
         const float* din = static_cast<const float*>(
@@ -539,6 +545,7 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
         a.out_bf16 = false;
         a.split_bf16 = prec == SGX_PREC_BF16;
         a.scale = 1.0f;
+        if (!ft.empty()) a.fft = g + npad;
         sgx::launch_sino_filter(a, ctx->stream);
         finish(ctx, out, dout, 4 * nout, flags);
     });
@@ -653,6 +660,7 @@ struct sgx_plan {
           *wview = nullptr, *s_axis = nullptr, *t_axis = nullptr;
     float* wview_owned = nullptr;
     float* ptab = nullptr;          // short-scan Parker weights (ns, na), formed once in FP64
+    float* fft = nullptr;           // FFT-form filter tables (npad = 1024), in `tables`
     float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -709,6 +717,7 @@ struct sgx_plan {
             a.t_axis = t_axis;
         }
         a.ptab = ptab;
+        a.fft = fft;
         return a;
     }
 };
@@ -943,9 +952,11 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         for (int64_t i = 0; i < ns; ++i) gamv[i] = (G.kind == SGX_SINO_FAN_ARC || G.kind == SGX_CT_FAN_ARC)
                                                        ? s[i] / G.dsd : std::atan(s[i] / G.dsd);
         size_t o_wbin = 0, o_gam = 0, o_bet = 0, o_s = 0, o_t = 0;
+        std::vector<float> wbin_host;
         if (G.kind == SGX_SINO_FAN_ARC || G.kind == SGX_SINO_FAN_FLAT) {
             // fbp_sino_weight (src/fbp2/sino-weight.jl:12-16)
-            std::vector<float> w(ns);
+            std::vector<float>& w = wbin_host;
+            w.resize(ns);
             for (int64_t i = 0; i < ns; ++i)
                 w[i] = (float)(std::fabs(G.dso * std::cos(gamv[i]) -
                                          G.source_offset * std::sin(gamv[i])) / G.dsd);
@@ -969,6 +980,11 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             o_s = pack.add(sf.data(), ns * 4);
             o_t = pack.add(tf.data(), nt * 4);
         }
+        // FFT-form filter tables at npad = 1024 (with the per-bin weight factors)
+        const auto ft = sgx::fft_filter_tables(
+            Hk, npad, ns, wbin_host.empty() ? nullptr : wbin_host.data(), s.data(),
+            is3d ? ((G.kind == SGX_CT_FAN_ARC) ? 2 : 1) : 0, G.dsd);
+        const size_t o_fft = ft.empty() ? 0 : pack.add(ft.data(), ft.size() * 4);
         SGX_HIP(hipMalloc(&P->tables, pack.host.size()));
         SGX_HIP(hipMemcpy(P->tables, pack.host.data(), pack.host.size(), hipMemcpyHostToDevice));
         auto* d = static_cast<unsigned char*>(P->tables);
@@ -977,6 +993,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->yds = reinterpret_cast<float*>(d + o_y);
         if (is3d) P->zds = reinterpret_cast<float*>(d + o_z);
         P->g = reinterpret_cast<float*>(d + o_g);
+        if (!ft.empty()) P->fft = reinterpret_cast<float*>(d + o_fft);
         if (o_wbin) P->wbin = reinterpret_cast<float*>(d + o_wbin);
         if (G.parker) {
             P->gam = reinterpret_cast<double*>(d + o_gam);

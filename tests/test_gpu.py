@@ -8,6 +8,7 @@ differs) and its storage error against Float64 is reported.
 
 import ctypes
 import math
+import os
 import sys
 
 import numpy as np
@@ -286,6 +287,9 @@ def test_back2d_window_fallbacks(sg, orc, torch_cuda, kind):
 
 @pytest.mark.parametrize("nb,ncols,npad,extra", [(37, 5, 64, 3), (367, 360, 512, 0),
                                                  (888, 70, 1024, 0), (512, 1000, 1024, 2),
+                                                 (300, 37, 1024, 3), (512, 33, 1024, 0),
+                                                 (1024, 9, 1024, 0),
+                                                 (1, 3, 1024, 0),
                                                  (3000, 40, 4096, 5), (8192, 6, 8192, 0)])
 def test_sino_filter_fp32_vs_oracle(sg, orc, torch_cuda, nb, ncols, npad, extra):
     _say("filter fp32", nb, ncols, npad, extra)
@@ -379,6 +383,82 @@ def test_wide_detector_plans_vs_oracle(sg, orc, torch_cuda):
         e = rel_rmse(sg.fdk(sg.plan_fbp(cg, cig, precision=prec), proj), r32)
         _say(f"  cone ns=2304 {prec} vs FP32 {e:.3g}")
         assert e < tol
+
+
+def _toeplitz_plan(sg, rg, ig, **kw):
+    """The same plan with the Toeplitz (MFMA) form of the filter."""
+    os.environ["SGX_FILTER_TOEPLITZ"] = "1"
+    try:
+        return sg.plan_fbp(rg, ig, **kw)
+    finally:
+        del os.environ["SGX_FILTER_TOEPLITZ"]
+
+
+@pytest.mark.parametrize("arc", [True, False])
+@pytest.mark.parametrize("short", [False, True])
+def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
+    # 600 bins: npad = nextpow2(nb + 1) = 1024, so the plan filters with the
+    # FFT form (fan weight, Parker table and angular weight fused into its
+    # input load)
+    _say("fft-form fan plan", arc, short)
+    cls = sg.SinoFanArc if arc else sg.SinoFanFlat
+    kw = dict(nb=600, d=0.5, dsd=400.0, dod=150.0)
+    rg = cls.short(na=200, **kw) if short else cls(na=200, **kw)
+    ig = sg.ImageGeom((96, 96), (1.0, 1.0))
+    ig = ig.with_mask(ig.circle(rg.rfov()))
+    ell = sg.phantom.random_ellipses(7, 6, 90.0)
+    sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
+    plan = sg.plan_fbp(rg, ig, window=sg.Window(sg.Hann()))
+    assert len(plan.filter) == 1024
+    got = sg.fbp(plan, sino)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.angle_weight(ar) * (orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())
+                                 if short else 1.0)
+    sw = orc.fan_sino_weight(rg.gamma(), rg.dso, rg.dsd)
+    xc, yc = ig.axes()
+    args = (plan.filter, vw, sw, ar, rg.dsd, rg.dso, 0.0, arc, rg.d, 0.0, xc, yc, ig.mask)
+    e32 = rel_rmse(got, orc.fbp_fan(sino, *args, dtype=np.float32))
+    et = rel_rmse(got, sg.fbp(_toeplitz_plan(sg, rg, ig, window=sg.Window(sg.Hann())), sino))
+    _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs the Toeplitz form {et:.3g}")
+    assert e32 < TOL32
+    assert et < 3e-6
+
+
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("short", [False, True])
+def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
+    # 600-bin cone detector: npad = 1024, the FFT form with the cylinder
+    # weight (flat or arc) and the short-scan Parker table fused
+    _say("fft-form fdk plan", arc, short)
+    cls = sg.CtFanArc if arc else sg.CtFanFlat
+    kw = dict(ns=600, nt=24, ds=0.5, dt=1.2)
+    rg = cls.short(na=59, **kw) if short else cls(na=32, **kw)
+    ig = sg.ImageGeom((30, 28, 16), (1.0, 1.0, 1.0))
+    ell = np.array([[2.0, 3.0, 1.0, 8.0, 7.0, 6.0, 1.0], [-4.0, 1.0, -1.0, 3.0, 4.0, 5.0, 0.5]])
+    proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
+    plan = sg.plan_fbp(rg, ig)
+    assert len(plan.filter) == 1024
+    got = sg.fdk(plan, proj)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, arc)[:, :, None] * \
+        orc.angle_weight(ar)
+    if short:
+        vw = vw * orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())[:, None, :]
+    xc, yc, zc = ig.axes()
+    args = (plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, arc, xc, yc, zc,
+            ig.mask)
+    e32 = rel_rmse(got, orc.fdk(proj, *args, dtype=np.float32))
+    ref64 = orc.fdk(proj.astype(np.float64), *args)
+    et = rel_rmse(got, sg.fdk(_toeplitz_plan(sg, rg, ig), proj))
+    e16 = rel_rmse(sg.fdk(sg.plan_fbp(rg, ig, precision="f16s"), proj), ref64)
+    _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs the Toeplitz form {et:.3g}, "
+         f"F16S vs Float64 {e16:.3g}")
+    assert e32 < TOL32
+    assert et < 3e-6
+    # F16S: the FFT form's scaled FP16 output.  Its error is the FP16 storage
+    # rounding averaged over only 32-59 views here (about 1e-4; 3.4e-5 over
+    # the 360 views of the 256^2 x 128 workload)
+    assert e16 < 2.5e-4
 
 
 def test_parker_device_vs_oracle(sg, orc, torch_cuda):
