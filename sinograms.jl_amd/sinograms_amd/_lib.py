@@ -188,8 +188,18 @@ class Context:
             raise SgxError(rc, f"sgx_ctx_create({device}) failed ({rc})")
         self.h = h
         self.device = device
+        self.stream = stream
         if stream is not None:
             self.check(L.sgx_ctx_set_stream(self.h, ctypes.c_void_p(stream)))
+
+    def on_current_stream(self) -> bool:
+        """True when this context's stream is torch's current stream of its
+        device: device-array calls may then return without synchronising,
+        ordered with torch's work like any torch operation."""
+        if self.stream is None:
+            return False
+        import torch
+        return torch.cuda.current_stream(self.device).cuda_stream == self.stream
 
     def check(self, rc):
         if rc != SGX_OK:

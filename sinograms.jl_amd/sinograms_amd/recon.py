@@ -505,7 +505,11 @@ class _Plan:
             img, iptr = out, out.data_ptr()
         else:
             img, iptr = a.out(self.ig.dims)
-        self._ctx.check(_lib.load().sgx_plan_execute(self.h, a.ptr, iptr, a.flags))
+        # device arrays on torch's current stream: stream-ordered, no host wait
+        flags = a.flags
+        if not a.host and self._ctx.on_current_stream():
+            flags |= _lib.SGX_ASYNC
+        self._ctx.check(_lib.load().sgx_plan_execute(self.h, a.ptr, iptr, flags))
         return img
 
     def times(self):
