@@ -86,11 +86,11 @@ def time_steps(fn, steps, warmup):
 
 
 def filter_form(plan):
-    """Which ramp-filter kernel a plan runs: the FFT form at npad = 1024,
-    else the Toeplitz form on MFMA (also when SGX_FILTER_TOEPLITZ=1 was set
-    while the plan was made)."""
-    return "toeplitz" if len(plan.filter) != 1024 or getattr(plan, "_toeplitz", False) \
-        else "fft"
+    """Which ramp-filter kernel a plan runs: the FFT form at npad = 1024 or
+    2048, else the Toeplitz form on MFMA (also when SGX_FILTER_TOEPLITZ=1 was
+    set while the plan was made)."""
+    return "toeplitz" if len(plan.filter) not in (1024, 2048) or \
+        getattr(plan, "_toeplitz", False) else "fft"
 
 
 def filter_figures(plan, fm, nb, ncols, out_bytes):
@@ -113,6 +113,36 @@ def plan_toeplitz(sg, rg, ig, **kw):
         del os.environ["SGX_FILTER_TOEPLITZ"]
     plan._toeplitz = True
     return plan
+
+
+def filter_forms_leg(sg, device, reps):
+    """The plan's filter kernel alone (HIP events around it), the FFT form
+    against the Toeplitz form, for flat-panel cone-beam plans with a 512-bin
+    (npad 1024) and a 1500-bin (npad 2048) detector, 128 rows x 360 views
+    (46080 columns), onto a small volume."""
+    import torch
+    out = {}
+    for ns in (512, 1500):
+        rg = sg.CtFanFlat(ns=ns, nt=128, ds=600.0 / ns, dt=1.2, na=360, dsd=1200.0,
+                          dod=400.0)
+        ig = sg.ImageGeom((32, 32, 8), (4.0, 4.0, 4.0))
+        proj = torch.rand(rg.na, rg.nt, rg.ns, device=device).permute(2, 1, 0)
+        y = sg.recon._empty_colmajor(ig.dims, device)
+        r = {"npad": None}
+        for form in ("fft", "toeplitz"):
+            plan = plan_toeplitz(sg, rg, ig) if form == "toeplitz" else sg.plan_fbp(rg, ig)
+            r["npad"] = len(plan.filter)
+            plan.execute(proj, out=y)
+            fm, _ = kernel_times(plan, proj, y, reps)
+            r[form + "_ms"] = fm
+            assert filter_form(plan) == form
+            del plan
+        r["speedup"] = r["toeplitz_ms"] / r["fft_ms"]
+        r["hbm_GBs_fft"] = 8.0 * ns * rg.nt * rg.na / (r["fft_ms"] * 1e-3) / 1e9
+        out[f"CtFanFlat_{ns}x128x360"] = r
+        log(f"  filter ns={ns} npad={r['npad']}: FFT {r['fft_ms']:.3f} ms, "
+            f"Toeplitz {r['toeplitz_ms']:.3f} ms")
+    return out
 
 
 def kernel_times(plan, proj, out, n):
@@ -542,6 +572,8 @@ def main():
         result["dd_projector"] = dd_leg(sg, dev, max(3, min(args.steps, 10)))
         log("timing the analytic projector ...")
         result["analytic_projector"] = phantom_leg(sg, dev, max(3, min(args.steps, 10)))
+        log("timing the filter forms ...")
+        result["filter_forms"] = filter_forms_leg(sg, dev, max(3, min(args.steps, 10)))
     if not args.no_batch:
         log("batch leg ...")
         result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))

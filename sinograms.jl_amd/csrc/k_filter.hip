@@ -538,160 +538,164 @@ void launch_mode(const Params& p, hipStream_t st)
 // Bound: VALU (about 500 wave instructions per column for both transforms)
 // and HBM (each column read once and written once).
 
-constexpr int kFftN = 1024;
 constexpr int kFftCols = 16;          // columns per block: 8 transforms of 2 columns
 constexpr int kFftTP = 33;            // transpose pitch (floats): conflict-free rows and columns
 
-__host__ __device__ constexpr int bitrev5(int i)
+template <int BITS>
+__host__ __device__ constexpr int bitrev(int i)
 {
-    return ((i & 1) << 4) | ((i & 2) << 2) | (i & 4) | ((i & 8) >> 2) | ((i & 16) >> 4);
+    int r = 0;
+    for (int b = 0; b < BITS; ++b) r |= ((i >> b) & 1) << (BITS - 1 - b);
+    return r;
 }
+constexpr int log2i(int m) { return m <= 1 ? 0 : 1 + log2i(m / 2); }
 
-// cos and sin of 2 pi t / 32, t = 0..15
-__host__ __device__ constexpr float tw32_cos(int t)
+// cos and sin of 2 pi t / 64, t = 0..31 (W_M^t = W_64^(t 64 / M))
+__host__ __device__ constexpr float tw64_cos(int t)
 {
-    constexpr float c[16] = {1.0f, 9.807852804e-01f, 9.238795325e-01f, 8.314696123e-01f,
-                             7.071067812e-01f, 5.555702330e-01f, 3.826834324e-01f,
-                             1.950903220e-01f, 0.0f, -1.950903220e-01f, -3.826834324e-01f,
-                             -5.555702330e-01f, -7.071067812e-01f, -8.314696123e-01f,
-                             -9.238795325e-01f, -9.807852804e-01f};
+    constexpr float c[32] = {
+        1.0f, 9.951847267e-01f, 9.807852804e-01f, 9.569403357e-01f, 9.238795325e-01f,
+        8.819212643e-01f, 8.314696123e-01f, 7.730104534e-01f, 7.071067812e-01f,
+        6.343932842e-01f, 5.555702330e-01f, 4.713967368e-01f, 3.826834324e-01f,
+        2.902846773e-01f, 1.950903220e-01f, 9.801714033e-02f, 0.0f, -9.801714033e-02f,
+        -1.950903220e-01f, -2.902846773e-01f, -3.826834324e-01f, -4.713967368e-01f,
+        -5.555702330e-01f, -6.343932842e-01f, -7.071067812e-01f, -7.730104534e-01f,
+        -8.314696123e-01f, -8.819212643e-01f, -9.238795325e-01f, -9.569403357e-01f,
+        -9.807852804e-01f, -9.951847267e-01f};
     return c[t];
 }
-__host__ __device__ constexpr float tw32_sin(int t)
+__host__ __device__ constexpr float tw64_sin(int t) { return t <= 16 ? tw64_cos(16 - t) : tw64_cos(t - 16); }
+
+// (re, im) *= W_M^t = e^(-2 pi i t / M), 0 <= t < M / 2
+template <int M>
+__device__ __forceinline__ void rot(float& re, float& im, int t)
 {
-    constexpr float s[16] = {0.0f, 1.950903220e-01f, 3.826834324e-01f, 5.555702330e-01f,
-                             7.071067812e-01f, 8.314696123e-01f, 9.238795325e-01f,
-                             9.807852804e-01f, 1.0f, 9.807852804e-01f, 9.238795325e-01f,
-                             8.314696123e-01f, 7.071067812e-01f, 5.555702330e-01f,
-                             3.826834324e-01f, 1.950903220e-01f};
-    return s[t];
+    if (t == 0) return;
+    const float r = re, i = im;
+    if (4 * t == M) {                         // -i
+        re = i;
+        im = -r;
+    } else {
+        const float c = tw64_cos(t * (64 / M)), sn = tw64_sin(t * (64 / M));
+        re = fmaf(r, c, i * sn);
+        im = fmaf(i, c, -(r * sn));
+    }
 }
 
-// One radix-2 DIF stage of a 32-point forward DFT (W = e^(-2 pi i / 32)).
-template <int SPAN>
-__device__ __forceinline__ void dif_stage(float (&re)[32], float (&im)[32])
+// 32-point forward DFT, radix-2 decimation in frequency, in place: natural
+// input order, X[k] ends in register bitrev<5>(k).
+__device__ __forceinline__ void dft32_dif(float* re, float* im)
 {
 #pragma unroll
-    for (int g = 0; g < 32; g += 2 * SPAN) {
+    for (int span = 16; span >= 1; span /= 2) {
 #pragma unroll
-        for (int j = 0; j < SPAN; ++j) {
-            const int a = g + j, b = a + SPAN, t = j * (16 / SPAN);
-            const float dr = re[a] - re[b], di = im[a] - im[b];
-            re[a] += re[b];
-            im[a] += im[b];
-            if (t == 0) {
+        for (int g = 0; g < 32; g += 2 * span) {
+#pragma unroll
+            for (int j = 0; j < span; ++j) {
+                const int a = g + j, b = a + span;
+                float dr = re[a] - re[b], di = im[a] - im[b];
+                re[a] += re[b];
+                im[a] += im[b];
+                rot<32>(dr, di, j * (16 / span));
                 re[b] = dr;
                 im[b] = di;
-            } else if (t == 8) {                 // W^8 = -i
-                re[b] = di;
-                im[b] = -dr;
-            } else {                             // (dr + i di)(c - i s)
-                const float c = tw32_cos(t), sn = tw32_sin(t);
-                re[b] = fmaf(dr, c, di * sn);
-                im[b] = fmaf(di, c, -(dr * sn));
             }
         }
     }
 }
 
-// The first stage when inputs 16..31 are zero (the zero-padded half).
-__device__ __forceinline__ void dif_stage16_half(float (&re)[32], float (&im)[32])
+// M-point forward DFT, radix-2 decimation in time, in place: element n in
+// register bitrev(n), X[k] ends in register k.  HALF: elements M/2 .. M-1
+// (the odd registers) are zero and not read.
+template <int M, bool HALF>
+__device__ __forceinline__ void dft_dit(float (&re)[M], float (&im)[M])
 {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int b = j + 16;
-        const float dr = re[j], di = im[j];
-        if (j == 0) {
-            re[b] = dr;
-            im[b] = di;
-        } else if (j == 8) {
-            re[b] = di;
-            im[b] = -dr;
-        } else {
-            const float c = tw32_cos(j), sn = tw32_sin(j);
-            re[b] = fmaf(dr, c, di * sn);
-            im[b] = fmaf(di, c, -(dr * sn));
+    for (int span = 1; span < M; span *= 2) {
+#pragma unroll
+        for (int g = 0; g < M; g += 2 * span) {
+#pragma unroll
+            for (int j = 0; j < span; ++j) {
+                const int a = g + j, b = a + span;
+                if (HALF && span == 1) {          // x_b = 0
+                    re[b] = re[a];
+                    im[b] = im[a];
+                    continue;
+                }
+                float br = re[b], bi = im[b];
+                rot<M>(br, bi, j * (M / (2 * span)));
+                re[b] = re[a] - br;
+                im[b] = im[a] - bi;
+                re[a] += br;
+                im[a] += bi;
+            }
         }
     }
 }
 
-// In-place 32-point forward DFT; X[k] ends in register bitrev5(k).  HALF:
-// inputs 16..31 are zero (not read).
-template <bool HALF = false>
-__device__ __forceinline__ void dft32(float (&re)[32], float (&im)[32])
-{
-    if (HALF) dif_stage16_half(re, im);
-    else dif_stage<16>(re, im);
-    dif_stage<8>(re, im);
-    dif_stage<4>(re, im);
-    dif_stage<2>(re, im);
-    dif_stage<1>(re, im);
-}
-
-// Steps B and C: twiddle the DFT output (natural index k2 in register
-// bitrev5(k2), lane n1) and transpose it through this half-wave's LDS rows,
-// so that lane k2 holds element n1 in register n1.
-__device__ __forceinline__ void twiddle_transpose(float (&re)[32], float (&im)[32],
+// Steps B and C: twiddle the step-A output (k2 in register k2, lane n1 = l)
+// by W_N^(n1 k2) and transpose it through this half-wave's LDS rows, 32 rows
+// of k2 at a time, so that lane l holds element n1 of k2 = l + 32 h in
+// register 32 h + n1.  In place: a round's registers are dead once written.
+template <int M>
+__device__ __forceinline__ void twiddle_transpose(float (&re)[M], float (&im)[M],
                                                   const float* tw_re, const float* tw_im,
                                                   float* tb, int l)
 {
-    float tr[32], ti[32];
 #pragma unroll
-    for (int k2 = 0; k2 < 32; ++k2) {
-        const int r = bitrev5(k2);
-        if (k2 == 0) {
-            tr[0] = re[r];
-            ti[0] = im[r];
-        } else {
-            const float wr = tw_re[k2 * 32 + l], wi = tw_im[k2 * 32 + l];
-            tr[k2] = fmaf(re[r], wr, -(im[r] * wi));
-            ti[k2] = fmaf(re[r], wi, im[r] * wr);
+    for (int k2 = 1; k2 < M; ++k2) {
+        const float wr = tw_re[k2 * 32 + l], wi = tw_im[k2 * 32 + l];
+        const float r = re[k2], i = im[k2];
+        re[k2] = fmaf(r, wr, -(i * wi));
+        im[k2] = fmaf(r, wi, i * wr);
+    }
+    // LDS operations of a wave complete in order; the wave barriers keep the
+    // compiler from moving them across one another.
+#pragma unroll
+    for (int h = 0; h < M / 32; ++h) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            float* v = c ? im : re;
+#pragma unroll
+            for (int k = 0; k < 32; ++k) tb[k * kFftTP + l] = v[32 * h + k];
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int n1 = 0; n1 < 32; ++n1) v[32 * h + n1] = tb[l * kFftTP + n1];
+            __builtin_amdgcn_wave_barrier();
         }
     }
-    // The half-wave's own LDS rows: LDS operations of a wave complete in
-    // order; the wave barrier keeps the compiler from moving them across.
-#pragma unroll
-    for (int k2 = 0; k2 < 32; ++k2) tb[k2 * kFftTP + l] = tr[k2];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int n1 = 0; n1 < 32; ++n1) re[n1] = tb[l * kFftTP + n1];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int k2 = 0; k2 < 32; ++k2) tb[k2 * kFftTP + l] = ti[k2];
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int n1 = 0; n1 < 32; ++n1) im[n1] = tb[l * kFftTP + n1];
-    __builtin_amdgcn_wave_barrier();
 }
 
-// Buffer resource over one column: loads past `bytes` return 0 and stores
-// past it are dropped, without touching memory.
+// Buffer resource over a range: loads past `bytes` return 0 and stores past
+// it are dropped, without touching memory.
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base, uint32_t bytes)
 {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), 0, (int)bytes,
                                              0x00020000);
 }
 
-// HALF: nb <= 512 and nb + extra <= 512 (npad = 1024 from 2 nb - 1): the
-// upper half of every input is zero and of every output column unused.
-template <int OUT, bool HALF>
-__global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
+// N = 32 M (M = 32: npad 1024, M = 64: npad 2048).  HALF: nb + extra <= N / 2,
+// the upper half of every input is zero and of every output column unused.
+template <int OUT, bool HALF, int M>
+__global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params p)
 {
+    constexpr int N = 32 * M, H = M / 32, BITS = log2i(M);
     // the table: twiddles (re, im), spectrum, bin factors (a, s^2), 0 past nb
-    __shared__ __attribute__((aligned(16))) float tab[5 * kFftN];
-    __shared__ float tbuf[8 * 32 * kFftTP];                       // a region per transform
-    const float* tw_re = tab;                                     // tw[k2 * 32 + n1]
-    const float* tw_im = tab + kFftN;
-    const float* spec = tab + 2 * kFftN;
-    const float* bin_a = tab + 3 * kFftN;
-    const float* bin_s2 = tab + 4 * kFftN;
+    __shared__ __attribute__((aligned(16))) float tab[5 * N];
+    __shared__ float tbuf[8 * 32 * kFftTP];                      // a region per transform
+    const float* tw_re = tab;                                    // tw[k2 * 32 + n1]
+    const float* tw_im = tab + N;
+    const float* spec = tab + 2 * N;
+    const float* bin_a = tab + 3 * N;
+    const float* bin_s2 = tab + 4 * N;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     {
-        float4 v[5];
+        constexpr int R = 5 * N / 1024;                          // float4 per thread
+        float4 v[R];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) v[r] = reinterpret_cast<const float4*>(p.fft)[tid + 256 * r];
+        for (int r = 0; r < R; ++r) v[r] = reinterpret_cast<const float4*>(p.fft)[tid + 256 * r];
 #pragma unroll
-        for (int r = 0; r < 5; ++r) reinterpret_cast<float4*>(tab)[tid + 256 * r] = v[r];
+        for (int r = 0; r < R; ++r) reinterpret_cast<float4*>(tab)[tid + 256 * r] = v[r];
     }
     __syncthreads();
 
@@ -707,21 +711,23 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
     const int64_t rem = p.ncols - cw;
     const int nw = rem <= 0 ? 0 : rem >= 4 ? 4 : (int)rem;
 
-    // Step A input: lane n1 = l holds z[l + 32 j] in register j.  Every
-    // load is issued before the first is used (no branches between them);
-    // loads past a column's nb samples return 0.
-    constexpr int JN = HALF ? 16 : 32;
+    // Step A input: lane n1 = l holds z[l + 32 j] in register bitrev(j).
+    // Every load is issued before the first is used; loads past the wave's
+    // columns return 0.
+    constexpr int JN = HALF ? M / 2 : M;
     const __amdgpu_buffer_rsrc_t rin =
         col_rsrc(p.in + (int64_t)p.nb * (nw ? cw : 0), 4u * (uint32_t)p.nb * (uint32_t)nw);
     const uint32_t oa = 4u * (uint32_t)(2 * half * p.nb), ob = oa + 4u * (uint32_t)p.nb;
-    float re[32], im[32];
+    float re[M], im[M];
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
         const uint32_t off = 4u * (uint32_t)(l + 32 * j);
-        re[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, oa + off, 0, 0));
-        im[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, ob + off, 0, 0));
+        re[bitrev<BITS>(j)] =
+            __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, oa + off, 0, 0));
+        im[bitrev<BITS>(j)] =
+            __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rin, ob + off, 0, 0));
     }
-    // Column factors of the fused weight (see the Toeplitz form): colw, the
+    // Column factors of the fused weight (as in the Toeplitz form): colw, the
     // cylinder constants and the Parker row.
     float colw[2] = {0.f, 0.f}, cyl_c[2] = {1.f, 1.f}, cyl_k[2] = {1.f, 1.f};
     const float* prow[2] = {nullptr, nullptr};
@@ -741,18 +747,17 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
         }
         if (p.ptab) prow[h] = p.ptab + (int64_t)view * p.nb;
     }
-
     if (p.ptab) {
 #pragma unroll
         for (int j = 0; j < JN; ++j) {
             const int mm = min(l + 32 * j, p.nb - 1);
-            re[j] *= prow[0][mm];
-            im[j] *= prow[1][mm];
+            re[bitrev<BITS>(j)] *= prow[0][mm];
+            im[bitrev<BITS>(j)] *= prow[1][mm];
         }
     }
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
-        const int m = l + 32 * j;                       // < 1024: bin tables are 0 past nb
+        const int m = l + 32 * j, r = bitrev<BITS>(j);     // m < N: bin tables are 0 past nb
         const float a = bin_a[m];
         float wa = colw[0] * a, wb = colw[1] * a;
         if (p.cyl == 1) {
@@ -763,34 +768,44 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
             wb *= cyl_k[1];
         }
         // samples past nb belong to the next column: weight 0, value 0
-        re[j] = m < p.nb ? re[j] * wa : 0.f;
-        im[j] = m < p.nb ? im[j] * wb : 0.f;
+        re[r] = m < p.nb ? re[r] * wa : 0.f;
+        im[r] = m < p.nb ? im[r] * wb : 0.f;
     }
 
-    dft32<HALF>(re, im);                                        // A
-    twiddle_transpose(re, im, tw_re, tw_im, tb, l);             // B, C
-    dft32(re, im);                                              // D: X[32 k1 + l] in bitrev5(k1)
-    // Spectrum (He / N) and swap: the inverse's step A input, natural order.
-    float ar[32], ai[32];
+    dft_dit<M, HALF>(re, im);                                   // A
+    twiddle_transpose<M>(re, im, tw_re, tw_im, tb, l);          // B, C
 #pragma unroll
-    for (int k1 = 0; k1 < 32; ++k1) {
-        const float h = spec[32 * k1 + l];
-        ar[k1] = im[bitrev5(k1)] * h;
-        ai[k1] = re[bitrev5(k1)] * h;
+    for (int h = 0; h < H; ++h) dft32_dif(re + 32 * h, im + 32 * h);   // D
+    // X[M k1 + l + 32 h] is in register 32 h + bitrev5(k1).  Spectrum (He / N)
+    // and swap give the inverse's step-A input: element n2 = H k1 + h of lane
+    // l, in register bitrev(n2).
+    float ar[M], ai[M];
+#pragma unroll
+    for (int h = 0; h < H; ++h) {
+#pragma unroll
+        for (int k1 = 0; k1 < 32; ++k1) {
+            const float sp = spec[M * k1 + l + 32 * h];
+            const int src = 32 * h + bitrev<5>(k1), dst = bitrev<BITS>(H * k1 + h);
+            ar[dst] = im[src] * sp;
+            ai[dst] = re[src] * sp;
+        }
     }
-    dft32(ar, ai);
-    twiddle_transpose(ar, ai, tw_re, tw_im, tb, l);
-    dft32(ar, ai);                                              // R[32 j1 + l] in bitrev5(j1)
+    dft_dit<M, false>(ar, ai);
+    twiddle_transpose<M>(ar, ai, tw_re, tw_im, tb, l);
+#pragma unroll
+    for (int h = 0; h < H; ++h) dft32_dif(ar + 32 * h, ai + 32 * h);
+    // R[M j1 + l + 32 h] in register 32 h + bitrev5(j1)
 
     // Output: column a is Im R, column b is Re R; rows nb .. nrows_out - 1 are 0.
     constexpr uint32_t esz = OUT == OUT_F32 ? 4u : 2u;
     const __amdgpu_buffer_rsrc_t ro =
         col_rsrc(static_cast<char*>(p.out) + (int64_t)p.nrows_out * (nw ? cw : 0) * esz,
                  esz * (uint32_t)p.nrows_out * (uint32_t)nw);
+    constexpr int J1N = HALF ? 16 : 32;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int64_t c = h ? cb : ca;
-        if (!(h ? vb : va)) continue;
+    for (int c2 = 0; c2 < 2; ++c2) {
+        const int64_t c = c2 ? cb : ca;
+        if (!(c2 ? vb : va)) continue;
         float inv = 1.f;
         if (OUT == OUT_F16S) {
             const int view = (int)(c / p.nt);
@@ -798,22 +813,35 @@ __global__ __launch_bounds__(256) void fft_filter_kernel(Params p)
             inv = 1.f / sc;                      // a power of two: exact
             if (c % p.nt == 0 && l == 0) p.vscale[view] = sc;
         }
-        const uint32_t oc = esz * (uint32_t)((2 * half + h) * p.nrows_out);
+        const uint32_t oc = esz * (uint32_t)((2 * half + c2) * p.nrows_out);
 #pragma unroll
-        for (int j1 = 0; j1 < JN; ++j1) {
-            const int n = 32 * j1 + l;
-            if (n >= p.nrows_out) continue;              // the next column's rows
-            const float v = n < p.nb ? (h ? ar[bitrev5(j1)] : ai[bitrev5(j1)]) * inv : 0.f;
-            if (OUT == OUT_F32) {
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), ro, oc + esz * n, 0,
-                                                      0);
-            } else {
-                const uint16_t u = OUT == OUT_BF16 ? __builtin_bit_cast(uint16_t, bf16_rne(v))
-                                                   : __half_as_ushort(__float2half_rn(v));
-                __builtin_amdgcn_raw_buffer_store_b16(u, ro, oc + esz * n, 0, 0);
+        for (int h = 0; h < H; ++h) {
+#pragma unroll
+            for (int j1 = 0; j1 < J1N; ++j1) {
+                const int n = M * j1 + 32 * h + l;
+                if (n >= p.nrows_out) continue;          // the next column's rows
+                const int r = 32 * h + bitrev<5>(j1);
+                const float v = n < p.nb ? (c2 ? ar[r] : ai[r]) * inv : 0.f;
+                if (OUT == OUT_F32) {
+                    __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), ro, oc + esz * n,
+                                                          0, 0);
+                } else {
+                    const uint16_t u = OUT == OUT_BF16 ? __builtin_bit_cast(uint16_t, bf16_rne(v))
+                                                       : __half_as_ushort(__float2half_rn(v));
+                    __builtin_amdgcn_raw_buffer_store_b16(u, ro, oc + esz * n, 0, 0);
+                }
             }
         }
     }
+}
+
+template <int OUT, int M>
+void launch_fft_m(const Params& p, hipStream_t st, dim3 grid)
+{
+    if (p.nrows_out <= 16 * M)
+        hipLaunchKernelGGL((fft_filter_kernel<OUT, true, M>), grid, dim3(256), 0, st, p);
+    else
+        hipLaunchKernelGGL((fft_filter_kernel<OUT, false, M>), grid, dim3(256), 0, st, p);
 }
 
 template <int OUT>
@@ -822,12 +850,9 @@ void launch_fft(const Params& p, hipStream_t st)
     const int64_t blocks = (p.ncols + kFftCols - 1) / kFftCols;
     if (blocks >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many columns for one launch");
-    if (p.nrows_out <= kFftN / 2)
-        hipLaunchKernelGGL((fft_filter_kernel<OUT, true>), dim3((unsigned)blocks), dim3(256),
-                           0, st, p);
-    else
-        hipLaunchKernelGGL((fft_filter_kernel<OUT, false>), dim3((unsigned)blocks), dim3(256),
-                           0, st, p);
+    const dim3 grid((unsigned)blocks);
+    if (p.npad == 1024) launch_fft_m<OUT, 32>(p, st, grid);
+    else launch_fft_m<OUT, 64>(p, st, grid);
     SGX_HIP(hipGetLastError());
 }
 
@@ -1000,24 +1025,24 @@ std::vector<float> fft_filter_tables(const float* Hk, int64_t npad, int64_t nb,
 {
     // SGX_FILTER_TOEPLITZ=1 keeps every filter on the Toeplitz form (A/B runs)
     const char* env = std::getenv("SGX_FILTER_TOEPLITZ");
-    if (npad != kFftN || nb > kFftN || (env && env[0] == '1')) return {};
-    std::vector<float> t(5 * kFftN, 0.f);
+    if ((npad != 1024 && npad != 2048) || nb > npad || (env && env[0] == '1')) return {};
+    const int N = (int)npad, M = N / 32;
+    std::vector<float> t(5 * (size_t)N, 0.f);
     for (int64_t m = 0; m < nb; ++m) {
         float a = wbin ? wbin[m] : 1.f;
         if (cyl == 2) a *= (float)std::cos(s[m] / dsd);
-        t[3 * kFftN + m] = a;
-        t[4 * kFftN + m] = cyl == 1 ? (float)(s[m] * s[m]) : 0.f;
+        t[3 * N + m] = a;
+        t[4 * N + m] = cyl == 1 ? (float)(s[m] * s[m]) : 0.f;
     }
-    const double w = 2.0 * M_PI / (double)kFftN;
-    for (int k2 = 0; k2 < 32; ++k2)
+    const double w = 2.0 * M_PI / (double)N;
+    for (int k2 = 0; k2 < M; ++k2)
         for (int n1 = 0; n1 < 32; ++n1) {
             const double ph = w * (double)(n1 * k2);      // W^(n1 k2) = e^(-i ph)
             t[k2 * 32 + n1] = (float)std::cos(ph);
-            t[kFftN + k2 * 32 + n1] = (float)-std::sin(ph);
+            t[N + k2 * 32 + n1] = (float)-std::sin(ph);
         }
-    for (int k = 0; k < kFftN; ++k)
-        t[2 * kFftN + k] = (float)(0.5 * ((double)Hk[k] + (double)Hk[(kFftN - k) % kFftN]) /
-                                   (double)kFftN);
+    for (int k = 0; k < N; ++k)
+        t[2 * N + k] = (float)(0.5 * ((double)Hk[k] + (double)Hk[(N - k) % N]) / (double)N);
     return t;
 }
 
@@ -1044,8 +1069,8 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     const int out = a.out_f16s ? OUT_F16S : a.out_bf16 ? OUT_BF16 : OUT_F32;
     if (a.fft) {
         // FP32 transforms; the split-product flags do not apply
-        if (a.npad != kFftN || a.nb > kFftN || p.nrows_out > kFftN)
-            throw Error(SGX_ERR_ARG, "sino_filter: the FFT form takes npad = 1024");
+        if ((a.npad != 1024 && a.npad != 2048) || p.nrows_out > a.npad)
+            throw Error(SGX_ERR_ARG, "sino_filter: the FFT form takes npad = 1024 or 2048");
         switch (out) {
         case OUT_F32: launch_fft<OUT_F32>(p, st); break;
         case OUT_BF16: launch_fft<OUT_BF16>(p, st); break;

@@ -288,7 +288,8 @@ def test_back2d_window_fallbacks(sg, orc, torch_cuda, kind):
 @pytest.mark.parametrize("nb,ncols,npad,extra", [(37, 5, 64, 3), (367, 360, 512, 0),
                                                  (888, 70, 1024, 0), (512, 1000, 1024, 2),
                                                  (300, 37, 1024, 3), (512, 33, 1024, 0),
-                                                 (1024, 9, 1024, 0),
+                                                 (1024, 9, 1024, 0), (1500, 40, 2048, 0),
+                                                 (1024, 33, 2048, 0), (2047, 9, 2048, 1),
                                                  (1, 3, 1024, 0),
                                                  (3000, 40, 4096, 5), (8192, 6, 8192, 0)])
 def test_sino_filter_fp32_vs_oracle(sg, orc, torch_cuda, nb, ncols, npad, extra):
@@ -396,20 +397,21 @@ def _toeplitz_plan(sg, rg, ig, **kw):
 
 @pytest.mark.parametrize("arc", [True, False])
 @pytest.mark.parametrize("short", [False, True])
-def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
-    # 600 bins: npad = nextpow2(nb + 1) = 1024, so the plan filters with the
-    # FFT form (fan weight, Parker table and angular weight fused into its
-    # input load)
-    _say("fft-form fan plan", arc, short)
+@pytest.mark.parametrize("nb", [600, 1300])
+def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short, nb):
+    # 600 / 1300 bins: npad = nextpow2(nb + 1) = 1024 / 2048, so the plan
+    # filters with the FFT form (fan weight, Parker table and angular weight
+    # fused into its input load)
+    _say("fft-form fan plan", arc, short, nb)
     cls = sg.SinoFanArc if arc else sg.SinoFanFlat
-    kw = dict(nb=600, d=0.5, dsd=400.0, dod=150.0)
+    kw = dict(nb=nb, d=300.0 / nb, dsd=400.0, dod=150.0)
     rg = cls.short(na=200, **kw) if short else cls(na=200, **kw)
     ig = sg.ImageGeom((96, 96), (1.0, 1.0))
     ig = ig.with_mask(ig.circle(rg.rfov()))
     ell = sg.phantom.random_ellipses(7, 6, 90.0)
     sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
     plan = sg.plan_fbp(rg, ig, window=sg.Window(sg.Hann()))
-    assert len(plan.filter) == 1024
+    assert len(plan.filter) == (1024 if nb < 1024 else 2048)
     got = sg.fbp(plan, sino)
     ar = rg.ar(f32=True).astype(np.float32)
     vw = orc.angle_weight(ar) * (orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())
@@ -421,23 +423,26 @@ def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
     et = rel_rmse(got, sg.fbp(_toeplitz_plan(sg, rg, ig, window=sg.Window(sg.Hann())), sino))
     _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs the Toeplitz form {et:.3g}")
     assert e32 < TOL32
-    assert et < 3e-6
+    # the Toeplitz form's FP32 sums over 1300 bins carry ~5e-6 of their own
+    # (the FFT form is the closer of the two to the oracle there)
+    assert et < (3e-6 if nb < 1024 else TOL32)
 
 
 @pytest.mark.parametrize("arc", [False, True])
 @pytest.mark.parametrize("short", [False, True])
-def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
-    # 600-bin cone detector: npad = 1024, the FFT form with the cylinder
-    # weight (flat or arc) and the short-scan Parker table fused
-    _say("fft-form fdk plan", arc, short)
+@pytest.mark.parametrize("ns", [600, 1100])
+def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short, ns):
+    # 600 / 1100-bin cone detector: npad = 1024 / 2048, the FFT form with the
+    # cylinder weight (flat or arc) and the short-scan Parker table fused
+    _say("fft-form fdk plan", arc, short, ns)
     cls = sg.CtFanArc if arc else sg.CtFanFlat
-    kw = dict(ns=600, nt=24, ds=0.5, dt=1.2)
+    kw = dict(ns=ns, nt=24, ds=300.0 / ns, dt=1.2)
     rg = cls.short(na=59, **kw) if short else cls(na=32, **kw)
     ig = sg.ImageGeom((30, 28, 16), (1.0, 1.0, 1.0))
     ell = np.array([[2.0, 3.0, 1.0, 8.0, 7.0, 6.0, 1.0], [-4.0, 1.0, -1.0, 3.0, 4.0, 5.0, 0.5]])
     proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
     plan = sg.plan_fbp(rg, ig)
-    assert len(plan.filter) == 1024
+    assert len(plan.filter) == (1024 if ns < 1024 else 2048)
     got = sg.fdk(plan, proj)
     ar = rg.ar(f32=True).astype(np.float32)
     vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, arc)[:, :, None] * \
@@ -454,7 +459,7 @@ def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short):
     _say(f"  rel-RMSE vs Float32 oracle {e32:.3g}, vs the Toeplitz form {et:.3g}, "
          f"F16S vs Float64 {e16:.3g}")
     assert e32 < TOL32
-    assert et < 3e-6
+    assert et < (3e-6 if ns < 1024 else TOL32)
     # F16S: the FFT form's scaled FP16 output.  Its error is the FP16 storage
     # rounding averaged over only 32-59 views here (about 1e-4; 3.4e-5 over
     # the 360 views of the 256^2 x 128 workload)
