@@ -1,7 +1,9 @@
-// K4 + K5: ramp filtering of sinogram / projection rows as a batched Toeplitz
-// contraction on MFMA, with the pre-weights (fan sinogram weight, FDK
-// cylinder weight, Parker short-scan weight in FP64, angular weight) fused
-// into the operand load.  For gfx950.
+// K4 + K5: ramp filtering of sinogram / projection rows, with the pre-weights
+// (fan sinogram weight, FDK cylinder weight, Parker short-scan weight in FP64,
+// angular weight) fused into the operand load.  For gfx950.  Two forms: an
+// FFT on registers at npad = 1024 or 2048 (the plans' usual case, below the
+// Toeplitz form), and for any other npad a batched Toeplitz contraction on
+// MFMA.
 //
 // Replaces fbp_sino_filter (src/fbp/filter.jl:122-147): the reference zero-pads
 // every column to npad, multiplies its FFT by the real, even Hk and keeps the
@@ -512,31 +514,32 @@ void launch_mode(const Params& p, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// FFT form, npad = 1024: the reference's own algorithm (zero-pad, fft, .* Hk,
-// ifft, real part), on two columns at once.  z = x_a + i x_b (the weighted
-// columns) is transformed; the response enters as He = (Hk[k] + Hk[-k]) / 2,
-// the even part of Hk (real(ifft(fft(x) .* Hk)) = x circularly convolved
-// with real(ifft(Hk)) = ifft(He)), and since He is real and even,
-// ifft(fft(z) .* He) = y_a + i y_b exactly: the real part is column a's
-// output and the imaginary part column b's.
+// FFT form, npad = 1024 or 2048: the reference's own algorithm (zero-pad,
+// fft, .* Hk, ifft, real part), on two columns at once.  z = x_a + i x_b (the
+// weighted columns) is transformed; the response enters as
+// He = (Hk[k] + Hk[-k]) / 2, the even part of Hk (real(ifft(fft(x) .* Hk)) =
+// x circularly convolved with real(ifft(Hk)) = ifft(He)), and since He is real
+// and even, ifft(fft(z) .* He) = y_a + i y_b exactly: the real part is column
+// a's output and the imaginary part column b's.
 //
-// A half-wave (32 lanes) owns one transform of N = 32 x 32 points, held in
-// registers (32 complex per lane), by the four-step method:
-//   n = n1 + 32 n2, k = 32 k1 + k2;   lane n1 holds z[n1 + 32 n2], n2 = 0..31
-//   A: 32-point DFT over n2 in registers        Y[n1, k2]
-//   B: Y[n1, k2] *= W_N^(n1 k2)                  (table in LDS)
-//   C: transpose through LDS (wave-local)         lane k2 holds Y[., k2]
-//   D: 32-point DFT over n1 in registers        X[32 k1 + k2] at lane k2
+// A half-wave (32 lanes) owns one transform of N = 32 M points (M = 32 or
+// 64), held in registers (M complex per lane), by the four-step method:
+//   n = n1 + 32 n2, k = M k1 + k2;   lane n1 holds z[n1 + 32 n2], n2 < M
+//   A: M-point DFT over n2 in registers           Y[n1, k2]
+//   B: Y[n1, k2] *= W_N^(n1 k2)                    (table in LDS)
+//   C: transpose through LDS (wave-local)           lane l holds Y[., l + 32 h]
+//   D: M/32 32-point DFTs over n1 in registers    X[M k1 + l + 32 h] at lane l
 // The inverse is the same forward transform applied to the swapped spectrum:
 // ifft(Y) = swap(fft(swap(Y))) / N with swap(a + ib) = b + ia, and 1/N is in
-// the spectrum table.  Lane k2 holding k1 is exactly the layout step A takes
-// (n1 = k2, n2 = k1), so the inverse needs no extra data movement, and its
-// output R[32 j1 + j2] lands at lane j2: rows 32 j1 + j2, stored coalesced.
-// The 32-point DFTs are radix-2 decimation in frequency on registers, their
-// output in bit-reversed register order (a renaming, resolved at compile time).
+// the spectrum table.  Element M k1 + l + 32 h of lane l is element
+// l + 32 n2 with n2 = (M/32) k1 + h: exactly the layout step A takes, so the
+// inverse needs no extra data movement, and its output R[M j1 + 32 h + l]
+// lands at lane l: rows stored coalesced.  Step A is radix-2 decimation in
+// time (inputs in bit-reversed registers), step D decimation in frequency
+// (outputs in bit-reversed registers): renamings resolved at compile time.
 //
-// Bound: VALU (about 500 wave instructions per column for both transforms)
-// and HBM (each column read once and written once).
+// Bound: VALU (about 680 wave instructions per column at M = 32, both
+// transforms and the weights) and HBM (each column read once and written once).
 
 constexpr int kFftCols = 16;          // columns per block: 8 transforms of 2 columns
 constexpr int kFftTP = 33;            // transpose pitch (floats): conflict-free rows and columns
