@@ -86,10 +86,10 @@ def time_steps(fn, steps, warmup):
 
 
 def filter_form(plan):
-    """Which ramp-filter kernel a plan runs: the FFT form at npad = 1024 or
-    2048, else the Toeplitz form on MFMA (also when SGX_FILTER_TOEPLITZ=1 was
+    """Which ramp-filter kernel a plan runs: the FFT form at npad = 512, 1024
+    or 2048, else the Toeplitz form on MFMA (also when SGX_FILTER_TOEPLITZ=1 was
     set while the plan was made)."""
-    return "toeplitz" if len(plan.filter) not in (1024, 2048) or \
+    return "toeplitz" if len(plan.filter) not in (512, 1024, 2048) or \
         getattr(plan, "_toeplitz", False) else "fft"
 
 

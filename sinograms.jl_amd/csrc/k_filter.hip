@@ -1,7 +1,7 @@
 // K4 + K5: ramp filtering of sinogram / projection rows, with the pre-weights
 // (fan sinogram weight, FDK cylinder weight, Parker short-scan weight in FP64,
 // angular weight) fused into the operand load.  For gfx950.  Two forms: an
-// FFT on registers at npad = 1024 or 2048 (the plans' usual case, below the
+// FFT on registers at npad = 512, 1024 or 2048 (the plans' usual case, below the
 // Toeplitz form), and for any other npad a batched Toeplitz contraction on
 // MFMA.
 //
@@ -71,7 +71,8 @@ struct Params {
     float gscale;                 // split-FP16: power of two applied to g
     float* vscale;                // F16S: per-view output scale (written)
     int row_tiles_per_block;
-    const float* fft;             // FFT form (npad = 1024): twiddles and spectrum, or null
+    const float* fft;             // FFT form: its table, or null
+    int fft_n;                    // FFT form: transform size (1024 or 2048)
 };
 
 // F16S output scale of a view: the power of two s >= bound * g1 / 2^14, so
@@ -514,7 +515,8 @@ void launch_mode(const Params& p, hipStream_t st)
 }
 
 // ---------------------------------------------------------------------------
-// FFT form, npad = 1024 or 2048: the reference's own algorithm (zero-pad,
+// FFT form, npad = 512, 1024 or 2048 (512 as a 1024-point transform, see
+// fft_filter_tables): the reference's own algorithm (zero-pad,
 // fft, .* Hk, ifft, real part), on two columns at once.  z = x_a + i x_b (the
 // weighted columns) is transformed; the response enters as
 // He = (Hk[k] + Hk[-k]) / 2, the even part of Hk (real(ifft(fft(x) .* Hk)) =
@@ -677,7 +679,7 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* base, uin
                                              0x00020000);
 }
 
-// N = 32 M (M = 32: npad 1024, M = 64: npad 2048).  HALF: nb + extra <= N / 2,
+// N = 32 M (M = 32: npad 512 or 1024, M = 64: npad 2048).  HALF: nb + extra <= N / 2,
 // the upper half of every input is zero and of every output column unused.
 template <int OUT, bool HALF, int M>
 __global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params p)
@@ -854,7 +856,7 @@ void launch_fft(const Params& p, hipStream_t st)
     if (blocks >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many columns for one launch");
     const dim3 grid((unsigned)blocks);
-    if (p.npad == 1024) launch_fft_m<OUT, 32>(p, st, grid);
+    if (p.fft_n == 1024) launch_fft_m<OUT, 32>(p, st, grid);
     else launch_fft_m<OUT, 64>(p, st, grid);
     SGX_HIP(hipGetLastError());
 }
@@ -1018,6 +1020,7 @@ Params make_params(const FilterArgs& a)
     p.vscale = a.vscale;
     p.row_tiles_per_block = 0;
     p.fft = a.fft;
+    p.fft_n = a.fft_n;
     return p;
 }
 
@@ -1028,8 +1031,13 @@ std::vector<float> fft_filter_tables(const float* Hk, int64_t npad, int64_t nb,
 {
     // SGX_FILTER_TOEPLITZ=1 keeps every filter on the Toeplitz form (A/B runs)
     const char* env = std::getenv("SGX_FILTER_TOEPLITZ");
-    if ((npad != 1024 && npad != 2048) || nb > npad || (env && env[0] == '1')) return {};
-    const int N = (int)npad, M = N / 32;
+    if ((npad != 512 && npad != 1024 && npad != 2048) || nb > npad || (env && env[0] == '1'))
+        return {};
+    // npad = 512 runs as a 1024-point transform: circular convolution modulo
+    // 512 equals convolution modulo 1024 with g extended periodically, whose
+    // spectrum is g's 512-point spectrum on the even bins and 0 on the odd
+    // ones (rows below 512 of the result are the reference's).
+    const int N = npad < 1024 ? 1024 : (int)npad, M = N / 32, r = N / (int)npad;
     std::vector<float> t(5 * (size_t)N, 0.f);
     for (int64_t m = 0; m < nb; ++m) {
         float a = wbin ? wbin[m] : 1.f;
@@ -1044,8 +1052,11 @@ std::vector<float> fft_filter_tables(const float* Hk, int64_t npad, int64_t nb,
             t[k2 * 32 + n1] = (float)std::cos(ph);
             t[N + k2 * 32 + n1] = (float)-std::sin(ph);
         }
-    for (int k = 0; k < N; ++k)
-        t[2 * N + k] = (float)(0.5 * ((double)Hk[k] + (double)Hk[(N - k) % N]) / (double)N);
+    const int P = (int)npad;
+    for (int k = 0; k < N; k += r) {
+        const int q = k / r;
+        t[2 * N + k] = (float)(0.5 * ((double)Hk[q] + (double)Hk[(P - q) % P]) / (double)P);
+    }
     return t;
 }
 
@@ -1072,8 +1083,8 @@ void launch_sino_filter(const FilterArgs& a, hipStream_t st)
     const int out = a.out_f16s ? OUT_F16S : a.out_bf16 ? OUT_BF16 : OUT_F32;
     if (a.fft) {
         // FP32 transforms; the split-product flags do not apply
-        if ((a.npad != 1024 && a.npad != 2048) || p.nrows_out > a.npad)
-            throw Error(SGX_ERR_ARG, "sino_filter: the FFT form takes npad = 1024 or 2048");
+        if ((a.fft_n != 1024 && a.fft_n != 2048) || p.nrows_out > a.npad || a.npad > a.fft_n)
+            throw Error(SGX_ERR_ARG, "sino_filter: bad FFT-form table");
         switch (out) {
         case OUT_F32: launch_fft<OUT_F32>(p, st); break;
         case OUT_BF16: launch_fft<OUT_BF16>(p, st); break;

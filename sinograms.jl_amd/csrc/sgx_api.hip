@@ -545,7 +545,10 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
         a.out_bf16 = false;
         a.split_bf16 = prec == SGX_PREC_BF16;
         a.scale = 1.0f;
-        if (!ft.empty()) a.fft = g + npad;
+        if (!ft.empty()) {
+            a.fft = g + npad;
+            a.fft_n = (int)(ft.size() / 5);
+        }
         sgx::launch_sino_filter(a, ctx->stream);
         finish(ctx, out, dout, 4 * nout, flags);
     });
@@ -660,7 +663,8 @@ struct sgx_plan {
           *wview = nullptr, *s_axis = nullptr, *t_axis = nullptr;
     float* wview_owned = nullptr;
     float* ptab = nullptr;          // short-scan Parker weights (ns, na), formed once in FP64
-    float* fft = nullptr;           // FFT-form filter tables (npad = 1024), in `tables`
+    float* fft = nullptr;           // FFT-form filter table (npad 512-2048), in `tables`
+    int fft_n = 0;                  // its transform size
     float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -718,6 +722,7 @@ struct sgx_plan {
         }
         a.ptab = ptab;
         a.fft = fft;
+        a.fft_n = fft_n;
         return a;
     }
 };
@@ -993,7 +998,10 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->yds = reinterpret_cast<float*>(d + o_y);
         if (is3d) P->zds = reinterpret_cast<float*>(d + o_z);
         P->g = reinterpret_cast<float*>(d + o_g);
-        if (!ft.empty()) P->fft = reinterpret_cast<float*>(d + o_fft);
+        if (!ft.empty()) {
+            P->fft = reinterpret_cast<float*>(d + o_fft);
+            P->fft_n = (int)(ft.size() / 5);
+        }
         if (o_wbin) P->wbin = reinterpret_cast<float*>(d + o_wbin);
         if (G.parker) {
             P->gam = reinterpret_cast<double*>(d + o_gam);

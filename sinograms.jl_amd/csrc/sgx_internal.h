@@ -208,16 +208,19 @@ struct FilterArgs {
     const unsigned int* vbound;
     float g1;               // sum |g| over the band the filter uses
     float* vscale;          // written per view
-    // FFT form (npad = 1024 or 2048): the table of fft_filter_tables; null
-    // selects the Toeplitz form
+    // FFT form (npad = 512, 1024 or 2048): the table of fft_filter_tables and
+    // its transform size (1024 or 2048); null selects the Toeplitz form
     const float* fft = nullptr;
+    int fft_n = 0;
 };
-// Host: the FFT form's table for the response Hk, 5 x npad floats: the
-// twiddles W_npad^(n1 k2) as re[k2 * 32 + n1] then im, the spectrum
-// (Hk[k] + Hk[-k]) / (2 npad), and the per-bin factors of the fused weight
+// Host: the FFT form's table for the response Hk, 5 x N floats with the
+// transform size N = max(npad, 1024): the twiddles W_N^(n1 k2) as
+// re[k2 * 32 + n1] then im, the spectrum (Hk[k] + Hk[-k]) / (2 npad) (at
+// npad = 512 on the even bins of N, 0 on the odd ones), and the per-bin
+// factors of the fused weight
 // (wbin[m], times cos(s[m] / dsd) for the arc cylinder weight; s[m]^2 for the
 // flat one), 0 past nb.  wbin and s may be null (no such factor).  Empty when
-// npad is not 1024 or 2048, or when the environment sets SGX_FILTER_TOEPLITZ=1.
+// npad is not 512, 1024 or 2048, or when the environment sets SGX_FILTER_TOEPLITZ=1.
 std::vector<float> fft_filter_tables(const float* Hk, int64_t npad, int64_t nb,
                                      const float* wbin, const double* s, int cyl, double dsd);
 void launch_sino_filter(const FilterArgs& a, hipStream_t st);

@@ -397,11 +397,11 @@ def _toeplitz_plan(sg, rg, ig, **kw):
 
 @pytest.mark.parametrize("arc", [True, False])
 @pytest.mark.parametrize("short", [False, True])
-@pytest.mark.parametrize("nb", [600, 1300])
+@pytest.mark.parametrize("nb", [400, 600, 1300])
 def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short, nb):
-    # 600 / 1300 bins: npad = nextpow2(nb + 1) = 1024 / 2048, so the plan
-    # filters with the FFT form (fan weight, Parker table and angular weight
-    # fused into its input load)
+    # 400 / 600 / 1300 bins: npad = nextpow2(nb + 1) = 512 / 1024 / 2048, so
+    # the plan filters with the FFT form (fan weight, Parker table and angular
+    # weight fused into its input load; npad 512 as a 1024-point transform)
     _say("fft-form fan plan", arc, short, nb)
     cls = sg.SinoFanArc if arc else sg.SinoFanFlat
     kw = dict(nb=nb, d=300.0 / nb, dsd=400.0, dod=150.0)
@@ -411,7 +411,7 @@ def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short, nb):
     ell = sg.phantom.random_ellipses(7, 6, 90.0)
     sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
     plan = sg.plan_fbp(rg, ig, window=sg.Window(sg.Hann()))
-    assert len(plan.filter) == (1024 if nb < 1024 else 2048)
+    assert len(plan.filter) == 1 << nb.bit_length()
     got = sg.fbp(plan, sino)
     ar = rg.ar(f32=True).astype(np.float32)
     vw = orc.angle_weight(ar) * (orc.parker_fan_short(rg.gamma(), ar, rg.gamma_max())
@@ -430,9 +430,9 @@ def test_fft_filter_fan_plan_vs_oracle(sg, orc, torch_cuda, arc, short, nb):
 
 @pytest.mark.parametrize("arc", [False, True])
 @pytest.mark.parametrize("short", [False, True])
-@pytest.mark.parametrize("ns", [600, 1100])
+@pytest.mark.parametrize("ns", [400, 600, 1100])
 def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short, ns):
-    # 600 / 1100-bin cone detector: npad = 1024 / 2048, the FFT form with the
+    # 400 / 600 / 1100-bin cone detector: npad = 512 / 1024 / 2048, the FFT form with the
     # cylinder weight (flat or arc) and the short-scan Parker table fused
     _say("fft-form fdk plan", arc, short, ns)
     cls = sg.CtFanArc if arc else sg.CtFanFlat
@@ -442,7 +442,7 @@ def test_fft_filter_fdk_plan_vs_oracle(sg, orc, torch_cuda, arc, short, ns):
     ell = np.array([[2.0, 3.0, 1.0, 8.0, 7.0, 6.0, 1.0], [-4.0, 1.0, -1.0, 3.0, 4.0, 5.0, 0.5]])
     proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
     plan = sg.plan_fbp(rg, ig)
-    assert len(plan.filter) == (1024 if ns < 1024 else 2048)
+    assert len(plan.filter) == 1 << ns.bit_length()
     got = sg.fdk(plan, proj)
     ar = rg.ar(f32=True).astype(np.float32)
     vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, arc)[:, :, None] * \
