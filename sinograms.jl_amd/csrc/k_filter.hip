@@ -73,6 +73,7 @@ struct Params {
     int row_tiles_per_block;
     const float* fft;             // FFT form: its table, or null
     int fft_n;                    // FFT form: transform size (1024 or 2048)
+    const float* wtab;            // FFT form, cone beam: (nb, nt) bin x cylinder weights
 };
 
 // F16S output scale of a view: the power of two s >= bound * g1 / 2^14, so
@@ -760,6 +761,25 @@ __global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params
             im[bitrev<BITS>(j)] *= prow[1][mm];
         }
     }
+    if (M == 32 && p.wtab) {
+        // cone beam: the bin and cylinder factors from the plan's (nb, nt)
+        // table, one coalesced load per sample (rows of the two columns).
+        // Not at M = 64: those registers are already full (the loads in
+        // flight would spill).
+        const int64_t ta = std::min(ca, p.ncols - 1) % p.nt, tb2 = std::min(cb, p.ncols - 1) % p.nt;
+        const __amdgpu_buffer_rsrc_t rw =
+            col_rsrc(p.wtab, 4u * (uint32_t)p.nb * (uint32_t)p.nt);
+        const uint32_t wa0 = 4u * (uint32_t)(ta * p.nb), wb0 = 4u * (uint32_t)(tb2 * p.nb);
+#pragma unroll
+        for (int j = 0; j < JN; ++j) {
+            const int m = l + 32 * j, r = bitrev<BITS>(j);
+            const uint32_t off = 4u * (uint32_t)m;
+            const float fa = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rw, wa0 + off, 0, 0));
+            const float fb = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rw, wb0 + off, 0, 0));
+            re[r] = m < p.nb ? re[r] * (colw[0] * fa) : 0.f;
+            im[r] = m < p.nb ? im[r] * (colw[1] * fb) : 0.f;
+        }
+    } else {
 #pragma unroll
     for (int j = 0; j < JN; ++j) {
         const int m = l + 32 * j, r = bitrev<BITS>(j);     // m < N: bin tables are 0 past nb
@@ -775,6 +795,7 @@ __global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params
         // samples past nb belong to the next column: weight 0, value 0
         re[r] = m < p.nb ? re[r] * wa : 0.f;
         im[r] = m < p.nb ? im[r] * wb : 0.f;
+    }
     }
 
     dft_dit<M, HALF>(re, im);                                   // A
@@ -1021,6 +1042,7 @@ Params make_params(const FilterArgs& a)
     p.row_tiles_per_block = 0;
     p.fft = a.fft;
     p.fft_n = a.fft_n;
+    p.wtab = a.wtab;
     return p;
 }
 

@@ -665,6 +665,7 @@ struct sgx_plan {
     float* ptab = nullptr;          // short-scan Parker weights (ns, na), formed once in FP64
     float* fft = nullptr;           // FFT-form filter table (npad 512-2048), in `tables`
     int fft_n = 0;                  // its transform size
+    float* wtab = nullptr;          // cone beam on the FFT form: (ns, nt) weights, in `tables`
     float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
@@ -723,6 +724,7 @@ struct sgx_plan {
         a.ptab = ptab;
         a.fft = fft;
         a.fft_n = fft_n;
+        a.wtab = wtab;
         return a;
     }
 };
@@ -990,6 +992,23 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             Hk, npad, ns, wbin_host.empty() ? nullptr : wbin_host.data(), s.data(),
             is3d ? ((G.kind == SGX_CT_FAN_ARC) ? 2 : 1) : 0, G.dsd);
         const size_t o_fft = ft.empty() ? 0 : pack.add(ft.data(), ft.size() * 4);
+        // cone beam on the FFT form: wbin x cylinder weight per (bin, row), in
+        // FP64 then rounded (fdk_weight_cyl, src/fbp3/fdk.jl:21-45)
+        size_t o_wtab = 0;
+        if (!ft.empty() && is3d) {
+            std::vector<float> wt((size_t)ns * nt);
+            const bool arc = G.kind == SGX_CT_FAN_ARC;
+            for (int64_t it = 0; it < nt; ++it) {
+                const double tt = G.dt * ((double)it - ((nt - 1) / 2.0 + (double)(float)G.offset_t));
+                for (int64_t i = 0; i < ns; ++i) {
+                    const double w = arc ? (G.dso / G.dsd) * std::cos(s[i] / G.dsd) /
+                                               std::sqrt((tt / G.dsd) * (tt / G.dsd) + 1.0)
+                                         : G.dso / std::sqrt(s[i] * s[i] + tt * tt + G.dsd * G.dsd);
+                    wt[it * ns + i] = (float)((wbin_host.empty() ? 1.0 : (double)wbin_host[i]) * w);
+                }
+            }
+            o_wtab = pack.add(wt.data(), wt.size() * 4);
+        }
         SGX_HIP(hipMalloc(&P->tables, pack.host.size()));
         SGX_HIP(hipMemcpy(P->tables, pack.host.data(), pack.host.size(), hipMemcpyHostToDevice));
         auto* d = static_cast<unsigned char*>(P->tables);
@@ -1001,6 +1020,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         if (!ft.empty()) {
             P->fft = reinterpret_cast<float*>(d + o_fft);
             P->fft_n = (int)(ft.size() / 5);
+            if (o_wtab) P->wtab = reinterpret_cast<float*>(d + o_wtab);
         }
         if (o_wbin) P->wbin = reinterpret_cast<float*>(d + o_wbin);
         if (G.parker) {

@@ -212,6 +212,9 @@ struct FilterArgs {
     // its transform size (1024 or 2048); null selects the Toeplitz form
     const float* fft = nullptr;
     int fft_n = 0;
+    // FFT form, cone beam: the per-(bin, row) part of the fused weight,
+    // wbin[m] times the cylinder weight of (s[m], t[row]), (nb, nt), or null
+    const float* wtab = nullptr;
 };
 // Host: the FFT form's table for the response Hk, 5 x N floats with the
 // transform size N = max(npad, 1024): the twiddles W_N^(n1 k2) as
