@@ -5,12 +5,16 @@ Workload (the configuration the headline metric is quoted on):
   360 views over 360 deg; 256 x 256 x 128 volume at 1 mm; 12 random
   ellipsoids (seed 0) with analytic cone-beam projections (Float64 -> Float32).
 
-A step is one fdk(plan, proj): fused weights + Toeplitz ramp filter + voxel-
-driven backprojection, with the projections already resident in HBM.  value
-is voxel-view updates per second of the whole step for the plain FP32 form;
-the BF16-storage form is reported beside it.  The roofline object describes
-the dominant kernel (the backprojector) under the gather-equivalent model:
-16 B per voxel-view update (4 taps x 4 B) for FP32 storage, 8 B for BF16.
+A step is one fdk(plan, proj): fused weights + ramp filter (FFT form) +
+voxel-driven backprojection, with the projections already resident in HBM.
+value is voxel-view updates per second of the whole step for the plain FP32
+form; the 16-bit storage forms are reported beside it.  The roofline object
+describes the dominant kernel (the backprojector).  Its taps come from LDS,
+so it is bound by instruction issue: `frac` is its VALU issue rate (rocprofv3
+instruction counts of the committed summary over the launch time measured
+here) against the MI355X issue ceiling; its HBM traffic, measured and
+compulsory, is reported beside it, and the gather-equivalent model figure
+(16 B per voxel-view update for FP32 storage) only as a labelled model.
 
 The 2-D configurations (parallel beam; arc fan beam, full and short scan)
 are reported beside it under "slices", the distance-driven projector pair
@@ -53,24 +57,93 @@ def make_problem(device):
     return sg, rg, ig, ell, proj
 
 
-TRAFFIC_FILE = "profiles/fdk256_counters.md"
+# rocprofv3 counter summaries (text, under profiles/).  Each holds lines
+#   counters <kernel> <source-tag> <sha16> key=value ...
+# with per-launch means (valu_insts, lds_insts, fetch_bytes, write_bytes)
+# and the hash of the kernel sources the counters were taken from.
+COUNTER_FILES = ("profiles/fdk256_counters.md", "profiles/slices_counters.md")
+KERNEL_SOURCES = {
+    "k_cbct": ("sinograms.jl_amd/csrc/k_cbct.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
+    "k_back2d": ("sinograms.jl_amd/csrc/k_back2d.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
+    "k_filter": ("sinograms.jl_amd/csrc/k_filter.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
+}
+# VALU issue ceiling of an MI355X: 256 CUs x 4 SIMDs, one wave64 VALU
+# instruction per SIMD every 4 cycles (16 lanes per cycle), 2.4 GHz.
+VALU_PEAK_G = 256 * 4 * 2.4e9 / 4 / 1e9
 
 
-def measured_traffic(kernel):
-    """HBM bytes per launch of `kernel` as recorded in the committed rocprofv3
-    counter summary (FETCH_SIZE and WRITE_SIZE, calibrated against a copy
-    kernel of known size), or None when the summary does not list it."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_FILE)
-    try:
-        with open(path) as f:
-            for line in f:
-                parts = line.split()
-                if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and \
-                        parts[1] == kernel:
-                    return float(parts[2])
-    except OSError:
-        return None
-    return None
+def source_hash(tag):
+    """sha256 (first 16 hex digits) of the sources a kernel is built from."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES[tag]:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()[:16]
+
+
+def counters(kernel):
+    """The committed per-launch counters of the first kernel whose name
+    starts with `kernel` ({} when no summary lists one), with "current" True
+    when the summary was taken from the kernel sources in this tree."""
+    for name in COUNTER_FILES:
+        try:
+            with open(os.path.join(ROOT, name)) as f:
+                for line in f:
+                    parts = line.split()
+                    if len(parts) >= 4 and parts[0] == "counters" and \
+                            parts[1].startswith(kernel):
+                        r = {k: float(v) for k, v in (kv.split("=") for kv in parts[4:])}
+                        r["source"] = name
+                        r["current"] = parts[3] == source_hash(parts[2])
+                        return r
+        except OSError:
+            continue
+    return {}
+
+
+def roofline(kernel, kernel_ms, compulsory_bytes, units, unit_name, model_bytes_per_unit=None):
+    """Roofline object of one launch of `kernel` (average time kernel_ms,
+    measured in this run with HIP events).
+
+    The staged backprojectors serve their taps from LDS, so they are bound by
+    instruction issue, not by HBM: the binding resource is VALU issue, and
+    `achieved` = VALU wave instructions per launch (rocprofv3 SQ_INSTS_VALU,
+    from the committed summary) over the measured launch time, against the
+    issue ceiling.  HBM is reported beside it, measured (FETCH_SIZE /
+    WRITE_SIZE of the summary, calibrated) and compulsory (every input and
+    output byte once).  When the summary was taken from other kernel sources
+    than these, its counts are not used: the object falls back to the
+    compulsory HBM bytes over the measured time."""
+    ks = kernel_ms * 1e-3
+    c = counters(kernel)
+    hbm = {"compulsory_bytes": compulsory_bytes,
+           "compulsory_GBs": compulsory_bytes / ks / 1e9,
+           "frac_compulsory": compulsory_bytes / ks / 1e9 / HBM_PEAK_GBS, "peak_GBs": HBM_PEAK_GBS}
+    r = {"kernel": kernel, "kernel_ms": kernel_ms}
+    if c.get("current") and "valu_insts" in c:
+        ach = c["valu_insts"] / ks / 1e9
+        traffic = c.get("fetch_bytes", 0.0) + c.get("write_bytes", 0.0) or None
+        r.update({"bound": "valu", "achieved": ach, "peak": VALU_PEAK_G,
+                  "unit": "G VALU wave-instructions/s", "frac": ach / VALU_PEAK_G,
+                  "traffic": traffic, "counters_source": c["source"],
+                  "valu_insts_per_launch": c["valu_insts"],
+                  f"valu_per_{unit_name}": c["valu_insts"] * 64.0 / units})
+        if "lds_insts" in c:
+            r["lds_insts_per_launch"] = c["lds_insts"]
+        if traffic:
+            hbm.update({"measured_bytes": traffic, "measured_GBs": traffic / ks / 1e9,
+                        "frac_measured": traffic / ks / 1e9 / HBM_PEAK_GBS})
+    else:
+        r.update({"bound": "hbm", "achieved": hbm["compulsory_GBs"], "peak": HBM_PEAK_GBS,
+                  "unit": "GB/s", "frac": hbm["frac_compulsory"], "traffic": None,
+                  "note": "no counter summary of these kernel sources: compulsory bytes only"})
+    r["hbm"] = hbm
+    if model_bytes_per_unit:
+        # the gather-equivalent figure (taps x stored bytes per update): a
+        # model of the data the update reads, most of it from LDS -- not HBM
+        r["gather_equivalent_GBs"] = model_bytes_per_unit * units / ks / 1e9
+    return r
 
 
 def time_steps(fn, steps, warmup):
@@ -157,15 +230,14 @@ def kernel_times(plan, proj, out, n):
     return f / n, b / n
 
 
-def batch_leg(sg, device, steps, seed=3):
-    """The batching form: 48 independent reconstructions of uneven size drawn
-    from a seed -- 32 fan-beam arc slices (image side in {128, 256, 384, 512},
-    views in {180, 360, 720}; the 888-bin arc detector at dso 541 / dsd 949)
-    and 16 FDK volumes (side in {64, 128, 192}, 120-360 views; the 512 x 384
-    flat panel at dso 800 / dsd 1200) -- run as one autotuned HIP graph, and
-    timed against the same executes issued one after another."""
+def c3_items(sg, device, seed=3):
+    """The batching workload: 48 independent reconstructions of uneven size
+    drawn from a seed -- 32 fan-beam arc slices (image side in {128, 256,
+    384, 512}, views in {180, 360, 720}; the 888-bin arc detector at dso 541
+    / dsd 949) and 16 FDK volumes (side in {64, 128, 192}, 120-360 views; the
+    512 x 384 flat panel at dso 800 / dsd 1200), random-ellipse / ellipsoid
+    phantoms with analytic projections on the device.  Returns [(rg, ig, x)]."""
     import numpy as np
-    import torch
     rng = np.random.default_rng(seed)
     items = []
     for i in range(32):
@@ -183,6 +255,16 @@ def batch_leg(sg, device, steps, seed=3):
         ig = sg.ImageGeom((n, n, n), (256.0 / n,) * 3)
         ell = sg.phantom.random_ellipsoids(seed * 1000 + 100 + i, 12, 256.0)
         items.append((rg, ig, sg.phantom.proj_ellipsoids(ell, rg, device=device).float()))
+    return items
+
+
+def batch_leg(sg, device, steps, seed=3):
+    """The batching form: the 48 reconstructions of c3_items run as one
+    autotuned HIP graph, timed against the same executes issued one after
+    another."""
+    import numpy as np
+    import torch
+    items = c3_items(sg, device, seed)
     updates = sum(int(np.prod(ig.dims)) * rg.na for rg, ig, _ in items)
     plans = [sg.plan_fbp(rg, ig) for rg, ig, _ in items]
     outs = [sg.recon._empty_colmajor(ig.dims, device) for _, ig, _ in items]
@@ -277,8 +359,8 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
     so the longest ray attenuates by about e^-4.3; counts of 0 clamped to 1),
     as a full scan (984 views over 360 deg) and a short scan (642 views over
     180 deg + fan angle, Parker weights).  Analytic Float64 sinograms cast to Float32.
-    The backprojector's roofline uses the gather-equivalent 8 B per update
-    (2 taps x 4 B).  With `with_cpu`, the oracle's Float32 backprojection is
+    The backprojector's roofline is its VALU issue rate (see roofline()),
+    with the gather-equivalent 8 B per update (2 taps x 4 B) beside it as a model.  With `with_cpu`, the oracle's Float32 backprojection is
     timed on the host and every result is compared with the Float64 and
     Float32 oracles."""
     import numpy as np
@@ -320,16 +402,23 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
         r = {"workload": desc, "updates_per_step": updates, "ms_per_step": t * 1e3,
              "value": updates / t / 1e9, "unit": "Gupdates/s", "projections_per_s": rg.na / t,
              "back_kernel_ms": bm, "filter_ms": fm,
-             "roofline": {"bound": "hbm", "achieved": 8.0 * updates / (bm * 1e-3) / 1e9,
-                          "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                          "frac": 8.0 * updates / (bm * 1e-3) / 1e9 / HBM_PEAK_GBS,
-                          "model": "gather-equivalent 8 B per pixel-view update"}}
+             "roofline": roofline(back2d_kernel_name(rg), bm,
+                                  4 * rg.nb * rg.na + 4 * int(np.prod(ig.dims)), updates,
+                                  "update", 8.0)}
         r.update(filter_figures(plan, fm, rg.nb, rg.na, 4))
         log(f"  {name}: step {t * 1e3:.3f} ms (filter {fm:.3f} ms, back {bm:.3f} ms)")
         if with_cpu:
             r.update(_slice_cpu(sg, rg, ig, sino, plan, y.cpu().numpy()))
         out[name] = r
     return out
+
+
+def back2d_kernel_name(rg):
+    """The 2-D backprojector a plan of `rg` launches (csrc/k_back2d.hip), as
+    a kernel-name prefix: parallel beam, or the arc fan's tile-relative form
+    (its series length is chosen per launch)."""
+    import sinograms_amd as sg
+    return "back2d_kernel<0," if isinstance(rg, sg.SinoPar) else "back2d_kernel<4,"
 
 
 def phantom_leg(sg, device, steps):
@@ -503,17 +592,11 @@ def main():
 
     t32 = step["fp32"]
     f32ms, b32ms = kt["fp32"]
-    achieved = 16.0 * updates / (b32ms * 1e-3) / 1e9
-    traffic = measured_traffic("cbct_staged_kernel<false,0>") \
-        if args.cbct_kernel == "staged" else None
     compulsory = 4 * rg.ns * rg.nt * rg.na + 4 * updates // rg.na
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-            "kernel": f"cbct_back FP32, {args.cbct_kernel} (gather-equivalent 16 B/update)",
-            "kernel_ms": b32ms, "filter_ms": f32ms,
-            "traffic_source": TRAFFIC_FILE if traffic else None,
-            "traffic_GBs_at_kernel_ms": traffic / (b32ms * 1e-3) / 1e9 if traffic else None,
-            "compulsory_bytes": compulsory}
+    kname = "cbct_staged_kernel<false,0,false>" if args.cbct_kernel == "staged" \
+        else "cbct_back_kernel<8,false,0,false>"
+    roof = roofline(kname, b32ms, compulsory, updates, "update", 16.0)
+    roof["filter_ms"] = f32ms
     ncols = rg.nt * rg.na
 
     def form(f, bpu):

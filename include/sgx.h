@@ -324,7 +324,9 @@ int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);
  * plans not yet tuned (through the context cache) and captures every item
  * into the graph, with independent plans on parallel branches (items of one
  * plan stay in order on one branch); later executes replay the graph with
- * the same pointers.  No memset nodes are captured. */
+ * the same pointers.  No memset nodes are captured.  A plan change that the
+ * graph would not see (sgx_plan_set_cbct_opts, sgx_plan_set_view_weight,
+ * sgx_plan_autotune) makes the next execute capture again. */
 typedef struct sgx_batch sgx_batch;
 int sgx_batch_create(sgx_ctx* ctx, sgx_batch** out);
 void sgx_batch_destroy(sgx_batch* batch);

@@ -24,9 +24,30 @@
 // are too few pixel tiles to fill the GPU the views are split over G groups;
 // each group writes a partial image and a second kernel adds the G partials
 // in a fixed order (deterministic, no atomics).
+//
+// Arc fan, tile-relative form (GEOM 4).  The bin coordinate of an arc fan is
+// bb = dsd theta + wb with theta = atan2(r, d): about 440 bins at the edge of a
+// 888-bin detector, where FP32 resolves only 3e-5 of a bin.  Per (tile, view)
+// one thread forms, in FP64, the tile centre's (d0, r0), theta0 =
+// atan2(r0, d0) and K = dsd theta0 + wb = I0 + f0 (I0 an integer).  A pixel at
+// (dx, dy) from the centre has, exactly,
+//   t = tan(theta - theta0) = (r d0 - r0 d) / (d d0 + r r0)
+//     = (dx A + dy B) / (R2 + E),  E = dx B - dy A,
+//   A = c d0 - s r0,  B = s d0 + c r0,  R2 = d0^2 + r0^2,
+//   d^2 + r^2 = R2 + 2 E + dx^2 + dy^2,
+// so bb = I0 + (f0 + dsd atan(t)), with |t| small (a tile seen from the
+// source) and atan(t) its odd Taylor series of NT terms (NT chosen on the host
+// from a bound on |t|).  The fractional coordinate f0 + dsd atan(t) is a few
+// bins, so FP32 keeps it to about 1e-6 of a bin; w2 = dsd^2 / (d^2 + r^2).
+// About 22 VALU instructions per update against about 36 for the per-pixel
+// atan2 form; the per-pixel form stays for geometries where |t| is not small,
+// and for sgx_plan_back / sgx_plan_back_t, whose pair is an exact transpose.
 
 #include "sgx_internal.h"
 #include "sgx.h"
+
+#include <algorithm>
+#include <cmath>
 
 namespace sgx {
 namespace {
@@ -69,14 +90,48 @@ int window_bins(int nb)
     return w;
 }
 
+// Per-(tile, view) constants of the tile-relative arc form (see above):
+// A, B, R2, f0; the in-detector range of the fractional coordinate
+// [blo, bhi) = [1 - I0, nb - I0); base = the LDS index of tap il - 1 when
+// floor(f0 + dsd atan(t)) = 0; I0.
+struct RelView {
+    float A, B, R2, f0, blo, bhi;
+    int base, I0;
+};
+
+// Views of one LDS chunk: at most vc_max and what fits in kLdsBytes with the
+// per-view tables; a multiple of kU once it reaches kU.
 int chunk_views(int nb, int vc_max)
 {
-    return std::max(1, std::min(vc_max, kLdsBytes / (4 * window_bins(nb) + 8)));
+    const int per = 4 * window_bins(nb) + 8 + (int)sizeof(RelView);
+    int vc = std::max(1, std::min(vc_max, kLdsBytes / per));
+    if (vc >= kU) vc -= vc % kU;
+    return vc;
 }
 
+// Coefficients of dsd atan(t) = dsd (t - t^3/3 + t^5/5 - ...), NT terms.
+template <int NT> struct RelPoly {
+    float c[NT];
+    __device__ explicit RelPoly(float dsd_ds)
+    {
+#pragma unroll
+        for (int k = 0; k < NT; ++k) c[k] = dsd_ds / (float)(2 * k + 1) * ((k & 1) ? -1.f : 1.f);
+    }
+    // f0 + dsd atan(t) (Horner in t^2)
+    __device__ __forceinline__ float bin(float t, float f0) const
+    {
+        const float t2 = t * t;
+        float p = c[NT - 1];
+#pragma unroll
+        for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, c[k]);
+        return fmaf(t, p, f0);
+    }
+};
+
 // GEOM: 0 parallel, 1 fan arc, 2 fan flat, 3 fan arc whose detector spans
-// less than +-45 degrees (atan2_narrow)
-template <int GEOM>
+// less than +-45 degrees (atan2_narrow), 4 fan arc in the tile-relative form
+// with NT series terms (NT is used by GEOM 4 only)
+template <int GEOM, int NT = 1>
 __global__ __launch_bounds__(kThreads) void back2d_kernel(
     const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ xds,
@@ -85,10 +140,13 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     float wb, float dsd_ds, float dso_ds, float so_ds)
 {
     // vc window slots of wm bins, then per view int2 {lo + 1, n - 1}: the
-    // window holds 0-based bins lo .. lo + n - 1 (n = 0: nothing staged)
+    // window holds 0-based bins lo .. lo + n - 1 (n = 0: nothing staged),
+    // then (GEOM 4) per view a RelView
     extern __shared__ float lds[];
+    constexpr bool REL = GEOM == 4;
     const int wm = 1 << wshift;
     int2* win = reinterpret_cast<int2*>(lds + vc * wm);
+    RelView* rel = reinterpret_cast<RelView*>(win + vc);
     const int tid = threadIdx.x;
     const int tiles_x = (nx + kTile - 1) / kTile;
     const int tx0 = (blockIdx.x % tiles_x) * kTile, ty0 = (blockIdx.x / tiles_x) * kTile;
@@ -116,24 +174,34 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
         bx0 = fminf(bx0, xv); bx1 = fmaxf(bx1, xv);
         by0 = fminf(by0, yv); by1 = fmaxf(by1, yv);
     }
+    // GEOM 4: the pixel relative to the tile centre
+    const float xt = 0.5f * (bx0 + bx1), yt = 0.5f * (by0 + by1);
+    const float dx = x - xt, dy = y - yt;
+    const float p2 = dx * dx + dy * dy;
+    const float dsd2 = dsd_ds * dsd_ds;
+    const RelPoly<NT> poly(dsd_ds);
     float acc = 0.f;
 
     for (int c0 = v_begin; c0 < v_end; c0 += vc) {
         const int nv = min(vc, v_end - c0);
+        // GEOM 4 pads the chunk to a multiple of kU views of zero data and weight
+        const int nvp = REL ? min(vc, (nv + kU - 1) / kU * kU) : nv;
         __syncthreads();                       // the previous chunk is consumed
-        if (tid < nv) {
-            const ViewCS t = cs[c0 + tid];
+        bool interior = true;                  // GEOM 4: this view needs no per-lane tests
+        if (tid < nvp) {
+            int lo = 0, n = 0;
             float bmin = 3.4e38f, bmax = -3.4e38f;
-            bool ok = true;
+            const bool real = tid < nv;
+            const ViewCS t = cs[c0 + min(tid, nv - 1)];
+            bool ok = real;
             for (int k = 0; k < 4; ++k) {
                 const float cx = (k & 1) ? bx1 : bx0, cy = (k & 2) ? by1 : by0;
                 if (GEOM != 0) ok = ok && dso_ds + cx * t.s - cy * t.c > 0.f;   // d > 0
                 float bb, w2;
-                pixel_bin<GEOM>(cx, cy, t, wb, dsd_ds, dso_ds, so_ds, bb, w2);
+                pixel_bin<REL ? 1 : GEOM>(cx, cy, t, wb, dsd_ds, dso_ds, so_ds, bb, w2);
                 bmin = fminf(bmin, bb);
                 bmax = fmaxf(bmax, bb);
             }
-            int lo = 0, n = 0;
             if (ok && bmax >= 1.f && bmin < nb1) {      // false for NaN
                 lo = max((int)floorf(fmaxf(bmin, 0.f)) - 2, 0);
                 const int hi = min((int)floorf(fminf(bmax, nb1)) + 1, nb - 1);
@@ -141,9 +209,43 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                 if (n > wm) n = 0;                       // too wide: taps from global memory
             }
             win[tid] = make_int2(lo + 1, n - 1);
+            if (REL) {
+                RelView r;
+                if (real) {
+                    // the tile centre in FP64 from the FP32 inputs
+                    const double c = t.c, sn = t.s;
+                    const double d0 = (double)dso_ds + (double)xt * sn - (double)yt * c;
+                    const double r0 = (double)xt * c + (double)yt * sn - (double)so_ds;
+                    const double K = (double)dsd_ds * atan2(r0, d0) + (double)wb;
+                    const double I0 = floor(K);
+                    r.A = (float)(c * d0 - sn * r0);
+                    r.B = (float)(sn * d0 + c * r0);
+                    r.R2 = (float)(d0 * d0 + r0 * r0);
+                    r.f0 = (float)(K - I0);
+                    r.blo = (float)(1.0 - I0);
+                    r.bhi = (float)((double)nb - I0);
+                    r.I0 = (int)I0;
+                    r.base = (tid << wshift) + (r.I0 - 1 - lo);
+                    // every pixel's bb is inside [bmin, bmax] up to FP32 rounding
+                    // of the corners; one bin of margin on each side
+                    interior = ok && n > 0 && bmin >= 2.f && bmax < nb1 - 1.f &&
+                               (int)floorf(bmin) - 2 >= 0 && (int)floorf(bmax) + 1 <= nb - 1;
+                } else {
+                    // padding view: zero data (n = 0), zero weight
+                    r.A = r.B = 0.f;
+                    r.R2 = 1e30f;
+                    r.f0 = 0.5f;
+                    r.blo = r.bhi = 0.f;
+                    r.I0 = 1;
+                    r.base = tid << wshift;
+                }
+                rel[tid] = r;
+            }
         }
-        __syncthreads();
-        const int total = nv << wshift;
+        // GEOM 4: the chunk is interior when all its views are
+        const bool chunk_interior =
+            __builtin_amdgcn_readfirstlane(__syncthreads_and(interior ? 1 : 0)) != 0;
+        const int total = nvp << wshift;
         for (int i0 = tid; i0 < total; i0 += kThreads * kStage) {
             float t[kStage];
 #pragma unroll
@@ -163,7 +265,70 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
             }
         }
         __syncthreads();
-        if (active) {
+        if (!active) continue;
+        if (REL && chunk_interior) {
+            // every tap of every lane is in its view's window and on the detector
+            for (int v = 0; v < nv; v += kU) {
+                float w[kU], wr[kU], a[kU], b[kU];
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const RelView r = rel[v + u];
+                    const float E = fmaf(dx, r.B, -dy * r.A);
+                    const float num = fmaf(dx, r.A, dy * r.B);
+                    const float tt = num * __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float bbf = poly.bin(tt, r.f0);
+                    wr[u] = __builtin_amdgcn_fractf(bbf);
+                    const float* p = lds + (r.base + (int)floorf(bbf));
+                    a[u] = p[0];
+                    b[u] = p[1];
+                    w[u] = dsd2 * __builtin_amdgcn_rcpf(fmaf(2.f, E, r.R2 + p2));
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
+            }
+        } else if (REL) {
+            // per-lane tests: bins off the detector weigh 0, taps outside the
+            // window come from global memory
+            for (int v = 0; v < nv; v += kU) {
+                float w[kU], wr[kU], a[kU], b[kU];
+                int il[kU];
+                uint32_t miss = 0;
+#pragma unroll
+                for (int u = 0; u < kU; ++u) {
+                    const RelView r = rel[v + u];
+                    const float E = fmaf(dx, r.B, -dy * r.A);
+                    const float num = fmaf(dx, r.A, dy * r.B);
+                    const float tt = num * __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float bbf = poly.bin(tt, r.f0);
+                    const bool ok = bbf >= r.blo && bbf < r.bhi;   // 1 <= floor(bb) < nb
+                    w[u] = ok ? dsd2 * __builtin_amdgcn_rcpf(fmaf(2.f, E, r.R2 + p2)) : 0.f;
+                    const float bbc = ok ? bbf : r.blo;
+                    const int fi = (int)floorf(bbc);
+                    wr[u] = __builtin_amdgcn_fractf(bbc);
+                    il[u] = r.I0 + fi;                            // taps il - 1 and il (0-based)
+                    const int local = r.base - ((v + u) << wshift) + fi;
+                    const int2 wv = win[v + u];
+                    const bool in = local >= 0 && local < wv.y;
+                    miss |= (uint32_t)(!in) << u;
+                    const float* p = lds + ((v + u) << wshift) + (in ? local : 0);
+                    a[u] = p[0];
+                    b[u] = p[1];
+                }
+                if (__any(miss != 0u)) {
+#pragma unroll
+                    for (int u = 0; u < kU; ++u) {
+                        if ((miss >> u) & 1u) {
+                            const float* qv = q + view_stride * (int64_t)(c0 + min(v + u, nv - 1));
+                            const uint32_t o = (uint32_t)(il[u] - 1);
+                            a[u] = qv[o];
+                            b[u] = qv[o + 1u];
+                        }
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
+            }
+        } else {
             // kU views at a time, their LDS reads in flight together; the
             // last batch of a chunk repeats view nv - 1 with weight 0.  Out of
             // range (1 <= floor(bb) <= nb - 1 fails): bin 1, weight 0.  Taps
@@ -288,6 +453,24 @@ int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_block
     return g > 1 ? (int64_t)g * nx * ny : 0;
 }
 
+int back2d_rel_terms(const Back2dArgs& a)
+{
+    if (a.geom != 1 || a.pixel_form || a.rel_rmax < 0.f || a.rel_h < 0.f) return 0;
+    if (chunk_views(a.nb, a.vc_max) % kU != 0) return 0;
+    // The source is at distance >= dso from the origin in every view, so a
+    // tile centre is at least dso - rmax from it and its pixels (within
+    // rel_h of the centre) subtend |theta - theta0| <= asin(z); t = tan of it.
+    const double rho = (double)a.dso_ds - (double)a.rel_rmax;
+    if (!(rho > 0.0)) return 0;
+    const double z = (double)a.rel_h / rho;
+    if (!(z <= 0.25)) return 0;
+    const double T = z / std::sqrt(1.0 - z * z);
+    // the first omitted term of dsd atan(t): below 1e-6 of a bin
+    for (int nt : {3, 4, 5, 6, 8})
+        if ((double)a.dsd_ds * std::pow(T, 2 * nt + 1) / (2 * nt + 1) <= 1e-6) return nt;
+    return 0;
+}
+
 void launch_back2d(const Back2dArgs& a, hipStream_t st)
 {
     if (a.want_blocks < 1 || a.vc_max < 1) throw Error(SGX_ERR_ARG, "back2d: bad shape");
@@ -305,37 +488,75 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
     const int wm = window_bins(a.nb);
     int wshift = 0;
     while ((1 << wshift) < wm) ++wshift;
-    const size_t lds = (size_t)vc * wm * sizeof(float) + (size_t)vc * sizeof(int2);
+    const size_t lds = (size_t)vc * (wm * sizeof(float) + sizeof(int2) + sizeof(RelView));
+#define SGX_B2D(G, NT)                                                                          \
+    hipLaunchKernelGGL((back2d_kernel<G, NT>), grid, block, lds, st, a.q, a.nb, a.nviews,       \
+                       a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg, vc, \
+                       wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds)
     switch (a.geom) {
     case 0:
-        hipLaunchKernelGGL(back2d_kernel<0>, grid, block, lds, st, a.q, a.nb, a.nviews,
-                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        SGX_B2D(0, 1);
         break;
     case 1:
-        // bins 1 <= bb < nb are at |gamma| = |bb - wb| / dsd_ds; below 0.75 rad
-        // (43 degrees) on both sides the narrow arctangent serves
-        if (std::max(std::fabs(1.f - a.wb), std::fabs((float)a.nb - a.wb)) < 0.75f * a.dsd_ds)
-            hipLaunchKernelGGL(back2d_kernel<3>, grid, block, lds, st, a.q, a.nb, a.nviews,
-                               a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix,
-                               vpg, vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
-        else
-            hipLaunchKernelGGL(back2d_kernel<1>, grid, block, lds, st, a.q, a.nb, a.nviews,
-                               a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix,
-                               vpg, vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        switch (back2d_rel_terms(a)) {
+        case 3: SGX_B2D(4, 3); break;
+        case 4: SGX_B2D(4, 4); break;
+        case 5: SGX_B2D(4, 5); break;
+        case 6: SGX_B2D(4, 6); break;
+        case 8: SGX_B2D(4, 8); break;
+        default:
+            // bins 1 <= bb < nb are at |gamma| = |bb - wb| / dsd_ds; below 0.75 rad
+            // (43 degrees) on both sides the narrow arctangent serves
+            if (std::max(std::fabs(1.f - a.wb), std::fabs((float)a.nb - a.wb)) < 0.75f * a.dsd_ds)
+                SGX_B2D(3, 1);
+            else
+                SGX_B2D(1, 1);
+            break;
+        }
         break;
     default:
-        hipLaunchKernelGGL(back2d_kernel<2>, grid, block, lds, st, a.q, a.nb, a.nviews,
-                           a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg,
-                           vc, wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds);
+        SGX_B2D(2, 1);
         break;
     }
+#undef SGX_B2D
     SGX_HIP(hipGetLastError());
     if (groups > 1) {
         hipLaunchKernelGGL(sum_partials_kernel, dim3((unsigned)((npix + 255) / 256)), dim3(256),
                            0, st, a.partial, groups, npix, a.mask, a.image);
         SGX_HIP(hipGetLastError());
     }
+}
+
+void back2d_rel_bounds(const double* xc, int64_t nx, const double* yc, int64_t ny,
+                       const uint8_t* mask, double ds, float& rmax, float& h)
+{
+    // rmax: the largest |(x, y)| / ds over the pixels that are backprojected
+    double r2 = 0.0;
+    for (int64_t j = 0; j < ny; ++j) {
+        const double y = yc[j] / ds;
+        for (int64_t i = 0; i < nx; ++i) {
+            if (mask && !mask[i + nx * j]) continue;
+            const double x = xc[i] / ds;
+            r2 = std::max(r2, x * x + y * y);
+        }
+    }
+    // h: the largest half-diagonal of a 16 x 16 tile's box of pixel centres
+    auto half = [&](const double* c, int64_t n) {
+        double m = 0.0;
+        for (int64_t t0 = 0; t0 < n; t0 += kTile) {
+            double lo = c[t0], hi = c[t0];
+            for (int64_t k = t0; k < std::min<int64_t>(n, t0 + kTile); ++k) {
+                lo = std::min(lo, c[k]);
+                hi = std::max(hi, c[k]);
+            }
+            m = std::max(m, 0.5 * (hi - lo));
+        }
+        return m / std::fabs(ds);
+    };
+    const double hx = half(xc, nx), hy = half(yc, ny);
+    // pixel centres are rounded to FP32 in the kernel: a small relative margin
+    rmax = (float)(std::sqrt(r2) * (1.0 + 1e-6));
+    h = (float)(std::sqrt(hx * hx + hy * hy) * (1.0 + 1e-6));
 }
 
 void launch_back2d_t(const Back2dArgs& a, const float* img, float* qt, hipStream_t st)

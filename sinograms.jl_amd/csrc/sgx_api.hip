@@ -218,6 +218,9 @@ void back2d_common(sgx_ctx* ctx, int geom, const float* sino, int64_t nb, int64_
                                                   a.vc_max);
     a.partial = static_cast<float*>(ctx->get(W_PARTIAL, 4 * pf));
     a.partial_cap = pf;
+    if (geom == 1)
+        sgx::back2d_rel_bounds(xc, nx, yc, ny, (flags & SGX_HOST) ? mask : nullptr, ds,
+                               a.rel_rmax, a.rel_h);
     sgx::launch_back2d(a, ctx->stream);
     finish(ctx, image, img, 4 * nx * ny, flags);
 }
@@ -655,6 +658,7 @@ struct sgx_plan {
     float gscale = 1.f;             // power of two with max|g| * gscale <= 2^14 (split FP16)
     float* partial = nullptr;
     int64_t partial_cap = 0;
+    float rel_rmax = -1.f, rel_h = -1.f;  // arc fan: bounds of the tile-relative form
     // table offsets
     sgx::ViewCS* cs = nullptr;
     double* gam = nullptr;
@@ -670,6 +674,9 @@ struct sgx_plan {
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
+    // Bumped by every change that a captured batch graph would not see
+    // (options, view weights, kernel shapes): batches recapture on a change.
+    uint64_t gen = 0;
 
     ~sgx_plan()
     {
@@ -801,6 +808,8 @@ void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool tim
         a.partial_cap = P.partial_cap;
         a.want_blocks = P.tune.want_blocks;
         a.vc_max = P.tune.vc_max;
+        a.rel_rmax = P.rel_rmax;
+        a.rel_h = P.rel_h;
         sgx::launch_back2d(a, st);
     }
     if (timed) {
@@ -852,6 +861,7 @@ void autotune(sgx_plan& P, const float* in, float* img, int reps, bool use_cache
         if (it != ctx->tune_cache.end()) {
             P.tune = it->second;
             P.tuned = true;
+            ++P.gen;
             plan_run(P, in, img, ctx->stream, false);
             return;
         }
@@ -886,6 +896,7 @@ void autotune(sgx_plan& P, const float* in, float* img, int reps, bool use_cache
     (void)hipEventDestroy(e1);
     P.tune = best_t;
     P.tuned = true;
+    ++P.gen;
     ctx->tune_cache[key] = best_t;
     plan_run(P, in, img, st, false);          // leave the chosen shape's result in img
 }
@@ -1061,6 +1072,8 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             P->partial_cap = sgx::back2d_partial_floats((int)ns, (int)na, (int)nx, (int)ny,
                                                         kTuneMaxBlocks, kTuneMinChunk);
             if (P->partial_cap) SGX_HIP(hipMalloc(&P->partial, 4 * P->partial_cap));
+            if (G.kind == SGX_SINO_FAN_ARC)
+                sgx::back2d_rel_bounds(xc, nx, yc, ny, mask, G.ds, P->rel_rmax, P->rel_h);
         }
         if (is3d) {
             // the staged backprojector's detector windows, fixed by the geometry
@@ -1088,6 +1101,7 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview)
     return guard(plan->ctx, [&] {
         sgx_plan& P = *plan;
         const int64_t na = P.geom.na;
+        ++P.gen;
         if (!wview) {
             P.wview = nullptr;
             return;
@@ -1126,6 +1140,7 @@ int sgx_plan_set_cbct_opts(sgx_plan* plan, const sgx_cbct_opts* opts)
             P.zs = nullptr;
         }
         P.arc_t_exact = opts && opts->arc_t_exact;
+        ++P.gen;
         // the staged backprojector's windows follow the t mapping
         sgx::launch_cbct_windows(cbct_args_of(P, nullptr), P.win, plan->ctx->stream);
         SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
@@ -1258,6 +1273,7 @@ struct sgx_batch {
     };
     sgx_ctx* ctx = nullptr;
     std::vector<Item> items;
+    std::vector<uint64_t> gens;              // each item's plan generation at capture
     std::vector<hipStream_t> branches;       // capture streams (default priority)
     std::vector<hipEvent_t> joins;
     hipStream_t origin = nullptr;            // capture origin
@@ -1335,8 +1351,19 @@ struct sgx_batch {
         }
         SGX_HIP(hipStreamEndCapture(origin, &graph));
         SGX_HIP(hipGraphInstantiateWithFlags(&exec, graph, 0));
+        gens.clear();
+        for (const Item& it : items) gens.push_back(it.plan->gen);
         nbranch = nb;
         dirty = false;
+    }
+
+    // True when some item's plan changed since the capture.
+    bool stale() const
+    {
+        if (gens.size() != items.size()) return true;
+        for (size_t i = 0; i < items.size(); ++i)
+            if (items[i].plan->gen != gens[i]) return true;
+        return false;
     }
 };
 
@@ -1383,6 +1410,7 @@ int sgx_batch_execute(sgx_batch* batch, uint32_t flags)
     return guard(ctx, [&] {
         require(!batch->items.empty(), SGX_ERR_ARG, "empty batch");
         require(!(flags & SGX_HOST), SGX_ERR_ARG, "batches take device pointers");
+        if (!batch->dirty && batch->stale()) batch->dirty = true;
         if (batch->dirty) {
             if (!(flags & SGX_BATCH_NO_TUNE)) {
                 for (auto& it : batch->items)
@@ -1450,6 +1478,9 @@ sgx::Back2dArgs back2d_args_of(const sgx_plan& P, const float* q, float* img)
     a.partial_cap = P.partial_cap;
     a.want_blocks = P.tune.want_blocks;
     a.vc_max = P.tune.vc_max;
+    // the per-pixel arc form: sgx_plan_back and sgx_plan_back_t are an exact
+    // transpose pair
+    a.pixel_form = true;
     return a;
 }
 

@@ -154,8 +154,21 @@ struct Back2dArgs {
     // want_blocks; at most vc_max views per LDS chunk (and 48 KB)
     int want_blocks = 2048;
     int vc_max = 64;
+    // arc fan: bounds for the tile-relative form (back2d_rel_bounds; negative:
+    // not known, the per-pixel form runs), and pixel_form = true to keep the
+    // per-pixel form (the exact transpose pair of sgx_plan_back / _back_t)
+    float rel_rmax = -1.f, rel_h = -1.f;
+    bool pixel_form = false;
 };
 void launch_back2d(const Back2dArgs& a, hipStream_t st);
+// Series terms of the tile-relative arc form launch_back2d takes for `a`
+// (0: the per-pixel form).
+int back2d_rel_terms(const Back2dArgs& a);
+// Host bounds of the tile-relative arc form from the image axes and mask:
+// the largest |(x, y)| / ds of a backprojected pixel and the largest
+// half-diagonal / ds of a 16 x 16 tile.
+void back2d_rel_bounds(const double* xc, int64_t nx, const double* yc, int64_t ny,
+                       const uint8_t* mask, double ds, float& rmax, float& h);
 int64_t back2d_partial_floats(int nb, int nviews, int nx, int ny, int want_blocks, int vc_max);
 // Exact transpose of launch_back2d (view_stride = nb): qt (nb, nviews) from an
 // image; uses the geometry fields of `a` (q, image and partial are unused).

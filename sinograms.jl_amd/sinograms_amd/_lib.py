@@ -6,6 +6,7 @@ raises.  There is no CPU fallback.
 
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import threading
@@ -200,6 +201,36 @@ class Context:
             return False
         import torch
         return torch.cuda.current_stream(self.device).cuda_stream == self.stream
+
+    def _torch_stream(self):
+        import torch
+        if self.stream:
+            return torch.cuda.ExternalStream(self.stream, device=self.device)
+        return torch.cuda.default_stream(self.device)
+
+    @contextlib.contextmanager
+    def ordered(self, device_arrays: bool = True):
+        """Order a call on this context's stream with torch's current stream.
+
+        Before the call, the context's stream waits for the work queued so
+        far on torch's current stream (an event recorded there), so that
+        device arrays produced on that stream are read after they are
+        written; after it, torch's current stream waits for the call's work.
+        Yields True when the two streams are the same one (a call on device
+        arrays may then return without a host wait).  With
+        device_arrays=False (host arrays only) nothing is ordered."""
+        if not device_arrays:
+            yield False
+            return
+        if self.on_current_stream():
+            yield True
+            return
+        import torch
+        cur = torch.cuda.current_stream(self.device)
+        mine = self._torch_stream()
+        mine.wait_stream(cur)
+        yield False
+        cur.wait_stream(mine)
 
     def check(self, rc):
         if rc != SGX_OK:

@@ -67,7 +67,8 @@ class Batch:
 
     def execute(self, tune: bool = True, wait: bool = True) -> None:
         flags = (0 if tune else _lib.SGX_BATCH_NO_TUNE) | (0 if wait else _lib.SGX_ASYNC)
-        self._ctx.check(_lib.load().sgx_batch_execute(self.h, flags))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_batch_execute(self.h, flags))
 
     def last_ms(self) -> float:
         """Milliseconds of the last graph launch (HIP events)."""

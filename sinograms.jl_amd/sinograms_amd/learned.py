@@ -82,12 +82,14 @@ class ConvCorrection:
         n, rows = q.shape
         if out is None:
             out = _empty_colmajor((n, rows), q.device)
-        self._ctx.check(_lib.load().sgx_net_forward(self.h, _ptr(q), rows, n, _ptr(out),
-                                                    int(keep)))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_net_forward(self.h, _ptr(q), rows, n, _ptr(out),
+                                                        int(keep)))
         return out
 
     def backward(self, gout) -> None:
-        self._ctx.check(_lib.load().sgx_net_backward(self.h, _ptr(gout)))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_net_backward(self.h, _ptr(gout)))
 
     def adam(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8):
         self._ctx.check(_lib.load().sgx_net_adam(self.h, lr, beta1, beta2, eps))
@@ -104,8 +106,9 @@ class ConvCorrection:
 def mse(ctx, x, t, grad=None, scale: float = 1.0) -> float:
     """sum (x - t)^2 of two CUDA tensors; grad (optional) = scale * 2 (x - t)."""
     s = ctypes.c_double()
-    ctx.check(_lib.load().sgx_mse(ctx.h, _ptr(x), _ptr(t), x.numel(), scale,
-                                  _ptr(grad) if grad is not None else None, ctypes.byref(s)))
+    with ctx.ordered():
+        ctx.check(_lib.load().sgx_mse(ctx.h, _ptr(x), _ptr(t), x.numel(), scale,
+                                      _ptr(grad) if grad is not None else None, ctypes.byref(s)))
     return s.value
 
 
@@ -177,19 +180,24 @@ class SparseViewProblem:
     def filter(self, sino, q=None):
         if q is None:
             q = _empty_colmajor(tuple(sino.shape), sino.device)
-        self._ctx.check(_lib.load().sgx_plan_filter(self.plan_sparse.h, _ptr(sino), _ptr(q), 0))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_plan_filter(self.plan_sparse.h, _ptr(sino), _ptr(q),
+                                                        0))
         return q
 
     def back(self, q, out=None):
         if out is None:
             out = _empty_colmajor(self.ig.dims, q.device)
-        self._ctx.check(_lib.load().sgx_plan_back(self.plan_sparse.h, _ptr(q), _ptr(out), 0))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_plan_back(self.plan_sparse.h, _ptr(q), _ptr(out), 0))
         return out
 
     def back_t(self, img, q=None):
         if q is None:
             q = _empty_colmajor(self.rg_sparse.dims(), img.device)
-        self._ctx.check(_lib.load().sgx_plan_back_t(self.plan_sparse.h, _ptr(img), _ptr(q), 0))
+        with self._ctx.ordered():
+            self._ctx.check(_lib.load().sgx_plan_back_t(self.plan_sparse.h, _ptr(img), _ptr(q),
+                                                        0))
         return q
 
     def reconstruct(self, sino, net=None):

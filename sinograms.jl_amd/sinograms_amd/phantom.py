@@ -44,7 +44,8 @@ def _radon_hip(fn, dev, *args):
     """Call an sgx_radon_* entry point on the device's shared context."""
     from . import _lib
     ctx = _lib.context(dev.index if dev.index is not None else 0)
-    ctx.check(getattr(_lib.load(), fn)(ctx.h, *args))
+    with ctx.ordered():
+        ctx.check(getattr(_lib.load(), fn)(ctx.h, *args))
 
 
 def _p(a):

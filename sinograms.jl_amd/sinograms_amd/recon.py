@@ -336,8 +336,9 @@ def fbp_sino_filter(sino, filter, extra: int = 0, precision: str = "fp32"):
     out, optr = a.out((nb + extra,) + tuple(a.shape[1:]))
     ctx = _ctx_for(a)
     prec = _lib.SGX_PREC_BF16 if precision == "bf16x3" else _lib.SGX_PREC_FP32
-    ctx.check(_lib.load().sgx_sino_filter(ctx.h, a.ptr, nb, ncols, _dptr(Hk), npad, extra, prec,
-                                          optr, a.flags))
+    with ctx.ordered(not a.host):
+        ctx.check(_lib.load().sgx_sino_filter(ctx.h, a.ptr, nb, ncols, _dptr(Hk), npad, extra,
+                                              prec, optr, a.flags))
     return out
 
 
@@ -375,18 +376,19 @@ def fbp_back(rg, ig: ImageGeom, sino, ia_skip: int = 1):
     img, iptr = a.out(ig.dims)
     ctx = _ctx_for(a)
     L = _lib.load()
-    if isinstance(rg, SinoPar):
-        rc = L.sgx_back_par(ctx.h, a.ptr, nb, na, ia_skip, _dptr(sn), _dptr(cs), rg.d,
-                            rg.offset, _dptr(xc), len(xc), _dptr(yc), len(yc), _mask_ptr(mask),
-                            iptr, a.flags)
-    elif isinstance(rg, SinoFan):
-        rc = L.sgx_back_fan(ctx.h, a.ptr, nb, na, ia_skip, _dptr(sn), _dptr(cs), rg.dsd,
-                            rg.dso, rg.source_offset, int(rg.is_arc), rg.d, rg.offset,
-                            _dptr(xc), len(xc), _dptr(yc), len(yc), _mask_ptr(mask), iptr,
-                            a.flags)
-    else:
+    if not isinstance(rg, (SinoPar, SinoFan)):
         raise TypeError(type(rg))
-    ctx.check(rc)
+    with ctx.ordered(not a.host):
+        if isinstance(rg, SinoPar):
+            rc = L.sgx_back_par(ctx.h, a.ptr, nb, na, ia_skip, _dptr(sn), _dptr(cs), rg.d,
+                                rg.offset, _dptr(xc), len(xc), _dptr(yc), len(yc),
+                                _mask_ptr(mask), iptr, a.flags)
+        else:
+            rc = L.sgx_back_fan(ctx.h, a.ptr, nb, na, ia_skip, _dptr(sn), _dptr(cs), rg.dsd,
+                                rg.dso, rg.source_offset, int(rg.is_arc), rg.d, rg.offset,
+                                _dptr(xc), len(xc), _dptr(yc), len(yc), _mask_ptr(mask), iptr,
+                                a.flags)
+        ctx.check(rc)
     return img
 
 
@@ -422,10 +424,11 @@ def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1, use_source_zs: b
     args = (ctx.h, a.ptr, ns, nt, na, ia_skip, _dptr(betas), rg.dsd, rg.dso, rg.ds, rg.dt,
             rg.offset_s, rg.offset_t, int(rg.is_arc), _dptr(xc), len(xc), _dptr(yc), len(yc),
             _dptr(zc), len(zc), _mask_ptr(mask), iptr, a.flags)
-    if opts is None:
-        ctx.check(_lib.load().sgx_cbct_back(*args))
-    else:
-        ctx.check(_lib.load().sgx_cbct_back_opts(*args, ctypes.byref(opts)))
+    with ctx.ordered(not a.host):
+        if opts is None:
+            ctx.check(_lib.load().sgx_cbct_back(*args))
+        else:
+            ctx.check(_lib.load().sgx_cbct_back_opts(*args, ctypes.byref(opts)))
     return img
 
 
@@ -505,11 +508,11 @@ class _Plan:
             img, iptr = out, out.data_ptr()
         else:
             img, iptr = a.out(self.ig.dims)
-        # device arrays on torch's current stream: stream-ordered, no host wait
-        flags = a.flags
-        if not a.host and self._ctx.on_current_stream():
-            flags |= _lib.SGX_ASYNC
-        self._ctx.check(_lib.load().sgx_plan_execute(self.h, a.ptr, iptr, flags))
+        # device arrays on torch's current stream: stream-ordered, no host
+        # wait; on another stream the call waits for that stream's queued work
+        with self._ctx.ordered(not a.host) as same:
+            flags = a.flags | (_lib.SGX_ASYNC if same else 0)
+            self._ctx.check(_lib.load().sgx_plan_execute(self.h, a.ptr, iptr, flags))
         return img
 
     def times(self):

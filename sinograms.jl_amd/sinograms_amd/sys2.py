@@ -37,10 +37,11 @@ def project_bdd(phantom, geo):
         optr = out.data_ptr()
     ctx = _ctx_for(a)
     flags = a.flags | (_lib.SGX_IN_F32 if is32 else 0)
-    ctx.check(_lib.load().sgx_project_bdd(
-        ctx.h, a.ptr, nPix, nDet, _dptr(ang), len(ang), float(_g(geo, "DSD")),
-        float(_g(geo, "DS0")), float(_g(geo, "pSize")), float(_g(geo, "dSize")),
-        ctypes.c_void_p(optr), flags))
+    with ctx.ordered(not a.host):
+        ctx.check(_lib.load().sgx_project_bdd(
+            ctx.h, a.ptr, nPix, nDet, _dptr(ang), len(ang), float(_g(geo, "DSD")),
+            float(_g(geo, "DS0")), float(_g(geo, "pSize")), float(_g(geo, "dSize")),
+            ctypes.c_void_p(optr), flags))
     return out
 
 
@@ -63,8 +64,9 @@ def backproject_bdd(sinogram, geo, ref_alias: bool = True):
         out = _empty_colmajor((nPix, nPix), a.device, torch.float64)
         optr = out.data_ptr()
     ctx = _ctx_for(a)
-    ctx.check(_lib.load().sgx_backproject_bdd(
-        ctx.h, a.ptr, nPix, nDet, _dptr(ang), len(ang), float(_g(geo, "DSD")),
-        float(_g(geo, "DS0")), float(_g(geo, "pSize")), float(_g(geo, "dSize")),
-        int(bool(ref_alias)), ctypes.c_void_p(optr), a.flags))
+    with ctx.ordered(not a.host):
+        ctx.check(_lib.load().sgx_backproject_bdd(
+            ctx.h, a.ptr, nPix, nDet, _dptr(ang), len(ang), float(_g(geo, "DSD")),
+            float(_g(geo, "DS0")), float(_g(geo, "pSize")), float(_g(geo, "dSize")),
+            int(bool(ref_alias)), ctypes.c_void_p(optr), a.flags))
     return out
