@@ -149,6 +149,24 @@ int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, i
                        const double* zc, int64_t nz, const uint8_t* mask, float* image,
                        uint32_t flags, const sgx_cbct_opts* opts);
 
+/* sgx_cbct_back without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (tight serving loops,
+ * harnesses that forbid hipMalloc).  Same computation and kernels as
+ * sgx_cbct_back, with the call's geometry tables and the staged
+ * backprojector's window table (16 bytes per 16 x 16 x 16 voxel block and
+ * view) in a fixed 16 MiB device scratch array that the library holds on
+ * every GPU it runs on until the process ends.  Device FP32 data only
+ * (flags: 0 or SGX_ASYNC), ns a multiple of 4 and proj 16-byte aligned;
+ * a call that needs more than the scratch returns SGX_ERR_UNSUPPORTED before
+ * any launch.  Every call on one GPU shares that GPU's scratch: calls that
+ * overlap on one GPU (other streams, other threads or contexts) race. */
+int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t nt, int64_t na,
+                          int64_t ia_skip, const double* betas, double dsd, double dso,
+                          double ds, double dt, double offset_s, double offset_t, int is_arc,
+                          const double* xc, int64_t nx, const double* yc, int64_t ny,
+                          const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                          uint32_t flags);
+
 /* ---- row filter ------------------------------------------------------------ */
 
 /* fbp_sino_filter(sino, filter; extra)

@@ -13,11 +13,48 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <sstream>
 #include <string>
 #include <vector>
 
 using sgx::Error;
+
+// Device scratch of sgx_cbct_back_noalloc: the call's geometry tables and
+// the staged backprojector's per-(block, view) window table, in one
+// file-scope array of kScratchBytes on every GPU the library runs on (held
+// until the process ends).  Host code never passes `scratch` itself to a
+// kernel or a copy: it uses the per-device address that
+// hipGetSymbolAddress returns, taken once per device when a context is
+// created (scratch_ptr).
+constexpr size_t kScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: record the address of that device's `scratch`.
+hipError_t scratch_setup(int device)
+{
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e == hipSuccess) g_scratch_ptr[device] = p;
+    return e;
+}
+
+unsigned char* scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return (device >= 0 && device < kMaxDevices)
+               ? static_cast<unsigned char*>(g_scratch_ptr[device])
+               : nullptr;
+}
+}  // namespace
 
 // Kernel shapes of one plan: the cone-beam backprojector (0 staged, 1
 // gather); the 2-D backprojector's view grouping target and LDS view chunk.
@@ -35,6 +72,7 @@ struct sgx_ctx {
     int device = 0;
     std::map<std::string, PlanTune> tune_cache;   // autotuned shapes per size bucket
     int cbct_kernel = 0;          // SGX_OPT_CBCT_KERNEL
+    std::string scratch_err;      // why the device scratch is unusable (empty: usable)
     hipStream_t stream = nullptr;
     bool own_stream = false;
     std::string err;
@@ -245,6 +283,12 @@ int sgx_ctx_create(int device, sgx_ctx** out)
         return SGX_ERR_DEVICE;
     }
     c->own_stream = true;
+    // the address of this device's scratch (sgx_cbct_back_noalloc); a failure
+    // here leaves that entry point unusable on the device, nothing else
+    const hipError_t se = scratch_setup(device);
+    c->scratch_err = se == hipSuccess ? std::string()
+                                      : std::string("hipGetSymbolAddress(scratch): ") +
+                                            hipGetErrorString(se);
     *out = c;
     return SGX_OK;
 }
@@ -313,7 +357,95 @@ int sgx_back_fan(sgx_ctx* ctx, const float* sino, int64_t nb, int64_t na, int64_
     });
 }
 
+}  // extern "C"
+
 namespace {
+
+// Host tables of one cone-beam call: the used views' angles, the axes / ds,
+// room for the per-view cos/sin (formed on the device) and the source z.
+struct CbctCall {
+    TablePack pack;
+    size_t o_b = 0, o_x = 0, o_y = 0, o_z = 0, o_cs = 0, o_zs = 0;
+    int64_t nv = 0;
+    bool has_zs = false;
+};
+
+void cbct_check(const void* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_skip,
+                const double* betas, double ds, double dt, const double* xc, int64_t nx,
+                const double* yc, int64_t ny, const double* zc, int64_t nz, const float* image,
+                uint32_t flags)
+{
+    require(proj && betas && xc && yc && zc && image, SGX_ERR_ARG, "null pointer argument");
+    require(ns > 1 && nt > 1 && na > 0 && nx > 0 && ny > 0 && nz > 0 && ia_skip >= 1,
+            SGX_ERR_ARG, "bad size");
+    require(ds != 0.0 && dt != 0.0, SGX_ERR_ARG, "ds and dt must be nonzero");
+    require(!((flags & SGX_Q_BF16) && (flags & SGX_HOST)), SGX_ERR_UNSUPPORTED,
+            "bf16 projections must be device memory");
+    require(nx * ny < (int64_t(1) << 31) / std::max<int64_t>(nz, 1), SGX_ERR_UNSUPPORTED,
+            "volume too large");
+}
+
+CbctCall cbct_call_tables(int64_t na, int64_t ia_skip, const double* betas, double ds,
+                          const double* xc, int64_t nx, const double* yc, int64_t ny,
+                          const double* zc, int64_t nz, const sgx_cbct_opts* opts)
+{
+    CbctCall T;
+    T.nv = (na + ia_skip - 1) / ia_skip;
+    std::vector<double> b(T.nv);
+    for (int64_t i = 0; i < T.nv; ++i) b[i] = betas[i * ia_skip];
+    T.o_b = T.pack.add(b.data(), b.size() * 8);
+    const auto xv = axis_over(xc, nx, ds), yv = axis_over(yc, ny, ds), zv = axis_over(zc, nz, ds);
+    T.o_x = T.pack.add(xv.data(), xv.size() * 4);
+    T.o_y = T.pack.add(yv.data(), yv.size() * 4);
+    T.o_z = T.pack.add(zv.data(), zv.size() * 4);
+    T.o_cs = T.pack.add(b.data(), T.nv * sizeof(sgx::ViewCS));   // space only
+    T.has_zs = opts && opts->source_zs;
+    if (T.has_zs) {
+        std::vector<float> zsv(T.nv);
+        for (int64_t i = 0; i < T.nv; ++i)
+            zsv[i] = (float)((float)opts->source_zs[i * ia_skip] / (float)ds);
+        T.o_zs = T.pack.add(zsv.data(), T.nv * 4);
+    }
+    return T;
+}
+
+// The kernels' arguments for tables uploaded at device address d (the
+// per-view cos/sin still to be formed by launch_view_trig).
+sgx::CbctArgs cbct_call_args(const CbctCall& T, unsigned char* d, const void* q, bool bf16,
+                             int64_t ns, int64_t nt, int64_t pitch, int64_t ia_skip, double dsd,
+                             double dso, double ds, double dt, double offset_s, double offset_t,
+                             int is_arc, int64_t nx, int64_t ny, int64_t nz, const uint8_t* m,
+                             float* img, const sgx_cbct_opts* opts)
+{
+    sgx::CbctArgs a{};
+    a.q = q;
+    a.fmt = bf16 ? sgx::SGX_Q_FMT_BF16 : sgx::SGX_Q_FMT_F32;
+    a.vscale = nullptr;
+    a.ns = (int)ns;
+    a.nt = (int)nt;
+    a.pitch = (int)pitch;
+    a.nviews = (int)T.nv;
+    a.view_stride = pitch * nt * ia_skip;
+    a.cs = reinterpret_cast<sgx::ViewCS*>(d + T.o_cs);
+    a.xds = reinterpret_cast<const float*>(d + T.o_x);
+    a.yds = reinterpret_cast<const float*>(d + T.o_y);
+    a.zds = reinterpret_cast<const float*>(d + T.o_z);
+    a.nx = (int)nx;
+    a.ny = (int)ny;
+    a.nz = (int)nz;
+    a.mask = m;
+    a.image = img;
+    a.dsd_ds = (float)dsd / (float)ds;
+    a.dso_ds = (float)dso / (float)ds;
+    a.ws = (float)((ns + 1) / 2.0 + (double)(float)offset_s);
+    a.wt = (float)((nt + 1) / 2.0 + (double)(float)offset_t);
+    a.inv_dt_ds = 1.0f / ((float)dt / (float)ds);
+    a.is_arc = is_arc != 0;
+    a.zs = T.has_zs ? reinterpret_cast<const float*>(d + T.o_zs) : nullptr;
+    a.arc_t_exact = opts && opts->arc_t_exact;
+    return a;
+}
+
 void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
                     int64_t ia_skip, const double* betas, double dsd, double dso,
                     double ds, double dt, double offset_s, double offset_t, int is_arc,
@@ -321,98 +453,48 @@ void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int6
                     const double* zc, int64_t nz, const uint8_t* mask, float* image,
                     uint32_t flags, const sgx_cbct_opts* opts)
 {
-    {
-        require(proj && betas && xc && yc && zc && image, SGX_ERR_ARG, "null pointer argument");
-        require(ns > 1 && nt > 1 && na > 0 && nx > 0 && ny > 0 && nz > 0 && ia_skip >= 1,
-                SGX_ERR_ARG, "bad size");
-        require(ds != 0.0 && dt != 0.0, SGX_ERR_ARG, "ds and dt must be nonzero");
-        require(!((flags & SGX_Q_BF16) && (flags & SGX_HOST)), SGX_ERR_UNSUPPORTED,
-                "bf16 projections must be device memory");
-        require(nx * ny < (int64_t(1) << 31) / std::max<int64_t>(nz, 1), SGX_ERR_UNSUPPORTED,
-                "volume too large");
-        const int64_t nv = (na + ia_skip - 1) / ia_skip;
+    cbct_check(proj, ns, nt, na, ia_skip, betas, ds, dt, xc, nx, yc, ny, zc, nz, image, flags);
+    CbctCall T = cbct_call_tables(na, ia_skip, betas, ds, xc, nx, yc, ny, zc, nz, opts);
+    unsigned char* d = T.pack.upload(ctx, W_TABLES);
+    sgx::launch_view_trig(reinterpret_cast<const double*>(d + T.o_b), T.nv, 1,
+                          reinterpret_cast<sgx::ViewCS*>(d + T.o_cs), ctx->stream);
 
-        TablePack pack;
-        std::vector<double> b(nv);
-        for (int64_t i = 0; i < nv; ++i) b[i] = betas[i * ia_skip];
-        const size_t o_b = pack.add(b.data(), b.size() * 8);
-        const auto xv = axis_over(xc, nx, ds), yv = axis_over(yc, ny, ds),
-                   zv = axis_over(zc, nz, ds);
-        const size_t o_x = pack.add(xv.data(), xv.size() * 4);
-        const size_t o_y = pack.add(yv.data(), yv.size() * 4);
-        const size_t o_z = pack.add(zv.data(), zv.size() * 4);
-        const size_t o_cs = pack.add(b.data(), nv * sizeof(sgx::ViewCS));   // space only
-        const bool has_zs = opts && opts->source_zs;
-        std::vector<float> zsv;
-        if (has_zs) {
-            zsv.resize(nv);
-            for (int64_t i = 0; i < nv; ++i)
-                zsv[i] = (float)((float)opts->source_zs[i * ia_skip] / (float)ds);
-        }
-        const size_t o_zs = has_zs ? pack.add(zsv.data(), nv * 4) : 0;
-        unsigned char* d = pack.upload(ctx, W_TABLES);
-        auto* cs = reinterpret_cast<sgx::ViewCS*>(d + o_cs);
-        sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), nv, 1, cs, ctx->stream);
-
-        // The kernels take rows padded to a multiple of 4 bins (16-byte aligned)
-        // and, for 16-bit data, readable bytes after the last view: device FP32
-        // input that already has that shape is used in place, anything else is
-        // copied into a padded buffer (pad bins are never read as taps).
-        const size_t esz = (flags & SGX_Q_BF16) ? 2 : 4;
-        const int64_t pitch = (ns + 3) / 4 * 4;
-        const void* q = proj;
-        if ((flags & (SGX_HOST | SGX_Q_BF16)) || pitch != ns ||
-            reinterpret_cast<uintptr_t>(proj) % 16 != 0) {
-            const size_t body = esz * pitch * nt * na;
-            void* dq = ctx->get(W_IN, body + sgx::kQ16Pad);
-            SGX_HIP(hipMemsetAsync(static_cast<char*>(dq) + body, 0, sgx::kQ16Pad, ctx->stream));
-            SGX_HIP(hipMemcpy2DAsync(dq, esz * pitch, proj, esz * ns, esz * ns, nt * na,
-                                     (flags & SGX_HOST) ? hipMemcpyHostToDevice
-                                                        : hipMemcpyDeviceToDevice,
-                                     ctx->stream));
-            q = dq;
-        }
-        const int64_t nvox = nx * ny * nz;
-        const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nvox, flags));
-        float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
-
-        sgx::CbctArgs a{};
-        a.q = q;
-        a.fmt = (flags & SGX_Q_BF16) ? sgx::SGX_Q_FMT_BF16 : sgx::SGX_Q_FMT_F32;
-        a.vscale = nullptr;
-        a.ns = (int)ns;
-        a.nt = (int)nt;
-        a.pitch = (int)pitch;
-        a.nviews = (int)nv;
-        a.view_stride = pitch * nt * ia_skip;
-        a.cs = cs;
-        a.xds = reinterpret_cast<const float*>(d + o_x);
-        a.yds = reinterpret_cast<const float*>(d + o_y);
-        a.zds = reinterpret_cast<const float*>(d + o_z);
-        a.nx = (int)nx;
-        a.ny = (int)ny;
-        a.nz = (int)nz;
-        a.mask = m;
-        a.image = img;
-        a.dsd_ds = (float)dsd / (float)ds;
-        a.dso_ds = (float)dso / (float)ds;
-        a.ws = (float)((ns + 1) / 2.0 + (double)(float)offset_s);
-        a.wt = (float)((nt + 1) / 2.0 + (double)(float)offset_t);
-        a.inv_dt_ds = 1.0f / ((float)dt / (float)ds);
-        a.is_arc = is_arc != 0;
-        a.zs = has_zs ? reinterpret_cast<const float*>(d + o_zs) : nullptr;
-        a.arc_t_exact = opts && opts->arc_t_exact;
-        if (ctx->cbct_kernel == 0) {
-            int4* win = static_cast<int4*>(
-                ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
-            sgx::launch_cbct_windows(a, win, ctx->stream);
-            a.win = win;
-        }
-        sgx::launch_cbct_back(a, ctx->stream);
-        finish(ctx, image, img, 4 * nvox, flags);
+    // The kernels take rows padded to a multiple of 4 bins (16-byte aligned)
+    // and, for 16-bit data, readable bytes after the last view: device FP32
+    // input that already has that shape is used in place, anything else is
+    // copied into a padded buffer (pad bins are never read as taps).
+    const size_t esz = (flags & SGX_Q_BF16) ? 2 : 4;
+    const int64_t pitch = (ns + 3) / 4 * 4;
+    const void* q = proj;
+    if ((flags & (SGX_HOST | SGX_Q_BF16)) || pitch != ns ||
+        reinterpret_cast<uintptr_t>(proj) % 16 != 0) {
+        const size_t body = esz * pitch * nt * na;
+        void* dq = ctx->get(W_IN, body + sgx::kQ16Pad);
+        SGX_HIP(hipMemsetAsync(static_cast<char*>(dq) + body, 0, sgx::kQ16Pad, ctx->stream));
+        SGX_HIP(hipMemcpy2DAsync(dq, esz * pitch, proj, esz * ns, esz * ns, nt * na,
+                                 (flags & SGX_HOST) ? hipMemcpyHostToDevice
+                                                    : hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+        q = dq;
     }
+    const int64_t nvox = nx * ny * nz;
+    const uint8_t* m = static_cast<const uint8_t*>(stage_in(ctx, W_MASK, mask, nvox, flags));
+    float* img = (flags & SGX_HOST) ? static_cast<float*>(ctx->get(W_OUT, 4 * nvox)) : image;
+    sgx::CbctArgs a = cbct_call_args(T, d, q, (flags & SGX_Q_BF16) != 0, ns, nt, pitch, ia_skip,
+                                     dsd, dso, ds, dt, offset_s, offset_t, is_arc, nx, ny, nz, m,
+                                     img, opts);
+    if (ctx->cbct_kernel == 0) {
+        int4* win = static_cast<int4*>(
+            ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
+        sgx::launch_cbct_windows(a, win, ctx->stream);
+        a.win = win;
+    }
+    sgx::launch_cbct_back(a, ctx->stream);
+    finish(ctx, image, img, 4 * nvox, flags);
 }
 }  // namespace
+
+extern "C" {
 
 int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_t na,
                   int64_t ia_skip, const double* betas, double dsd, double dso,
@@ -437,6 +519,55 @@ int sgx_cbct_back_opts(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, i
     return guard(ctx, [&] {
         cbct_back_impl(ctx, proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s,
                        offset_t, is_arc, xc, nx, yc, ny, zc, nz, mask, image, flags, opts);
+    });
+}
+
+int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t nt, int64_t na,
+                          int64_t ia_skip, const double* betas, double dsd, double dso,
+                          double ds, double dt, double offset_s, double offset_t, int is_arc,
+                          const double* xc, int64_t nx, const double* yc, int64_t ny,
+                          const double* zc, int64_t nz, const uint8_t* mask, float* image,
+                          uint32_t flags)
+{
+    return guard(ctx, [&] {
+        require((flags & ~(uint32_t)SGX_ASYNC) == 0, SGX_ERR_UNSUPPORTED,
+                "sgx_cbct_back_noalloc: device FP32 data only (flags: SGX_ASYNC)");
+        cbct_check(proj, ns, nt, na, ia_skip, betas, ds, dt, xc, nx, yc, ny, zc, nz, image,
+                   flags);
+        require(ns % 4 == 0 && reinterpret_cast<uintptr_t>(proj) % 16 == 0,
+                SGX_ERR_UNSUPPORTED,
+                "sgx_cbct_back_noalloc: ns must be a multiple of 4 and proj 16-byte aligned "
+                "(other shapes need a padded copy: use sgx_cbct_back)");
+        require(ctx->scratch_err.empty(), SGX_ERR_DEVICE, ctx->scratch_err.c_str());
+        unsigned char* base = scratch_ptr(ctx->device);
+        require(base != nullptr, SGX_ERR_DEVICE, "device scratch not set up");
+        CbctCall T = cbct_call_tables(na, ia_skip, betas, ds, xc, nx, yc, ny, zc, nz, nullptr);
+        const size_t tbytes = (T.pack.host.size() + 255) & ~size_t(255);
+        sgx::CbctArgs a = cbct_call_args(T, base, proj, false, ns, nt, ns, ia_skip, dsd, dso, ds,
+                                         dt, offset_s, offset_t, is_arc, nx, ny, nz, mask, image,
+                                         nullptr);
+        const bool staged = ctx->cbct_kernel == 0;
+        const size_t need =
+            tbytes + (staged ? sizeof(int4) * (size_t)sgx::cbct_window_entries(a) : 0);
+        if (need > kScratchBytes) {
+            std::ostringstream os;
+            os << "sgx_cbct_back_noalloc: this call needs " << need << " bytes of scratch, "
+               << "more than its " << kScratchBytes << " (use sgx_cbct_back)";
+            throw Error(SGX_ERR_UNSUPPORTED, os.str());
+        }
+        SGX_HIP(hipMemcpyAsync(base, T.pack.host.data(), T.pack.host.size(),
+                               hipMemcpyHostToDevice, ctx->stream));
+        // the tables' host memory must outlive the copy
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        sgx::launch_view_trig(reinterpret_cast<const double*>(base + T.o_b), T.nv, 1,
+                              reinterpret_cast<sgx::ViewCS*>(base + T.o_cs), ctx->stream);
+        if (staged) {
+            int4* win = reinterpret_cast<int4*>(base + tbytes);
+            sgx::launch_cbct_windows(a, win, ctx->stream);
+            a.win = win;
+        }
+        sgx::launch_cbct_back(a, ctx->stream);
+        if (!(flags & SGX_ASYNC)) SGX_HIP(hipStreamSynchronize(ctx->stream));
     });
 }
 

@@ -51,7 +51,7 @@ EXPORTS = (
     "sgx_plan_back_t", "sgx_net_create", "sgx_net_destroy", "sgx_net_num_params",
     "sgx_net_params", "sgx_net_grads", "sgx_net_zero_grad", "sgx_net_forward",
     "sgx_net_backward", "sgx_net_adam", "sgx_mse",
-    "sgx_project_bdd", "sgx_backproject_bdd",
+    "sgx_project_bdd", "sgx_backproject_bdd", "sgx_cbct_back_noalloc",
 )
 
 
@@ -118,6 +118,8 @@ def _declare(L):
                                 i32, vp, i64, vp, i64, vp, i64, vp, vp, u32]
     L.sgx_cbct_back_opts.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64,
                                      f64, i32, vp, i64, vp, i64, vp, i64, vp, vp, u32, vp]
+    L.sgx_cbct_back_noalloc.argtypes = [vp, vp, i64, i64, i64, i64, vp, f64, f64, f64, f64, f64,
+                                        f64, i32, vp, i64, vp, i64, vp, i64, vp, vp, u32]
     L.sgx_sino_filter.argtypes = [vp, vp, i64, i64, vp, i64, i64, i32, vp, u32]
     L.sgx_radon_ellipses.argtypes = [vp, i32, vp, i64, vp, i64, f64, f64, vp, i64, vp, u32]
     L.sgx_radon_ellipsoids.argtypes = [vp, i32, vp, i64, vp, i64, vp, vp, i64, f64, f64, vp,
