@@ -126,17 +126,23 @@ int sgx_cbct_back(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int64_
 
 /* Cone-beam options that the reference leaves commented out in cbct_back
  * (src/fbp3/cbct-back.jl:36-59 and 188: source_zs; :180: the arc detector's
- * t magnification).  All zero (or a NULL pointer to this struct) is the
- * reference's geometry.
- *   source_zs    na source z positions (host, units of xc / ds, e.g. the
- *                helical _source_zs of src/geom/ct-geom.jl:134-140), or NULL
- *                for z = 0: t' = mag (z - source_zs[ia]);
- *   arc_t_exact  1: for the arc detector the t magnification of the ray that
- *                meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2),
- *                instead of the flat detector's dsd / y_beta. */
+ * t magnification; :36, :53, :157, :178: the source offset).  All zero (or a
+ * NULL pointer to this struct) is the reference's geometry.
+ *   source_zs      na source z positions (host, units of xc / ds, e.g. the
+ *                  helical _source_zs of src/geom/ct-geom.jl:134-140), or
+ *                  NULL for z = 0: t' = mag (z - source_zs[ia]);
+ *   arc_t_exact    1: for the arc detector the t magnification of the ray
+ *                  that meets the detector cylinder,
+ *                  dsd / sqrt(r_loop^2 + y_beta^2), instead of the flat
+ *                  detector's dsd / y_beta;
+ *   source_offset  the lateral source offset (units of ds), rg.source_offset:
+ *                  r_loop = x_beta - source_offset / ds in place of x_beta in
+ *                  s' and w2, for both detectors, as fbp_back_fan_xy applies
+ *                  it (src/fbp2/back-fan.jl:303-311). */
 typedef struct {
     const double* source_zs;
     int32_t arc_t_exact;
+    double source_offset;
 } sgx_cbct_opts;
 
 /* sgx_cbct_back with the options above; no counterpart in the reference (it

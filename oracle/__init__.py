@@ -77,6 +77,9 @@ def _declare(L):
         getattr(L, f"orc_cbct_back_opts_{suf}").argtypes = [
             _vp, _i64, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _f64, _f64, _int,
             _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _int, _vp]
+        getattr(L, f"orc_cbct_back_opts2_{suf}").argtypes = [
+            _vp, _i64, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _f64, _f64, _int,
+            _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp, _int, _f64, _vp]
         getattr(L, f"orc_cbct_back_rp_{suf}").argtypes = [
             _vp, _vp, _i64, _i64, _i64, _vp, _f64, _f64, _f64, _f64, _int,
             _vp, _i64, _vp, _i64, _vp, _i64, _vp, _vp]
@@ -320,18 +323,22 @@ def back_fan(sino, sinb, cosb, dsd, dso, source_offset, is_arc, ds, offset, xc, 
 
 
 def cbct_back(proj, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc, zc, mask,
-              ia_skip=1, dtype=np.float64, source_zs=None, arc_t_exact=False):
+              ia_skip=1, dtype=np.float64, source_zs=None, arc_t_exact=False,
+              source_offset=0.0):
     """``cbct_back_fan!`` (src/fbp3/cbct-back.jl:95-144).  ``source_zs`` (na
-    source z positions) and ``arc_t_exact`` are the options the reference
-    leaves commented out (:36-59, :180, :188): t' = mag (z - source_zs[ia]), and
-    for the arc detector mag = dsd / sqrt(x_beta^2 + y_beta^2) in t."""
+    source z positions), ``arc_t_exact`` and ``source_offset`` are the options
+    the reference leaves commented out (:36-59, :157, :178-180, :188):
+    t' = mag (z - source_zs[ia]); for the arc detector mag = dsd /
+    sqrt(r_loop^2 + y_beta^2) in t; r_loop = x_beta - source_offset / ds in s'
+    and w2 (the flat detector takes it as fbp_back_fan_xy does,
+    src/fbp2/back-fan.jl:303, 309-311)."""
     proj = _F(proj, dtype)
     ns, nt, na = proj.shape
     xc = _F(xc, np.float64); yc = _F(yc, np.float64); zc = _F(zc, np.float64)
     mask = _F(mask, np.uint8)
     betas = _F(betas, np.float64)
     img = np.zeros((len(xc), len(yc), len(zc)), dtype=dtype, order="F")
-    if source_zs is None and not arc_t_exact:
+    if source_zs is None and not arc_t_exact and source_offset == 0.0:
         getattr(lib(), "orc_cbct_back_" + _suffix(dtype))(
             _p(proj), ns, nt, na, ia_skip, _p(betas), dsd, dso, ds, dt, offset_s, offset_t,
             int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask),
@@ -340,10 +347,10 @@ def cbct_back(proj, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc,
     zs = None if source_zs is None else _F(source_zs, np.float64)
     if zs is not None and zs.shape != (na,):
         raise ValueError("source_zs needs one z per view")
-    getattr(lib(), "orc_cbct_back_opts_" + _suffix(dtype))(
+    getattr(lib(), "orc_cbct_back_opts2_" + _suffix(dtype))(
         _p(proj), ns, nt, na, ia_skip, _p(betas), dsd, dso, ds, dt, offset_s, offset_t,
         int(bool(is_arc)), _p(xc), len(xc), _p(yc), len(yc), _p(zc), len(zc), _p(mask),
-        None if zs is None else _p(zs), int(bool(arc_t_exact)), _p(img))
+        None if zs is None else _p(zs), int(bool(arc_t_exact)), float(source_offset), _p(img))
     return img
 
 

@@ -124,8 +124,11 @@ void back_fan(const T* sino, int64_t nb, int64_t na, int64_t ia_skip,
 
 // ---------------------------------------------------------------------------
 // cbct_back_fan! / cbct_back_fan_voxel   (src/fbp3/cbct-back.jl:95-213)
-// source_offset is ignored, as in the reference (:36,:53).  Two options that
-// the reference leaves commented out, off by default:
+// Three options that the reference leaves commented out, off by default:
+//   source_offset: the lateral source offset of the commented arc branch,
+//   r_loop = x_beta - offset_source / ds (:157, :178), in s' and w2; applied
+//   to the flat detector as fbp_back_fan_xy does (src/fbp2/back-fan.jl:303,
+//   309-311: s' = mag r_loop);
 //   zs (na source z positions, or null): t' = mag (z - zs[ia]) (:188);
 //   arc_t_exact: for the arc detector, the t magnification of the ray that
 //   meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2), instead of
@@ -137,7 +140,8 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
                double offset_s, double offset_t, int is_arc,
                const double* xc, int64_t nx, const double* yc, int64_t ny,
                const double* zc, int64_t nz, const uint8_t* mask, T* image,
-               const T* dnext = nullptr, const double* zs = nullptr, int arc_t_exact = 0)
+               const T* dnext = nullptr, const double* zs = nullptr, int arc_t_exact = 0,
+               double source_offset = 0.0)
 {
     // dnext (optional, same shape as proj): the stored row-pair differences;
     // the taps of row it+1 are then proj[., it] + dnext[., it] (a checker for
@@ -146,6 +150,7 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
     const T ws = (T)(float)((ns + 1) / 2.0 + (double)(float)offset_s);  // :119
     const T wt = (T)(float)((nt + 1) / 2.0 + (double)(float)offset_t);  // :120
     const T dt_ds = (T)dt / (T)ds, dsd_ds = (T)dsd / (T)ds, dso_ds = (T)dso / (T)ds;
+    const T so_ds = (T)source_offset / (T)ds;
     std::vector<T> s(na_sub), c(na_sub), xd(nx), yd(ny), zd(nz), zsd(na_sub, (T)0);
     for (int64_t i = 0; i < na_sub; ++i) {
         const T b = (T)betas[i * ia_skip];
@@ -166,18 +171,19 @@ void cbct_back(const T* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_ski
             const T sb = s[ja], cb = c[ja];
             const T x_beta = x * cb + y * sb;                             // :174
             const T y_beta = dso_ds - (-x * sb + y * cb);                 // :175
+            const T r_loop = source_offset != 0.0 ? x_beta - so_ds : x_beta;   // :178
             T sprime, mag, w2;
             if (is_arc) {
-                sprime = dsd_ds * std::atan2(x_beta, y_beta);             // :179
+                sprime = dsd_ds * std::atan2(r_loop, y_beta);             // :179
                 mag = dsd_ds / y_beta;
-                w2 = (dsd_ds * dsd_ds) / (x_beta * x_beta + y_beta * y_beta);
+                w2 = (dsd_ds * dsd_ds) / (r_loop * r_loop + y_beta * y_beta);
             } else {
                 mag = dsd_ds / y_beta;                                    // :183
-                sprime = mag * x_beta;
+                sprime = mag * r_loop;
                 w2 = mag * mag;
             }
             const T magt = (is_arc && arc_t_exact)
-                               ? dsd_ds / std::sqrt(x_beta * x_beta + y_beta * y_beta)
+                               ? dsd_ds / std::sqrt(r_loop * r_loop + y_beta * y_beta)
                                : mag;
             const T tprime = magt * (z - zsd[ja]);                        // :188-189
             const T bs = sprime + ws;
@@ -401,6 +407,22 @@ ORC_EXPORT void orc_cbct_back_opts_f32(const float* proj, int64_t ns, int64_t nt
     const uint8_t* mask, const double* zs, int arc_t_exact, float* image)
 { cbct_back<float>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
                    xc, nx, yc, ny, zc, nz, mask, image, nullptr, zs, arc_t_exact); }
+
+ORC_EXPORT void orc_cbct_back_opts2_f64(const double* proj, int64_t ns, int64_t nt, int64_t na,
+    int64_t ia_skip, const double* betas, double dsd, double dso, double ds, double dt,
+    double offset_s, double offset_t, int is_arc,
+    const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc, int64_t nz,
+    const uint8_t* mask, const double* zs, int arc_t_exact, double source_offset, double* image)
+{ cbct_back<double>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
+                    xc, nx, yc, ny, zc, nz, mask, image, nullptr, zs, arc_t_exact, source_offset); }
+
+ORC_EXPORT void orc_cbct_back_opts2_f32(const float* proj, int64_t ns, int64_t nt, int64_t na,
+    int64_t ia_skip, const double* betas, double dsd, double dso, double ds, double dt,
+    double offset_s, double offset_t, int is_arc,
+    const double* xc, int64_t nx, const double* yc, int64_t ny, const double* zc, int64_t nz,
+    const uint8_t* mask, const double* zs, int arc_t_exact, double source_offset, float* image)
+{ cbct_back<float>(proj, ns, nt, na, ia_skip, betas, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
+                   xc, nx, yc, ny, zc, nz, mask, image, nullptr, zs, arc_t_exact, source_offset); }
 
 ORC_EXPORT void orc_cbct_back_rp_f64(const double* proj, const double* dnext, int64_t ns,
     int64_t nt, int64_t na, const double* betas, double dsd, double dso, double ds, double dt,

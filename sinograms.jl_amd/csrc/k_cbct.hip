@@ -8,10 +8,13 @@
 //         w2 = dsd^2 / (x_beta^2 + y_beta^2)
 //   t' = mag z,  bs = s' + ws,  bt = t' / (dt/ds) + wt
 //   if 1 <= floor(bs) < ns and 1 <= floor(bt) < nt: add w2 * bilinear(q)
-// Options (off by default, commented out in the reference at :36-59, :180,
-// :188): t' = mag_t (z - zs[view]) with a per-view source z, and for the arc
-// detector mag_t = dsd / sqrt(x_beta^2 + y_beta^2) (the ray meets the
-// detector cylinder) instead of dsd / y_beta.  Both enter as per-view,
+// Options (off by default, commented out in the reference at :36-59, :157,
+// :178-180, :188): t' = mag_t (z - zs[view]) with a per-view source z; for
+// the arc detector mag_t = dsd / sqrt(r^2 + y_beta^2) (the ray meets the
+// detector cylinder) instead of dsd / y_beta; and a lateral source offset,
+// r = x_beta - source_offset / ds in place of x_beta in s' and w2 (as the
+// commented arc branch has it, and as fbp_back_fan_xy applies it to both
+// detectors, src/fbp2/back-fan.jl:303-311).  The t options enter as per-view,
 // z-independent terms: bt = mz z + wtv with mz = mag_t / (dt/ds) and
 // wtv = wt - mz zs (= wt exactly when zs = 0).
 //
@@ -139,12 +142,12 @@ struct ColView {
 // mz and the t offset of one column and view (see the options above); OPT
 // false compiles the options out (the reference's geometry).
 template <bool ARC, bool OPT>
-__device__ __forceinline__ void t_map(float xb, float yb, float mag, float dsd_ds,
+__device__ __forceinline__ void t_map(float rb, float yb, float mag, float dsd_ds,
                                       float inv_dt_ds, float wt, const float* zs, int ia,
                                       int arc_t_exact, float& mz, float& wtv)
 {
     float m = mag;
-    if (ARC && OPT && arc_t_exact) m = dsd_ds * __builtin_amdgcn_rsqf(xb * xb + yb * yb);
+    if (ARC && OPT && arc_t_exact) m = dsd_ds * __builtin_amdgcn_rsqf(rb * rb + yb * yb);
     mz = m * inv_dt_ds;
     wtv = wt;
     if (OPT && zs) wtv = fmaf(-mz, zs[ia], wt);
@@ -154,10 +157,11 @@ template <bool ARC, int FMT, bool OPT>
 __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float dsd_ds,
                                             float dso_ds, float ws, float nsf, float inv_dt_ds,
                                             float vs, float wt, const float* zs, int ia,
-                                            int arc_t_exact, int narrow)
+                                            int arc_t_exact, int narrow, float so_ds)
 {
     ColView r;
-    const float xb = x * v.c + y * v.s;
+    const float xb0 = x * v.c + y * v.s;
+    const float xb = OPT ? xb0 - so_ds : xb0;          // r_loop (source offset)
     const float yb = dso_ds - (-x * v.s + y * v.c);
     const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
     float sp, w2;
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow)
+    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow, float so_ds)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
             const ColView cv = col_view<ARC, FMT, OPT>(x, y, cs[ia], dsd_ds, dso_ds, ws, nsf,
                                                        inv_dt_ds,
                                                        FMT == F_F16S ? vscale[ia] : 1.f, wt, zs,
-                                                       ia, arc_t_exact, narrow);
+                                                       ia, arc_t_exact, narrow, so_ds);
             if (!cv.hit) continue;                        // the column misses the detector
             const float blo = fmaf(cv.mz, zlo, cv.wt), bhi = fmaf(cv.mz, zhi, cv.wt);
             const bool fast = blo >= 1.f && blo < ntf && bhi >= 1.f && bhi < ntf;
@@ -340,7 +344,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
                                    const float* __restrict__ zds, int nx, int ny, int nz, int ns,
                                    int nt, float dsd_ds, float dso_ds, float ws, float wt,
                                    float inv_dt_ds, int is_arc, const float* __restrict__ zs,
-                                   int arc_t_exact, int4* __restrict__ win)
+                                   int arc_t_exact, float so_ds, int4* __restrict__ win)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nblocks * nviews) return;
@@ -374,7 +378,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     bool ok = true;
     for (int k = 0; k < 4; ++k) {
         const float x = (k & 1) ? x1 : x0, y = (k & 2) ? y1 : y0;
-        const float xb = x * v.c + y * v.s;
+        const float xb = x * v.c + y * v.s - so_ds;
         const float yb = dso_ds - (-x * v.s + y * v.c);
         ok = ok && yb > 0.f;
         const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
@@ -423,7 +427,7 @@ cbct_staged_kernel(
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
-    int arc_t_exact, int narrow)
+    int arc_t_exact, int narrow, float so_ds)
 {
     using L = Lds<FMT>;
     constexpr int kLP = L::pitch;
@@ -536,7 +540,8 @@ cbct_staged_kernel(
     auto lane_view = [&](int ia, int4 w, bool check) {
         LaneView r;
         const ViewCS v = cs[ia];
-        const float xb = x * v.c + y * v.s;
+        const float xb0 = x * v.c + y * v.s;
+        const float xb = OPT ? xb0 - so_ds : xb0;      // r_loop (source offset)
         const float yb = dso_ds - (-x * v.s + y * v.c);
         const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
         float sp;
@@ -676,7 +681,7 @@ void launch_gather(const CbctArgs& a, hipStream_t st)
                        dim3(64 * kWaves), 0, st, a.q, a.ns, a.nt, a.pitch, a.nviews,
                        a.view_stride, a.cs, a.vscale, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz,
                        a.mask, a.image, a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.zs,
-                       a.arc_t_exact ? 1 : 0, arc_narrow(a));
+                       a.arc_t_exact ? 1 : 0, arc_narrow(a), a.so_ds);
     SGX_HIP(hipGetLastError());
 }
 
@@ -690,7 +695,7 @@ void launch_staged(const CbctArgs& a, hipStream_t st)
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
                        a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0,
-                       arc_narrow(a));
+                       arc_narrow(a), a.so_ds);
     SGX_HIP(hipGetLastError());
 }
 
@@ -707,7 +712,7 @@ template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
 
 template <int FMT> void launch_fmt(const CbctArgs& a, hipStream_t st)
 {
-    if (a.zs || a.arc_t_exact) launch_opt<FMT, true>(a, st);
+    if (a.zs || a.arc_t_exact || a.so_ds != 0.f) launch_opt<FMT, true>(a, st);
     else launch_opt<FMT, false>(a, st);
 }
 
@@ -743,7 +748,7 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
     hipLaunchKernelGGL(cbct_window_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        (int)nb, a.nviews, a.cs, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.ns, a.nt,
                        a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, a.zs,
-                       a.arc_t_exact ? 1 : 0, win);
+                       a.arc_t_exact ? 1 : 0, a.so_ds, win);
     SGX_HIP(hipGetLastError());
 }
 

@@ -443,6 +443,7 @@ sgx::CbctArgs cbct_call_args(const CbctCall& T, unsigned char* d, const void* q,
     a.is_arc = is_arc != 0;
     a.zs = T.has_zs ? reinterpret_cast<const float*>(d + T.o_zs) : nullptr;
     a.arc_t_exact = opts && opts->arc_t_exact;
+    a.so_ds = opts ? (float)opts->source_offset / (float)ds : 0.f;
     return a;
 }
 
@@ -803,6 +804,7 @@ struct sgx_plan {
     float* wtab = nullptr;          // cone beam on the FFT form: (ns, nt) weights, in `tables`
     float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
+    float so_ds = 0.f;              // cone-beam option: source offset / ds
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
     // Bumped by every change that a captured batch graph would not see
@@ -899,6 +901,7 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     a.win = P.win;
     a.zs = P.zs;
     a.arc_t_exact = P.arc_t_exact;
+    a.so_ds = P.so_ds;
     return a;
 }
 
@@ -1271,6 +1274,7 @@ int sgx_plan_set_cbct_opts(sgx_plan* plan, const sgx_cbct_opts* opts)
             P.zs = nullptr;
         }
         P.arc_t_exact = opts && opts->arc_t_exact;
+        P.so_ds = opts ? (float)opts->source_offset / (float)P.geom.ds : 0.f;
         ++P.gen;
         // the staged backprojector's windows follow the t mapping
         sgx::launch_cbct_windows(cbct_args_of(P, nullptr), P.win, plan->ctx->stream);

@@ -112,6 +112,8 @@ struct CbctArgs {
     // meets the detector cylinder, dsd / sqrt(x_beta^2 + y_beta^2)
     const float* zs;
     bool arc_t_exact;
+    // the lateral source offset / ds (r_loop = x_beta - so_ds), 0 for none
+    float so_ds = 0.f;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 // Entries of the window table the staged kernel takes for these arguments.

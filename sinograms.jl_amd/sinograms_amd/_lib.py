@@ -68,7 +68,8 @@ class SgxGeom(ctypes.Structure):
 
 
 class SgxCbctOpts(ctypes.Structure):
-    _fields_ = [("source_zs", ctypes.c_void_p), ("arc_t_exact", ctypes.c_int32)]
+    _fields_ = [("source_zs", ctypes.c_void_p), ("arc_t_exact", ctypes.c_int32),
+                ("source_offset", ctypes.c_double)]
 
 
 class SgxError(RuntimeError):

@@ -392,24 +392,29 @@ def fbp_back(rg, ig: ImageGeom, sino, ia_skip: int = 1):
     return img
 
 
-def _cbct_opts(rg: CtFan, use_source_zs: bool, arc_t_exact: bool):
+def _cbct_opts(rg: CtFan, use_source_zs: bool, arc_t_exact: bool,
+               use_source_offset: bool = False):
     """The cone-beam options (None: the reference's geometry) and the array
     they point to, which must stay alive for the call."""
-    if not use_source_zs and not arc_t_exact:
+    so = float(rg.source_offset) if use_source_offset else 0.0
+    if not use_source_zs and not arc_t_exact and so == 0.0:
         return None, None
     zs = _f64(rg.source_zs()) if use_source_zs else None
-    return _lib.SgxCbctOpts(_dptr(zs), int(bool(arc_t_exact))), zs
+    return _lib.SgxCbctOpts(_dptr(zs), int(bool(arc_t_exact)), so), zs
 
 
 def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1, use_source_zs: bool = False,
-              arc_t_exact: bool = False):
+              arc_t_exact: bool = False, use_source_offset: bool = False):
     """Voxel-driven cone-beam backprojection (cbct-back.jl:25-45).
 
-    Options the reference keeps commented out (:36-59, :180, :188), off by
-    default: ``use_source_zs`` backprojects with the source z of every view,
-    ``rg.source_zs()`` (a helical ``CtSourceHelix`` trajectory), t' = mag
-    (z - zs); ``arc_t_exact`` uses, for the arc detector, the t magnification
-    of the ray that meets the detector cylinder, dsd / sqrt(x_b^2 + y_b^2).
+    Options the reference keeps commented out (:36-59, :157, :178-180, :188),
+    off by default: ``use_source_zs`` backprojects with the source z of every
+    view, ``rg.source_zs()`` (a helical ``CtSourceHelix`` trajectory), t' =
+    mag (z - zs); ``arc_t_exact`` uses, for the arc detector, the t
+    magnification of the ray that meets the detector cylinder,
+    dsd / sqrt(r^2 + y_b^2); ``use_source_offset`` applies ``rg.source_offset``,
+    r = x_b - source_offset / ds in s' and w2 (for both detectors, as the 2-D
+    fan backprojector does, back-fan.jl:303-311).
     """
     a = _Arr(proj)
     if a.shape != rg.dims() or len(ig.dims) != 3:
@@ -420,7 +425,7 @@ def cbct_back(proj, rg: CtFan, ig: ImageGeom, ia_skip: int = 1, use_source_zs: b
     mask = _mask_arr(ig, a)
     img, iptr = a.out(ig.dims)
     ctx = _ctx_for(a)
-    opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact)
+    opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact, use_source_offset)
     args = (ctx.h, a.ptr, ns, nt, na, ia_skip, _dptr(betas), rg.dsd, rg.dso, rg.ds, rg.dt,
             rg.offset_s, rg.offset_t, int(rg.is_arc), _dptr(xc), len(xc), _dptr(yc), len(yc),
             _dptr(zc), len(zc), _mask_ptr(mask), iptr, a.flags)
@@ -604,7 +609,8 @@ def _fdk_weights(rg, device=0):
 
 def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1: bool = True,
              filter=None, precision: str = "fp32", device: int = 0,
-             use_source_zs: bool = False, arc_t_exact: bool = False, **_ignored):
+             use_source_zs: bool = False, arc_t_exact: bool = False,
+             use_source_offset: bool = False, **_ignored):
     """``plan_fbp`` (plan2.jl:106-138 for SinoGeom, plan3.jl:60-72 for CtGeom).
 
     precision: "fp32" (FP32 filter and storage), or for cone-beam plans the
@@ -613,8 +619,9 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
     scaled operands, FP16 filtered projections scaled per view by a power of
     two); backprojection always accumulates in FP32.
 
-    use_source_zs, arc_t_exact (cone-beam plans): the backprojection options
-    of ``cbct_back``, which the reference keeps commented out; off by default.
+    use_source_zs, arc_t_exact, use_source_offset (cone-beam plans): the
+    backprojection options of ``cbct_back``, which the reference keeps
+    commented out; off by default.
     """
     nb = rg.nb if isinstance(rg, SinoGeom) else rg.ns
     npad = npad or _nextpow2(nb + 1)
@@ -623,12 +630,13 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
     if isinstance(rg, CtFan):
         vw = np.float32(_angle_weights(rg.ar(f32=True)))   # the full array is on the device
         plan = FDKplan(rg, ig, filter, vw, precision, device)
-        opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact)
+        opts, _keep = _cbct_opts(rg, use_source_zs, arc_t_exact, use_source_offset)
         if opts is not None:
             plan._ctx.check(_lib.load().sgx_plan_set_cbct_opts(plan.h, ctypes.byref(opts)))
         return plan
-    if use_source_zs or arc_t_exact:
-        raise ValueError("use_source_zs and arc_t_exact apply to cone-beam geometries")
+    if use_source_zs or arc_t_exact or use_source_offset:
+        raise ValueError("use_source_zs, arc_t_exact and use_source_offset apply to cone-beam "
+                         "geometries")
     vw = parker_weight_par(rg.orbit, rg.angles() - rg.orbit_start) if isinstance(rg, SinoPar) \
         else np.ones((1, 1), dtype=np.float32)
     return FBPNormalPlan(rg, ig, filter, vw, precision, device)

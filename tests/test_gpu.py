@@ -1136,3 +1136,50 @@ def test_c4_learned_workload(sg, torch_cuda):
     assert e < 1e-6
     assert r["psnr_learned"] > r["psnr_plain"] + 3.0
     assert r["ssim_learned"] > r["ssim_plain"] + 0.1
+
+
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("exact", [False, True])
+def test_cbct_back_source_offset_vs_oracle(sg, orc, torch_cuda, cbct_kernel, arc, exact):
+    # the lateral source offset the reference keeps commented out
+    # (cbct-back.jl:178), pinned on the CPU to its fan-beam formula; with the
+    # exact arc t magnification and a helix in the second case
+    _say("cbct_back source offset", cbct_kernel, arc, exact)
+    rng = np.random.default_rng(14)
+    src = sg.CtSourceHelix(pitch=0.8, source_z0=-6.0) if exact else sg.CtSourceCircle()
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=48, nt=36, ds=1.1, dt=1.3, na=30,
+                                                 offset_s=0.25, offset_t=-0.5, dsd=120.0,
+                                                 dod=50.0, source_offset=3.3, src=src)
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    ig = sg.ImageGeom((29, 23, 26), (1.0, 0.9, 1.1), (0.3, 0.0, -0.2))
+    xc, yc, zc = ig.axes()
+    for ia_skip in (1, 2):
+        got = sg.cbct_back(proj, rg, ig, ia_skip=ia_skip, use_source_offset=True,
+                           use_source_zs=exact, arc_t_exact=exact)
+        ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, rg.offset_s,
+                            rg.offset_t, arc, xc, yc, zc, ig.mask, ia_skip=ia_skip,
+                            dtype=np.float32, source_zs=rg.source_zs() if exact else None,
+                            arc_t_exact=exact, source_offset=rg.source_offset)
+        assert rel_rmse(got, ref) < TOL32, ia_skip
+    base = sg.cbct_back(proj, rg, ig, ia_skip=2, use_source_zs=exact, arc_t_exact=exact)
+    assert rel_rmse(base, got) > 1e-3
+
+
+@pytest.mark.parametrize("arc", [False, True])
+def test_fdk_plan_source_offset_vs_oracle(sg, orc, torch_cuda, arc):
+    _say("fdk plan source offset", arc)
+    rng = np.random.default_rng(15)
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=64, nt=40, ds=1.0, dt=1.2, na=32,
+                                                 source_offset=-2.5)
+    ig = sg.ImageGeom((30, 28, 26), (1.0, 1.0, 1.0))
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    plan = sg.plan_fbp(rg, ig, use_source_offset=True)
+    got = sg.fdk(plan, proj)
+    ar = rg.ar(f32=True).astype(np.float32)
+    vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, arc)[:, :, None] * \
+        orc.angle_weight(ar)
+    xc, yc, zc = ig.axes()
+    ref = orc.fdk(proj, plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, arc, xc,
+                  yc, zc, ig.mask, dtype=np.float32, source_offset=rg.source_offset)
+    assert rel_rmse(got, ref) < TOL32
+    assert rel_rmse(sg.fdk(sg.plan_fbp(rg, ig), proj), got) > 1e-3

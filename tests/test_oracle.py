@@ -474,3 +474,30 @@ def test_nn_oracle_backward_matches_finite_differences():
     assert np.allclose(onn.gelu(x), x * 0.5 * (1 + np.vectorize(math.erf)(x / math.sqrt(2))))
     assert np.allclose(onn.gelu_grad(x), (onn.gelu(x + 1e-6) - onn.gelu(x - 1e-6)) / 2e-6,
                        atol=1e-6)
+
+
+@pytest.mark.parametrize("arc", [False, True])
+def test_cbct_source_offset_midplane_is_the_fan_backprojector(orc, arc):
+    # The cone-beam source offset (the commented r_loop = x_beta - offset of
+    # cbct-back.jl:178) pinned on the reference's own fan-beam formula: with
+    # every detector row equal to one 2-D sinogram, the voxels of the z = 0
+    # plane (t' = 0, both row taps equal) see exactly fbp_back_fan_xy with the
+    # same source_offset (back-fan.jl:299-333), for both detectors.
+    rng = np.random.default_rng(5)
+    ns, nt, na = 41, 9, 23
+    dsd, dso, ds, so = 150.0, 100.0, 1.3, 4.7
+    q2 = rng.standard_normal((ns, na))
+    proj = np.repeat(q2[:, None, :], nt, axis=1)
+    betas = np.linspace(0, 2 * np.pi, na, endpoint=False) + 0.1
+    xc = np.arange(-9, 10) * 1.1 + 0.3
+    yc = np.arange(-8, 9) * 0.9 - 0.2
+    zc = np.array([0.0])
+    m3 = np.ones((len(xc), len(yc), 1), bool)
+    cone = orc.cbct_back(proj, betas, dsd, dso, ds, 1.0, 0.0, 0.0, arc, xc, yc, zc, m3,
+                         source_offset=so)[:, :, 0]
+    fan = orc.back_fan(q2, np.sin(betas), np.cos(betas), dsd, dso, so, arc, ds, 0.0, xc, yc,
+                       np.ones((len(xc), len(yc)), bool))
+    assert np.max(np.abs(cone - fan)) <= 1e-12 * np.max(np.abs(fan))
+    # and the offset matters
+    plain = orc.cbct_back(proj, betas, dsd, dso, ds, 1.0, 0.0, 0.0, arc, xc, yc, zc, m3)[:, :, 0]
+    assert np.max(np.abs(plain - cone)) > 1e-3 * np.max(np.abs(fan))
