@@ -51,13 +51,18 @@ enum {
 #define SGX_Q_BF16      0x10u  /* filtered data held in bfloat16 (device ptrs)    */
 
 /* geometry kinds (the reference's SinoPar, SinoFanArc, SinoFanFlat,
- * CtFanArc, CtFanFlat; src/geom/type2.jl:95-159, type3.jl:145-184) */
+ * CtFanArc, CtFanFlat, SinoMoj; src/geom/type2.jl:95-159, type3.jl:145-184).
+ * SGX_SINO_MOJ is the mojette geometry: ds is the pixel size d, and view
+ * beta samples the radial axis at d max(|cos beta|, |sin beta|) (_d_moj,
+ * src/geom/common.jl:128); its plans are the :mojette FBP (plan_fbp's `how`,
+ * src/fbp2/plan2.jl:95, 129-130). */
 enum {
     SGX_SINO_PAR = 0,
     SGX_SINO_FAN_ARC = 1,
     SGX_SINO_FAN_FLAT = 2,
     SGX_CT_FAN_ARC = 3,
-    SGX_CT_FAN_FLAT = 4
+    SGX_CT_FAN_FLAT = 4,
+    SGX_SINO_MOJ = 5
 };
 
 /* precision forms of a plan */
@@ -268,6 +273,9 @@ typedef struct {
 
 /* Create a plan.  betas: na angles (radians).  Image grid: axes xc, yc (, zc
  * for 3-D; pass nz = 1 and zc = NULL for 2-D) and an optional host mask.
+ * A SGX_SINO_MOJ plan filters each view with Hk / max(|cos|, |sin|) (the
+ * response for the view's radial spacing, with Hk that of spacing d) and
+ * backprojects with the view's spacing; Hk is fbp_filter for spacing d. 
  * Hk: the npad filter response of fbp_filter (host, Float32).  prec:
  * SGX_PREC_FP32 or SGX_PREC_BF16.  The plan owns device copies of all of
  * these plus the filtered-data buffer. */

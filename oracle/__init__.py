@@ -440,3 +440,44 @@ def fdk(proj, Hk, view_weight, ar, dsd, dso, ds, dt, offset_s, offset_t, is_arc,
     f = sino_filter(p, np.asarray(Hk, dtype=dtype), dtype=dtype)
     return cbct_back(f, ar, dsd, dso, ds, dt, offset_s, offset_t, is_arc, xc, yc, zc, mask,
                      dtype=dtype, **back_options)
+
+
+# ---------------------------------------------------------------------------
+# the :mojette FBP of a SinoMoj geometry
+# ---------------------------------------------------------------------------
+
+def d_moj(d, ar):
+    """``_d_moj(rg)(ar)`` = d max(|cos|, |sin|) (src/geom/common.jl:128)."""
+    ar = np.asarray(ar, np.float64)
+    return d * np.maximum(np.abs(np.cos(ar)), np.abs(np.sin(ar)))
+
+
+def fbp_moj(sino, ar, d, offset, xc, yc, mask, npad, window=None, decon1=True):
+    """Float64 restatement of the :mojette FBP of a SinoMoj sinogram (nb, na):
+    each view is ramp-filtered with the response built for its own radial
+    spacing dr = _d_moj(ar) (ramp_flat(N, dr), ramp.jl:26-29, through
+    fbp_filter, filter.jl:32-57; the per-view filter of the reference's
+    commented fbp2_make_sino_filter_moj, fbp.jl:150-170), weighted by the
+    angular step (_angle_weights, d-angle.jl:55-65; the "dphi" of fbp.jl:138-145)
+    and backprojected pixel by pixel at tau = (x cos + y sin) / dr (tau.jl:22-25)
+    with the linear interpolation and bounds of fbp_back_par_xy
+    (back-par.jl:292-312)."""
+    sino = np.asarray(sino, np.float64)
+    nb, na = sino.shape
+    ar = np.asarray(ar, np.float64)
+    dr = d_moj(d, ar)
+    aw = float(angle_weight(np.asarray(ar, np.float32)))
+    wb = (nb + 1) / 2 + float(np.float32(offset))
+    x = np.asarray(xc, np.float64)[:, None]
+    y = np.asarray(yc, np.float64)[None, :]
+    img = np.zeros((x.shape[0], y.shape[1]))
+    for j in range(na):
+        Hk = fbp_filter(ramp_flat(npad, dr[j])[0], npad, dr[j], window, decon1)
+        q = sino_filter(sino[:, j:j + 1] * aw, Hk)[:, 0]
+        bb = (x * np.cos(ar[j]) + y * np.sin(ar[j])) / dr[j] + wb
+        il = np.floor(bb).astype(np.int64)
+        ok = (il >= 1) & (il + 1 <= nb)
+        ilc = np.clip(il, 1, nb - 1)
+        wr = bb - il
+        img += np.where(ok, (1 - wr) * q[ilc - 1] + wr * q[ilc], 0.0)
+    return np.where(np.asarray(mask, bool), img, 0.0)

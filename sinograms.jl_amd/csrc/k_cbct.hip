@@ -717,12 +717,18 @@ template <int FMT> void launch_fmt(const CbctArgs& a, hipStream_t st)
 }
 
 __global__ void view_trig_kernel(const double* __restrict__ betas, int64_t n, int64_t stride,
-                                 ViewCS* __restrict__ out)
+                                 int mojette, ViewCS* __restrict__ out)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n) {
         double s, c;
         sincos(betas[i * stride], &s, &c);
+        if (mojette) {
+            // the view's radial spacing over d: max(|cos|, |sin|) (_d_moj)
+            const double f = fmax(fabs(c), fabs(s));
+            c /= f;
+            s /= f;
+        }
         out[i].c = (float)c;
         out[i].s = (float)s;
     }
@@ -731,11 +737,11 @@ __global__ void view_trig_kernel(const double* __restrict__ betas, int64_t n, in
 }  // namespace
 
 void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* out,
-                      hipStream_t st)
+                      hipStream_t st, bool mojette)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(view_trig_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       betas, n, stride, out);
+                       betas, n, stride, mojette ? 1 : 0, out);
     SGX_HIP(hipGetLastError());
 }
 

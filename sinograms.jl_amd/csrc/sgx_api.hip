@@ -933,7 +933,8 @@ void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool tim
         a.ny = (int)P.ny;
         a.mask = P.mask;
         a.image = img;
-        a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+        a.geom = (G.kind == SGX_SINO_PAR || G.kind == SGX_SINO_MOJ) ? 0
+                                                                  : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
         a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
         a.dsd_ds = (float)G.dsd / (float)G.ds;
         a.dso_ds = (float)G.dso / (float)G.ds;
@@ -1046,8 +1047,10 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         require(geom && betas && xc && yc && Hk && out, SGX_ERR_ARG, "null pointer argument");
         *out = nullptr;
         const sgx_geom& G = *geom;
-        require(G.kind >= SGX_SINO_PAR && G.kind <= SGX_CT_FAN_FLAT, SGX_ERR_ARG, "bad kind");
-        const bool is3d = G.kind >= SGX_CT_FAN_ARC;
+        require(G.kind >= SGX_SINO_PAR && G.kind <= SGX_SINO_MOJ, SGX_ERR_ARG, "bad kind");
+        const bool is3d = G.kind == SGX_CT_FAN_ARC || G.kind == SGX_CT_FAN_FLAT;
+        const bool moj = G.kind == SGX_SINO_MOJ;
+        require(!moj || !G.parker, SGX_ERR_ARG, "no Parker weighting for a mojette geometry");
         require(G.ns > 1 && G.na > 0 && nx > 0 && ny > 0, SGX_ERR_ARG, "bad size");
         require(!is3d || (zc && nz > 0 && G.nt > 1), SGX_ERR_ARG, "3-D plan needs zc, nz, nt");
         require(is3d || G.nt == 1, SGX_ERR_ARG, "2-D plan needs nt == 1");
@@ -1115,7 +1118,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             o_wbin = pack.add(w.data(), ns * 4);
         }
         if (G.parker) {
-            require(G.kind != SGX_SINO_PAR, SGX_ERR_ARG,
+            require(G.kind != SGX_SINO_PAR && G.kind != SGX_SINO_MOJ, SGX_ERR_ARG,
                     "parallel-beam Parker weights are passed per view");
             double amin = betas[0];
             for (int64_t i = 1; i < na; ++i) amin = std::min(amin, betas[i]);
@@ -1177,7 +1180,18 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             P->t_axis = reinterpret_cast<float*>(d + o_t);
         }
         sgx::launch_view_trig(reinterpret_cast<const double*>(d + o_b), na, 1, P->cs,
-                              ctx->stream);
+                              ctx->stream, moj);
+        if (moj) {
+            // the filter for the view's radial spacing d_moj = d f is Hk(d) / f
+            // (the ramp scales as 1 / spacing): a per-view weight 1 / f
+            std::vector<float> w(na);
+            for (int64_t i = 0; i < na; ++i)
+                w[i] = (float)(1.0 / std::max(std::fabs(std::cos(betas[i])),
+                                              std::fabs(std::sin(betas[i]))));
+            SGX_HIP(hipMalloc(&P->wview_owned, 4 * na));
+            SGX_HIP(hipMemcpy(P->wview_owned, w.data(), 4 * na, hipMemcpyHostToDevice));
+            P->wview = P->wview_owned;
+        }
         if (G.parker) {
             // parker_weight_fan_short (parker.jl:96-106) in FP64, cast to Float32
             // once per plan; the filter multiplies it into its operand load
@@ -1235,6 +1249,8 @@ int sgx_plan_set_view_weight(sgx_plan* plan, const float* wview)
     return guard(plan->ctx, [&] {
         sgx_plan& P = *plan;
         const int64_t na = P.geom.na;
+        require(P.geom.kind != SGX_SINO_MOJ, SGX_ERR_ARG,
+                "a mojette plan's per-view weight is its radial-spacing factor");
         ++P.gen;
         if (!wview) {
             P.wview = nullptr;
@@ -1604,7 +1620,8 @@ sgx::Back2dArgs back2d_args_of(const sgx_plan& P, const float* q, float* img)
     a.ny = (int)P.ny;
     a.mask = P.mask;
     a.image = img;
-    a.geom = G.kind == SGX_SINO_PAR ? 0 : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+    a.geom = (G.kind == SGX_SINO_PAR || G.kind == SGX_SINO_MOJ) ? 0
+                                                                  : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
     a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
     a.dsd_ds = (float)G.dsd / (float)G.ds;
     a.dso_ds = (float)G.dso / (float)G.ds;

@@ -79,9 +79,10 @@ struct ViewCS {
 
 // --- launch wrappers (defined in the .hip kernel files) -------------------
 
-// Per-view cos/sin from angles (FP64 on the device).
+// Per-view cos/sin from angles (FP64 on the device); with `mojette` both are
+// divided by max(|cos|, |sin|), the mojette view's radial spacing over d.
 void launch_view_trig(const double* betas, int64_t n, int64_t stride, ViewCS* out,
-                      hipStream_t st);
+                      hipStream_t st, bool mojette = false);
 
 // K3: cone-beam voxel-driven backprojection.
 enum { SGX_Q_FMT_F32 = 0, SGX_Q_FMT_BF16 = 1, SGX_Q_FMT_F16S = 2 };

@@ -7,8 +7,8 @@ in libsgx; there is no CPU fallback.
 """
 
 from .geom import (CtFan, CtFanArc, CtFanFlat, CtSourceCircle, CtSourceHelix, ImageGeom,
-                   SinoFan, SinoFanArc, SinoFanFlat, SinoGeom, SinoPar)
-from .recon import (Boxcar, FBPNormalPlan, FDKplan, Hamming, Hann, NoWindow, SizeMismatch,
+                   SinoFan, SinoFanArc, SinoFanFlat, SinoGeom, SinoMoj, SinoPar)
+from .recon import (Boxcar, FBPNormalPlan, FDKplan, MojPlan, Hamming, Hann, NoWindow, SizeMismatch,
                     Window, WindowVect, _angle_weights, _fbp_weights, _fdk_weights, cbct_back,
                     fbp, fbp_back, fbp_filter, fbp_ramp, fbp_sino_filter, fbp_sino_weight,
                     fbp_window, fdk, fdk_weight_cyl, parker_weight, parker_weight_fan_short,
@@ -21,7 +21,7 @@ from .raygeom import footprint_size, rays
 from ._lib import LIB_PATH, SgxError
 
 __all__ = [
-    "SinoGeom", "SinoPar", "SinoFan", "SinoFanArc", "SinoFanFlat", "CtFan", "CtFanArc",
+    "SinoGeom", "SinoPar", "SinoMoj", "MojPlan", "SinoFan", "SinoFanArc", "SinoFanFlat", "CtFan", "CtFanArc",
     "CtFanFlat", "CtSourceCircle", "CtSourceHelix", "ImageGeom", "plan_fbp", "fbp", "fdk", "fbp_back", "cbct_back",
     "fbp_sino_filter", "fbp_filter", "fbp_ramp", "ramp_flat", "ramp_arc", "fbp_window",
     "Window", "NoWindow", "Boxcar", "Hamming", "Hann", "WindowVect", "parker_weight",

@@ -31,6 +31,7 @@ SGX_SINO_FAN_ARC = 1
 SGX_SINO_FAN_FLAT = 2
 SGX_CT_FAN_ARC = 3
 SGX_CT_FAN_FLAT = 4
+SGX_SINO_MOJ = 5
 
 SGX_PREC_FP32 = 0
 SGX_PREC_BF16 = 1

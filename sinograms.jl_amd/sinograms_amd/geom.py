@@ -3,6 +3,7 @@
 Mirrors the reference's geometry records and the derived quantities the hot
 path consumes:
   SinoPar / SinoFanArc / SinoFanFlat   src/geom/type2.jl:95-159, ctors :182-345
+  SinoMoj, _d_moj, _d_ang              src/geom/type2.jl:108-121, common.jl:60, 128-129
   CtFanArc / CtFanFlat                 src/geom/type3.jl:145-184, ctors :306-409
   angles, _ar, _dso, _gamma, _rfov ... src/geom/common.jl:27-99
   _s, _w                               src/geom/sino-geom.jl:18-20
@@ -108,6 +109,30 @@ class SinoPar(SinoGeom):
 
     def rfov(self) -> float:
         return float(np.max(np.abs(self.s())))
+
+
+@dataclasses.dataclass(frozen=True)
+class SinoMoj(SinoGeom):
+    """2-D mojette sinogram geometry (type2.jl:108-121, ctor :199-229): ``d``
+    is the (square) pixel size, and view phi samples the radial axis at
+    ``d_moj(phi) = d max(|cos phi|, |sin phi|)`` (``_d_moj``, common.jl:128)."""
+
+    def __post_init__(self):
+        if self.na <= 0:
+            object.__setattr__(self, "na", 2 * int(math.floor(self.nb * math.pi / 2 / 2)))
+
+    def d_moj(self, ar) -> np.ndarray:
+        """``_d_moj(rg)(ar)`` (common.jl:128)."""
+        ar = np.asarray(ar, np.float64)
+        return self.d * np.maximum(np.abs(np.cos(ar)), np.abs(np.sin(ar)))
+
+    def d_ang(self) -> np.ndarray:
+        """``_d_ang(rg)``: the radial spacing of every view (common.jl:129)."""
+        return self.d_moj(self.ar())
+
+    def rfov(self) -> float:
+        """``_rfov(rg::SinoMoj)`` = nb / 2 min(d_ang) (common.jl:60, ignores offset)."""
+        return self.nb / 2 * float(np.min(self.d_ang()))
 
 
 @dataclasses.dataclass(frozen=True)

@@ -19,7 +19,7 @@ import math
 
 import numpy as np
 
-from .geom import CtFan, SinoFan, SinoPar
+from .geom import CtFan, SinoFan, SinoMoj, SinoPar
 
 
 def _torch():
@@ -100,10 +100,11 @@ def _ellipse_rays(ell, r, phi):
 
 
 def sino_ellipses(ell, rg, device="cpu"):
-    """Exact sinogram (nb, na) of ellipses for SinoPar or SinoFan (source_offset 0)."""
+    """Exact sinogram (nb, na) of ellipses for SinoPar, SinoMoj or SinoFan
+    (source_offset 0)."""
     torch = _torch()
     dev = _gpu(device)
-    if dev is not None:
+    if dev is not None and not isinstance(rg, SinoMoj):
         from . import _lib
         if isinstance(rg, SinoFan) and rg.source_offset != 0:
             raise ValueError("sino_ellipses: source_offset must be 0")
@@ -127,6 +128,9 @@ def _sino_ellipses_torch(ell, rg, device):
     beta = torch.as_tensor(rg.ar(), device=device)[None, :]
     if isinstance(rg, SinoPar):
         return _ellipse_rays(ell, s, beta)
+    if isinstance(rg, SinoMoj):
+        # the mojette rays (sino-geom.jl:107-115): r = s max(|sin|, |cos|)
+        return _ellipse_rays(ell, s * torch.maximum(beta.cos().abs(), beta.sin().abs()), beta)
     if isinstance(rg, SinoFan):
         gam = s / rg.dsd if rg.is_arc else torch.atan(s / rg.dsd)
         return _ellipse_rays(ell, rg.dso * torch.sin(gam), gam + beta)

@@ -3,6 +3,7 @@
 Mirrors the reference's ray helpers (host side, float64, vectorised):
 
   rays(SinoPar)    (r, phi) = (s_i, beta_j)                  src/geom/sino-geom.jl:99-104
+  rays(SinoMoj)    (r, phi) = (s_i max|sincos beta_j|, beta_j) src/geom/sino-geom.jl:107-115
   rays(SinoFan)    (dso sin(gamma), gamma + beta)            src/geom/sino-geom.jl:118-132
   rays(CtFan)      (u, v, phi, theta) = cb_*_to_par(s, t, beta, dso, dod)
                                                              src/geom/ct-geom.jl:98-110
@@ -33,7 +34,7 @@ import math
 
 import numpy as np
 
-from .geom import CtFan, CtSourceCircle, ImageGeom, SinoFan, SinoGeom, SinoPar
+from .geom import CtFan, CtSourceCircle, ImageGeom, SinoFan, SinoGeom, SinoMoj, SinoPar
 
 
 def cb_flat_to_par(s, t, beta, dso: float, dod: float):
@@ -72,6 +73,10 @@ def rays(rg):
     if isinstance(rg, SinoPar):
         s, b = np.meshgrid(rg.s(), rg.ar(), indexing="ij")
         return s, b
+    if isinstance(rg, SinoMoj):
+        # _moj_to_par (sino-geom.jl:107-115): r = s max(|sin|, |cos|)
+        s, b = np.meshgrid(rg.s(), rg.ar(), indexing="ij")
+        return s * np.maximum(np.abs(np.sin(b)), np.abs(np.cos(b))), b
     if isinstance(rg, SinoFan):
         s, b = np.meshgrid(rg.s(), rg.ar(), indexing="ij")
         g = rg.gamma(s)
@@ -99,6 +104,9 @@ def tau(rg, x=None, y=None):
     x, y = x[:, None], y[:, None]
     if isinstance(rg, SinoPar):
         return (x * np.cos(b) + y * np.sin(b)) / rg.d
+    if isinstance(rg, SinoMoj):
+        # tau.jl:22-25: divided by the view's radial spacing
+        return (x * np.cos(b) + y * np.sin(b)) / rg.d_moj(b)
     if isinstance(rg, (SinoFan, CtFan)):
         xb = x * np.cos(b) + y * np.sin(b)
         yb = -x * np.sin(b) + y * np.cos(b)

@@ -33,7 +33,7 @@ import warnings
 import numpy as np
 
 from . import _lib
-from .geom import (CtFan, ImageGeom, SinoFan, SinoGeom, SinoPar, is_full_orbit)
+from .geom import (CtFan, ImageGeom, SinoFan, SinoGeom, SinoMoj, SinoPar, is_full_orbit)
 
 
 class SizeMismatch(ValueError):
@@ -141,6 +141,8 @@ def fbp_ramp(rg, N: int):
     """``fbp_ramp`` (ramp.jl:171-182)."""
     if isinstance(rg, SinoPar):
         return ramp_flat(N, rg.d)
+    if isinstance(rg, SinoMoj):
+        return ramp_flat(N, rg.d)           # dr = rg.d (ramp.jl:26-29)
     if isinstance(rg, (SinoFan, CtFan)):
         ds = rg.d if isinstance(rg, SinoFan) else rg.ds
         return ramp_arc(N, ds, rg.dsd) if rg.is_arc else ramp_flat(N, ds)
@@ -294,6 +296,11 @@ def parker_weight(rg, device: int = 0) -> np.ndarray:
     """``parker_weight`` (parker.jl:58-59, 111-119, 147-154)."""
     if isinstance(rg, SinoPar):
         return parker_weight_par(rg.orbit, rg.angles() - rg.orbit_start)
+    if isinstance(rg, SinoMoj):
+        # parker.jl:122-129: only whole multiples of 180 degrees
+        if math.trunc(rg.orbit / 180) * 180 != abs(rg.orbit):
+            raise ValueError(f"No Parker weighting for Mojette geometry with orbit={rg.orbit}")
+        return np.ones((1, 1), dtype=np.float32)
     if isinstance(rg, SinoFan):
         if is_full_orbit(rg.orbit, 360):
             return np.ones((1, 1), dtype=np.float32)
@@ -453,6 +460,8 @@ class _Plan:
         G = _lib.SgxGeom()
         if isinstance(rg, SinoPar):
             G.kind = _lib.SGX_SINO_PAR
+        elif isinstance(rg, SinoMoj):
+            G.kind = _lib.SGX_SINO_MOJ
         elif isinstance(rg, SinoFan):
             G.kind = _lib.SGX_SINO_FAN_ARC if rg.is_arc else _lib.SGX_SINO_FAN_FLAT
         elif isinstance(rg, CtFan):
@@ -588,6 +597,19 @@ class FBPNormalPlan(_Plan):
                 f"V {np.shape(self.view_weight)})")
 
 
+class MojPlan(_Plan):
+    """The ``:mojette`` FBP plan of a ``SinoMoj`` geometry (the reference's
+    commented ``MojPlan``, plan2.jl:56-62, 125-135, and ``fpb2(plan::MojPlan)``,
+    fbp.jl:138-145): every view is ramp-filtered for its own radial spacing
+    d_moj = d max(|cos|, |sin|) (the response of spacing d divided by
+    max(|cos|, |sin|), fbp.jl:150-160) and backprojected with that spacing,
+    times the angular weight dphi."""
+
+    def __repr__(self):
+        return (f"MojPlan(SinoMoj {self.rg.dims()}, ig {self.ig.dims}, "
+                f"H extrema {self.filter.min():.4g} {self.filter.max():.4g})")
+
+
 class FDKplan(_Plan):
     """``FDKplan`` (plan3.jl:19-29)."""
 
@@ -607,11 +629,23 @@ def _fdk_weights(rg, device=0):
             * _angle_weights(rg.ar(f32=True)))
 
 
+# plan_fbp's `how` (plan2.jl:93-96, 119-133): the reference builds only
+# :normal; :mojette, :dsc, :df and :table are names in a commented dispatch
+# onto MIRT objects (Gtomo2_table, Gtomo2_dscex, a NUFFT) that it does not
+# contain.  :mojette is built here from the mojette geometry it does define.
+_HOW_UNDEFINED = ("dsc", "df", "table")
+
+
 def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1: bool = True,
              filter=None, precision: str = "fp32", device: int = 0,
              use_source_zs: bool = False, arc_t_exact: bool = False,
-             use_source_offset: bool = False, **_ignored):
+             use_source_offset: bool = False, how: str = "normal", **_ignored):
     """``plan_fbp`` (plan2.jl:106-138 for SinoGeom, plan3.jl:60-72 for CtGeom).
+
+    how: "normal" (the reference's FBP), or "mojette" for a ``SinoMoj``
+    geometry (also its default: the reference has no other FBP for it), the
+    per-view-spacing FBP of ``MojPlan``.  "dsc", "df" and "table" raise: the
+    reference names them without an algorithm.
 
     precision: "fp32" (FP32 filter and storage), or for cone-beam plans the
     16-bit storage forms "bf16" (split-BF16x3 MFMA filter, BF16 filtered
@@ -623,10 +657,25 @@ def plan_fbp(rg, ig: ImageGeom, window: Window = None, npad: int = None, decon1:
     backprojection options of ``cbct_back``, which the reference keeps
     commented out; off by default.
     """
+    how = str(how).lstrip(":")
+    if how in _HOW_UNDEFINED:
+        raise NotImplementedError(f"how=:{how}: the reference defines no algorithm for it "
+                                  "(plan2.jl:125-135 keep it commented out)")
+    if how not in ("normal", "mojette"):
+        raise ValueError(f"unknown type: {how}")                # plan2.jl:134
+    if how == "mojette" and not isinstance(rg, SinoMoj):
+        raise ValueError("how=:mojette needs a SinoMoj geometry (the reference's rebinning of "
+                         "other geometries onto mojette sampling is not defined)")
     nb = rg.nb if isinstance(rg, SinoGeom) else rg.ns
     npad = npad or _nextpow2(nb + 1)
     if filter is None:
         filter = fbp_filter(rg, npad=npad, window=window or Window(), decon1=decon1)
+    if isinstance(rg, SinoMoj):
+        parker_weight(rg)                                       # orbit check (parker.jl:122-129)
+        if use_source_zs or arc_t_exact or use_source_offset:
+            raise ValueError("use_source_zs, arc_t_exact and use_source_offset apply to "
+                             "cone-beam geometries")
+        return MojPlan(rg, ig, filter, np.ones((1, 1), np.float32), precision, device)
     if isinstance(rg, CtFan):
         vw = np.float32(_angle_weights(rg.ar(f32=True)))   # the full array is on the device
         plan = FDKplan(rg, ig, filter, vw, precision, device)

@@ -1183,3 +1183,30 @@ def test_fdk_plan_source_offset_vs_oracle(sg, orc, torch_cuda, arc):
                   yc, zc, ig.mask, dtype=np.float32, source_offset=rg.source_offset)
     assert rel_rmse(got, ref) < TOL32
     assert rel_rmse(sg.fdk(sg.plan_fbp(rg, ig), proj), got) > 1e-3
+
+
+@pytest.mark.parametrize("window", [None, "hann"])
+def test_mojette_plan_vs_oracle(sg, orc, torch_cuda, window):
+    # plan_fbp(rg::SinoMoj; how=:mojette) + fbp against the oracle's
+    # restatement, which builds each view's ramp for its own spacing
+    _say("mojette plan", window)
+    d, nb, na = 1.0, 128, 200
+    rg = sg.SinoMoj(nb=nb, d=d, na=na, offset=0.25)
+    ig = sg.ImageGeom((96, 90), (d, d))
+    ig = ig.with_mask(ig.circle(rg.rfov()))
+    ell = sg.phantom.random_ellipses(9, 6, 60.0)
+    sino = sg.phantom.sino_ellipses(ell, rg).numpy().astype(np.float32)
+    win = sg.Window(sg.Hann()) if window else None
+    plan = sg.plan_fbp(rg, ig, how="mojette", window=win)
+    assert isinstance(plan, sg.MojPlan)
+    got = sg.fbp(plan, sino)
+    xc, yc = ig.axes()
+    w = orc.fbp_window("hann", 256) if window else None
+    ref = orc.fbp_moj(sino.astype(np.float64), rg.ar(f32=True), d, rg.offset, xc, yc, ig.mask,
+                      npad=256, window=w)
+    e = rel_rmse(got, ref)
+    _say(f"  rel-RMSE vs the Float64 oracle {e:.3g}")
+    assert e < TOL32
+    assert np.all(got[~ig.mask] == 0)
+    # the default plan of a mojette geometry is the same
+    assert np.array_equal(sg.fbp(sg.plan_fbp(rg, ig, window=win), sino), got)

@@ -501,3 +501,59 @@ def test_cbct_source_offset_midplane_is_the_fan_backprojector(orc, arc):
     # and the offset matters
     plain = orc.cbct_back(proj, betas, dsd, dso, ds, 1.0, 0.0, 0.0, arc, xc, yc, zc, m3)[:, :, 0]
     assert np.max(np.abs(plain - cone)) > 1e-3 * np.max(np.abs(fan))
+
+
+def test_mojette_geometry_and_rays(sg):
+    # _d_moj (common.jl:128), _rfov (common.jl:60), and rays / tau agree:
+    # a point on mojette ray (ib, ia) projects to bin s[ib] / d (tau.jl:22-25)
+    from sinograms_amd import raygeom
+    rg = sg.SinoMoj(nb=40, d=0.7, offset=0.25, na=36, orbit=180)
+    ar = rg.ar()
+    assert np.allclose(rg.d_moj([0.0, np.pi / 4, np.pi / 2]), [0.7, 0.7 / np.sqrt(2), 0.7])
+    assert rg.rfov() == pytest.approx(20 * 0.7 / np.sqrt(2), rel=1e-3)
+    r, phi = raygeom.rays(rg)
+    rng = np.random.default_rng(3)
+    ib, ia = rng.integers(0, 40, 20), rng.integers(0, 36, 20)
+    tt = rng.uniform(-5, 5, 20)
+    c, s = np.cos(phi[ib, ia]), np.sin(phi[ib, ia])
+    x = r[ib, ia] * c - tt * s
+    y = r[ib, ia] * s + tt * c
+    tau = raygeom.tau(rg, x, y)[np.arange(20), ia]
+    assert np.allclose(tau, rg.s()[ib] / rg.d, atol=1e-9)
+    # Parker: whole multiples of 180 degrees only (parker.jl:122-129)
+    assert sg.parker_weight(rg).shape == (1, 1)
+    with pytest.raises(ValueError):
+        sg.parker_weight(sg.SinoMoj(nb=40, na=36, orbit=200))
+    # plan_fbp's undefined variants raise before any device work
+    ig = sg.ImageGeom((16, 16), (1.0, 1.0))
+    for how in ("dsc", ":table", "df"):
+        with pytest.raises(NotImplementedError):
+            sg.plan_fbp(rg, ig, how=how)
+    with pytest.raises(ValueError):
+        sg.plan_fbp(rg, ig, how="bogus")
+    with pytest.raises(ValueError):
+        sg.plan_fbp(sg.SinoPar(nb=40, na=36), ig, how="mojette")
+
+
+def test_fbp_moj_disk_area(sg, orc):
+    # The :mojette FBP restatement reconstructs an analytic disk projected
+    # along the mojette rays: its area within the reference's parallel-beam
+    # FBP tolerance scale (test/fbp2/fbp-par.jl:27-33), and the image matches
+    # the plain parallel-beam FBP of the same disk.
+    d, nb, na = 1.0, 128, 200
+    rg = sg.SinoMoj(nb=nb, d=d, na=na)
+    ig = sg.ImageGeom((96, 96), (d, d))
+    ig = ig.with_mask(ig.circle(rg.rfov()))      # the mojette field of view (common.jl:60)
+    disk = np.array([[4.0, -3.0, 20.0, 20.0, 0.0, 1.0]])
+    sino = sg.phantom.sino_ellipses(disk, rg).numpy()
+    xc, yc = ig.axes()
+    img = orc.fbp_moj(sino, rg.ar(), d, 0.0, xc, yc, ig.mask, npad=1 << (nb).bit_length())
+    area = img.sum() * d * d
+    assert abs(area - np.pi * 400) / (np.pi * 400) < 2e-3
+    par = sg.SinoPar(nb=nb, d=d, na=na)
+    sp = sg.phantom.sino_ellipses(disk, par).numpy()
+    Hk = orc.fbp_filter(orc.ramp_flat(256, d)[0], 256, d)
+    ip = orc.fbp_par(sp, Hk, orc.angle_weight(par.ar(f32=True).astype(np.float32)),
+                     par.ar(f32=True), d, 0.0, xc, yc, ig.mask)
+    # the two samplings differ at the disk edge only (Gibbs ringing of each)
+    assert np.sqrt(np.mean((img - ip) ** 2)) / np.sqrt(np.mean(ip ** 2)) < 0.12
