@@ -34,13 +34,14 @@
 //   t = tan(theta - theta0) = (r d0 - r0 d) / (d d0 + r r0)
 //     = (dx A + dy B) / (R2 + E),  E = dx B - dy A,
 //   A = c d0 - s r0,  B = s d0 + c r0,  R2 = d0^2 + r0^2,
-//   d^2 + r^2 = R2 + 2 E + dx^2 + dy^2,
 // so bb = I0 + (f0 + dsd atan(t)), with |t| small (a tile seen from the
 // source) and atan(t) its odd Taylor series of NT terms (NT chosen on the host
 // from a bound on |t|).  The fractional coordinate f0 + dsd atan(t) is a few
-// bins, so FP32 keeps it to about 1e-6 of a bin; w2 = dsd^2 / (d^2 + r^2).
-// About 22 VALU instructions per update against about 36 for the per-pixel
-// atan2 form; the per-pixel form stays for geometries where |t| is not small,
+// bins, so FP32 keeps it to about 1e-6 of a bin.  The weight
+// w2 = dsd^2 / (d^2 + r^2) comes without a second reciprocal: by Lagrange's
+// identity num^2 + den^2 = (d^2 + r^2) R2, so w2 = dsd^2 R2 / den^2 / (1 + t^2),
+// with 1 / (1 + t^2) its series of NT terms.  About 21 VALU instructions per
+// update against about 36 for the per-pixel atan2 form; the per-pixel form stays for geometries where |t| is not small,
 // and for sgx_plan_back / sgx_plan_back_t, whose pair is an exact transpose.
 
 #include "sgx_internal.h"
@@ -90,14 +91,15 @@ int window_bins(int nb)
     return w;
 }
 
-// Per-(tile, view) constants of the tile-relative arc form (see above):
-// A, B, R2, f0; the in-detector range of the fractional coordinate
-// [blo, bhi) = [1 - I0, nb - I0); base = the LDS index of tap il - 1 when
-// floor(f0 + dsd atan(t)) = 0; I0.
+// Per-(tile, view) constants of the tile-relative arc form (see above), 32
+// bytes: A, B, R2, f0; KR = dsd^2 R2; the in-detector range of the fractional
+// coordinate [blo, bhi) = [1 - I0, nb - I0) (so I0 = 1 - blo); base = the LDS
+// byte offset of tap il - 1 when floor(f0 + dsd atan(t)) = 0.
 struct RelView {
-    float A, B, R2, f0, blo, bhi;
-    int base, I0;
+    float A, B, R2, f0, KR, blo, bhi;
+    int base;
 };
+static_assert(sizeof(RelView) == 32, "RelView");
 
 // Views of one LDS chunk: at most vc_max and what fits in kLdsBytes with the
 // per-view tables; a multiple of kU once it reaches kU.
@@ -118,15 +120,31 @@ template <int NT> struct RelPoly {
         for (int k = 0; k < NT; ++k) c[k] = dsd_ds / (float)(2 * k + 1) * ((k & 1) ? -1.f : 1.f);
     }
     // f0 + dsd atan(t) (Horner in t^2)
-    __device__ __forceinline__ float bin(float t, float f0) const
+    __device__ __forceinline__ float bin(float t, float t2, float f0) const
     {
-        const float t2 = t * t;
         float p = c[NT - 1];
 #pragma unroll
         for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, c[k]);
         return fmaf(t, p, f0);
     }
+    // 1 / (1 + t^2) = 1 - t^2 + t^4 - ..., NT terms
+    __device__ __forceinline__ static float inv1p(float t2)
+    {
+        float p = (NT & 1) ? 1.f : -1.f;
+#pragma unroll
+        for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, (k & 1) ? -1.f : 1.f);
+        return p;
+    }
 };
+
+// floor(x) as an integer in one instruction (v_cvt_flr_i32_f32) for the
+// small, finite coordinates of the tile-relative form
+__device__ __forceinline__ int cvt_floor(float x)
+{
+    int r;
+    asm volatile("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 
 // GEOM: 0 parallel, 1 fan arc, 2 fan flat, 3 fan arc whose detector spans
 // less than +-45 degrees (atan2_narrow), 4 fan arc in the tile-relative form
@@ -177,8 +195,6 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     // GEOM 4: the pixel relative to the tile centre
     const float xt = 0.5f * (bx0 + bx1), yt = 0.5f * (by0 + by1);
     const float dx = x - xt, dy = y - yt;
-    const float p2 = dx * dx + dy * dy;
-    const float dsd2 = dsd_ds * dsd_ds;
     const RelPoly<NT> poly(dsd_ds);
     float acc = 0.f;
 
@@ -221,11 +237,11 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                     r.A = (float)(c * d0 - sn * r0);
                     r.B = (float)(sn * d0 + c * r0);
                     r.R2 = (float)(d0 * d0 + r0 * r0);
+                    r.KR = (float)((double)dsd_ds * (double)dsd_ds * (d0 * d0 + r0 * r0));
                     r.f0 = (float)(K - I0);
                     r.blo = (float)(1.0 - I0);
                     r.bhi = (float)((double)nb - I0);
-                    r.I0 = (int)I0;
-                    r.base = (tid << wshift) + (r.I0 - 1 - lo);
+                    r.base = 4 * ((tid << wshift) + ((int)I0 - 1 - lo));
                     // every pixel's bb is inside [bmin, bmax] up to FP32 rounding
                     // of the corners; one bin of margin on each side
                     interior = ok && n > 0 && bmin >= 2.f && bmax < nb1 - 1.f &&
@@ -234,17 +250,22 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                     // padding view: zero data (n = 0), zero weight
                     r.A = r.B = 0.f;
                     r.R2 = 1e30f;
+                    r.KR = 0.f;
                     r.f0 = 0.5f;
-                    r.blo = r.bhi = 0.f;
-                    r.I0 = 1;
-                    r.base = tid << wshift;
+                    r.blo = r.bhi = 0.f;      // I0 = 1
+                    r.base = 4 * (tid << wshift);
                 }
                 rel[tid] = r;
             }
         }
-        // GEOM 4: the chunk is interior when all its views are
-        const bool chunk_interior =
-            __builtin_amdgcn_readfirstlane(__syncthreads_and(interior ? 1 : 0)) != 0;
+        // GEOM 4: the chunk is interior when all its views are (a reduction
+        // barrier only where it is used)
+        bool chunk_interior = false;
+        if constexpr (REL)
+            chunk_interior =
+                __builtin_amdgcn_readfirstlane(__syncthreads_and(interior ? 1 : 0)) != 0;
+        else
+            __syncthreads();
         const int total = nvp << wshift;
         for (int i0 = tid; i0 < total; i0 += kThreads * kStage) {
             float t[kStage];
@@ -275,13 +296,16 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                     const RelView r = rel[v + u];
                     const float E = fmaf(dx, r.B, -dy * r.A);
                     const float num = fmaf(dx, r.A, dy * r.B);
-                    const float tt = num * __builtin_amdgcn_rcpf(r.R2 + E);
-                    const float bbf = poly.bin(tt, r.f0);
+                    const float rt = __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float tt = num * rt;
+                    const float t2 = tt * tt;
+                    const float bbf = poly.bin(tt, t2, r.f0);
                     wr[u] = __builtin_amdgcn_fractf(bbf);
-                    const float* p = lds + (r.base + (int)floorf(bbf));
+                    const float* p = reinterpret_cast<const float*>(
+                        reinterpret_cast<const char*>(lds) + (r.base + 4 * cvt_floor(bbf)));
                     a[u] = p[0];
                     b[u] = p[1];
-                    w[u] = dsd2 * __builtin_amdgcn_rcpf(fmaf(2.f, E, r.R2 + p2));
+                    w[u] = r.KR * rt * rt * RelPoly<NT>::inv1p(t2);
                 }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
@@ -298,15 +322,17 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
                     const RelView r = rel[v + u];
                     const float E = fmaf(dx, r.B, -dy * r.A);
                     const float num = fmaf(dx, r.A, dy * r.B);
-                    const float tt = num * __builtin_amdgcn_rcpf(r.R2 + E);
-                    const float bbf = poly.bin(tt, r.f0);
+                    const float rt = __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float tt = num * rt;
+                    const float t2 = tt * tt;
+                    const float bbf = poly.bin(tt, t2, r.f0);
                     const bool ok = bbf >= r.blo && bbf < r.bhi;   // 1 <= floor(bb) < nb
-                    w[u] = ok ? dsd2 * __builtin_amdgcn_rcpf(fmaf(2.f, E, r.R2 + p2)) : 0.f;
+                    w[u] = ok ? r.KR * rt * rt * RelPoly<NT>::inv1p(t2) : 0.f;
                     const float bbc = ok ? bbf : r.blo;
                     const int fi = (int)floorf(bbc);
                     wr[u] = __builtin_amdgcn_fractf(bbc);
-                    il[u] = r.I0 + fi;                            // taps il - 1 and il (0-based)
-                    const int local = r.base - ((v + u) << wshift) + fi;
+                    il[u] = (int)(1.f - r.blo) + fi;              // taps il - 1 and il (0-based)
+                    const int local = (r.base >> 2) - ((v + u) << wshift) + fi;
                     const int2 wv = win[v + u];
                     const bool in = local >= 0 && local < wv.y;
                     miss |= (uint32_t)(!in) << u;
@@ -465,9 +491,12 @@ int back2d_rel_terms(const Back2dArgs& a)
     const double z = (double)a.rel_h / rho;
     if (!(z <= 0.25)) return 0;
     const double T = z / std::sqrt(1.0 - z * z);
-    // the first omitted term of dsd atan(t): below 1e-6 of a bin
+    // the first omitted terms: of dsd atan(t) below 1e-6 of a bin, of
+    // 1 / (1 + t^2) below 1e-6 relative
     for (int nt : {3, 4, 5, 6, 8})
-        if ((double)a.dsd_ds * std::pow(T, 2 * nt + 1) / (2 * nt + 1) <= 1e-6) return nt;
+        if ((double)a.dsd_ds * std::pow(T, 2 * nt + 1) / (2 * nt + 1) <= 1e-6 &&
+            std::pow(T, 2 * nt) <= 1e-6)
+            return nt;
     return 0;
 }
 

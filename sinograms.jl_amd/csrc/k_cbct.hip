@@ -123,6 +123,12 @@ __device__ __forceinline__ float4 taps(__amdgpu_buffer_rsrc_t r, uint32_t voff, 
         const auto b = __builtin_amdgcn_raw_buffer_load_b64(r, voff, rowb, 0);
         lo = make_float2(__uint_as_float(a[0]), __uint_as_float(a[1]));
         hi = make_float2(__uint_as_float(b[0]), __uint_as_float(b[1]));
+    } else if (FMT == F_F16S) {
+        // the row difference rounded in FP16, as the staged LDS entries hold
+        // it: a voxel gets the same value whichever path serves its view
+        const uint32_t a = pair16(r, voff, 0), d = half2_sub(pair16(r, voff, rowb), a);
+        const float2 av = dec2<FMT>(a), dv = dec2<FMT>(d);
+        return make_float4(av.x, dv.x, av.y, dv.y);
     } else {
         lo = dec2<FMT>(pair16(r, voff, 0));
         hi = dec2<FMT>(pair16(r, voff, rowb));
@@ -309,6 +315,9 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #endif
 #ifndef SGX_STAGED_PITCH
 #define SGX_STAGED_PITCH 58
+#endif
+#ifndef SGX_STAGED_SCHED
+#define SGX_STAGED_SCHED 1
 #endif
 constexpr int kSP = 16;               // block patch: 16 x 16 columns (2 x 2 waves of 8 x 8)
 constexpr int kSZ = 16;               // voxels per lane along z
@@ -612,6 +621,9 @@ cbct_staged_kernel(
                             tp[j] = make_float4(e0.x, e0.y, e1.x, e1.y);
                         }
                     }
+                    // every LDS read of the batch is issued before its
+                    // arithmetic (the scheduler otherwise keeps two in flight)
+                    if (SGX_STAGED_SCHED) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                     for (int j = 0; j < kSB; ++j) {
                         const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
