@@ -85,14 +85,19 @@ def source_hash(tag):
 def counters(kernel):
     """The committed per-launch counters of the first kernel whose name
     starts with `kernel` ({} when no summary lists one), with "current" True
-    when the summary was taken from the kernel sources in this tree."""
+    when the summary was taken from the kernel sources in this tree.  A name
+    may carry a workload tag after "@" (two workloads launching one grid of
+    one kernel, told apart in the summary); `kernel` "k@tag" matches the
+    tagged line only."""
+    kernel, _, tag = kernel.partition("@")
     for name in COUNTER_FILES:
         try:
             with open(os.path.join(ROOT, name)) as f:
                 for line in f:
                     parts = line.split()
                     if len(parts) >= 4 and parts[0] == "counters" and \
-                            parts[1].startswith(kernel):
+                            parts[1].startswith(kernel) and \
+                            (not tag or parts[1].endswith("@" + tag)):
                         r = {k: float(v) for k, v in (kv.split("=") for kv in parts[4:])}
                         r["source"] = name
                         r["current"] = parts[3] == source_hash(parts[2])
@@ -402,7 +407,7 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
         r = {"workload": desc, "updates_per_step": updates, "ms_per_step": t * 1e3,
              "value": updates / t / 1e9, "unit": "Gupdates/s", "projections_per_s": rg.na / t,
              "back_kernel_ms": bm, "filter_ms": fm,
-             "roofline": roofline(back2d_kernel_name(rg), bm,
+             "roofline": roofline(back2d_kernel_name(rg) + SCAN_TAG.get(name, ""), bm,
                                   4 * rg.nb * rg.na + 4 * int(np.prod(ig.dims)), updates,
                                   "update", 8.0)}
         r.update(filter_figures(plan, fm, rg.nb, rg.na, 4))
@@ -411,6 +416,10 @@ def slice_legs(sg, device, steps, warmup, with_cpu):
             r.update(_slice_cpu(sg, rg, ig, sino, plan, y.cpu().numpy()))
         out[name] = r
     return out
+
+
+# the C1 scans launch one grid of one kernel; the counter summary tags them
+SCAN_TAG = {"C1_fan_arc_full": "@full", "C1_fan_arc_short": "@short"}
 
 
 def back2d_kernel_name(rg):
