@@ -184,8 +184,12 @@ int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t n
  * Replaces src/fbp/filter.jl:122-147.  in is (nb, ncols), ncols = product of
  * the trailing dimensions; out is (nb + extra, ncols) with the extra rows 0.
  * Computed as the Toeplitz contraction out[n,c] = sum_m g[(n-m) mod npad] in[m,c]
- * with g = real(ifft(Hk)), on MFMA.  prec selects SGX_PREC_FP32 (FP32 output)
- * or SGX_PREC_BF16 (split-BF16x3 products, FP32 accumulation, FP32 output).
+ * with g = real(ifft(Hk)), on MFMA, or for SGX_PREC_FP32 at npad 512, 1024
+ * or 2048 as the zero-padded FFT product itself.  prec selects SGX_PREC_FP32
+ * (FP32 output) or SGX_PREC_BF16 (split-BF16x3 products, FP32 accumulation,
+ * FP32 output).  The context keeps the device form of the last 8 filters
+ * (by the contents of Hk, nb and prec): a repeated call with the same Hk does
+ * no host transform and no upload.
  * Returns SGX_ERR_VALUE for the reference's error("nb + extra > npad"), and
  * SGX_ERR_UNSUPPORTED for nb > 8192 (the reference has no such limit; the
  * band of g lives in one workgroup's LDS).  Plans share this limit. */

@@ -82,6 +82,22 @@ struct sgx_ctx {
     };
     Buf ws[12];
 
+    // Device filter kernels of sgx_sino_filter (g, then the FFT tables), kept
+    // per (Hk contents, nb, precision) so that repeated calls with one filter
+    // skip the O(npad^2) host transform and the upload.  Least recently used
+    // entries are dropped beyond kFilterCache.
+    struct FilterEntry {
+        std::vector<float> hk;
+        int64_t nb = 0;
+        int prec = 0;
+        void* dev = nullptr;
+        int64_t fft_n = 0;
+        uint64_t last = 0;
+    };
+    static constexpr size_t kFilterCache = 8;
+    std::vector<FilterEntry> filters;
+    uint64_t filter_clock = 0;
+
     // Grow-only device workspace, one slot per role.  Reuse is ordered by the
     // context's stream; a grow waits for the stream before freeing.
     void* get(int slot, size_t bytes)
@@ -104,6 +120,8 @@ struct sgx_ctx {
     {
         for (Buf& b : ws)
             if (b.p) (void)hipFree(b.p);
+        for (FilterEntry& e : filters)
+            if (e.dev) (void)hipFree(e.dev);
         if (own_stream && stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -116,7 +134,6 @@ enum Slot {
     W_IN,           // staged input
     W_OUT,          // staged output
     W_MASK,         // staged mask
-    W_G,            // filter kernel g
     W_WIN,          // cone-beam staged windows
     W_EXTRA,
 };
@@ -207,6 +224,56 @@ std::vector<double> filter_kernel_g(const float* Hk, int64_t npad)
         g[k] = acc / (double)npad;
     }
     return g;
+}
+
+// The context's device filter kernel for (Hk, nb, prec): g = real(ifft(Hk))
+// in FP32, then (FP32 at an FFT size) the tables of the FFT form.  Built and
+// uploaded on a miss; a hit is a compare of npad floats.
+const sgx_ctx::FilterEntry& filter_entry(sgx_ctx* ctx, const float* Hk, int64_t npad,
+                                         int64_t nb, int prec)
+{
+    auto& fs = ctx->filters;
+    const uint64_t now = ++ctx->filter_clock;
+    for (auto& e : fs) {
+        if (e.nb == nb && e.prec == prec && (int64_t)e.hk.size() == npad &&
+            std::memcmp(e.hk.data(), Hk, sizeof(float) * npad) == 0) {
+            e.last = now;
+            return e;
+        }
+    }
+    const auto gd = filter_kernel_g(Hk, npad);
+    std::vector<float> gf(gd.begin(), gd.end());
+    const auto ft = prec == SGX_PREC_FP32
+                        ? sgx::fft_filter_tables(Hk, npad, nb, nullptr, nullptr, 0, 0.0)
+                        : std::vector<float>();
+    gf.insert(gf.end(), ft.begin(), ft.end());
+    if (fs.size() >= sgx_ctx::kFilterCache) {
+        // drop the least recently used entry (kernels queued on the stream
+        // may still read it)
+        auto old = std::min_element(fs.begin(), fs.end(), [](const auto& a, const auto& b) {
+            return a.last < b.last;
+        });
+        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        SGX_HIP(hipFree(old->dev));
+        fs.erase(old);
+    }
+    sgx_ctx::FilterEntry e;
+    e.hk.assign(Hk, Hk + npad);
+    e.nb = nb;
+    e.prec = prec;
+    e.fft_n = (int64_t)(ft.size() / 5);
+    e.last = now;
+    SGX_HIP(hipMalloc(&e.dev, 4 * gf.size()));
+    try {
+        SGX_HIP(hipMemcpyAsync(e.dev, gf.data(), 4 * gf.size(), hipMemcpyHostToDevice,
+                               ctx->stream));
+        SGX_HIP(hipStreamSynchronize(ctx->stream));   // gf is a local
+    } catch (...) {
+        (void)hipFree(e.dev);
+        throw;
+    }
+    fs.push_back(std::move(e));
+    return fs.back();
 }
 
 void back2d_common(sgx_ctx* ctx, int geom, const float* sino, int64_t nb, int64_t na,
@@ -652,17 +719,8 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
             os << "nb=" << nb << " + extra=" << extra << " > npad=" << npad;
             throw Error(SGX_ERR_VALUE, os.str());
         }
-        const auto gd = filter_kernel_g(Hk, npad);
-        std::vector<float> gf(gd.begin(), gd.end());
-        // FP32 at npad = 1024: the FFT form (its tables after g)
-        const auto ft = prec == SGX_PREC_FP32
-                            ? sgx::fft_filter_tables(Hk, npad, nb, nullptr, nullptr, 0, 0.0)
-                            : std::vector<float>();
-        gf.insert(gf.end(), ft.begin(), ft.end());
-        float* g = static_cast<float*>(ctx->get(W_G, 4 * gf.size()));
-        SGX_HIP(hipMemcpyAsync(g, gf.data(), 4 * gf.size(), hipMemcpyHostToDevice,
-                               ctx->stream));
-        SGX_HIP(hipStreamSynchronize(ctx->stream));
+        const sgx_ctx::FilterEntry& fe = filter_entry(ctx, Hk, npad, nb, prec);
+        float* g = static_cast<float*>(fe.dev);
         const int64_t nout = (nb + extra) * ncols;
         const float* din = static_cast<const float*>(
             stage_in(ctx, W_IN, in, 4 * nb * ncols, flags));
@@ -680,9 +738,9 @@ int sgx_sino_filter(sgx_ctx* ctx, const float* in, int64_t nb, int64_t ncols,
         a.out_bf16 = false;
         a.split_bf16 = prec == SGX_PREC_BF16;
         a.scale = 1.0f;
-        if (!ft.empty()) {
+        if (fe.fft_n > 0) {
             a.fft = g + npad;
-            a.fft_n = (int)(ft.size() / 5);
+            a.fft_n = (int)fe.fft_n;
         }
         sgx::launch_sino_filter(a, ctx->stream);
         finish(ctx, out, dout, 4 * nout, flags);
