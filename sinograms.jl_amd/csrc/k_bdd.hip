@@ -23,9 +23,22 @@
 // interpolates a strided set of detector boundaries with a binary search in
 // LDS.  The group's per-cell sums go to a partial buffer; a second kernel
 // adds the row groups in a fixed order and applies L (deterministic).
-// Backprojection: a block owns one view and a group of rows; the detector
-// prefix integral is built once per block in LDS and each thread
-// interpolates pixel boundaries; image sums over views use FP64 atomics.
+// Backprojection, owner-computes (no atomics, the reference's summation
+// order).  A first kernel (one block per view) forms the view's mapped
+// detector boundaries and the prefix integral of sino .* L .* detSize at them
+// (ascending knots, as the reference's reversal makes them) and, for every
+// 16 x 16 image tile, the range of knots that the tile's mapped pixel
+// boundaries fall in (the mapping is monotone along each axis, so the tile's
+// corners bound it; one knot of margin).  The second kernel gives each block
+// one tile and walks the views in order: the tile's knots are staged in LDS,
+// the 16 x 17 mapped pixel boundaries are interpolated once each, and every
+// thread adds its pixel's |difference| to a register.  A boundary outside the
+// staged knots (or a range wider than the LDS slot) is interpolated from the
+// view's full table in global memory, so the result never depends on the
+// range.  The reference's first-view quirk (bdd_2d.jl:139-140: the first
+// view's row buffer aliases the image, so it is added once more in the
+// untransposed frame) is an extra pass before view 0 in the identity frame.
+// Each pixel's sum runs over the passes in the reference's order.
 
 #include "sgx_internal.h"
 #include "sgx.h"
@@ -98,27 +111,29 @@ __device__ __forceinline__ double interp_flat(const double* xs, const double* ys
     return ys[lo] + t * (ys[lo + 1] - ys[lo]);
 }
 
-// Block-wide inclusive scan of LDS values v[0..n) (n <= 4096), in place.
+// Block-wide inclusive scan of LDS values v[0..n) (n <= 4096), in place:
+// per-thread serial chunks, their totals scanned across each wave with
+// shuffles, then the wave totals (tmp holds kThreads / 64 of them).
 __device__ void block_scan(double* v, double* tmp, int n)
 {
-    // per-thread serial chunk, then a scan of the chunk totals
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int per = (n + kThreads - 1) / kThreads;
     const int b = t * per, e = min(n, b + per);
     double s = 0;
     for (int i = b; i < e; ++i) s += v[i];
-    tmp[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        double a = 0;
-        for (int i = 0; i < kThreads; ++i) {
-            const double x = tmp[i];
-            tmp[i] = a;
-            a += x;
-        }
+    double x = s;                              // inclusive scan over the wave
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
     }
+    if (lane == 63) tmp[w] = x;
+    double ex = __shfl_up(x, 1, 64);           // exclusive within the wave
+    if (lane == 0) ex = 0;
     __syncthreads();
-    double a = tmp[t];
+    double a = 0;
+    for (int k = 0; k < w; ++k) a += tmp[k];
+    a += ex;
     for (int i = b; i < e; ++i) {
         a += v[i];
         v[i] = a;
@@ -207,21 +222,54 @@ __global__ void bdd_finish_kernel(const double* __restrict__ partial, int groups
     sino[view + (int64_t)nangle * k] = s * L;                           // (nangle, nDet)
 }
 
-// Backprojection: block = (view, group of kRows image rows).
-__global__ __launch_bounds__(kThreads) void bdd_back_kernel(
+constexpr int kBT = 16;             // owner tile edge (pixels)
+constexpr int kKnotMax = 256;       // staged knots per (tile, pass)
+
+// The view frame's rows [R0, R0 + kBT) and pixel boundaries [J0, J0 + kBT]
+// that image tile (ta, tb) covers.  Identity frame: image[r, j]; transposed
+// (views mapped on the y axis): image[nPix - 1 - j, r].
+__device__ __forceinline__ void tile_frame(bool ident, int ta, int tb, int nPix, int& R0, int& J0)
+{
+    if (ident) {
+        R0 = kBT * ta;
+        J0 = kBT * tb;
+    } else {
+        R0 = kBT * tb;
+        J0 = nPix - kBT - kBT * ta;
+    }
+}
+
+// interp_flat on a view's full table of {x, P} knots (ascending)
+__device__ __forceinline__ double interp_knots(const double2* __restrict__ kn, int n, double x)
+{
+    if (x <= kn[0].x) return kn[0].y;
+    if (x >= kn[n - 1].x) return kn[n - 1].y;
+    int lo = 0, hi = n - 1;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (kn[mid].x <= x) lo = mid;
+        else hi = mid;
+    }
+    const double2 p = kn[lo], q = kn[lo + 1];
+    const double t = (x - p.x) / (q.x - p.x);
+    return p.y + t * (q.y - p.y);
+}
+
+// Per view: knots[view] = {mapped detector boundary, prefix integral}
+// (nDet + 1, ascending), and per (pass, tile) the knot range of the tile.
+// Passes: [view 0 in the identity frame when ref_alias], then views in order.
+__global__ __launch_bounds__(kThreads) void bdd_back_table_kernel(
     const double* __restrict__ sino, int nPix, int nDet, const double* __restrict__ angle,
-    int nangle, double DSD, double DS0, double pSize, double dSize, int ref_alias,
-    double* __restrict__ image)
+    int nangle, double DSD, double DS0, double pSize, double dSize, int ref_alias, int tiles_a,
+    int ntiles, double2* __restrict__ knots, int2* __restrict__ ranges)
 {
     extern __shared__ double sm[];
     double* xs = sm;                           // nDet + 1 knots (ascending)
     double* ys = xs + (nDet + 1);              // nDet + 1 prefix integral
-    double* pdk = ys + (nDet + 1);             // nPix + 1
-    double* tmp = pdk + (nPix + 1);            // kThreads
-    double* cell = tmp + kThreads;             // nDet: sino .* L .* detSize, then detm
+    double* tmp = ys + (nDet + 1);             // kThreads
+    double* cell = tmp + kThreads;             // nDet: sino .* L .* detSize
 
     const int view = blockIdx.x;
-    const int r0 = blockIdx.y * kRows;
     const double beta = angle[view];
     const ViewGeo v = view_geo(beta, DS0);
     // detector boundaries (into xs), then per-cell sino * L * detSize
@@ -259,45 +307,152 @@ __global__ __launch_bounds__(kThreads) void bdd_back_kernel(
         }
     }
     __syncthreads();
+    double2* kv = knots + (int64_t)view * (nDet + 1);
+    for (int k = threadIdx.x; k <= nDet; k += kThreads) kv[k] = make_double2(xs[k], ys[k]);
 
-    for (int r = r0; r < min(nPix, r0 + kRows); ++r) {
-        for (int j = threadIdx.x; j <= nPix; j += kThreads)
-            pdk[j] = interp_flat(xs, ys, nDet + 1, pix_boundary(v, nPix, pSize, r, j));
-        __syncthreads();
-        for (int j = threadIdx.x; j < nPix; j += kThreads) {
-            const double val = fabs(pdk[j + 1] - pdk[j]);                 // :205
-            // axis_x: image[r, j]; else reverse(tmp, dims=2)' -> image[nPix-1-j, r]
-            const int64_t o = v.axis_x ? r + (int64_t)nPix * j : (nPix - 1 - j) + (int64_t)nPix * r;
-            atomicAdd(image + o, val);
-            // bdd_2d.jl:139-140: the first view's raw buffer is added once more
-            if (ref_alias && view == 0) atomicAdd(image + r + (int64_t)nPix * j, val);
+    const int npasses_here = (ref_alias && view == 0) ? 2 : 1;
+    for (int q = 0; q < npasses_here; ++q) {
+        const bool alias = q == 1;
+        const bool ident = alias || v.axis_x;
+        const int pass = alias ? 0 : view + (ref_alias ? 1 : 0);
+        for (int t = threadIdx.x; t < ntiles; t += kThreads) {
+            int R0, J0;
+            tile_frame(ident, t % tiles_a, t / tiles_a, nPix, R0, J0);
+            const int r0 = max(R0, 0), r1 = min(R0 + kBT - 1, nPix - 1);
+            const int j0 = max(J0, 0), j1 = min(J0 + kBT, nPix);
+            double lo = 1e308, hi = -1e308;
+            for (int c = 0; c < 4; ++c) {
+                const double x = pix_boundary(v, nPix, pSize, (c & 1) ? r1 : r0, (c & 2) ? j1 : j0);
+                lo = fmin(lo, x);
+                hi = fmax(hi, x);
+            }
+            // largest k with xs[k] <= lo, smallest k with xs[k] >= hi
+            int a = 0, b = nDet;
+            while (b - a > 1) {
+                const int m = (a + b) >> 1;
+                if (xs[m] <= lo) a = m;
+                else b = m;
+            }
+            const int klo = max(a - 1, 0);
+            a = 0;
+            b = nDet;
+            while (b - a > 1) {
+                const int m = (a + b) >> 1;
+                if (xs[m] >= hi) b = m;
+                else a = m;
+            }
+            const int khi = min(b + 1, nDet);
+            ranges[(int64_t)pass * ntiles + t] = make_int2(klo, khi);
         }
-        __syncthreads();
     }
 }
 
-__global__ void zero_f64_kernel(double* p, int64_t n)
+// Block = one 16 x 16 image tile, all passes in order (see above); one
+// thread per mapped pixel boundary of the tile (16 x 17), the first 256 of
+// them also own a pixel.
+constexpr int kOwnThreads = kBT * (kBT + 1);
+__global__ __launch_bounds__(kOwnThreads) void bdd_back_kernel(
+    const double2* __restrict__ knots, const int2* __restrict__ ranges, int npass,
+    int ref_alias, int nPix, int nDet, const double* __restrict__ angle, double DS0,
+    double pSize, int tiles_a, int ntiles, double* __restrict__ image)
 {
-    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
+    __shared__ double2 kn[2][kKnotMax];
+    __shared__ double P[2][kBT][kBT + 1];      // [row][boundary] of the view frame
+    const int t = blockIdx.x;
+    const int ta = t % tiles_a, tb = t / tiles_a;
+    const int tid = threadIdx.x;
+    const int tx = tid % kBT, ty = tid / kBT;
+    const int a = kBT * ta + tx, b = kBT * tb + ty;      // image[a, b]
+    const int nk = nDet + 1;
+    double acc = 0;
+
+    auto view_of = [&](int p) { return ref_alias ? max(p - 1, 0) : p; };
+    auto stage = [&](int p, int buf) {
+        const int2 rg = ranges[(int64_t)p * ntiles + t];
+        const int m = rg.y - rg.x + 1;
+        if (m <= kKnotMax) {
+            const double2* src = knots + (int64_t)view_of(p) * nk + rg.x;
+            for (int i = tid; i < m; i += kOwnThreads) kn[buf][i] = src[i];
+        }
+    };
+    stage(0, 0);
+    __syncthreads();
+    for (int p = 0; p < npass; ++p) {
+        const int buf = p & 1;
+        const int view = view_of(p);
+        const double beta = angle[view];
+        const ViewGeo v = view_geo(beta, DS0);
+        const bool ident = (ref_alias && p == 0) || v.axis_x;
+        int R0, J0;
+        tile_frame(ident, ta, tb, nPix, R0, J0);
+        const int2 rg = ranges[(int64_t)p * ntiles + t];
+        const int m = rg.y - rg.x + 1;
+        const double2* full = knots + (int64_t)view * nk;
+        const double2* st = kn[buf];
+        const int rho = tid / (kBT + 1), be = tid % (kBT + 1);
+        const int r = R0 + rho, j = J0 + be;
+        if (r >= 0 && r < nPix && j >= 0 && j <= nPix) {
+            const double x = pix_boundary(v, nPix, pSize, r, j);
+            const bool in_lo = x > st[0].x || rg.x == 0;
+            const bool in_hi = x < st[m - 1].x || rg.y == nDet;
+            double val;
+            if (m > kKnotMax || !in_lo || !in_hi) {
+                val = interp_knots(full, nk, x);
+            } else if (x <= st[0].x) {
+                val = st[0].y;                   // the table's first knot: flat
+            } else if (x >= st[m - 1].x) {
+                val = st[m - 1].y;               // the table's last knot: flat
+            } else {
+                int lo = 0, hi = m - 1;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (st[mid].x <= x) lo = mid;
+                    else hi = mid;
+                }
+                const double2 u = st[lo], w = st[lo + 1];
+                const double tt = (x - u.x) / (w.x - u.x);
+                val = u.y + tt * (w.y - u.y);
+            }
+            P[buf][rho][be] = val;
+        }
+        if (p + 1 < npass) stage(p + 1, buf ^ 1);
+        __syncthreads();
+        if (tid < kBT * kBT && a < nPix && b < nPix) {
+            acc += ident ? fabs(P[buf][tx][ty + 1] - P[buf][tx][ty])
+                         : fabs(P[buf][ty][kBT - tx] - P[buf][ty][kBT - 1 - tx]);
+        }
+    }
+    if (tid < kBT * kBT && a < nPix && b < nPix) image[a + (int64_t)nPix * b] = acc;
 }
 
 }  // namespace
 
+int64_t bdd_back_work_bytes(int nPix, int nDet, int nangle, bool ref_alias)
+{
+    const int64_t tiles = (int64_t)((nPix + kBT - 1) / kBT) * ((nPix + kBT - 1) / kBT);
+    const int64_t knots = (int64_t)nangle * (nDet + 1) * (int64_t)sizeof(double2);
+    return knots + (nangle + (ref_alias ? 1 : 0)) * tiles * (int64_t)sizeof(int2);
+}
+
 void launch_bdd_backproject(const double* sino, int nPix, int nDet, const double* angle,
                             int nangle, double DSD, double DS0, double pSize, double dSize,
-                            bool ref_alias, double* image, hipStream_t st)
+                            bool ref_alias, double* image, void* work, hipStream_t st)
 {
-    const size_t lds = sizeof(double) * (2 * (nDet + 1) + (nPix + 1) + kThreads + nDet);
+    const size_t lds = sizeof(double) * (2 * (nDet + 1) + kThreads + nDet);
     if (lds > 64 * 1024)
-        throw Error(SGX_ERR_UNSUPPORTED, "backproject_bdd: needs 3 nDet + nPix + 259 <= 8192");
-    const int64_t n = (int64_t)nPix * nPix;
-    hipLaunchKernelGGL(zero_f64_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                       image, n);
+        throw Error(SGX_ERR_UNSUPPORTED, "backproject_bdd: needs 3 nDet + 258 <= 8192");
+    const int tiles_a = (nPix + kBT - 1) / kBT;
+    const int ntiles = tiles_a * tiles_a;
+    const int npass = nangle + (ref_alias ? 1 : 0);
+    double2* knots = static_cast<double2*>(work);
+    int2* ranges = reinterpret_cast<int2*>(knots + (int64_t)nangle * (nDet + 1));
+    hipLaunchKernelGGL(bdd_back_table_kernel, dim3(nangle), dim3(kThreads), lds, st, sino, nPix,
+                       nDet, angle, nangle, DSD, DS0, pSize, dSize, (int)ref_alias, tiles_a,
+                       ntiles, knots, ranges);
     SGX_HIP(hipGetLastError());
-    dim3 grid(nangle, (nPix + kRows - 1) / kRows);
-    hipLaunchKernelGGL(bdd_back_kernel, grid, dim3(kThreads), lds, st, sino, nPix, nDet, angle,
-                       nangle, DSD, DS0, pSize, dSize, (int)ref_alias, image);
+    hipLaunchKernelGGL(bdd_back_kernel, dim3(ntiles), dim3(kOwnThreads), 0, st, knots, ranges,
+                       npass, (int)ref_alias, nPix, nDet, angle, DS0, pSize, tiles_a, ntiles,
+                       image);
     SGX_HIP(hipGetLastError());
 }
 

@@ -816,9 +816,11 @@ int sgx_backproject_bdd(sgx_ctx* ctx, const double* sino, int64_t nPix, int64_t 
             stage_in(ctx, W_IN, sino, 8 * nangle * nDet, flags));
         const int64_t nout = nPix * nPix;
         double* out = (flags & SGX_HOST) ? static_cast<double*>(ctx->get(W_OUT, 8 * nout)) : image;
+        void* work = ctx->get(W_PARTIAL, (size_t)sgx::bdd_back_work_bytes(
+                                             (int)nPix, (int)nDet, (int)nangle, ref_alias != 0));
         sgx::launch_bdd_backproject(sn, (int)nPix, (int)nDet,
                                     reinterpret_cast<const double*>(d + o_a), (int)nangle, DSD,
-                                    DS0, pSize, dSize, ref_alias != 0, out, ctx->stream);
+                                    DS0, pSize, dSize, ref_alias != 0, out, work, ctx->stream);
         finish(ctx, image, out, 8 * nout, flags);
     });
 }

@@ -261,7 +261,10 @@ void launch_bdd_project(const void* img, bool img_f32, int nPix, int nDet,
 // K6 adjoint: distance-driven backprojector (FP64, atomics over views).
 void launch_bdd_backproject(const double* sino, int nPix, int nDet, const double* angle,
                             int nangle, double DSD, double DS0, double pSize, double dSize,
-                            bool ref_alias, double* image, hipStream_t st);
+                            bool ref_alias, double* image, void* work, hipStream_t st);
+// bytes of the workspace of launch_bdd_backproject (per-view knot tables and
+// per-(view, tile) knot ranges)
+int64_t bdd_back_work_bytes(int nPix, int nDet, int nangle, bool ref_alias);
 
 // Parker weights (nb, na) in FP64 on the device, stored FP32.
 void launch_parker(const double* gam, const double* bet, int nb, int na, double gamma_max,

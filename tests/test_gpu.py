@@ -733,17 +733,23 @@ def test_project_bdd_vs_oracle(sg, orc, torch_cuda, nPix, nDet, na, f32):
     assert np.allclose(got_d, got, rtol=0, atol=1e-12 * np.abs(got).max())
 
 
-@pytest.mark.parametrize("nPix,nDet,na", [(32, 57, 24), (128, 226, 90)])
-def test_backproject_bdd_vs_oracle_and_adjoint(sg, orc, torch_cuda, nPix, nDet, na):
-    _say("backproject_bdd", nPix, nDet, na)
+@pytest.mark.parametrize("nPix,nDet,na,a0", [(32, 57, 24, 0.0), (128, 226, 90, 0.0),
+                                             (40, 57, 24, math.pi / 2), (50, 80, 18, 3.5)])
+def test_backproject_bdd_vs_oracle_and_adjoint(sg, orc, torch_cuda, nPix, nDet, na, a0):
+    # a0 = pi / 2: the first view maps on the y axis, so the reference's
+    # first-view aliasing adds it in the other frame; 40 and 50 pixels leave
+    # partial 16 x 16 tiles at the image edge.
+    _say("backproject_bdd", nPix, nDet, na, a0)
     geo = dict(DSD=949.0, DS0=541.0, pSize=512.0 / nPix, dSize=1.0239 * 910 / nDet,
-               nPix=nPix, nDet=nDet, angle=np.arange(na) * 2 * math.pi / na)
+               nPix=nPix, nDet=nDet, angle=(a0 + np.arange(na) * 2 * math.pi / na) % (2 * math.pi))
     rng = np.random.default_rng(41)
     y = rng.random((na, nDet))
     for alias in (True, False):
         got = sg.backproject_bdd(y, geo, ref_alias=alias)
         ref = orc.backproject_bdd(y, **geo, ref_alias=alias)
         assert rel_rmse(got, ref) < 1e-12
+        # owner-computes: no atomics, the same bits every call
+        assert np.array_equal(sg.backproject_bdd(y, geo, ref_alias=alias), got)
     # adjoint identity of the GPU pair (without the reference's aliasing quirk)
     x = rng.random((nPix, nPix))
     lhs = np.dot(sg.project_bdd(x, geo).ravel(), y.ravel())
