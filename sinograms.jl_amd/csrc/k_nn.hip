@@ -16,6 +16,12 @@
 //                   contraction with the channels swapped and the taps
 //                   reversed), GELU applied to the input on load and GELU'
 //                   applied to the output in the backward epilogue.
+// conv_mfma_kernel  the same for 32 -> 32 channel layers on MFMA
+//                   (v_mfma_f32_32x32x16_bf16): per 32 bins a 32 x 288 x 32
+//                   product, K = (tap, input channel).  The weights are BF16
+//                   (as everywhere); each FP32 activation is split exactly into
+//                   three BF16 parts, so that the three products per K step
+//                   are exact and only the FP32 accumulation rounds.
 // wgrad_kernel      weight and bias gradients: per-block partial sums over a
 //                   strided set of (row, bin tile) items, then a fixed-order
 //                   reduction (deterministic).
@@ -79,17 +85,25 @@ __global__ __launch_bounds__(kThreads) void conv_kernel(
         const float v = TRANS ? w[(i * cout + o) * kW + (kW - 1 - k)] : w[e];
         wt[e] = bf16r(v);
     }
-    // input tile with halo, zero outside [0, n)
+    // input tile with halo, zero outside [0, n); a thread's loads in flight
+    // together
     const float* src = in + (int64_t)r * cin * n;
-    for (int e = tid; e < cin * (kTileN + kW - 1); e += kThreads) {
-        const int i = e / (kTileN + kW - 1), j = e % (kTileN + kW - 1);
+    constexpr int kSpan = kTileN + kW - 1;
+    constexpr int kLoads = (kMaxC * kSpan + kThreads - 1) / kThreads;
+    const int total = cin * kSpan;
+    float xv[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int e = tid + u * kThreads;
+        const int i = e / kSpan, j = e % kSpan;
         const int m = n0 + j - kHalf;
-        float v = 0.f;
-        if (m >= 0 && m < n) {
-            v = src[(int64_t)i * n + m];
-            if (GELU_IN) v = gelu(v);
-        }
-        tile[i][j] = v;
+        xv[u] = (e < total && m >= 0 && m < n) ? src[(int64_t)i * n + m] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int e = tid + u * kThreads;
+        if (e >= total) break;
+        tile[e / kSpan][e % kSpan] = GELU_IN ? gelu(xv[u]) : xv[u];
     }
     __syncthreads();
 
@@ -131,6 +145,289 @@ __global__ __launch_bounds__(kThreads) void conv_kernel(
             if (!TRANS && resid) v += resid[(int64_t)r * n + m];
             dst[(int64_t)o * n + m] = v;
         }
+    }
+}
+
+// ---- MFMA form of the 32 -> 32 channel layers --------------------------------
+
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr int kMC = 32;                    // channels of the MFMA form
+constexpr int kMK = kW * kMC;              // 288 = K of one output bin
+constexpr int kMTile = 128;                // bins per block: 4 waves x 32
+constexpr int kMPos = kMTile + kW - 1;     // input positions with the halo
+constexpr int kMStride = 40;               // halves per position: 32 channels + 8 pad
+                                           // (16-byte reads of 8 lanes hit distinct banks)
+
+__device__ __forceinline__ uint16_t bf16_bits(float x)
+{
+    return (uint16_t)(__float_as_uint(bf16r(x)) >> 16);
+}
+
+// x = h + m + l exactly, each a BF16 (FP32 has 24 significant bits)
+__device__ __forceinline__ void split3(float x, uint16_t& h, uint16_t& m, uint16_t& l)
+{
+    const float fh = bf16r(x);
+    const float r1 = x - fh;
+    const float fm = bf16r(r1);
+    h = (uint16_t)(__float_as_uint(fh) >> 16);
+    m = (uint16_t)(__float_as_uint(fm) >> 16);
+    l = bf16_bits(r1 - fm);
+}
+
+// A operand of conv_mfma_kernel: wa[o][kk] = BF16 of Wt[o][i][k], kk = 32 k + i;
+// forward Wt = W (cout x cin x 9), TRANS Wt[o][i][k] = W[i][o][8 - k].
+template <bool TRANS>
+__global__ void wprep_kernel(const float* __restrict__ w, uint16_t* __restrict__ wa)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kMC * kMK) return;
+    const int o = e / kMK, kk = e % kMK, k = kk / kMC, i = kk % kMC;
+    const float v = TRANS ? w[(i * kMC + o) * kW + (kW - 1 - k)] : w[(o * kMC + i) * kW + k];
+    wa[e] = bf16_bits(v);
+}
+
+// out[r, o, n] as conv_kernel, cin = cout = 32.  Block: 128 bins of one row,
+// wave w the 32 bins n0 + 32 w ..; lane (col, half) = (lane & 31, lane >> 5)
+// holds A rows o = col, B columns n = col and K elements 8 half .. 8 half + 7
+// of each 16-step; acc[v] is output channel 8 (v / 4) + 4 half + v % 4.
+template <bool TRANS, bool GELU_IN>
+__global__ __launch_bounds__(kThreads) void conv_mfma_kernel(
+    const float* __restrict__ in, int n, const uint16_t* __restrict__ wa,
+    const float* __restrict__ bias, const float* __restrict__ zprev, float* __restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3][kMPos * kMStride];
+    const int r = blockIdx.y;
+    const int n0 = blockIdx.x * kMTile;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+
+    // input tile with halo (zero outside [0, n)), split, position-major; all
+    // of a thread's loads in flight before the first is used
+    const float* src = in + (int64_t)r * kMC * n;
+    constexpr int kLoads = (kMC * kMPos + kThreads - 1) / kThreads;
+    float xv[kLoads];
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int e = tid + u * kThreads;
+        const int i = e / kMPos, p = e % kMPos;
+        const int m = n0 + p - kHalf;
+        xv[u] = (e < kMC * kMPos && m >= 0 && m < n) ? src[(int64_t)i * n + m] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < kLoads; ++u) {
+        const int e = tid + u * kThreads;
+        if (e >= kMC * kMPos) break;
+        const int i = e / kMPos, p = e % kMPos;
+        const float v = GELU_IN ? gelu(xv[u]) : xv[u];     // gelu(0) = 0 in the halo
+        uint16_t h, md, l;
+        split3(v, h, md, l);
+        xs[0][p * kMStride + i] = h;
+        xs[1][p * kMStride + i] = md;
+        xs[2][p * kMStride + i] = l;
+    }
+    // this lane's A fragments (row o = col), all 18 K steps
+    bf16x8 a[kMK / 16];
+    const uint4* wrow = reinterpret_cast<const uint4*>(wa + col * kMK + 8 * half);
+#pragma unroll
+    for (int ks = 0; ks < kMK / 16; ++ks) a[ks] = __builtin_bit_cast(bf16x8, wrow[2 * ks]);
+    __syncthreads();
+
+    floatx16 acc;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+    for (int ks = 0; ks < kMK / 16; ++ks) {
+        const int k = ks / 2, i0 = (ks & 1) * 16 + 8 * half;
+        const int pos = 32 * wave + col + k;
+#pragma unroll
+        for (int pl = 0; pl < 3; ++pl) {
+            const bf16x8 b = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const uint4*>(&xs[pl][pos * kMStride + i0]));
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks], b, acc, 0, 0, 0);
+        }
+    }
+
+    const int m = n0 + 32 * wave + col;
+    if (m >= n) return;
+    float* dst = out + (int64_t)r * kMC * n + m;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) {
+        const int o = 8 * (v / 4) + 4 * half + (v % 4);
+        float y = acc[v];
+        if (!TRANS && bias) y += bias[o];
+        if (TRANS && zprev) y *= gelu_grad(zprev[((int64_t)r * kMC + o) * n + m]);
+        dst[(int64_t)o * n] = y;
+    }
+}
+
+// ---- fused inference: all layers of a 32-channel network in one kernel ------
+//
+// Block = one row and T = 128 - 8 (L - 2) output bins.  Layer 1 (1 -> 32)
+// runs on the VALU over 136 positions, the 32 -> 32 layers on MFMA over 128
+// positions each (4 waves x 32), the last layer (32 -> 1) on the VALU;
+// activations stay in LDS (exact three-way BF16 splits, FP32 before the last
+// layer), so HBM sees only q in and q + f(q) out.  Each layer's output
+// window starts 4 bins after its input's; positions a layer did not compute
+// (past its 128) read as 0 and only reach outputs outside the final T.  As
+// in the per-layer kernels, every activation outside [0, n) is 0 (zero
+// padding), weights are BF16 and sums FP32.
+constexpr int kNetMaxL = 16;
+struct NetArgs {
+    const float* q;
+    float* out;
+    int n;
+    int layers;
+    const float* w;                       // all parameters
+    int woff[kNetMaxL], boff[kNetMaxL];
+    uint16_t* wa;                         // A operands of layers 1 .. L-2
+};
+
+constexpr int kAStride = kMK + 8;         // halves per A row in LDS (conflict-free reads)
+
+// the A operands (wprep_kernel<false> layout) of layers 1 .. L-2: block
+// (x, l - 1) converts 256 weights of layer l
+__global__ void wprep_net_kernel(NetArgs a)
+{
+    const int l = blockIdx.y + 1;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= kMC * kMK) return;
+    const int o = e / kMK, kk = e % kMK, k = kk / kMC, i = kk % kMC;
+    a.wa[(int64_t)(l - 1) * kMC * kMK + e] = bf16_bits(a.w[a.woff[l] + (o * kMC + i) * kW + k]);
+}
+
+__global__ __launch_bounds__(kThreads) void net_infer_kernel(NetArgs a)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3][kMPos * kMStride];
+    // A operand of the current MFMA layer; after the last one, the FP32
+    // gelu'd activations g[32][kMPos] of the last layer's input
+    __shared__ __attribute__((aligned(16))) uint16_t abuf[kMC * kAStride];
+    __shared__ float qt[kMPos + kW - 1];
+    __shared__ float w1[kMC * kW], b1[kMC], wl[kMC * kW];
+    __shared__ float red[kMTile];
+    static_assert(sizeof(float) * kMC * kMPos <= sizeof(uint16_t) * kMC * kAStride, "g");
+    float* g = reinterpret_cast<float*>(abuf);
+
+    const int L = a.layers, n = a.n;
+    const int T = kMTile - 8 * (L - 2);
+    const int r = blockIdx.y, n0 = blockIdx.x * T;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    int s = -kHalf * (L - 1);                 // layer 1's first output position - n0
+    const float* qr = a.q + (int64_t)r * n;
+
+    for (int j = tid; j < kMPos + kW - 1; j += kThreads) {
+        const int m = n0 + s - kHalf + j;
+        qt[j] = (m >= 0 && m < n) ? qr[m] : 0.f;
+    }
+    for (int e = tid; e < kMC * kW; e += kThreads) {
+        w1[e] = bf16r(a.w[a.woff[0] + e]);            // W1[o][0][k]
+        wl[e] = bf16r(a.w[a.woff[L - 1] + e]);        // WL[0][i][k]
+    }
+    if (tid < kMC) b1[tid] = a.w[a.boff[0] + tid];
+    __syncthreads();
+
+    // layer 1 (1 -> 32): z[o][p] at n0 + s + p, p < kMPos
+    for (int e = tid; e < kMC * kMPos; e += kThreads) {
+        const int p = e / kMC, o = e % kMC;
+        const int m = n0 + s + p;
+        float z = 0.f;
+        if (m >= 0 && m < n) {
+            float acc = 0.f;
+#pragma unroll
+            for (int k = 0; k < kW; ++k) acc = fmaf(w1[o * kW + k], qt[p + k], acc);
+            z = acc + b1[o];
+        }
+        const float v = gelu(z);
+        if (L == 2) {
+            g[o * kMPos + p] = v;
+        } else {
+            uint16_t h, md, lo;
+            split3(v, h, md, lo);
+            xs[0][p * kMStride + o] = h;
+            xs[1][p * kMStride + o] = md;
+            xs[2][p * kMStride + o] = lo;
+        }
+    }
+
+    // layers 1 .. L-2 (32 -> 32) on MFMA
+    for (int l = 1; l <= L - 2; ++l) {
+        const uint16_t* wsrc = a.wa + (int64_t)(l - 1) * kMC * kMK;
+        for (int e = 8 * tid; e < kMC * kMK; e += 8 * kThreads) {
+            const int o = e / kMK, kk = e % kMK;
+            *reinterpret_cast<uint4*>(&abuf[o * kAStride + kk]) =
+                *reinterpret_cast<const uint4*>(wsrc + e);
+        }
+        __syncthreads();                      // xs (previous layer) and A staged
+        bf16x8 af[kMK / 16];
+#pragma unroll
+        for (int ks = 0; ks < kMK / 16; ++ks)
+            af[ks] = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const uint4*>(&abuf[col * kAStride + 16 * ks + 8 * half]));
+        floatx16 acc;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[v] = 0.f;
+#pragma unroll
+        for (int ks = 0; ks < kMK / 16; ++ks) {
+            const int k = ks / 2, i0 = (ks & 1) * 16 + 8 * half;
+            const int pos = 32 * wave + col + k;
+#pragma unroll
+            for (int pl = 0; pl < 3; ++pl) {
+                const bf16x8 b = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(&xs[pl][pos * kMStride + i0]));
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(af[ks], b, acc, 0, 0, 0);
+            }
+        }
+        __syncthreads();                      // every read of xs and abuf is done
+        s += kHalf;                           // this layer's first output position - n0
+        const int idx = 32 * wave + col;
+        const int m = n0 + s + idx;
+        const bool in = m >= 0 && m < n;
+        const bool last_mfma = l == L - 2;
+        const float* bias = a.w + a.boff[l];
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int o = 8 * (v / 4) + 4 * half + (v % 4);
+            const float y = gelu(in ? acc[v] + bias[o] : 0.f);
+            if (last_mfma) {
+                g[o * kMPos + idx] = y;
+            } else {
+                uint16_t h, md, lo;
+                split3(y, h, md, lo);
+                xs[0][idx * kMStride + o] = h;
+                xs[1][idx * kMStride + o] = md;
+                xs[2][idx * kMStride + o] = lo;
+            }
+        }
+        // positions kMTile .. kMPos - 1 were not computed: 0
+        for (int e = tid; e < kMC * (kMPos - kMTile); e += kThreads) {
+            const int o = e / (kMPos - kMTile), p = kMTile + e % (kMPos - kMTile);
+            if (last_mfma) {
+                g[o * kMPos + p] = 0.f;
+            } else {
+                xs[0][p * kMStride + o] = 0;
+                xs[1][p * kMStride + o] = 0;
+                xs[2][p * kMStride + o] = 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    // last layer (32 -> 1) + bias + q: thread (bin m, channel half hh)
+    const int m = tid % kMTile, hh = tid / kMTile;
+    float acc = 0.f;
+    if (m < T) {
+        for (int i = 16 * hh; i < 16 * hh + 16; ++i)
+#pragma unroll
+            for (int k = 0; k < kW; ++k) acc = fmaf(wl[i * kW + k], g[i * kMPos + m + k], acc);
+    }
+    if (hh == 1) red[m] = acc;
+    __syncthreads();
+    if (hh == 0 && m < T && n0 + m < n) {
+        float y = (acc + red[m]) + a.w[a.boff[L - 1]];
+        y += qt[m + kHalf * L];
+        a.out[(int64_t)r * n + n0 + m] = y;
     }
 }
 
@@ -254,12 +551,34 @@ __global__ __launch_bounds__(256) void mse_kernel(const float* __restrict__ x,
 
 }  // namespace
 
+int64_t nn_wprep_halves() { return (int64_t)kMC * kMK; }
+
+// the MFMA form takes 32 -> 32 channel layers
+static bool mfma_layer(int cin, int cout, const float* resid)
+{
+    return cin == kMC && cout == kMC && resid == nullptr;
+}
+
 void launch_conv(const float* in, int rows, int cin, int cout, int n, const float* w,
                  const float* bias, const float* resid, bool gelu_in, float* out,
-                 hipStream_t st)
+                 uint16_t* wscratch, hipStream_t st)
 {
     if (cin < 1 || cin > kMaxC || cout < 1 || cout > kMaxC)
         throw Error(SGX_ERR_UNSUPPORTED, "net: channels must be 1..32");
+    if (mfma_layer(cin, cout, resid)) {
+        hipLaunchKernelGGL(wprep_kernel<false>, dim3((kMC * kMK + 255) / 256), dim3(256), 0, st,
+                           w, wscratch);
+        SGX_HIP(hipGetLastError());
+        dim3 g((unsigned)((n + kMTile - 1) / kMTile), (unsigned)rows);
+        if (gelu_in)
+            hipLaunchKernelGGL((conv_mfma_kernel<false, true>), g, dim3(kThreads), 0, st, in, n,
+                               wscratch, bias, nullptr, out);
+        else
+            hipLaunchKernelGGL((conv_mfma_kernel<false, false>), g, dim3(kThreads), 0, st, in,
+                               n, wscratch, bias, nullptr, out);
+        SGX_HIP(hipGetLastError());
+        return;
+    }
     dim3 grid((unsigned)((n + kTileN - 1) / kTileN), (unsigned)rows);
     if (gelu_in)
         hipLaunchKernelGGL((conv_kernel<false, true>), grid, dim3(kThreads), 0, st, in, cin,
@@ -270,12 +589,64 @@ void launch_conv(const float* in, int rows, int cin, int cout, int n, const floa
     SGX_HIP(hipGetLastError());
 }
 
+bool net_fused_ok(int layers, int channels)
+{
+    return channels == kMC && layers >= 2 && layers <= kNetMaxL;
+}
+
+void launch_net_prep(const float* w, int layers, const int64_t* woff, uint16_t* wa,
+                     hipStream_t st)
+{
+    if (layers <= 2) return;
+    NetArgs a{};
+    a.layers = layers;
+    a.w = w;
+    for (int l = 0; l < layers; ++l) a.woff[l] = (int)woff[l];
+    a.wa = wa;
+    hipLaunchKernelGGL(wprep_net_kernel, dim3((kMC * kMK + 255) / 256, layers - 2), dim3(256), 0,
+                       st, a);
+    SGX_HIP(hipGetLastError());
+}
+
+void launch_net_infer(const float* q, int rows, int n, const float* w, int layers,
+                      const int64_t* woff, const int64_t* boff, const uint16_t* wa, float* out,
+                      hipStream_t st)
+{
+    if (!net_fused_ok(layers, kMC))
+        throw Error(SGX_ERR_UNSUPPORTED, "fused inference: 32 channels, 2..16 layers");
+    NetArgs a{};
+    a.q = q;
+    a.out = out;
+    a.n = n;
+    a.layers = layers;
+    a.w = w;
+    for (int l = 0; l < layers; ++l) {
+        a.woff[l] = (int)woff[l];
+        a.boff[l] = (int)boff[l];
+    }
+    a.wa = const_cast<uint16_t*>(wa);
+    const int T = kMTile - 8 * (layers - 2);
+    dim3 grid((unsigned)((n + T - 1) / T), (unsigned)rows);
+    hipLaunchKernelGGL(net_infer_kernel, grid, dim3(kThreads), 0, st, a);
+    SGX_HIP(hipGetLastError());
+}
+
 void launch_conv_dgrad(const float* dout, int rows, int cin, int cout, int n, const float* w,
-                       const float* zprev, float* din, hipStream_t st)
+                       const float* zprev, float* din, uint16_t* wscratch, hipStream_t st)
 {
     // layer (cin -> cout) with weights w[cout][cin][9]: din has cin channels
     if (cin < 1 || cin > kMaxC || cout < 1 || cout > kMaxC)
         throw Error(SGX_ERR_UNSUPPORTED, "net: channels must be 1..32");
+    if (mfma_layer(cin, cout, nullptr)) {
+        hipLaunchKernelGGL(wprep_kernel<true>, dim3((kMC * kMK + 255) / 256), dim3(256), 0, st, w,
+                           wscratch);
+        SGX_HIP(hipGetLastError());
+        dim3 g((unsigned)((n + kMTile - 1) / kMTile), (unsigned)rows);
+        hipLaunchKernelGGL((conv_mfma_kernel<true, false>), g, dim3(kThreads), 0, st, dout, n,
+                           wscratch, nullptr, zprev, din);
+        SGX_HIP(hipGetLastError());
+        return;
+    }
     dim3 grid((unsigned)((n + kTileN - 1) / kTileN), (unsigned)rows);
     hipLaunchKernelGGL((conv_kernel<true, false>), grid, dim3(kThreads), 0, st, dout, cout, cin,
                        n, w, nullptr, nullptr, zprev, din);

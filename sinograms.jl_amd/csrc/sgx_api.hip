@@ -1715,12 +1715,16 @@ struct sgx_net {
     int64_t d_cap = 0;
     float* part = nullptr;
     int64_t part_cap = 0;
+    uint16_t* wbf = nullptr;               // BF16 weights of the MFMA layers
+    int64_t wver = 1;                      // bumped by every parameter write
+    int64_t wbf_ver = 0;                   // parameters the fused form's wbf holds
     bool kept = false;
 
     ~sgx_net()
     {
         for (float* p : {w, g, m, v, z, d, part})
             if (p) (void)hipFree(p);
+        if (wbf) (void)hipFree(wbf);
     }
     void grow(float*& p, int64_t& cap, int64_t need)
     {
@@ -1809,6 +1813,7 @@ int sgx_net_create(sgx_ctx* ctx, int layers, int channels, uint64_t seed, sgx_ne
                 h[N->woff[l] + e] = (float)((2.0 * uni() - 1.0) * a);
         }
         for (float** p : {&N->w, &N->g, &N->m, &N->v}) SGX_HIP(hipMalloc(p, 4 * N->nparams));
+        SGX_HIP(hipMalloc(&N->wbf, 2 * sgx::nn_wprep_halves() * std::max(1, layers - 2)));
         SGX_HIP(hipMemcpy(N->w, h.data(), 4 * N->nparams, hipMemcpyHostToDevice));
         for (float* p : {N->g, N->m, N->v}) sgx::launch_fill(p, N->nparams, 0.f, ctx->stream);
         SGX_HIP(hipStreamSynchronize(ctx->stream));
@@ -1832,7 +1837,10 @@ int sgx_net_params(sgx_net* net, float* host, int set)
     return guard(net->ctx, [&] {
         require(host, SGX_ERR_ARG, "null pointer argument");
         SGX_HIP(hipStreamSynchronize(net->ctx->stream));
-        if (set) SGX_HIP(hipMemcpy(net->w, host, 4 * net->nparams, hipMemcpyHostToDevice));
+        if (set) {
+            SGX_HIP(hipMemcpy(net->w, host, 4 * net->nparams, hipMemcpyHostToDevice));
+            net->wver += 1;
+        }
         else SGX_HIP(hipMemcpy(host, net->w, 4 * net->nparams, hipMemcpyDeviceToHost));
     });
 }
@@ -1863,6 +1871,19 @@ int sgx_net_forward(sgx_net* net, const float* q, int64_t rows, int64_t n, float
         sgx_net& N = *net;
         const int L = N.layers;
         const int64_t per = rows * N.channels * n;          // one hidden activation
+        if (!keep && sgx::net_fused_ok(L, N.channels)) {
+            // inference: one fused launch; the BF16 weights are converted
+            // once per parameter update
+            if (N.wbf_ver != N.wver) {
+                sgx::launch_net_prep(N.w, L, N.woff.data(), N.wbf, ctx->stream);
+                N.wbf_ver = N.wver;
+            }
+            sgx::launch_net_infer(q, (int)rows, (int)n, N.w, L, N.woff.data(), N.boff.data(),
+                                  N.wbf, out, ctx->stream);
+            N.kept = false;
+            return;
+        }
+        N.wbf_ver = 0;                       // the per-layer kernels reuse wbf
         // hidden pre-activations: all L-1 kept for training, else two in turn
         N.grow(N.z, N.z_cap, per * (keep ? (L - 1) : std::min(L - 1, 2)));
         const float* in = q;
@@ -1870,7 +1891,8 @@ int sgx_net_forward(sgx_net* net, const float* q, int64_t rows, int64_t n, float
             const bool last = l == L - 1;
             float* dst = last ? out : N.z + per * (keep ? l : (l % 2));
             sgx::launch_conv(in, (int)rows, N.cin[l], N.cout[l], (int)n, N.w + N.woff[l],
-                             N.w + N.boff[l], last ? q : nullptr, l > 0, dst, ctx->stream);
+                             N.w + N.boff[l], last ? q : nullptr, l > 0, dst, N.wbf,
+                             ctx->stream);
             in = dst;
         }
         N.q = q;
@@ -1888,6 +1910,7 @@ int sgx_net_backward(sgx_net* net, const float* gout)
         require(gout, SGX_ERR_ARG, "null pointer argument");
         sgx_net& N = *net;
         require(N.kept, SGX_ERR_ARG, "sgx_net_backward needs a kept forward");
+        N.wbf_ver = 0;                       // the data-gradient kernels reuse wbf
         const int L = N.layers;
         const int64_t per = N.rows * N.channels * N.n;
         N.grow(N.d, N.d_cap, 2 * per);
@@ -1901,7 +1924,8 @@ int sgx_net_backward(sgx_net* net, const float* gout)
             if (l == 0) break;
             float* din = N.d + per * (l % 2);
             sgx::launch_conv_dgrad(delta, (int)N.rows, N.cin[l], N.cout[l], (int)N.n,
-                                   N.w + N.woff[l], N.z + per * (l - 1), din, ctx->stream);
+                                   N.w + N.woff[l], N.z + per * (l - 1), din, N.wbf,
+                                   ctx->stream);
             delta = din;
         }
     });
@@ -1912,6 +1936,7 @@ int sgx_net_adam(sgx_net* net, float lr, float beta1, float beta2, float eps)
     if (!net) return SGX_ERR_ARG;
     return guard(net->ctx, [&] {
         net->step += 1;
+        net->wver += 1;
         sgx::launch_adam(net->w, net->g, net->m, net->v, net->nparams, lr, beta1, beta2, eps,
                          net->step, net->ctx->stream);
     });

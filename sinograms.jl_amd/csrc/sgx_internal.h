@@ -179,11 +179,23 @@ void launch_back2d_t(const Back2dArgs& a, const float* img, float* qt, hipStream
 
 // K8: the learned correction's kernels (csrc/k_nn.hip).  Activations are
 // (rows, channels, n); weights (cout, cin, 9) FP32, rounded to BF16 in use.
+// wscratch: nn_wprep_halves() device halves for the MFMA form's BF16 weights
+// (32 -> 32 channel layers); stream-ordered reuse across layers is safe
+int64_t nn_wprep_halves();
 void launch_conv(const float* in, int rows, int cin, int cout, int n, const float* w,
                  const float* bias, const float* resid, bool gelu_in, float* out,
-                 hipStream_t st);
+                 uint16_t* wscratch, hipStream_t st);
 void launch_conv_dgrad(const float* dout, int rows, int cin, int cout, int n, const float* w,
-                       const float* zprev, float* din, hipStream_t st);
+                       const float* zprev, float* din, uint16_t* wscratch, hipStream_t st);
+// Fused inference of a whole network (32 channels, 2..16 layers): q (rows, n)
+// -> out = q + f(q), one launch.  wa: the BF16 A operands of the 32 -> 32
+// layers, (layers - 2) * nn_wprep_halves() halves, written by launch_net_prep.
+bool net_fused_ok(int layers, int channels);
+void launch_net_prep(const float* w, int layers, const int64_t* woff, uint16_t* wa,
+                     hipStream_t st);
+void launch_net_infer(const float* q, int rows, int n, const float* w, int layers,
+                      const int64_t* woff, const int64_t* boff, const uint16_t* wa, float* out,
+                      hipStream_t st);
 int nn_wgrad_blocks(int rows, int n);
 void launch_conv_wgrad(const float* dout, const float* x, bool gelu_in, int rows, int cin,
                        int cout, int n, float* part, float* gw, float* gb, hipStream_t st);

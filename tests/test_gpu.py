@@ -889,6 +889,45 @@ def test_net_forward_backward_vs_oracle(sg, torch_cuda):
     assert float((net0.forward(qd) - qd).abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("layers", [2, 3, 4, 5])
+def test_net_fused_inference(sg, torch_cuda, layers):
+    # Inference (keep=False) of a 32-channel network is one fused kernel
+    # (activations in LDS); training forwards (keep=True) run layer by layer.
+    # Both against the float64 oracle, on rows shorter than one fused tile,
+    # spanning several tiles with a ragged end, and after parameter updates
+    # (the fused form's BF16 weights are rebuilt when the parameters change).
+    from oracle import nn as onn
+    from sinograms_amd.learned import ConvCorrection
+    _say("fused net inference, layers", layers)
+    rng = np.random.default_rng(50 + layers)
+    ch = 32
+    net = ConvCorrection(layers, ch, seed=5)
+    p = (rng.standard_normal(net.num_params) * 0.05).astype(np.float32)
+    net.set_params(p)
+    for N, R in ((37, 3), (300, 4), (888, 2)):
+        q = rng.standard_normal((N, R)).astype(np.float32)
+        qd = _colmajor(torch_cuda, q)
+        fused = net.forward(qd).cpu().numpy()
+        layered = net.forward(qd, keep=True).cpu().numpy()
+        ref, _ = onn.forward(onn.split_params(p.astype(np.float64), layers, ch),
+                             q.T.astype(np.float64))
+        assert rel_rmse(fused.T, ref) < 1e-6, N
+        assert rel_rmse(layered.T, ref) < 1e-6, N
+        assert rel_rmse(fused, layered) < 1e-6, N
+    # a backward pass and an Adam step change the parameters: the next fused
+    # forward uses the new ones
+    net.zero_grad()
+    net.backward(_colmajor(torch_cuda, rng.standard_normal((888, 2)).astype(np.float32)))
+    net.adam(1e-2)
+    p2 = net.params()
+    assert not np.array_equal(p2, p)
+    q = rng.standard_normal((300, 4)).astype(np.float32)
+    qd = _colmajor(torch_cuda, q)
+    ref2, _ = onn.forward(onn.split_params(p2.astype(np.float64), layers, ch),
+                          q.T.astype(np.float64))
+    assert rel_rmse(net.forward(qd).cpu().numpy().T, ref2) < 1e-6
+
+
 def test_back_t_is_the_exact_transpose(sg, orc, torch_cuda):
     from sinograms_amd.learned import _empty_colmajor
     _say("back_fan transpose")
