@@ -496,6 +496,138 @@ __global__ __launch_bounds__(kThreads) void wgrad_kernel(
     if (ig == 0) p[cout * cin * kW + o] = accb;
 }
 
+// Weight / bias gradient partials of a 32 -> 32 layer on MFMA:
+//   dW[o][i][k] = sum_{r,m} d[r][o][m] gelu(x[r][i][m + k - 4]),  db[o] = sum d[r][o][m]
+// as per-item 32 x 32 x 128 products, K = the item's 128 bins: A = d (o x m),
+// one N tile per tap k (B[m][i] = gelu(x)[i][m + k - 4]) and one of ones (the
+// bias).  Both operands are split exactly into three BF16 parts; the six
+// products down to 2^-16 relative are summed, so the result matches FP32
+// products to within FP32 accumulation.  The tap shift makes the B rows start
+// at any bin: two copies of the activations, the second shifted by one bin,
+// keep every 8-element read on a 4-byte boundary.  Taps go to waves k % 4
+// (wave 1 also takes the bias tile).  Block b takes items b, b + gridDim.x,
+// ... and writes its partial sums as wgrad_kernel does.
+constexpr int kGS = 136;          // halves per LDS row of the d and x tiles (68 dwords:
+                                  // 16-byte reads of 8 lanes hit distinct banks)
+
+__global__ __launch_bounds__(kThreads) void wgrad_mfma_kernel(
+    const float* __restrict__ d, const float* __restrict__ x, int rows, int n,
+    float* __restrict__ part)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t ds[3][kMC * kGS];     // [plane][o][m]
+    __shared__ __attribute__((aligned(16))) uint16_t xs2[2][3][kMC * kGS]; // [copy][plane][i][p]
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int col = lane & 31, half = lane >> 5;
+    const int tiles = (n + kMTile - 1) / kMTile;
+    const int64_t items = (int64_t)rows * tiles;
+    const int ntap = wave == 0 ? 3 : 2;             // taps wave, wave + 4 (, 8)
+    floatx16 acc[3], accb;
+#pragma unroll
+    for (int t = 0; t < 3; ++t)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[t][v] = 0.f;
+#pragma unroll
+    for (int v = 0; v < 16; ++v) accb[v] = 0.f;
+    bf16x8 ones;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) ones[j] = (__bf16)1.0f;
+
+    constexpr int kLd = kMC * kMTile / kThreads;     // 16 d loads per thread
+    constexpr int kLx = (kMC * kGS + kThreads - 1) / kThreads;   // 17 x loads
+    for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const int r = (int)(it / tiles), n0 = (int)(it % tiles) * kMTile;
+        float dv[kLd], xv[kLx];
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int e = tid + u * kThreads, o = e / kMTile, m = n0 + e % kMTile;
+            dv[u] = m < n ? d[((int64_t)r * kMC + o) * n + m] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLx; ++u) {
+            const int e = tid + u * kThreads, i = e / kGS, m = n0 - kHalf + e % kGS;
+            xv[u] = (e < kMC * kGS && m >= 0 && m < n) ? x[((int64_t)r * kMC + i) * n + m] : 0.f;
+        }
+        __syncthreads();                             // the previous item's reads are done
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int e = tid + u * kThreads, o = e / kMTile, m = e % kMTile;
+            uint16_t h, md, l;
+            split3(dv[u], h, md, l);
+            ds[0][o * kGS + m] = h;
+            ds[1][o * kGS + m] = md;
+            ds[2][o * kGS + m] = l;
+        }
+#pragma unroll
+        for (int u = 0; u < kLx; ++u) {
+            const int e = tid + u * kThreads;
+            if (e >= kMC * kGS) break;
+            const int i = e / kGS, p = e % kGS;
+            uint16_t h, md, l;
+            split3(gelu(xv[u]), h, md, l);           // gelu(0) = 0 outside [0, n)
+            xs2[0][0][i * kGS + p] = h;
+            xs2[0][1][i * kGS + p] = md;
+            xs2[0][2][i * kGS + p] = l;
+            if (p > 0) {                             // copy 1: position p - 1
+                xs2[1][0][i * kGS + p - 1] = h;
+                xs2[1][1][i * kGS + p - 1] = md;
+                xs2[1][2][i * kGS + p - 1] = l;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < kMTile / 16; ++ks) {
+            const int m0 = 16 * ks + 8 * half;
+            bf16x8 a[3];
+#pragma unroll
+            for (int pl = 0; pl < 3; ++pl)
+                a[pl] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4*>(&ds[pl][col * kGS + m0]));
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                if (t >= ntap) break;
+                const int k = wave + 4 * t;
+                const int s0 = m0 + k;               // first position of the 8
+                const int cp = s0 & 1;
+                const uint32_t* src[3];
+                bf16x8 b[3];
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl) {
+                    src[pl] = reinterpret_cast<const uint32_t*>(&xs2[cp][pl][col * kGS + s0 - cp]);
+                    const uint4 w4 = make_uint4(src[pl][0], src[pl][1], src[pl][2], src[pl][3]);
+                    b[pl] = __builtin_bit_cast(bf16x8, w4);
+                }
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[0], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[1], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[0], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[0], b[2], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[2], b[0], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[1], b[1], acc[t], 0, 0, 0);
+            }
+            if (wave == 1) {
+#pragma unroll
+                for (int pl = 0; pl < 3; ++pl)
+                    accb = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[pl], ones, accb, 0, 0, 0);
+            }
+        }
+    }
+    // partials: acc[t][v] = dW[o][i = col][k = wave + 4 t], o = 8 (v / 4) + 4 half + v % 4
+    float* p = part + (int64_t)blockIdx.x * (kMC * kMC * kW + kMC);
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        if (t >= ntap) break;
+        const int k = wave + 4 * t;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int o = 8 * (v / 4) + 4 * half + (v % 4);
+            p[(o * kMC + col) * kW + k] = acc[t][v];
+        }
+    }
+    if (wave == 1 && col == 0) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) p[kMC * kMC * kW + 8 * (v / 4) + 4 * half + (v % 4)] = accb[v];
+    }
+}
+
 // Sum the partials of `blocks` blocks in block order, add into grad (per
 // element: the weights then the biases of one layer).
 __global__ void reduce_kernel(const float* __restrict__ part, int blocks, int len,
@@ -504,7 +636,14 @@ __global__ void reduce_kernel(const float* __restrict__ part, int blocks, int le
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= len) return;
     float s = 0.f;
-    for (int b = 0; b < blocks; ++b) s += part[(int64_t)b * len + e];
+    for (int b0 = 0; b0 < blocks; b0 += 8) {
+        float v[8];                               // loads in flight together
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = b0 + u < blocks ? part[(int64_t)(b0 + u) * len + e] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (b0 + u < blocks) s += v[u];           // block order
+    }
     if (e < nw) gw[e] += s;
     else gb[e - nw] += s;
 }
@@ -656,14 +795,17 @@ void launch_conv_dgrad(const float* dout, int rows, int cin, int cout, int n, co
 int nn_wgrad_blocks(int rows, int n)
 {
     const int64_t items = (int64_t)rows * ((n + kTileN - 1) / kTileN);
-    return (int)std::min<int64_t>(items, 1024);
+    return (int)std::min<int64_t>(items, 512);
 }
 
 void launch_conv_wgrad(const float* dout, const float* x, bool gelu_in, int rows, int cin,
                        int cout, int n, float* part, float* gw, float* gb, hipStream_t st)
 {
     const int blocks = nn_wgrad_blocks(rows, n);
-    if (gelu_in)
+    if (gelu_in && mfma_layer(cin, cout, nullptr))
+        hipLaunchKernelGGL(wgrad_mfma_kernel, dim3(blocks), dim3(kThreads), 0, st, dout, x, rows,
+                           n, part);
+    else if (gelu_in)
         hipLaunchKernelGGL((wgrad_kernel<true>), dim3(blocks), dim3(kThreads), 0, st, dout, x,
                            rows, cin, cout, n, part);
     else
