@@ -628,6 +628,86 @@ __global__ __launch_bounds__(kThreads) void wgrad_mfma_kernel(
     }
 }
 
+// Weight / bias gradient partials of a layer with few (o, i) pairs (the
+// 1 -> C and C -> 1 layers, P = cout * cin <= 128): thread t owns pair t % P
+// and every (256 / P)-th bin of each item (phase t / P), all taps; the
+// phases are added in order at the end of the block.
+constexpr int kSmallPairs = 128;
+
+template <bool GELU_IN>
+__global__ __launch_bounds__(kThreads) void wgrad_small_kernel(
+    const float* __restrict__ d, const float* __restrict__ x, int rows, int cin, int cout,
+    int n, float* __restrict__ part)
+{
+    __shared__ float dt[kMaxC][kTileN];
+    __shared__ float xt[kMaxC][kTileN + kW - 1];
+    __shared__ float red[kThreads][kW + 1];
+    const int tid = threadIdx.x;
+    const int P = cout * cin, phases = kThreads / P;
+    const int pair = tid % P, phase = tid / P;
+    const bool own = phase < phases;
+    const int o = pair / cin, i = pair % cin;
+    float acc[kW], accb = 0.f;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) acc[k] = 0.f;
+    const int tiles = (n + kTileN - 1) / kTileN;
+    const int64_t items = (int64_t)rows * tiles;
+    constexpr int kSpan = kTileN + kW - 1;
+    constexpr int kLd = (kMaxC * kTileN + kThreads - 1) / kThreads;
+    constexpr int kLx = (kMaxC * kSpan + kThreads - 1) / kThreads;
+    for (int64_t it = blockIdx.x; it < items; it += gridDim.x) {
+        const int r = (int)(it / tiles), n0 = (int)(it % tiles) * kTileN;
+        float dv[kLd], xv[kLx];
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int e = tid + u * kThreads, c = e / kTileN, m = n0 + e % kTileN;
+            dv[u] = (c < cout && m < n) ? d[((int64_t)r * cout + c) * n + m] : 0.f;
+        }
+#pragma unroll
+        for (int u = 0; u < kLx; ++u) {
+            const int e = tid + u * kThreads, c = e / kSpan, m = n0 - kHalf + e % kSpan;
+            xv[u] = (c < cin && m >= 0 && m < n) ? x[((int64_t)r * cin + c) * n + m] : 0.f;
+        }
+        __syncthreads();                          // the previous item is consumed
+#pragma unroll
+        for (int u = 0; u < kLd; ++u) {
+            const int e = tid + u * kThreads, c = e / kTileN;
+            if (c < cout) dt[c][e % kTileN] = dv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < kLx; ++u) {
+            const int e = tid + u * kThreads, c = e / kSpan;
+            if (c < cin) xt[c][e % kSpan] = GELU_IN ? gelu(xv[u]) : xv[u];
+        }
+        __syncthreads();
+        if (own) {
+            for (int m = phase; m < kTileN; m += phases) {
+                const float dvm = dt[o][m];
+                if (i == 0) accb += dvm;
+#pragma unroll
+                for (int k = 0; k < kW; ++k) acc[k] = fmaf(dvm, xt[i][m + k], acc[k]);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kW; ++k) red[tid][k] = acc[k];
+    red[tid][kW] = accb;
+    __syncthreads();
+    if (tid >= P) return;
+    float sw[kW], sb = 0.f;
+#pragma unroll
+    for (int k = 0; k < kW; ++k) sw[k] = 0.f;
+    for (int ph = 0; ph < phases; ++ph) {
+#pragma unroll
+        for (int k = 0; k < kW; ++k) sw[k] += red[ph * P + tid][k];
+        sb += red[ph * P + tid][kW];
+    }
+    float* pp = part + (int64_t)blockIdx.x * (cout * cin * kW + cout);
+#pragma unroll
+    for (int k = 0; k < kW; ++k) pp[(o * cin + i) * kW + k] = sw[k];
+    if (i == 0) pp[cout * cin * kW + o] = sb;
+}
+
 // Sum the partials of `blocks` blocks in block order, add into grad (per
 // element: the weights then the biases of one layer).
 __global__ void reduce_kernel(const float* __restrict__ part, int blocks, int len,
@@ -805,6 +885,12 @@ void launch_conv_wgrad(const float* dout, const float* x, bool gelu_in, int rows
     if (gelu_in && mfma_layer(cin, cout, nullptr))
         hipLaunchKernelGGL(wgrad_mfma_kernel, dim3(blocks), dim3(kThreads), 0, st, dout, x, rows,
                            n, part);
+    else if (cin * cout <= kSmallPairs && gelu_in)
+        hipLaunchKernelGGL(wgrad_small_kernel<true>, dim3(blocks), dim3(kThreads), 0, st, dout,
+                           x, rows, cin, cout, n, part);
+    else if (cin * cout <= kSmallPairs)
+        hipLaunchKernelGGL(wgrad_small_kernel<false>, dim3(blocks), dim3(kThreads), 0, st, dout,
+                           x, rows, cin, cout, n, part);
     else if (gelu_in)
         hipLaunchKernelGGL((wgrad_kernel<true>), dim3(blocks), dim3(kThreads), 0, st, dout, x,
                            rows, cin, cout, n, part);
