@@ -266,7 +266,10 @@ def c3_items(sg, device, seed=3):
 def batch_leg(sg, device, steps, seed=3):
     """The batching form: the 48 reconstructions of c3_items run as one
     autotuned HIP graph, timed against the same executes issued one after
-    another."""
+    another.  Two graph forms: queue launches (filters on parallel branches,
+    then one launch per backprojector variant over a size-ordered table of
+    tile / voxel-block tasks) and whole plan executes on parallel branches;
+    the capture times both and keeps the faster, and both are reported."""
     import numpy as np
     import torch
     items = c3_items(sg, device, seed)
@@ -283,14 +286,33 @@ def batch_leg(sg, device, steps, seed=3):
         for p, (_, _, x), r in zip(plans, items, refs):
             p.execute(x, out=r)
     t_serial = time_steps(serial, steps, 1)
+    forced = {}
+    for q in (False, True):
+        batch.execute(queues=q)
+        forced[q] = time_steps(lambda: batch.execute(queues=q), steps, 1)
+        forced[q, "err"] = max(float((y - r).norm() / r.norm()) for y, r in zip(outs, refs))
+        if q:
+            qinfo = batch.info()
+    batch.execute()                                  # capture with the timed choice
+    info = batch.info()
     t_graph = time_steps(batch.execute, steps, 1)
     err = max(float((y - r).norm() / r.norm()) for y, r in zip(outs, refs))
     tunes = [sg.tuning(p) for p in plans]
     return {"workload": "48 reconstructions of uneven size (seed 3): 32 fan-beam arc slices "
                         "128-512 px x 180-720 views, 16 FDK volumes 64-192 voxels x 120-360 views",
-            "items": len(items), "updates": updates, "branches": batch.info()["branches"],
+            "items": len(items), "updates": updates, "branches": info["branches"],
+            "chosen_form": "queues" if info["queues"] else "launches",
+            "capture_timed_ms": {"queues": info["timed_ms_queues"],
+                                 "launches": info["timed_ms_launches"]},
             "graph_ms": t_graph * 1e3, "serial_ms": t_serial * 1e3,
             "speedup_vs_serial": t_serial / t_graph,
+            "queue_form": {"ms": forced[True] * 1e3, "speedup_vs_serial": t_serial / forced[True],
+                           "queues": qinfo["queues"], "queued_items": qinfo["queued_items"],
+                           "tasks": qinfo["tasks"],
+                           "max_rel_rmse_vs_unbatched": forced[True, "err"]},
+            "launch_form": {"ms": forced[False] * 1e3,
+                            "speedup_vs_serial": t_serial / forced[False],
+                            "max_rel_rmse_vs_unbatched": forced[False, "err"]},
             "value": updates / t_graph / 1e9, "unit": "Gupdates/s",
             "max_rel_rmse_vs_unbatched": err,
             "cbct_kernels": sorted({t["cbct_kernel"] for t, (rg, _, _) in zip(tunes, items)
@@ -490,6 +512,54 @@ def dd_leg(sg, device, steps):
     return out
 
 
+def learned_leg(sg, device, steps):
+    """The learned sparse-view correction (C4): the committed network (4
+    layers, 32 channels, width 9) on the 512 x 512 problem of 90 short-scan
+    views of the 888-bin flat fan.  Quality on 8 held-out phantoms (PSNR and
+    SSIM of plain and corrected sparse-view FBP against the dense-view FBP),
+    inference time (the fused kernel on one 888 x 90 filtered sinogram, and
+    the corrected reconstruction against the plain one) and training
+    throughput (batch 8, 16 phantoms sampled beforehand, forward, backward
+    and Adam on the device)."""
+    import numpy as np
+    import torch
+    from sinograms_amd import learned
+    path = os.path.join(ROOT, "sinograms.jl_amd", "data", "sparse_view_correction_fan90.npy")
+    params = np.load(path)
+    prob = learned.SparseViewProblem()
+    net = learned.ConvCorrection(layers=4, channels=32)
+    net.set_params(params)
+    log("  quality on 8 held-out phantoms ...")
+    quality = learned.evaluate(prob, net, range(10 ** 6, 10 ** 6 + 8))
+    sino, _ = prob.sample(10 ** 6)
+    q = prob.filter(sino)
+    out = net.forward(q)
+    t_inf = time_steps(lambda: net.forward(q, out=out), steps, 3)
+    t_plain = time_steps(lambda: prob.reconstruct(sino), steps, 3)
+    t_learned = time_steps(lambda: prob.reconstruct(sino, net), steps, 3)
+    nb, na = prob.rg_sparse.dims()
+    flop = 2.0 * nb * na * (9 * 32 + 2 * 9 * 32 * 32 + 9 * 32)
+    log("  training (batch 8) ...")
+    tnet = learned.ConvCorrection(layers=4, channels=32)
+    tnet.set_params(params)
+    hist = learned.train(prob, tnet, range(16), batch=8, lr=1e-4, epochs=2)
+    r = {"workload": "4-layer 32-channel width-9 correction on 90 short-scan views x 888 bins "
+                     "(flat fan, dso 541, dsd 949) -> 512 x 512 at 0.98 mm; committed weights",
+         "psnr_plain_db": quality["psnr_plain"], "psnr_learned_db": quality["psnr_learned"],
+         "ssim_plain": quality["ssim_plain"], "ssim_learned": quality["ssim_learned"],
+         "eval_phantoms": 8,
+         "inference_ms": t_inf * 1e3, "inference_tflops": flop / t_inf / 1e12,
+         "recon_plain_ms": t_plain * 1e3, "recon_learned_ms": t_learned * 1e3,
+         "correction_overhead_ms": (t_learned - t_plain) * 1e3,
+         "train_steps": hist["steps"], "train_batch": 8,
+         "train_ms_per_step": hist["train_s"] / max(hist["steps"], 1) * 1e3,
+         "train_phantoms_per_s": hist["phantoms_per_s"]}
+    log(f"  learned: PSNR {r['psnr_plain_db']:.2f} -> {r['psnr_learned_db']:.2f} dB, inference "
+        f"{r['inference_ms']:.3f} ms, overhead {r['correction_overhead_ms']:.3f} ms, training "
+        f"{r['train_ms_per_step']:.2f} ms per batch-8 step")
+    return r
+
+
 def _slice_cpu(sg, rg, ig, sino, plan, got):
     """Oracle timing (Float32 backprojection of the filtered sinogram) and the
     GPU result's rel-RMSE against the Float32 and Float64 oracles."""
@@ -558,6 +628,8 @@ def main():
                     help="skip the host-buffer (PCIe-inclusive) leg")
     ap.add_argument("--no-slices", action="store_true",
                     help="skip the 2-D legs (parallel-beam and arc fan-beam FBP)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the learned sparse-view correction leg")
     ap.add_argument("--cpu-slices", type=int, default=128)
     ap.add_argument("--cbct-kernel", choices=("staged", "gather"), default="staged",
                     help="cone-beam backprojector (LDS-staged windows or global gathers)")
@@ -666,6 +738,9 @@ def main():
         result["analytic_projector"] = phantom_leg(sg, dev, max(3, min(args.steps, 10)))
         log("timing the filter forms ...")
         result["filter_forms"] = filter_forms_leg(sg, dev, max(3, min(args.steps, 10)))
+    if not args.no_learned:
+        log("learned correction leg ...")
+        result["learned"] = learned_leg(sg, dev, max(3, min(args.steps, 20)))
     if not args.no_batch:
         log("batch leg ...")
         result["batch"] = batch_leg(sg, dev, max(3, min(args.steps, 10)))

@@ -358,23 +358,44 @@ int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);
 /* A batch: many reconstructions (plan, input, output; device pointers) run as
  * one HIP graph.  The first execute after items were added autotunes the
  * plans not yet tuned (through the context cache) and captures every item
- * into the graph, with independent plans on parallel branches (items of one
- * plan stay in order on one branch); later executes replay the graph with
- * the same pointers.  No memset nodes are captured.  A plan change that the
- * graph would not see (sgx_plan_set_cbct_opts, sgx_plan_set_view_weight,
- * sgx_plan_autotune) makes the next execute capture again. */
+ * into the graph; later executes replay the graph with the same pointers.
+ * Backprojections run as queue launches: the filter phases of the queued
+ * items run first on parallel branches, then one launch per backprojector
+ * variant walks a table of (item, pixel tile or voxel block) tasks, items
+ * ordered by views (largest tasks first), one workgroup per task, dispatched
+ * in table order.  Items whose plan
+ * occurs more than once in the batch (they share the plan's filtered
+ * buffer) run as whole plan executes on the parallel branches instead
+ * (independent plans in parallel, items of one plan in order).  No memset nodes are captured.  A plan change
+ * that the graph would not see (sgx_plan_set_cbct_opts,
+ * sgx_plan_set_view_weight, sgx_plan_autotune) makes the next execute
+ * capture again.  The reference's counterpart is the serial slice loop of
+ * fbp(plan, aa), src/fbp2/fbp.jl:46-68. */
 typedef struct sgx_batch sgx_batch;
 int sgx_batch_create(sgx_ctx* ctx, sgx_batch** out);
 void sgx_batch_destroy(sgx_batch* batch);
 int sgx_batch_add(sgx_batch* batch, sgx_plan* plan, const float* sino, float* image);
 /* flags: SGX_ASYNC returns without waiting for the graph;
- * SGX_BATCH_NO_TUNE keeps the plans' current shapes. */
+ * SGX_BATCH_NO_TUNE keeps the plans' current shapes.  By default a capture
+ * times one replay of the graph with queue launches and one of the graph of
+ * whole plan executes (no queues) and keeps the faster (sgx_batch_choice);
+ * SGX_BATCH_QUEUE or SGX_BATCH_NO_QUEUE takes that form without timing.  A
+ * change of these flags between executes captures again. */
 #define SGX_BATCH_NO_TUNE 0x100u
+#define SGX_BATCH_NO_QUEUE 0x200u
+#define SGX_BATCH_QUEUE 0x400u
 int sgx_batch_execute(sgx_batch* batch, uint32_t flags);
 /* Milliseconds of the last execute's graph launch (HIP events). */
 int sgx_batch_last_ms(sgx_batch* batch, float* ms);
 /* Number of items and of parallel branches of the captured graph. */
 int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches);
+/* The captured graph's queues: their number, the items they hold and their
+ * tasks (all 0 without queues). */
+int sgx_batch_queues(const sgx_batch* batch, int64_t* queues, int64_t* queued_items,
+                     int64_t* tasks);
+/* The timed choice of the last capture: milliseconds of a replay with queue
+ * launches and without (-1 when not timed). */
+int sgx_batch_choice(const sgx_batch* batch, float* ms_queues, float* ms_launches);
 
 /* ---- learned sparse-view correction (no counterpart in the reference) -----
  * A 1-D convolutional network applied along detector rows of the filtered

@@ -149,38 +149,39 @@ __device__ __forceinline__ int cvt_floor(float x)
 // GEOM: 0 parallel, 1 fan arc, 2 fan flat, 3 fan arc whose detector spans
 // less than +-45 degrees (atan2_narrow), 4 fan arc in the tile-relative form
 // with NT series terms (NT is used by GEOM 4 only)
-template <int GEOM, int NT = 1>
-__global__ __launch_bounds__(kThreads) void back2d_kernel(
+// One block's work: pixel tile `tile` over view group `group` (views
+// group * views_per_group ..), with the block's dynamic LDS `lds`.
+template <int GEOM, int NT>
+__device__ __forceinline__ void back2d_tile(
     const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ xds,
     const float* __restrict__ yds, int nx, int ny, const uint8_t* __restrict__ mask,
     float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, int wshift,
-    float wb, float dsd_ds, float dso_ds, float so_ds)
+    float wb, float dsd_ds, float dso_ds, float so_ds, int tile, int group, float* lds)
 {
     // vc window slots of wm bins, then per view int2 {lo + 1, n - 1}: the
     // window holds 0-based bins lo .. lo + n - 1 (n = 0: nothing staged),
     // then (GEOM 4) per view a RelView
-    extern __shared__ float lds[];
     constexpr bool REL = GEOM == 4;
     const int wm = 1 << wshift;
     int2* win = reinterpret_cast<int2*>(lds + vc * wm);
     RelView* rel = reinterpret_cast<RelView*>(win + vc);
     const int tid = threadIdx.x;
     const int tiles_x = (nx + kTile - 1) / kTile;
-    const int tx0 = (blockIdx.x % tiles_x) * kTile, ty0 = (blockIdx.x / tiles_x) * kTile;
+    const int tx0 = (tile % tiles_x) * kTile, ty0 = (tile / tiles_x) * kTile;
     const int ix = tx0 + (tid % kTile);
     const int iy = ty0 + (tid / kTile);
     const int64_t pix = ix + (int64_t)nx * iy;
     const bool inside = ix < nx && iy < ny;
     const bool active = inside && (mask == nullptr || mask[pix]);
-    float* dst = out + out_group_stride * blockIdx.y;
+    float* dst = out + out_group_stride * group;
 
     if (!__syncthreads_or(active)) {          // whole tile outside the mask
         if (inside) dst[pix] = 0.f;
         return;
     }
 
-    const int v_begin = blockIdx.y * views_per_group;
+    const int v_begin = group * views_per_group;
     const int v_end = min(nviews, v_begin + views_per_group);
     const float x = inside ? xds[ix] : 0.f;
     const float y = inside ? yds[iy] : 0.f;
@@ -403,6 +404,37 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
     if (inside) dst[pix] = active ? acc : 0.f;
 }
 
+template <int GEOM, int NT = 1>
+__global__ __launch_bounds__(kThreads) void back2d_kernel(
+    const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ xds,
+    const float* __restrict__ yds, int nx, int ny, const uint8_t* __restrict__ mask,
+    float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, int wshift,
+    float wb, float dsd_ds, float dso_ds, float so_ds)
+{
+    extern __shared__ float lds[];
+    back2d_tile<GEOM, NT>(q, nb, nviews, view_stride, cs, xds, yds, nx, ny, mask, out,
+                          out_group_stride, views_per_group, vc, wshift, wb, dsd_ds, dso_ds,
+                          so_ds, blockIdx.x, blockIdx.y, lds);
+}
+
+// Batch form: one launch over the tasks (item, pixel tile) of many
+// reconstructions; block b does task b of the table (ordered by the host,
+// largest tasks first, interleaved over the XCDs); a task backprojects
+// every view of its tile (no view groups) into the item's image.
+template <int GEOM, int NT>
+__global__ __launch_bounds__(kThreads) void back2d_queue_kernel(
+    const Back2dItem* __restrict__ items, const int2* __restrict__ tasks)
+{
+    extern __shared__ float lds[];
+    const int2 tk = tasks[blockIdx.x];
+    using CItem = const __attribute__((address_space(4))) Back2dItem;
+    const CItem& it = ((CItem*)(items))[tk.x];
+    back2d_tile<GEOM, NT>(it.q, it.nb, it.nviews, it.view_stride, it.cs, it.xds, it.yds, it.nx,
+                          it.ny, it.mask, it.image, 0, it.nviews, it.vc, it.wshift, it.wb,
+                          it.dsd_ds, it.dso_ds, it.so_ds, tk.y, 0, lds);
+}
+
 // Exact transpose of back2d_kernel:
 //   qt[b, v] = sum over active pixels p of img[p] w2 ((1 - wr) [b = il - 1] + wr [b = il])
 // Block (view v, pixel group g): bins accumulate in LDS (atomics), then are
@@ -500,6 +532,96 @@ int back2d_rel_terms(const Back2dArgs& a)
     return 0;
 }
 
+// Kernel variant of a 2-D backprojection: GEOM * 16 + NT.
+int back2d_variant(const Back2dArgs& a)
+{
+    switch (a.geom) {
+    case 0:
+        return 0 * 16 + 1;
+    case 1: {
+        const int nt = back2d_rel_terms(a);
+        if (nt > 0) return 4 * 16 + nt;
+        // bins 1 <= bb < nb are at |gamma| = |bb - wb| / dsd_ds; below 0.75 rad
+        // (43 degrees) on both sides the narrow arctangent serves
+        if (std::max(std::fabs(1.f - a.wb), std::fabs((float)a.nb - a.wb)) < 0.75f * a.dsd_ds)
+            return 3 * 16 + 1;
+        return 1 * 16 + 1;
+    }
+    default:
+        return 2 * 16 + 1;
+    }
+}
+
+Back2dItem back2d_item(const Back2dArgs& a)
+{
+    Back2dItem it{};
+    it.q = a.q;
+    it.cs = a.cs;
+    it.xds = a.xds;
+    it.yds = a.yds;
+    it.mask = a.mask;
+    it.image = a.image;
+    it.view_stride = a.view_stride;
+    it.nb = a.nb;
+    it.nviews = a.nviews;
+    it.nx = a.nx;
+    it.ny = a.ny;
+    it.vc = chunk_views(a.nb, 64);
+    const int wm = window_bins(a.nb);
+    int wshift = 0;
+    while ((1 << wshift) < wm) ++wshift;
+    it.wshift = wshift;
+    it.wb = a.wb;
+    it.dsd_ds = a.dsd_ds;
+    it.dso_ds = a.dso_ds;
+    it.so_ds = a.so_ds;
+    return it;
+}
+
+int back2d_item_tiles_x(const Back2dItem& it) { return (it.nx + kTile - 1) / kTile; }
+
+int back2d_item_tiles(const Back2dItem& it)
+{
+    return ((it.nx + kTile - 1) / kTile) * ((it.ny + kTile - 1) / kTile);
+}
+
+size_t back2d_item_lds(const Back2dItem& it)
+{
+    return (size_t)it.vc * ((sizeof(float) << it.wshift) + sizeof(int2) + sizeof(RelView));
+}
+
+template <int GEOM, int NT>
+static const void* b2q_fn()
+{
+    return reinterpret_cast<const void*>(&back2d_queue_kernel<GEOM, NT>);
+}
+
+static const void* back2d_queue_fn(int variant)
+{
+    switch (variant) {
+    case 0 * 16 + 1: return b2q_fn<0, 1>();
+    case 1 * 16 + 1: return b2q_fn<1, 1>();
+    case 2 * 16 + 1: return b2q_fn<2, 1>();
+    case 3 * 16 + 1: return b2q_fn<3, 1>();
+    case 4 * 16 + 3: return b2q_fn<4, 3>();
+    case 4 * 16 + 4: return b2q_fn<4, 4>();
+    case 4 * 16 + 5: return b2q_fn<4, 5>();
+    case 4 * 16 + 6: return b2q_fn<4, 6>();
+    case 4 * 16 + 8: return b2q_fn<4, 8>();
+    default: throw Error(SGX_ERR_ARG, "back2d: unknown kernel variant");
+    }
+}
+
+void launch_back2d_queue(int variant, const Back2dItem* items, const int2* tasks, int ntasks,
+                         size_t lds, hipStream_t st)
+{
+    if (ntasks <= 0) return;
+    if (lds > (size_t)kLdsBytes) throw Error(SGX_ERR_ARG, "back2d queue: LDS");
+    void* args[] = {&items, &tasks};
+    SGX_HIP(hipLaunchKernel(back2d_queue_fn(variant), dim3((unsigned)ntasks), dim3(kThreads), args,
+                            lds, st));
+}
+
 void launch_back2d(const Back2dArgs& a, hipStream_t st)
 {
     if (a.want_blocks < 1 || a.vc_max < 1) throw Error(SGX_ERR_ARG, "back2d: bad shape");
@@ -522,30 +644,17 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
     hipLaunchKernelGGL((back2d_kernel<G, NT>), grid, block, lds, st, a.q, a.nb, a.nviews,       \
                        a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg, vc, \
                        wshift, a.wb, a.dsd_ds, a.dso_ds, a.so_ds)
-    switch (a.geom) {
-    case 0:
-        SGX_B2D(0, 1);
-        break;
-    case 1:
-        switch (back2d_rel_terms(a)) {
-        case 3: SGX_B2D(4, 3); break;
-        case 4: SGX_B2D(4, 4); break;
-        case 5: SGX_B2D(4, 5); break;
-        case 6: SGX_B2D(4, 6); break;
-        case 8: SGX_B2D(4, 8); break;
-        default:
-            // bins 1 <= bb < nb are at |gamma| = |bb - wb| / dsd_ds; below 0.75 rad
-            // (43 degrees) on both sides the narrow arctangent serves
-            if (std::max(std::fabs(1.f - a.wb), std::fabs((float)a.nb - a.wb)) < 0.75f * a.dsd_ds)
-                SGX_B2D(3, 1);
-            else
-                SGX_B2D(1, 1);
-            break;
-        }
-        break;
-    default:
-        SGX_B2D(2, 1);
-        break;
+    switch (back2d_variant(a)) {
+    case 0 * 16 + 1: SGX_B2D(0, 1); break;
+    case 1 * 16 + 1: SGX_B2D(1, 1); break;
+    case 2 * 16 + 1: SGX_B2D(2, 1); break;
+    case 3 * 16 + 1: SGX_B2D(3, 1); break;
+    case 4 * 16 + 3: SGX_B2D(4, 3); break;
+    case 4 * 16 + 4: SGX_B2D(4, 4); break;
+    case 4 * 16 + 5: SGX_B2D(4, 5); break;
+    case 4 * 16 + 6: SGX_B2D(4, 6); break;
+    case 4 * 16 + 8: SGX_B2D(4, 8); break;
+    default: throw Error(SGX_ERR_ARG, "back2d: unknown kernel variant");
     }
 #undef SGX_B2D
     SGX_HIP(hipGetLastError());

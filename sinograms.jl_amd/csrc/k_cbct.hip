@@ -234,22 +234,26 @@ __device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[
 // ---------------------------------------------------------------------------
 
 constexpr int kWaves = 4;      // waves per block, each on its own z-slab
+constexpr int kGatherVariant = 16;   // cbct_variant: the gather kernel
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
+// One block of the gather kernel: block `blk` (8 x 8 columns x 4 waves of
+// ZB z).
 template <int ZB, bool ARC, int FMT, bool OPT>
-__global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
+__device__ __forceinline__ void cbct_gather_block(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
-    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow, float so_ds)
+    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow, float so_ds,
+    int blk)
 {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int tiles_x = (nx + kPatch - 1) / kPatch;
     const int tiles = tiles_x * ((ny + kPatch - 1) / kPatch);
-    const int tile = blockIdx.x % tiles, zblk = blockIdx.x / tiles;
+    const int tile = blk % tiles, zblk = blk / tiles;
     const int ix = (tile % tiles_x) * kPatch + (lane & (kPatch - 1));
     const int iy = (tile / tiles_x) * kPatch + (lane / kPatch);
     const int iz0 = (zblk * kWaves + wave) * ZB;
@@ -297,6 +301,21 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
         if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
 }
 
+template <int ZB, bool ARC, int FMT, bool OPT>
+__global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds, const float* __restrict__ zs, int arc_t_exact, int narrow, float so_ds)
+{
+    cbct_gather_block<ZB, ARC, FMT, OPT>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds,
+                                         yds, zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws,
+                                         wt, inv_dt_ds, zs, arc_t_exact, narrow, so_ds,
+                                         blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // staged kernel
 // ---------------------------------------------------------------------------
@@ -313,6 +332,9 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #ifndef SGX_STAGED_WAVES
 #define SGX_STAGED_WAVES 3
 #endif
+#ifndef SGX_STAGED_PITCH16
+#define SGX_STAGED_PITCH16 52
+#endif
 #ifndef SGX_STAGED_PITCH
 #define SGX_STAGED_PITCH 58
 #endif
@@ -321,7 +343,10 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #endif
 constexpr int kSP = 16;               // block patch: 16 x 16 columns (2 x 2 waves of 8 x 8)
 constexpr int kSZ = 16;               // voxels per lane along z
-constexpr int kWmax = 48;             // staged window: bins (12 quads of 4) ...
+#ifndef SGX_STAGED_WMAX
+#define SGX_STAGED_WMAX 48
+#endif
+constexpr int kWmax = SGX_STAGED_WMAX; // staged window: bins (quads of 4) ...
 constexpr int kHmax = 32;             // ... x rows (16 groups of 2)
 constexpr int kSB = SGX_STAGED_BATCH;
 
@@ -333,13 +358,14 @@ constexpr int kSB = SGX_STAGED_BATCH;
 template <int FMT> struct Lds {
     static constexpr bool half = FMT == F_F16S;
     static constexpr uint32_t esz = half ? 4 : 8;
-    static constexpr int pitch = half ? 52 : SGX_STAGED_PITCH;
+    static constexpr int pitch = half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH;
     static constexpr int buf = kHmax * pitch;                 // entries per buffer
     static_assert(pitch >= kWmax + 2 && (pitch * esz) % 16 == 0, "pitch");
 };
 static_assert(kSZ % kSB == 0, "batch divides the voxels per lane");
 constexpr int kQuads = kWmax / 4;
-constexpr int kStagers = kQuads * (kHmax / 2);   // 192 threads stage
+constexpr int kStagers = kQuads * (kHmax / 2);   // threads that stage
+static_assert(kStagers <= 256, "one staging thread per quad and row pair");
 
 // Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
 // stored as int4 {clo | interior, rlo, W, H}.  clo is a multiple of 4;
@@ -427,25 +453,24 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     win[i] = w;
 }
 
+// One block's work: voxel block `blk` (16 x 16 columns x 16 z), all views,
+// with the block's LDS `lds` (2 * Lds<FMT>::buf entries).
 template <bool ARC, int FMT, bool OPT>
-__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
-cbct_staged_kernel(
+__device__ __forceinline__ void cbct_staged_block(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
-    int arc_t_exact, int narrow, float so_ds)
+    int arc_t_exact, int narrow, float so_ds, int blk, char* lds)
 {
     using L = Lds<FMT>;
     constexpr int kLP = L::pitch;
-    __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (nx + kSP - 1) / kSP;
     const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
-    const int blk = blockIdx.x;
     const int tile = blk % tiles, zblk = blk / tiles;
     const int ix = (tile % tiles_x) * kSP + (wave & 1) * 8 + (lane & 7);
     const int iy = (tile / tiles_x) * kSP + (wave >> 1) * 8 + (lane >> 3);
@@ -668,6 +693,48 @@ cbct_staged_kernel(
     }
 }
 
+template <bool ARC, int FMT, bool OPT>
+__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
+cbct_staged_kernel(
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
+    int arc_t_exact, int narrow, float so_ds)
+{
+    using L = Lds<FMT>;
+    __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
+    cbct_staged_block<ARC, FMT, OPT>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds, yds,
+                                     zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws, wt,
+                                     inv_dt_ds, win, zs, arc_t_exact, narrow, so_ds, blockIdx.x,
+                                     lds);
+}
+
+// Batch form: one launch over the tasks (item, voxel block) of many
+// reconstructions; block b does task b of the table, which the host orders
+// (largest tasks first, interleaved so that block b % 8 keeps to one XCD's
+// share).  The hardware dispatches blocks in index order, so the launch
+// walks the table in order and backfills the GPU as blocks finish.
+template <bool ARC, int FMT, bool OPT>
+__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
+cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
+{
+    using L = Lds<FMT>;
+    __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
+    const int2 tk = tasks[blockIdx.x];
+    // through the constant address space: invariant loads, as for kernel
+    // arguments
+    using CItem = const __attribute__((address_space(4))) CbctItem;
+    const CItem& it = ((CItem*)(items))[tk.x];
+    cbct_staged_block<ARC, FMT, OPT>(it.q, it.ns, it.nt, it.pitch, it.nviews, it.view_stride,
+                                     it.cs, it.vscale, it.xds, it.yds, it.zds, it.nx, it.ny, it.nz,
+                                     it.mask, it.image, it.dsd_ds, it.dso_ds, it.ws, it.wt,
+                                     it.inv_dt_ds, it.win, it.zs, it.arc_t_exact, it.narrow,
+                                     it.so_ds, tk.y, lds);
+}
+
 // An arc detector whose bins (1 <= bs < ns, |gamma| = |bs - ws| / dsd_ds)
 // span less than 0.75 rad (43 degrees) on both sides takes atan2_narrow.
 int arc_narrow(const CbctArgs& a)
@@ -768,6 +835,111 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
                        a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, a.zs,
                        a.arc_t_exact ? 1 : 0, a.so_ds, win);
     SGX_HIP(hipGetLastError());
+}
+
+int cbct_variant(const CbctArgs& a)
+{
+    const int fmt = a.fmt == SGX_Q_FMT_BF16 ? F_BF16 : a.fmt == SGX_Q_FMT_F16S ? F_F16S : F_F32;
+    const bool opt = a.zs || a.arc_t_exact || a.so_ds != 0.f;
+    return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant);
+}
+
+CbctItem cbct_item(const CbctArgs& a)
+{
+    CbctItem it{};
+    it.q = a.q;
+    it.cs = a.cs;
+    it.vscale = a.vscale;
+    it.xds = a.xds;
+    it.yds = a.yds;
+    it.zds = a.zds;
+    it.mask = a.mask;
+    it.image = a.image;
+    it.win = a.win;
+    it.zs = a.zs;
+    it.view_stride = a.view_stride;
+    it.ns = a.ns;
+    it.nt = a.nt;
+    it.pitch = a.pitch;
+    it.nviews = a.nviews;
+    it.nx = a.nx;
+    it.ny = a.ny;
+    it.nz = a.nz;
+    it.arc_t_exact = a.arc_t_exact ? 1 : 0;
+    it.narrow = arc_narrow(a);
+    it.dsd_ds = a.dsd_ds;
+    it.dso_ds = a.dso_ds;
+    it.ws = a.ws;
+    it.wt = a.wt;
+    it.inv_dt_ds = a.inv_dt_ds;
+    it.so_ds = a.so_ds;
+    return it;
+}
+
+int64_t cbct_item_blocks(const CbctArgs& a)
+{
+    if (a.win) return staged_blocks(a);
+    const int64_t tiles = (int64_t)((a.nx + kPatch - 1) / kPatch) * ((a.ny + kPatch - 1) / kPatch);
+    return tiles * ((a.nz + 8 * kWaves - 1) / (8 * kWaves));
+}
+
+
+// Batch form of the gather kernel (as cbct_queue_kernel).
+template <bool ARC, int FMT, bool OPT>
+__global__ __launch_bounds__(64 * kWaves) void cbct_gather_queue_kernel(
+    const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
+{
+    const int2 tk = tasks[blockIdx.x];
+    using CItem = const __attribute__((address_space(4))) CbctItem;
+    const CItem& it = ((CItem*)(items))[tk.x];
+    cbct_gather_block<8, ARC, FMT, OPT>(it.q, it.ns, it.nt, it.pitch, it.nviews, it.view_stride,
+                                        it.cs, it.vscale, it.xds, it.yds, it.zds, it.nx, it.ny,
+                                        it.nz, it.mask, it.image, it.dsd_ds, it.dso_ds, it.ws,
+                                        it.wt, it.inv_dt_ds, it.zs, it.arc_t_exact, it.narrow,
+                                        it.so_ds, tk.y);
+}
+
+template <bool ARC, int FMT, bool OPT>
+static const void* cbq_fn()
+{
+    return reinterpret_cast<const void*>(&cbct_queue_kernel<ARC, FMT, OPT>);
+}
+
+template <bool ARC, int FMT, bool OPT>
+static const void* cbg_fn()
+{
+    return reinterpret_cast<const void*>(&cbct_gather_queue_kernel<ARC, FMT, OPT>);
+}
+
+static const void* cbct_queue_fn(int variant)
+{
+    const bool arc = variant & 1, opt = variant & 2;
+    const int fmt = (variant >> 2) & 3;
+    if (variant & kGatherVariant) {
+#define SGX_CBG(F)                                                                   \
+    return arc ? (opt ? cbg_fn<true, F, true>() : cbg_fn<true, F, false>())          \
+               : (opt ? cbg_fn<false, F, true>() : cbg_fn<false, F, false>())
+        if (fmt == F_BF16) SGX_CBG(F_BF16);
+        if (fmt == F_F16S) SGX_CBG(F_F16S);
+        SGX_CBG(F_F32);
+#undef SGX_CBG
+    }
+#define SGX_CBQ(F)                                                                   \
+    return arc ? (opt ? cbq_fn<true, F, true>() : cbq_fn<true, F, false>())          \
+               : (opt ? cbq_fn<false, F, true>() : cbq_fn<false, F, false>())
+    if (fmt == F_BF16) SGX_CBQ(F_BF16);
+    if (fmt == F_F16S) SGX_CBQ(F_F16S);
+    SGX_CBQ(F_F32);
+#undef SGX_CBQ
+}
+
+void launch_cbct_queue(int variant, const CbctItem* items, const int2* tasks, int ntasks,
+                       hipStream_t st)
+{
+    if (ntasks <= 0) return;
+    void* args[] = {&items, &tasks};
+    SGX_HIP(hipLaunchKernel(cbct_queue_fn(variant), dim3((unsigned)ntasks), dim3(64 * kWaves),
+                            args, 0, st));
 }
 
 void launch_cbct_back(const CbctArgs& a, hipStream_t st)

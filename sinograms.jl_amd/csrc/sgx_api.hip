@@ -968,44 +968,56 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
 // One plan execute on stream `st` (device pointers): weights + filter, then
 // backprojection, with the plan's kernel shapes.  `timed` brackets the two
 // phases with the plan's events (left out under graph capture).
-void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool timed)
+// The weights + filter phase of a plan execute (into the plan's q).
+void plan_filter_phase(sgx_plan& P, const float* in, hipStream_t st)
 {
-    const sgx_geom& G = P.geom;
-    if (timed) SGX_HIP(hipEventRecord(P.ev[0], st));
     const sgx::FilterArgs fa = P.filter_args(in, P.q);
     if (P.prec == SGX_PREC_F16S) sgx::launch_filter_bound(fa, P.vbound, st);
     sgx::launch_sino_filter(fa, st);
+}
+
+// Backprojection arguments of a 2-D plan execute into img.
+sgx::Back2dArgs back2d_run_args(const sgx_plan& P, float* img)
+{
+    const sgx_geom& G = P.geom;
+    sgx::Back2dArgs a{};
+    a.q = static_cast<const float*>(P.q);
+    a.nb = (int)G.ns;
+    a.nviews = (int)G.na;
+    a.view_stride = G.ns;
+    a.cs = P.cs;
+    a.xds = P.xds;
+    a.yds = P.yds;
+    a.nx = (int)P.nx;
+    a.ny = (int)P.ny;
+    a.mask = P.mask;
+    a.image = img;
+    a.geom = (G.kind == SGX_SINO_PAR || G.kind == SGX_SINO_MOJ) ? 0
+                                                              : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
+    a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
+    a.dsd_ds = (float)G.dsd / (float)G.ds;
+    a.dso_ds = (float)G.dso / (float)G.ds;
+    a.so_ds = (float)G.source_offset / (float)G.ds;
+    a.partial = P.partial;
+    a.partial_cap = P.partial_cap;
+    a.want_blocks = P.tune.want_blocks;
+    a.vc_max = P.tune.vc_max;
+    a.rel_rmax = P.rel_rmax;
+    a.rel_h = P.rel_h;
+    return a;
+}
+
+void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool timed)
+{
+    if (timed) SGX_HIP(hipEventRecord(P.ev[0], st));
+    plan_filter_phase(P, in, st);
     if (timed) SGX_HIP(hipEventRecord(P.ev[1], st));
     if (P.is3d()) {
         sgx::CbctArgs a = cbct_args_of(P, img);
         if (P.tune.cbct == 1) a.win = nullptr;
         sgx::launch_cbct_back(a, st);
     } else {
-        sgx::Back2dArgs a{};
-        a.q = static_cast<const float*>(P.q);
-        a.nb = (int)G.ns;
-        a.nviews = (int)G.na;
-        a.view_stride = G.ns;
-        a.cs = P.cs;
-        a.xds = P.xds;
-        a.yds = P.yds;
-        a.nx = (int)P.nx;
-        a.ny = (int)P.ny;
-        a.mask = P.mask;
-        a.image = img;
-        a.geom = (G.kind == SGX_SINO_PAR || G.kind == SGX_SINO_MOJ) ? 0
-                                                                  : (G.kind == SGX_SINO_FAN_ARC ? 1 : 2);
-        a.wb = (float)((G.ns + 1) / 2.0 + (double)(float)G.offset_s);
-        a.dsd_ds = (float)G.dsd / (float)G.ds;
-        a.dso_ds = (float)G.dso / (float)G.ds;
-        a.so_ds = (float)G.source_offset / (float)G.ds;
-        a.partial = P.partial;
-        a.partial_cap = P.partial_cap;
-        a.want_blocks = P.tune.want_blocks;
-        a.vc_max = P.tune.vc_max;
-        a.rel_rmax = P.rel_rmax;
-        a.rel_h = P.rel_h;
-        sgx::launch_back2d(a, st);
+        sgx::launch_back2d(back2d_run_args(P, img), st);
     }
     if (timed) {
         SGX_HIP(hipEventRecord(P.ev[2], st));
@@ -1482,18 +1494,36 @@ struct sgx_batch {
         const float* in;
         float* out;
     };
+    // A backprojection queue: kind 2 (2-D) or 3 (cone beam), the kernel
+    // variant, and its items and task table in the batch's device buffer.
+    struct Queue {
+        int kind = 0, variant = 0;
+        std::vector<int> items;              // batch item indices, queue order
+        size_t items_off = 0, tasks_off = 0;
+        int ntasks = 0;
+        size_t lds = 0;
+    };
     sgx_ctx* ctx = nullptr;
     std::vector<Item> items;
     std::vector<uint64_t> gens;              // each item's plan generation at capture
     std::vector<hipStream_t> branches;       // capture streams (default priority)
     std::vector<hipEvent_t> joins;
+    std::vector<hipEvent_t> filtered;        // per (queue, branch): that queue's items on
+                                             // the branch are filtered
     hipStream_t origin = nullptr;            // capture origin
     hipEvent_t fork = nullptr, t0 = nullptr, t1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
     bool launched = false;
+    bool queued = true;                      // the captured graph uses queues
+    int mode = -1;                           // requested: 0 no queues, 1 queues, 2 timed choice
+    float ms_queues = -1.f, ms_launches = -1.f;   // the timed choice's replays
     int64_t nbranch = 0;
+    std::vector<Queue> queues;
+    bool gather_queues = false;              // queue gather-tuned cone-beam items too
+    void* qbuf = nullptr;                    // queue items and task tables
+    size_t qcap = 0;
 
     void drop_graph()
     {
@@ -1506,18 +1536,169 @@ struct sgx_batch {
     {
         drop_graph();
         for (hipEvent_t e : joins) (void)hipEventDestroy(e);
+        for (hipEvent_t e : filtered) (void)hipEventDestroy(e);
         for (hipStream_t s : branches) (void)hipStreamDestroy(s);
         for (hipEvent_t e : {fork, t0, t1})
             if (e) (void)hipEventDestroy(e);
         if (origin) (void)hipStreamDestroy(origin);
+        if (qbuf) (void)hipFree(qbuf);
     }
 
-    // Capture every item: plans are spread over at most kBranches parallel
-    // branches, largest work first onto the least loaded branch; the items of
-    // one plan stay on one branch, in the order they were added.
-    void capture()
+    void ensure_streams(size_t n)
+    {
+        while (branches.size() < n) {
+            hipStream_t s;
+            SGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+            branches.push_back(s);
+            hipEvent_t e;
+            SGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            joins.push_back(e);
+        }
+    }
+    void ensure_filter_events(size_t n)
+    {
+        while (filtered.size() < n) {
+            hipEvent_t f;
+            SGX_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+            filtered.push_back(f);
+        }
+    }
+
+    // Sort the items into backprojection queues (use_queues) or
+    // none.  An item goes to a queue when its plan occurs once in the batch
+    // (the filter phases of all queued items run before any queue, and a
+    // plan's items share its filtered buffer); the others run as whole plan
+    // executes.  A queue orders its items by the cost of one task
+    // (views), largest first, and each item's tasks in block order.
+    void build_queues(bool use_queues)
+    {
+        queues.clear();
+        if (!use_queues) return;
+        std::map<sgx_plan*, int> count;
+        for (const Item& it : items) count[it.plan] += 1;
+        std::map<std::pair<int, int>, size_t> qindex;
+        for (int i = 0; i < (int)items.size(); ++i) {
+            const Item& it = items[i];
+            sgx_plan& P = *it.plan;
+            if (count[it.plan] > 1) continue;
+            int kind, variant;
+            if (P.is3d()) {
+                // gather-tuned items (coarse voxels, the LDS windows miss)
+                // stay whole executes: their filter and backprojection overlap
+                // the other branches' work better than a queue that waits
+                // for every filter of its chunk
+                if (P.tune.cbct == 1 && !gather_queues) continue;
+                sgx::CbctArgs a = cbct_args_of(P, it.out);
+                if (P.tune.cbct == 1) a.win = nullptr;
+                kind = 3;
+                variant = sgx::cbct_variant(a);
+            } else {
+                sgx::Back2dArgs a = back2d_run_args(P, it.out);
+                a.vc_max = 64;
+                kind = 2;
+                variant = sgx::back2d_variant(a);
+            }
+            auto key = std::make_pair(kind, variant);
+            if (!qindex.count(key)) {
+                qindex[key] = queues.size();
+                queues.push_back(Queue{});
+                queues.back().kind = kind;
+                queues.back().variant = variant;
+            }
+            queues[qindex[key]].items.push_back(i);
+        }
+        // host images of the device buffer: per queue items then tasks, then
+        // one counter per queue
+        std::vector<unsigned char> host;
+        auto put = [&](const void* p, size_t n) {
+            const size_t off = (host.size() + 15) & ~size_t(15);
+            host.resize(off + n);
+            std::memcpy(host.data() + off, p, n);
+            return off;
+        };
+        // Items ordered by task cost (views), largest first; then each
+        // variant's items are cut into chunks of kChunkItems, one queue
+        // launch each, so that a chunk's backprojection overlaps the filter
+        // phases of the next.  2-D queues first: their filters are short.
+        constexpr size_t kChunkItems = 4;
+        auto cost = [&](int i) {
+            const sgx_plan& P = *items[i].plan;
+            return std::make_pair(P.geom.na, P.nx * P.ny * P.nz);
+        };
+        {
+            std::vector<Queue> chunked;
+            for (Queue& Q : queues) {
+                std::stable_sort(Q.items.begin(), Q.items.end(),
+                                 [&](int x, int y) { return cost(x) > cost(y); });
+                for (size_t c = 0; c < Q.items.size(); c += kChunkItems) {
+                    Queue C;
+                    C.kind = Q.kind;
+                    C.variant = Q.variant;
+                    C.items.assign(Q.items.begin() + c,
+                                   Q.items.begin() + std::min(Q.items.size(), c + kChunkItems));
+                    chunked.push_back(std::move(C));
+                }
+            }
+            queues.swap(chunked);
+        }
+        std::stable_sort(queues.begin(), queues.end(),
+                         [](const Queue& a, const Queue& b) { return a.kind < b.kind; });
+        for (Queue& Q : queues) {
+            // the items' tasks one item after another, each in block order
+            // (as a launch of the item alone would dispatch them)
+            std::vector<int2> tasks;
+            if (Q.kind == 3) {
+                std::vector<sgx::CbctItem> its;
+                for (int k = 0; k < (int)Q.items.size(); ++k) {
+                    const Item& it = items[Q.items[k]];
+                    sgx::CbctArgs a = cbct_args_of(*it.plan, it.out);
+                    if (it.plan->tune.cbct == 1) a.win = nullptr;
+                    its.push_back(sgx::cbct_item(a));
+                    const int64_t nb = sgx::cbct_item_blocks(a);
+                    for (int64_t b = 0; b < nb; ++b) tasks.push_back(make_int2(k, (int)b));
+                }
+                Q.items_off = put(its.data(), its.size() * sizeof(sgx::CbctItem));
+            } else {
+                std::vector<sgx::Back2dItem> its;
+                for (int k = 0; k < (int)Q.items.size(); ++k) {
+                    const Item& it = items[Q.items[k]];
+                    its.push_back(sgx::back2d_item(back2d_run_args(*it.plan, it.out)));
+                    Q.lds = std::max(Q.lds, sgx::back2d_item_lds(its.back()));
+                    const int nt = sgx::back2d_item_tiles(its.back());
+                    for (int b = 0; b < nt; ++b) tasks.push_back(make_int2(k, b));
+                }
+                Q.items_off = put(its.data(), its.size() * sizeof(sgx::Back2dItem));
+            }
+            require(tasks.size() < (size_t(1) << 31), SGX_ERR_UNSUPPORTED, "batch: too many tasks");
+            Q.ntasks = (int)tasks.size();
+            Q.tasks_off = put(tasks.data(), tasks.size() * sizeof(int2));
+        }
+        if (qcap < host.size()) {
+            SGX_HIP(hipStreamSynchronize(ctx->stream));
+            if (qbuf) SGX_HIP(hipFree(qbuf));
+            qbuf = nullptr;
+            qcap = 0;
+            SGX_HIP(hipMalloc(&qbuf, host.size()));
+            qcap = host.size();
+        }
+        if (queues.empty()) return;
+        SGX_HIP(hipMemcpy(qbuf, host.data(), host.size(), hipMemcpyHostToDevice));
+    }
+
+    // Capture every item.  Branches (at most kBranches) take the plans,
+    // largest work first onto the least loaded branch; the items of one plan
+    // stay on one branch, in the order they were added.  On each branch the
+    // filter phases of its queued items run first, queue by queue, then its
+    // whole plan executes.  Each queue runs as one launch (one block per
+    // task of its table) on a stream of its own, once its own items are
+    // filtered.
+    void capture(bool use_queues)
     {
         constexpr int kBranches = 4;
+        build_queues(use_queues);
+        std::vector<bool> in_queue(items.size(), false);
+        for (const Queue& Q : queues)
+            for (int i : Q.items) in_queue[i] = true;
         std::vector<sgx_plan*> plans;
         std::map<sgx_plan*, double> work;
         for (const Item& it : items) {
@@ -1535,22 +1716,45 @@ struct sgx_batch {
             branch_of[p] = b;
             load[b] += work[p];
         }
-        while ((int)branches.size() < nb) {
-            hipStream_t s;
-            SGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-            branches.push_back(s);
-            hipEvent_t e;
-            SGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-            joins.push_back(e);
-        }
+        const int nq = (int)queues.size();
+        ensure_streams((size_t)(nb + nq));
+        ensure_filter_events((size_t)(nb * nq));
         drop_graph();
         SGX_HIP(hipStreamBeginCapture(origin, hipStreamCaptureModeThreadLocal));
         try {
             SGX_HIP(hipEventRecord(fork, origin));
             for (int b = 0; b < nb; ++b) SGX_HIP(hipStreamWaitEvent(branches[b], fork, 0));
-            for (const Item& it : items)
-                plan_run(*it.plan, it.in, it.out, branches[branch_of[it.plan]], false);
-            for (int b = 0; b < nb; ++b) {
+            std::vector<std::vector<bool>> used(nq, std::vector<bool>(nb, false));
+            for (int j = 0; j < nq; ++j) {
+                for (int i : queues[j].items) {
+                    const int b = branch_of[items[i].plan];
+                    plan_filter_phase(*items[i].plan, items[i].in, branches[b]);
+                    used[j][b] = true;
+                }
+                for (int b = 0; b < nb; ++b)
+                    if (used[j][b]) SGX_HIP(hipEventRecord(filtered[j * nb + b], branches[b]));
+            }
+            for (int i = 0; i < (int)items.size(); ++i)
+                if (!in_queue[i])
+                    plan_run(*items[i].plan, items[i].in, items[i].out,
+                             branches[branch_of[items[i].plan]], false);
+            for (int j = 0; j < nq; ++j) {
+                const Queue& Q = queues[j];
+                hipStream_t st = branches[nb + j];
+                for (int b = 0; b < nb; ++b)
+                    if (used[j][b]) SGX_HIP(hipStreamWaitEvent(st, filtered[j * nb + b], 0));
+                const char* base = static_cast<const char*>(qbuf);
+                const int2* tasks = reinterpret_cast<const int2*>(base + Q.tasks_off);
+                if (Q.kind == 3)
+                    sgx::launch_cbct_queue(
+                        Q.variant, reinterpret_cast<const sgx::CbctItem*>(base + Q.items_off),
+                        tasks, Q.ntasks, st);
+                else
+                    sgx::launch_back2d_queue(
+                        Q.variant, reinterpret_cast<const sgx::Back2dItem*>(base + Q.items_off),
+                        tasks, Q.ntasks, Q.lds, st);
+            }
+            for (int b = 0; b < nb + nq; ++b) {
                 SGX_HIP(hipEventRecord(joins[b], branches[b]));
                 SGX_HIP(hipStreamWaitEvent(origin, joins[b], 0));
             }
@@ -1565,6 +1769,7 @@ struct sgx_batch {
         gens.clear();
         for (const Item& it : items) gens.push_back(it.plan->gen);
         nbranch = nb;
+        queued = use_queues;
         dirty = false;
     }
 
@@ -1621,14 +1826,38 @@ int sgx_batch_execute(sgx_batch* batch, uint32_t flags)
     return guard(ctx, [&] {
         require(!batch->items.empty(), SGX_ERR_ARG, "empty batch");
         require(!(flags & SGX_HOST), SGX_ERR_ARG, "batches take device pointers");
-        if (!batch->dirty && batch->stale()) batch->dirty = true;
+        const int mode = (flags & SGX_BATCH_NO_QUEUE) ? 0 : (flags & SGX_BATCH_QUEUE) ? 1 : 2;
+        if (!batch->dirty && (batch->stale() || batch->mode != mode)) batch->dirty = true;
         if (batch->dirty) {
             if (!(flags & SGX_BATCH_NO_TUNE)) {
                 for (auto& it : batch->items)
                     if (!it.plan->tuned) autotune(*it.plan, it.in, it.out, 3, true);
             }
             SGX_HIP(hipStreamSynchronize(ctx->stream));
-            batch->capture();
+            if (mode == 2) {
+                // time a replay of each form (after one untimed replay) and
+                // keep the faster
+                auto timed = [&]() {
+                    SGX_HIP(hipGraphLaunch(batch->exec, ctx->stream));
+                    SGX_HIP(hipEventRecord(batch->t0, ctx->stream));
+                    SGX_HIP(hipGraphLaunch(batch->exec, ctx->stream));
+                    SGX_HIP(hipEventRecord(batch->t1, ctx->stream));
+                    SGX_HIP(hipEventSynchronize(batch->t1));
+                    float ms = 0.f;
+                    SGX_HIP(hipEventElapsedTime(&ms, batch->t0, batch->t1));
+                    return ms;
+                };
+                batch->capture(true);
+                batch->ms_queues = batch->queues.empty() ? -1.f : timed();
+                batch->capture(false);
+                batch->ms_launches = timed();
+                if (batch->ms_queues >= 0.f && batch->ms_queues < batch->ms_launches)
+                    batch->capture(true);
+            } else {
+                batch->capture(mode == 1);
+                batch->ms_queues = batch->ms_launches = -1.f;
+            }
+            batch->mode = mode;
         }
         SGX_HIP(hipEventRecord(batch->t0, ctx->stream));
         SGX_HIP(hipGraphLaunch(batch->exec, ctx->stream));
@@ -1653,6 +1882,29 @@ int sgx_batch_info(const sgx_batch* batch, int64_t* items, int64_t* branches)
     if (!batch) return SGX_ERR_ARG;
     if (items) *items = (int64_t)batch->items.size();
     if (branches) *branches = batch->nbranch;
+    return SGX_OK;
+}
+
+int sgx_batch_choice(const sgx_batch* batch, float* ms_queues, float* ms_launches)
+{
+    if (!batch) return SGX_ERR_ARG;
+    if (ms_queues) *ms_queues = batch->ms_queues;
+    if (ms_launches) *ms_launches = batch->ms_launches;
+    return SGX_OK;
+}
+
+int sgx_batch_queues(const sgx_batch* batch, int64_t* queues, int64_t* queued_items,
+                     int64_t* tasks)
+{
+    if (!batch) return SGX_ERR_ARG;
+    int64_t n = 0, t = 0;
+    for (const auto& Q : batch->queues) {
+        n += (int64_t)Q.items.size();
+        t += Q.ntasks;
+    }
+    if (queues) *queues = (int64_t)batch->queues.size();
+    if (queued_items) *queued_items = n;
+    if (tasks) *tasks = t;
     return SGX_OK;
 }
 

@@ -117,6 +117,33 @@ struct CbctArgs {
     float so_ds = 0.f;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
+
+// Batch (task-table) form of the LDS-staged cone-beam backprojector: one
+// item per reconstruction (device array), tasks (item, voxel block); block b
+// does task b; the items of one table share a kernel variant.
+struct CbctItem {
+    const void* q;
+    const ViewCS* cs;
+    const float* vscale;
+    const float* xds;
+    const float* yds;
+    const float* zds;
+    const uint8_t* mask;
+    float* image;
+    const int4* win;
+    const float* zs;
+    int64_t view_stride;
+    int ns, nt, pitch, nviews, nx, ny, nz, arc_t_exact, narrow;
+    float dsd_ds, dso_ds, ws, wt, inv_dt_ds, so_ds;
+};
+// kernel variant of `a`: the staged kernel when a.win is set, else the
+// gather kernel
+int cbct_variant(const CbctArgs& a);
+CbctItem cbct_item(const CbctArgs& a);
+// blocks of the variant's kernel for `a`
+int64_t cbct_item_blocks(const CbctArgs& a);
+void launch_cbct_queue(int variant, const CbctItem* items, const int2* tasks, int ntasks,
+                       hipStream_t st);
 // Entries of the window table the staged kernel takes for these arguments.
 int64_t cbct_window_entries(const CbctArgs& a);
 // Fill the window table (needs cs, axes, geometry and sizes of `a`).
@@ -164,6 +191,35 @@ struct Back2dArgs {
     bool pixel_form = false;
 };
 void launch_back2d(const Back2dArgs& a, hipStream_t st);
+
+// Task tables of the batch launches are interleaved over kQueueXcds XCDs
+// (workgroups are dispatched to the XCDs round-robin: block b runs on XCD
+// b % kQueueXcds).
+constexpr int kQueueXcds = 8;
+
+// Batch (task-table) form of the 2-D backprojector: one item per
+// reconstruction (device array), tasks (item, pixel tile); block b does
+// task b.
+struct Back2dItem {
+    const float* q;
+    const ViewCS* cs;
+    const float* xds;
+    const float* yds;
+    const uint8_t* mask;
+    float* image;
+    int64_t view_stride;
+    int nb, nviews, nx, ny, vc, wshift;
+    float wb, dsd_ds, dso_ds, so_ds;
+};
+// kernel variant of `a` (items of one queue share it); set a.vc_max = 64
+// first, the queue's LDS chunk
+int back2d_variant(const Back2dArgs& a);
+Back2dItem back2d_item(const Back2dArgs& a);
+int back2d_item_tiles(const Back2dItem& it);
+int back2d_item_tiles_x(const Back2dItem& it);
+size_t back2d_item_lds(const Back2dItem& it);
+void launch_back2d_queue(int variant, const Back2dItem* items, const int2* tasks, int ntasks,
+                         size_t lds, hipStream_t st);
 // Series terms of the tile-relative arc form launch_back2d takes for `a`
 // (0: the per-pixel form).
 int back2d_rel_terms(const Back2dArgs& a);

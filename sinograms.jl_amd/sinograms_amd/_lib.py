@@ -48,7 +48,9 @@ EXPORTS = (
     "sgx_plan_stored_bytes", "sgx_plan_read_stored", "sgx_plan_pitch",
     "sgx_ctx_set_option", "sgx_ctx_get_option", "sgx_plan_autotune", "sgx_plan_tuning",
     "sgx_batch_create", "sgx_batch_destroy", "sgx_batch_add", "sgx_batch_execute",
-    "sgx_batch_last_ms", "sgx_batch_info", "sgx_plan_filter", "sgx_plan_back",
+    "sgx_batch_last_ms", "sgx_batch_info", "sgx_batch_queues", "sgx_batch_choice",
+    "sgx_plan_filter",
+    "sgx_plan_back",
     "sgx_plan_back_t", "sgx_net_create", "sgx_net_destroy", "sgx_net_num_params",
     "sgx_net_params", "sgx_net_grads", "sgx_net_zero_grad", "sgx_net_forward",
     "sgx_net_backward", "sgx_net_adam", "sgx_mse",
@@ -160,6 +162,10 @@ def _declare(L):
     L.sgx_batch_last_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_float)]
     L.sgx_batch_info.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
                                  ctypes.POINTER(ctypes.c_int64)]
+    L.sgx_batch_queues.argtypes = [vp, ctypes.POINTER(ctypes.c_int64),
+                                   ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+    L.sgx_batch_choice.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
+                                   ctypes.POINTER(ctypes.c_float)]
     L.sgx_plan_filter.argtypes = [vp, vp, vp, u32]
     L.sgx_plan_back.argtypes = [vp, vp, vp, u32]
     L.sgx_plan_back_t.argtypes = [vp, vp, vp, u32]
@@ -180,6 +186,8 @@ def _declare(L):
 
 SGX_OPT_CBCT_KERNEL = 1
 SGX_BATCH_NO_TUNE = 0x100
+SGX_BATCH_NO_QUEUE = 0x200
+SGX_BATCH_QUEUE = 0x400
 
 
 class Context:

@@ -65,8 +65,17 @@ class Batch:
                                                   _dev_ptr(out, "out")))
         self._keep.append((plan, sino, out))
 
-    def execute(self, tune: bool = True, wait: bool = True) -> None:
+    def execute(self, tune: bool = True, wait: bool = True, queues=None) -> None:
+        """Run the batch's graph (autotuning and capturing it first when items
+        were added or plans changed).  queues: None times both forms at
+        capture and keeps the faster; True takes the queue launches (one
+        launch per backprojector variant over a table of tasks); False the
+        graph of whole plan executes."""
         flags = (0 if tune else _lib.SGX_BATCH_NO_TUNE) | (0 if wait else _lib.SGX_ASYNC)
+        if queues is True:
+            flags |= _lib.SGX_BATCH_QUEUE
+        elif queues is False:
+            flags |= _lib.SGX_BATCH_NO_QUEUE
         with self._ctx.ordered():
             self._ctx.check(_lib.load().sgx_batch_execute(self.h, flags))
 
@@ -79,7 +88,14 @@ class Batch:
     def info(self) -> dict:
         n, b = ctypes.c_int64(), ctypes.c_int64()
         self._ctx.check(_lib.load().sgx_batch_info(self.h, ctypes.byref(n), ctypes.byref(b)))
-        return {"items": n.value, "branches": b.value}
+        q, qi, t = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        self._ctx.check(_lib.load().sgx_batch_queues(self.h, ctypes.byref(q), ctypes.byref(qi),
+                                                     ctypes.byref(t)))
+        mq, ml = ctypes.c_float(), ctypes.c_float()
+        self._ctx.check(_lib.load().sgx_batch_choice(self.h, ctypes.byref(mq), ctypes.byref(ml)))
+        return {"items": n.value, "branches": b.value, "queues": q.value,
+                "queued_items": qi.value, "tasks": t.value,
+                "timed_ms_queues": mq.value, "timed_ms_launches": ml.value}
 
     def __len__(self):
         return len(self._keep)

@@ -798,8 +798,18 @@ def test_batch_graph_matches_unbatched(sg, torch_cuda):
     batch = sg.Batch()
     for (p, _), x, y in zip(items, sinos, outs):
         batch.add(p, x, y)
-    batch.execute()                       # autotune + capture + first replay
-    assert batch.info() == {"items": len(items), "branches": 4}
+    batch.execute()                       # autotune + capture (both forms timed) + replay
+    info = batch.info()
+    assert (info["items"], info["branches"]) == (len(items), 4)
+    assert info["timed_ms_queues"] > 0 and info["timed_ms_launches"] > 0
+    assert (info["queues"] > 0) == (info["timed_ms_queues"] < info["timed_ms_launches"])
+    # in the queue form every item but the two of the shared plan and the
+    # gather-tuned cone-beam ones backprojects through a queue launch
+    batch.execute(queues=True)
+    info = batch.info()
+    gathered = sum(1 for p, spec in zip(plans, specs) if isinstance(spec[0], sg.CtFan)
+                   and sg.tuning(p)["cbct_kernel"] == "gather" and p is not plans[-1])
+    assert info["queued_items"] == len(items) - 2 - gathered and info["queues"] >= 2
     assert all(sg.tuning(p)["tuned"] for p in plans)
     refs = []
     for (p, spec), x in zip(items, sinos):
@@ -826,6 +836,15 @@ def test_batch_graph_matches_unbatched(sg, torch_cuda):
             e = float((y - r).norm() / r.norm())
             assert e <= 1e-6, ("graph", rep, i, e)
     assert batch.last_ms() > 0
+    # the graph of whole plan executes (no queues), then the queues again
+    for queues in (False, True):
+        for y in outs:
+            y.zero_()
+        batch.execute(queues=queues)
+        assert (batch.info()["queues"] > 0) == queues
+        for i, (y, r) in enumerate(zip(outs, refs)):
+            e = float((y - r).norm() / r.norm())
+            assert e <= 1e-6, ("queues" if queues else "plain", i, e)
 
 
 def test_autotune_choice_and_cache(sg, orc, torch_cuda):
