@@ -66,6 +66,8 @@ KERNEL_SOURCES = {
     "k_cbct": ("sinograms.jl_amd/csrc/k_cbct.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
     "k_back2d": ("sinograms.jl_amd/csrc/k_back2d.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
     "k_filter": ("sinograms.jl_amd/csrc/k_filter.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
+    "k_bdd": ("sinograms.jl_amd/csrc/k_bdd.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
+    "k_nn": ("sinograms.jl_amd/csrc/k_nn.hip", "sinograms.jl_amd/csrc/sgx_internal.h"),
 }
 # VALU issue ceiling of an MI355X: 256 CUs x 4 SIMDs, one wave64 VALU
 # instruction per SIMD every 4 cycles (16 lanes per cycle), 2.4 GHz.
@@ -221,6 +223,20 @@ def filter_forms_leg(sg, device, reps):
         log(f"  filter ns={ns} npad={r['npad']}: FFT {r['fft_ms']:.3f} ms, "
             f"Toeplitz {r['toeplitz_ms']:.3f} ms")
     return out
+
+
+def event_ms(fn, n):
+    """Average milliseconds of n calls of fn, from HIP events recorded on
+    torch's current stream (the stream the calls run on)."""
+    import torch
+    fn()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(n):
+        fn()
+    b.record()
+    b.synchronize()
+    return a.elapsed_time(b) / n
 
 
 def kernel_times(plan, proj, out, n):
@@ -503,10 +519,22 @@ def dd_leg(sg, device, steps):
                           generator=g).t().contiguous().t()
         tp = time_steps(lambda: sg.project_bdd(img, geo), steps, 2)
         tb = time_steps(lambda: sg.backproject_bdd(sino, geo), steps, 2)
-        out[f"{nPix}x{nPix}_to_{na}x{nDet}"] = {
-            "project_ms": tp * 1e3, "backproject_ms": tb * 1e3,
-            "rays_per_s": na * nDet / tp, "pixel_views_per_s": nPix * nPix * na / tp,
-            "dtype": "f64"}
+        r = {"project_ms": tp * 1e3, "backproject_ms": tb * 1e3,
+             "rays_per_s": na * nDet / tp, "pixel_views_per_s": nPix * nPix * na / tp,
+             "dtype": "f64"}
+        if nPix == 512:
+            # both calls run on torch's current stream: HIP events around
+            # each (the main kernel and a short finishing kernel)
+            units = nPix * nPix * na
+            io = 8 * (nPix * nPix + na * nDet)
+            cp = event_ms(lambda: sg.project_bdd(img, geo), steps)
+            cb = event_ms(lambda: sg.backproject_bdd(sino, geo), steps)
+            r["roofline"] = roofline("bdd_project_kernel<double>", cp, io, units, "update")
+            r["roofline"]["note_time"] = "HIP events around the call (bdd_project_kernel + bdd_finish_kernel)"
+            r["roofline_adjoint"] = roofline("bdd_back_kernel", cb, io, units, "update")
+            r["roofline_adjoint"]["note_time"] = ("HIP events around the call "
+                                                  "(bdd_back_table_kernel + bdd_back_kernel)")
+        out[f"{nPix}x{nPix}_to_{na}x{nDet}"] = r
         log(f"  project_bdd {nPix}^2 -> {na} x {nDet}: {tp * 1e3:.3f} ms, "
             f"adjoint {tb * 1e3:.3f} ms")
     return out
