@@ -377,6 +377,7 @@ def cpu_leg(sg, rg, ig, proj, Hk, vols, nslab):
         "fp32_vs_f64_oracle": err(g["fp32"], ref64),
         "f16s_vs_f64_oracle": err(g["f16s"], ref64),
         "bf16_vs_f64_oracle": err(g["bf16"], ref64),
+        "f16s_packed16_vs_f64_oracle": err(g["f16s_pk"], ref64) if "f16s_pk" in g else None,
         "f32_oracle_vs_f64_oracle": err(ref32, ref64),
     }
     cpu = {
@@ -676,9 +677,16 @@ def main():
     updates = int(np.prod(ig.dims)) * rg.na
 
     log("planning ...")
-    sg._lib.context(0).cbct_kernel = args.cbct_kernel
+    ctx = sg._lib.context(0)
+    ctx.cbct_kernel = args.cbct_kernel
     forms = ("fp32", "bf16", "f16s")
     plans = {f: sg.plan_fbp(rg, ig, precision=f) for f in forms}
+    if args.cbct_kernel == "staged":
+        # A/B: the scaled-FP16 form interpolating in packed FP16 arithmetic
+        ctx.cbct_kernel = "staged_packed16"
+        plans["f16s_pk"] = sg.plan_fbp(rg, ig, precision="f16s")
+        ctx.cbct_kernel = args.cbct_kernel
+        forms += ("f16s_pk",)
     outs = {f: sg.recon._empty_colmajor(ig.dims, dev) for f in forms}
     step, kt = {}, {}
     for f in forms:
@@ -756,6 +764,7 @@ def main():
             "note": "FP32 plan, host projections in and host volume out (pageable numpy arrays)"},
         "mixed_f16s": form("f16s", 8.0),
         "mixed_bf16": form("bf16", 8.0),
+        "mixed_f16s_packed16": form("f16s_pk", 8.0) if "f16s_pk" in plans else None,
         "toeplitz_filter_ab": toep,
     }
     if not args.no_slices:

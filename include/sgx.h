@@ -88,7 +88,10 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
 /* Kernel choices of a context (no counterpart in the reference; the results
  * agree to FP32 rounding whatever the choice).
  *   SGX_OPT_CBCT_KERNEL: 0 = LDS-staged cone-beam backprojector (default),
- *                        1 = gather backprojector.
+ *                        1 = gather backprojector,
+ *                        2 = LDS-staged, with scaled-FP16 plans interpolating
+ *                            in packed FP16 arithmetic (otherwise as 0;
+ *                            about twice the FP16 form's rounding error).
  * Plans take the context's choice when they are created (and an autotuned
  * choice after sgx_plan_autotune). */
 enum { SGX_OPT_CBCT_KERNEL = 1 };

@@ -235,6 +235,7 @@ __device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[
 
 constexpr int kWaves = 4;      // waves per block, each on its own z-slab
 constexpr int kGatherVariant = 16;   // cbct_variant: the gather kernel
+constexpr int kPackedVariant = 32;   // cbct_variant: the packed-FP16 kernel
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
 // One block of the gather kernel: block `blk` (8 x 8 columns x 4 waves of
@@ -735,6 +736,325 @@ cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ t
                                      it.so_ds, tk.y, lds);
 }
 
+// ---------------------------------------------------------------------------
+// packed-FP16 kernel (scaled-FP16 storage)
+// ---------------------------------------------------------------------------
+//
+// The blocks, window table and view loop of the staged kernel, with the
+// interpolation of the scaled-FP16 form in packed FP16 arithmetic (selected
+// by SGX_OPT_CBCT_KERNEL = 2):
+//
+// * LDS entry (c, r): {(q[c, r], q[c+1, r]), (d[c, r], d[c+1, r])}, two FP16
+//   pairs, d = fp16(q[., r+1] - q[., r]), rows kPkRow bytes apart (a pitch
+//   that staggers the banks of neighbouring rows).
+// * Position word: Y = fma(1024 mz, z, 1024 (wt - rlo - 1) + 2^23) in FP32.
+//   While 0 <= bt - rlo - 1 < 8192 its bits [22:10] are the entry row of the
+//   position rounded to 1/1024 of a row and bits [9:0] the fraction f in
+//   units of 1/1024: the row is one bit-field extract, and
+//   ((Y & 0x3ff) | 0x3c00) - 1 is f as FP16, exact, formed for two voxels at
+//   once.
+// * Update: (q0 + f d0, q1 + f d1) in one v_pk_fma_f16 (each rounded once to
+//   FP16), then acc += W . v in one v_dot2_f32_f16, W = FP16 ((1 - fs) w2,
+//   fs w2) per lane and view, with every view's scale taken relative to the
+//   largest so that W stays in FP16's normal range; the result is multiplied
+//   back by the largest scale.  6.5 VALU per update in the inner loop
+//   against 8 for the FP32 interpolation.
+// * Measured on the 256^2 x 128 FDK workload (360 views) it does not pay:
+//   the 8-byte entries repeat every bin (twice the staging work and LDS
+//   writes of the FP32 interpolation's 4-byte entries), so the whole kernel
+//   issues 9.7 VALU per update against 10.1 and runs 1.09 ms against
+//   0.98 ms.  The rounding of the position to 1/1024 row, of v and of W each
+//   add about as much error as the FP16 storage itself: 6.7e-5 from Float64
+//   against 3.4e-5.
+// * Lanes whose taps leave the window, or whose w2 would leave FP16's range,
+//   take the FP32 gather path after the view loop, as in the staged kernel.
+
+#ifndef SGX_PACKED_PITCH
+#define SGX_PACKED_PITCH 52
+#endif
+constexpr uint32_t kPkRow = 8 * SGX_PACKED_PITCH;  // LDS row pitch (bytes); staggers the banks
+constexpr uint32_t kPkBuf = kHmax * kPkRow;        // bytes per buffer
+static_assert(SGX_PACKED_PITCH >= kWmax + 2, "a window row fits an LDS row");
+constexpr int kPkPairs = kWmax / 2;                // staging threads per row group
+constexpr int kPkRowsPer = 4;                      // entry rows per staging thread
+constexpr int kPkStagers = kPkPairs * (kHmax / kPkRowsPer);
+static_assert(kPkStagers <= 256, "one staging thread per entry pair and row group");
+static_assert((kHmax & (kHmax - 1)) == 0, "the row field needs a power-of-two row count");
+
+typedef _Float16 hf2 __attribute__((ext_vector_type(2)));
+typedef __fp16 hp2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ hf2 as_hf2(uint32_t u) { return __builtin_bit_cast(hf2, u); }
+
+// (a & m) | c in one v_and_or_b32.  A VOP3 instruction takes no literal here
+// and at most one scalar operand, so c comes in a VGPR (the compiler would
+// otherwise split a pair of literal masks into v_and + v_or).
+__device__ __forceinline__ uint32_t and_or(uint32_t a, uint32_t m, uint32_t c)
+{
+    uint32_t r;
+    asm("v_and_or_b32 %0, %1, %2, %3" : "=v"(r) : "v"(a), "s"(m), "v"(c));
+    return r;
+}
+
+// Per-view lane geometry of the staged kernels (bs, w2 with the view's
+// scale, the t map, the lane's entry column and, for windows that are not
+// interior, whether every tap of the lane is inside the window).  `tmargin`
+// widens the row test for positions that are rounded after it.
+struct LaneGeom {
+    float bs, w2, mz, wt, blo, bhi;
+    bool inwin;
+    int ce;
+};
+template <bool ARC, bool OPT>
+__device__ __forceinline__ LaneGeom lane_geom(float x, float y, ViewCS v, int4 w, bool check,
+                                              float zlo, float zhi, float dsd_ds, float dso_ds,
+                                              float ws, float wt, float inv_dt_ds, float vs,
+                                              const float* zs, int ia, int arc_t_exact,
+                                              int narrow, float so_ds, float tmargin)
+{
+    LaneGeom r;
+    const float xb0 = x * v.c + y * v.s;
+    const float xb = OPT ? xb0 - so_ds : xb0;          // r_loop (source offset)
+    const float yb = dso_ds - (-x * v.s + y * v.c);
+    const float mag = dsd_ds * __builtin_amdgcn_rcpf(yb);
+    float sp;
+    if (ARC) {
+        sp = dsd_ds * (narrow ? atan2_narrow(xb, yb) : atan2_fast(xb, yb));
+        r.w2 = dsd_ds * dsd_ds * __builtin_amdgcn_rcpf(xb * xb + yb * yb);
+    } else {
+        sp = mag * xb;
+        r.w2 = mag * mag;
+    }
+    r.w2 *= vs;
+    r.bs = sp + ws;
+    t_map<ARC, OPT>(xb, yb, mag, dsd_ds, inv_dt_ds, wt, zs, ia, arc_t_exact, r.mz, r.wt);
+    r.ce = (int)r.bs - 1 - (w.x & ~3);
+    r.inwin = true;
+    r.blo = fmaf(r.mz, zlo, r.wt);
+    r.bhi = fmaf(r.mz, zhi, r.wt);
+    if (check) {
+        const float tl = fminf(r.blo, r.bhi) - tmargin, th = fmaxf(r.blo, r.bhi) + tmargin;
+        const int rl = (int)tl - 1 - w.y, rh = (int)th - 1 - w.y;
+        const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
+        r.inwin = tl >= 1.f && (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
+                  (uint32_t)rh < (uint32_t)w.w;
+    }
+    return r;
+}
+
+// One block's work (as cbct_staged_block) with the packed-FP16 update; `lds`
+// holds kHmax rows of kPkRow bytes.
+template <bool ARC, bool OPT>
+__device__ __forceinline__ void cbct_packed_block(
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
+    int arc_t_exact, int narrow, float so_ds, int blk, char* lds)
+{
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tiles_x = (nx + kSP - 1) / kSP;
+    const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
+    const int tile = blk % tiles, zblk = blk / tiles;
+    const int ix = (tile % tiles_x) * kSP + (wave & 1) * 8 + (lane & 7);
+    const int iy = (tile / tiles_x) * kSP + (wave >> 1) * 8 + (lane >> 3);
+    const int iz0 = zblk * kSZ;
+    const bool inxy = ix < nx && iy < ny;
+    const int64_t nxy = (int64_t)nx * ny;
+    const int64_t col = inxy ? ix + (int64_t)nx * iy : 0;
+
+    float z[kSZ], acc[kSZ];
+    uint32_t inbits = 0;
+#pragma unroll
+    for (int j = 0; j < kSZ; ++j) {
+        const int iz = iz0 + j;
+        const bool in = inxy && iz < nz && (mask == nullptr || mask[col + nxy * iz]);
+        inbits |= (uint32_t)in << j;
+        z[j] = zds[min(iz, nz - 1)];
+        acc[j] = 0.f;
+    }
+    const bool any = inbits != 0;
+    float zlo = z[0], zhi = z[0];
+#pragma unroll
+    for (int j = 1; j < kSZ; ++j) {
+        zlo = fminf(zlo, z[j]);
+        zhi = fmaxf(zhi, z[j]);
+    }
+    // The largest view scale (every wave forms it itself: no barrier).
+    float vmax = 0.f;
+    for (int i = lane; i < nviews; i += 64) vmax = fmaxf(vmax, vscale[i]);
+    for (int o = 32; o >= 1; o >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, o));
+    if (!(vmax > 0.f)) vmax = 1.f;
+    const float rvmax = 1.f / vmax;
+
+    const float x = inxy ? xds[ix] : 0.f, y = inxy ? yds[iy] : 0.f;
+    const float nsf = (float)ns, ntf = (float)nt;
+    const uint32_t rowb = (uint32_t)pitch * 2u;
+    const uint32_t vbytes = (uint32_t)nt * rowb + Fmt<F_F16S>::pad;
+    const int64_t viewb = view_stride * 2;
+    const char* qb = static_cast<const char*>(q);
+    const int4* bwin = win + (int64_t)blk * nviews;
+
+    // Staging role: thread (pair sp, group sg) builds entries 2sp, 2sp+1 of
+    // entry rows 4sg .. 4sg+3 from bins 2sp .. 2sp+3 of data rows 4sg ..
+    // 4sg+4, and stores each entry row's two entries as one 16-byte write:
+    // the lanes of a row write consecutive 16-byte slots (no bank conflicts).
+    const bool stager = tid < kPkStagers;
+    const int sp = tid % kPkPairs, sg = tid / kPkPairs;
+    const uint32_t toff = ((uint32_t)(kPkRowsPer * sg) * (uint32_t)pitch + 2u * sp) * 2u;
+    char* const sdst = lds + (kPkRowsPer * sg) * kPkRow + 16 * sp;
+    uint2 hv[kPkRowsPer + 1];     // bins (2sp, 2sp+1), (2sp+2, 2sp+3) of the data rows
+    auto stage_load = [&](int ia, int4 w) {
+        if (!stager || 2 * sp >= w.z || kPkRowsPer * sg >= w.w) return;
+        const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
+        const uint32_t off = toff + ((uint32_t)w.y * (uint32_t)pitch + (uint32_t)(w.x & ~3)) * 2u;
+#pragma unroll
+        for (int k = 0; k <= kPkRowsPer; ++k) {
+            if (k > 0 && kPkRowsPer * sg + k > w.w) break;     // data row needed up to H
+            hv[k].x = __builtin_amdgcn_raw_buffer_load_b32(rs, off + k * rowb, 0, 0);
+            hv[k].y = __builtin_amdgcn_raw_buffer_load_b32(rs, off + k * rowb + 4u, 0, 0);
+        }
+    };
+    auto stage_store = [&](int buf, int4 w) {
+        if (!stager || 2 * sp >= w.z || kPkRowsPer * sg >= w.w) return;
+        char* d = sdst + buf * kPkBuf;
+#pragma unroll
+        for (int k = 0; k < kPkRowsPer; ++k) {
+            if (kPkRowsPer * sg + k >= w.w) break;
+            const uint32_t q01 = hv[k].x, q23 = hv[k].y;
+            const uint32_t d01 = half2_sub(hv[k + 1].x, q01);
+            const uint32_t d23 = half2_sub(hv[k + 1].y, q23);
+            // the pair starting at the odd bin: the upper half of one pair and
+            // the lower half of the next
+            const uint32_t q12 = __builtin_amdgcn_alignbit(q23, q01, 16);
+            const uint32_t d12 = __builtin_amdgcn_alignbit(d23, d01, 16);
+            *reinterpret_cast<uint4*>(d + k * kPkRow) = make_uint4(q01, d01, q12, d12);
+        }
+    };
+
+    bool missed = false;
+    int4 wcur = bwin[0];
+    stage_load(0, wcur);
+    stage_store(0, wcur);
+    __syncthreads();
+    for (int ia = 0; ia < nviews; ++ia) {
+        const int buf = ia & 1;
+        const int4 wnext = ia + 1 < nviews ? bwin[ia + 1] : make_int4(0, 0, 0, 0);
+        stage_load(ia + 1, wnext);
+        if (any) {
+            const LaneGeom lv = lane_geom<ARC, OPT>(x, y, cs[ia], wcur, (wcur.x & 1) == 0, zlo,
+                                                    zhi, dsd_ds, dso_ds, ws, wt, inv_dt_ds,
+                                                    vscale[ia] * rvmax, zs, ia, arc_t_exact,
+                                                    narrow, so_ds, 1.f / 1024.f);
+            if (lv.inwin && lv.w2 < 32768.f) {
+                const float wb = lv.w2 * __builtin_amdgcn_fractf(lv.bs);
+                const float wa = lv.w2 - wb;
+                const hp2 W = {(__fp16)wa, (__fp16)wb};
+                const float mzs = lv.mz * 1024.f;
+                const float wts = fmaf(lv.wt - (float)(wcur.y + 1), 1024.f, 8388608.f);
+                // (the clamp keeps every address inside the LDS block whatever ce is)
+                const uint32_t colb = 8u * min((uint32_t)lv.ce, (uint32_t)SGX_PACKED_PITCH - 1u) +
+                                      kPkBuf * buf;
+                const uint32_t kOnes = 0x3c003c00u;                 // (1.0, 1.0) in FP16
+#pragma unroll
+                for (int h = 0; h < kSZ; h += kSB) {
+                    uint32_t Y[kSB];
+                    uint2 e[kSB];
+#pragma unroll
+                    for (int j = 0; j < kSB; ++j) {
+                        Y[j] = __float_as_uint(fmaf(mzs, z[h + j], wts));
+                        const uint32_t row = __builtin_amdgcn_ubfe(Y[j], 10, 5);
+                        e[j] = *reinterpret_cast<const uint2*>(lds + __umul24(row, kPkRow) + colb);
+                    }
+                    if (SGX_STAGED_SCHED) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int j = 0; j < kSB; j += 2) {
+                        const uint32_t P = __builtin_amdgcn_perm(Y[j + 1], Y[j], 0x05040100u);
+                        const hf2 f = as_hf2(and_or(P, 0x03ff03ffu, kOnes)) - (hf2)(1.0f16);
+                        const hf2 v0 = __builtin_elementwise_fma(f.xx, as_hf2(e[j].y),
+                                                                 as_hf2(e[j].x));
+                        const hf2 v1 = __builtin_elementwise_fma(f.yy, as_hf2(e[j + 1].y),
+                                                                 as_hf2(e[j + 1].x));
+                        acc[h + j] = __builtin_amdgcn_fdot2(__builtin_bit_cast(hp2, v0), W,
+                                                            acc[h + j], false);
+                        acc[h + j + 1] = __builtin_amdgcn_fdot2(__builtin_bit_cast(hp2, v1), W,
+                                                                acc[h + j + 1], false);
+                    }
+                }
+            } else {
+                missed = true;
+            }
+        }
+        stage_store(buf ^ 1, wnext);
+        __syncthreads();
+        wcur = wnext;
+    }
+
+    // Views whose taps missed the staged window: gathered from global memory
+    // in FP32 arithmetic, with the same relative scale.
+    if (missed) {
+        for (int ia = 0; ia < nviews; ++ia) {
+            // the view loop's decision, recomputed from the same inputs
+            const int4 w = bwin[ia];
+            const LaneGeom lv = lane_geom<ARC, OPT>(x, y, cs[ia], w, (w.x & 1) == 0, zlo, zhi,
+                                                    dsd_ds, dso_ds, ws, wt, inv_dt_ds,
+                                                    vscale[ia] * rvmax, zs, ia, arc_t_exact,
+                                                    narrow, so_ds, 1.f / 1024.f);
+            if ((lv.inwin && lv.w2 < 32768.f) || !(lv.bs >= 1.f && lv.bs < nsf)) continue;
+            ColView cv;
+            cv.isf = floorf(lv.bs);
+            cv.wb = lv.w2 * (lv.bs - cv.isf);
+            cv.wa = lv.w2 - cv.wb;
+            cv.mz = lv.mz;
+            cv.wt = lv.wt;
+            const float tlo = fminf(lv.blo, lv.bhi), thi = fmaxf(lv.blo, lv.bhi);
+            const bool fast = tlo >= 1.f && thi < ntf;
+            gather_view<kSZ, F_F16S>(cv, z, lv.wt, ntf, fast, view_rsrc(qb + ia * viewb, vbytes),
+                                     rowb, acc);
+        }
+    }
+
+    if (inxy) {
+#pragma unroll
+        for (int j = 0; j < kSZ; ++j)
+            if (iz0 + j < nz)
+                image[col + nxy * (iz0 + j)] = ((inbits >> j) & 1u) ? acc[j] * vmax : 0.f;
+    }
+}
+
+template <bool ARC, bool OPT>
+__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_packed_kernel(
+    const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
+    const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
+    const float* __restrict__ xds, const float* __restrict__ yds,
+    const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
+    float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
+    float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
+    int arc_t_exact, int narrow, float so_ds)
+{
+    __shared__ __attribute__((aligned(16))) char lds[2 * kPkBuf];
+    cbct_packed_block<ARC, OPT>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds, yds, zds,
+                                nx, ny, nz, mask, image, dsd_ds, dso_ds, ws, wt, inv_dt_ds, win,
+                                zs, arc_t_exact, narrow, so_ds, blockIdx.x, lds);
+}
+
+// Batch form (as cbct_queue_kernel).
+template <bool ARC, bool OPT>
+__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_packed_queue_kernel(
+    const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
+{
+    __shared__ __attribute__((aligned(16))) char lds[2 * kPkBuf];
+    const int2 tk = tasks[blockIdx.x];
+    using CItem = const __attribute__((address_space(4))) CbctItem;
+    const CItem& it = ((CItem*)(items))[tk.x];
+    cbct_packed_block<ARC, OPT>(it.q, it.ns, it.nt, it.pitch, it.nviews, it.view_stride, it.cs,
+                                it.vscale, it.xds, it.yds, it.zds, it.nx, it.ny, it.nz, it.mask,
+                                it.image, it.dsd_ds, it.dso_ds, it.ws, it.wt, it.inv_dt_ds,
+                                it.win, it.zs, it.arc_t_exact, it.narrow, it.so_ds, tk.y, lds);
+}
+
 // An arc detector whose bins (1 <= bs < ns, |gamma| = |bs - ws| / dsd_ds)
 // span less than 0.75 rad (43 degrees) on both sides takes atan2_narrow.
 int arc_narrow(const CbctArgs& a)
@@ -778,9 +1098,32 @@ void launch_staged(const CbctArgs& a, hipStream_t st)
     SGX_HIP(hipGetLastError());
 }
 
+template <bool ARC, bool OPT>
+void launch_packed(const CbctArgs& a, hipStream_t st)
+{
+    const int64_t nb = staged_blocks(a);
+    if (nb >= (int64_t(1) << 31))
+        throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
+    hipLaunchKernelGGL((cbct_packed_kernel<ARC, OPT>), dim3((unsigned)nb), dim3(256), 0, st,
+                       a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
+                       a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
+                       a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0,
+                       arc_narrow(a), a.so_ds);
+    SGX_HIP(hipGetLastError());
+}
+
+// The scaled-FP16 form takes the packed kernel when asked for it.
+bool packed_form(const CbctArgs& a)
+{
+    return a.win && a.fmt == SGX_Q_FMT_F16S && a.f16_packed;
+}
+
 template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
 {
-    if (a.win) {
+    if (packed_form(a)) {
+        if (a.is_arc) launch_packed<true, OPT>(a, st);
+        else launch_packed<false, OPT>(a, st);
+    } else if (a.win) {
         if (a.is_arc) launch_staged<true, FMT, OPT>(a, st);
         else launch_staged<false, FMT, OPT>(a, st);
     } else {
@@ -841,7 +1184,8 @@ int cbct_variant(const CbctArgs& a)
 {
     const int fmt = a.fmt == SGX_Q_FMT_BF16 ? F_BF16 : a.fmt == SGX_Q_FMT_F16S ? F_F16S : F_F32;
     const bool opt = a.zs || a.arc_t_exact || a.so_ds != 0.f;
-    return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant);
+    return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant) +
+           (packed_form(a) ? kPackedVariant : 0);
 }
 
 CbctItem cbct_item(const CbctArgs& a)
@@ -911,10 +1255,19 @@ static const void* cbg_fn()
     return reinterpret_cast<const void*>(&cbct_gather_queue_kernel<ARC, FMT, OPT>);
 }
 
+template <bool ARC, bool OPT>
+static const void* cbp_fn()
+{
+    return reinterpret_cast<const void*>(&cbct_packed_queue_kernel<ARC, OPT>);
+}
+
 static const void* cbct_queue_fn(int variant)
 {
     const bool arc = variant & 1, opt = variant & 2;
     const int fmt = (variant >> 2) & 3;
+    if (variant & kPackedVariant)
+        return arc ? (opt ? cbp_fn<true, true>() : cbp_fn<true, false>())
+                   : (opt ? cbp_fn<false, true>() : cbp_fn<false, false>());
     if (variant & kGatherVariant) {
 #define SGX_CBG(F)                                                                   \
     return arc ? (opt ? cbg_fn<true, F, true>() : cbg_fn<true, F, false>())          \

@@ -389,7 +389,7 @@ int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
 {
     return guard(ctx, [&] {
         require(option == SGX_OPT_CBCT_KERNEL, SGX_ERR_ARG, "unknown option");
-        require(value == 0 || value == 1, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0 or 1");
+        require(value >= 0 && value <= 2, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0, 1 or 2");
         ctx->cbct_kernel = (int)value;
     });
 }
@@ -551,7 +551,7 @@ void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int6
     sgx::CbctArgs a = cbct_call_args(T, d, q, (flags & SGX_Q_BF16) != 0, ns, nt, pitch, ia_skip,
                                      dsd, dso, ds, dt, offset_s, offset_t, is_arc, nx, ny, nz, m,
                                      img, opts);
-    if (ctx->cbct_kernel == 0) {
+    if (ctx->cbct_kernel != 1) {
         int4* win = static_cast<int4*>(
             ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
         sgx::launch_cbct_windows(a, win, ctx->stream);
@@ -614,7 +614,7 @@ int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t n
         sgx::CbctArgs a = cbct_call_args(T, base, proj, false, ns, nt, ns, ia_skip, dsd, dso, ds,
                                          dt, offset_s, offset_t, is_arc, nx, ny, nz, mask, image,
                                          nullptr);
-        const bool staged = ctx->cbct_kernel == 0;
+        const bool staged = ctx->cbct_kernel != 1;
         const size_t need =
             tbytes + (staged ? sizeof(int4) * (size_t)sgx::cbct_window_entries(a) : 0);
         if (need > kScratchBytes) {
@@ -962,6 +962,7 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     a.zs = P.zs;
     a.arc_t_exact = P.arc_t_exact;
     a.so_ds = P.so_ds;
+    a.f16_packed = P.tune.cbct == 2;
     return a;
 }
 
@@ -1031,6 +1032,7 @@ std::vector<PlanTune> tune_candidates(const sgx_plan& P)
     std::vector<PlanTune> c;
     if (P.is3d()) {
         for (int k : {0, 1}) c.push_back(PlanTune{k, 2048, 64});
+        if (P.prec == SGX_PREC_F16S) c.push_back(PlanTune{2, 2048, 64});
     } else {
         for (int want : {1024, 2048, kTuneMaxBlocks})
             for (int vc : {kTuneMinChunk, 32, 64}) c.push_back(PlanTune{0, want, vc});

@@ -115,6 +115,9 @@ struct CbctArgs {
     bool arc_t_exact;
     // the lateral source offset / ds (r_loop = x_beta - so_ds), 0 for none
     float so_ds = 0.f;
+    // scaled-FP16 data with windows: true takes the packed-FP16 kernel in
+    // place of the staged kernel's FP32 interpolation
+    bool f16_packed = false;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 

@@ -185,6 +185,8 @@ def _declare(L):
 
 
 SGX_OPT_CBCT_KERNEL = 1
+# values of SGX_OPT_CBCT_KERNEL
+CBCT_KERNELS = ("staged", "gather", "staged_packed16")
 SGX_BATCH_NO_TUNE = 0x100
 SGX_BATCH_NO_QUEUE = 0x200
 SGX_BATCH_QUEUE = 0x400
@@ -251,15 +253,16 @@ class Context:
     @property
     def cbct_kernel(self) -> str:
         """Cone-beam backprojector of this context: "staged" (LDS-staged
-        detector windows, the default) or "gather"."""
+        detector windows, the default), "gather", or "staged_packed16"
+        (staged, with scaled-FP16 plans interpolating in packed FP16)."""
         v = ctypes.c_int64()
         self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_CBCT_KERNEL, ctypes.byref(v)))
-        return ("staged", "gather")[v.value]
+        return CBCT_KERNELS[v.value]
 
     @cbct_kernel.setter
     def cbct_kernel(self, name: str):
         self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_CBCT_KERNEL,
-                                             {"staged": 0, "gather": 1}[name]))
+                                             CBCT_KERNELS.index(name)))
 
     def close(self):
         if self.h:

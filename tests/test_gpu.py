@@ -711,6 +711,56 @@ def test_fdk_stored_data_exact_vs_oracle(sg, orc, torch_cuda, prec):
     assert rel_rmse(got, ref) < TOL32
 
 
+@pytest.mark.parametrize("case", ["flat", "arc_opts_edges"])
+def test_fdk_f16s_packed16_vs_fp32_interp(sg, orc, torch_cuda, case):
+    # The packed-FP16 interpolation of the scaled-FP16 form (cone-beam kernel
+    # "staged_packed16") against the default FP32 interpolation of the same
+    # stored data and against Float64.  Its extra rounding (position to 1/1024
+    # row, FP16 interpolated values and weights) about doubles the FP16
+    # storage error (6.7e-5 against 3.4e-5 on the 256^2 x 128 FDK workload).
+    # "arc_opts_edges": a helical arc detector with the exact t magnification
+    # and a detector short enough that the volume's top and bottom leave it
+    # (windows that are not interior, lanes on the gather path).
+    _say("fdk f16s packed16", case)
+    if case == "flat":
+        rg = sg.CtFanFlat(ns=128, nt=96, ds=2.4, dt=2.4, na=90, dsd=1200.0, dod=400.0)
+        ig = sg.ImageGeom((64, 64, 32), (2.0, 2.0, 2.0))
+        ell = sg.phantom.random_ellipsoids(3, 12, 128.0)
+        opts = {}
+    else:
+        rg = sg.CtFanArc(ns=96, nt=30, ds=1.5, dt=1.5, na=72,
+                         src=sg.CtSourceHelix(pitch=0.3, source_z0=-2.0))
+        ig = sg.ImageGeom((48, 44, 40), (1.5, 1.5, 1.5))
+        ell = sg.phantom.random_ellipsoids(4, 10, 64.0)
+        opts = dict(use_source_zs=True, arc_t_exact=True)
+    proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
+    ctx = sg._lib.context(0)
+    old = ctx.cbct_kernel
+    try:
+        ctx.cbct_kernel = "staged_packed16"
+        packed = sg.fdk(sg.plan_fbp(rg, ig, precision="f16s", **opts), proj)
+    finally:
+        ctx.cbct_kernel = old
+    f16 = sg.fdk(sg.plan_fbp(rg, ig, precision="f16s", **opts), proj)
+    f32 = sg.fdk(sg.plan_fbp(rg, ig, **opts), proj)
+    d = rel_rmse(packed, f16)
+    e_pk, e16 = rel_rmse(packed, f32), rel_rmse(f16, f32)
+    _say(f"  packed vs FP32-interpolated {d:.3g}; from the FP32 plan: packed {e_pk:.3g}, "
+         f"FP32-interpolated {e16:.3g}")
+    assert d < 1e-3
+    assert e_pk < 2.5 * e16 + 1e-6
+    if case == "flat":
+        ar = rg.ar(f32=True).astype(np.float32)
+        vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
+            orc.angle_weight(ar)
+        xc, yc, zc = ig.axes()
+        ref64 = orc.fdk(proj.astype(np.float64), sg.plan_fbp(rg, ig).filter, vw, ar, rg.dsd,
+                        rg.dso, rg.ds, rg.dt, 0.0, 0.0, False, xc, yc, zc, ig.mask)
+        e64 = rel_rmse(packed, ref64)
+        _say(f"  packed vs Float64 {e64:.3g}")
+        assert e64 < 2.5e-4
+
+
 # --- K6 distance-driven projector ------------------------------------------------------
 
 @pytest.mark.parametrize("nPix,nDet,na,f32", [(32, 57, 24, False), (128, 226, 90, True),
