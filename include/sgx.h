@@ -92,9 +92,17 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
  *                        2 = LDS-staged, with scaled-FP16 plans interpolating
  *                            in packed FP16 arithmetic (otherwise as 0;
  *                            about twice the FP16 form's rounding error).
- * Plans take the context's choice when they are created (and an autotuned
- * choice after sgx_plan_autotune). */
-enum { SGX_OPT_CBCT_KERNEL = 1 };
+ *   SGX_OPT_CBCT_MIRROR: the staged backprojector's mirror form, which pairs
+ *                        each voxel z > 0 with its image at -z where the
+ *                        geometry is symmetric (z axis symmetric about 0, no
+ *                        per-view source z, offset_t = 0, nz a multiple of
+ *                        16), one detector position serving both:
+ *                        1 = for scaled-FP16 plans (default; the autotuner
+ *                            times it for the other forms),
+ *                        2 = wherever the geometry allows, 0 = never.
+ * Plans take the context's choices when they are created (and an autotuned
+ * kernel after sgx_plan_autotune). */
+enum { SGX_OPT_CBCT_KERNEL = 1, SGX_OPT_CBCT_MIRROR = 2 };
 int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value);
 int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value);
 
@@ -354,8 +362,9 @@ int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms);
 int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
                       int use_cache);
 /* The plan's kernel shape: values[0] = cone-beam kernel (0 staged, 1
- * gather), values[1] = 2-D view-grouping target, values[2] = 2-D view chunk,
- * values[3] = 1 when autotuned. */
+ * gather, 2 staged with packed-FP16 interpolation, 3 the staged kernel's
+ * mirror form), values[1] = 2-D view-grouping target, values[2] = 2-D view
+ * chunk, values[3] = 1 when autotuned. */
 int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);
 
 /* A batch: many reconstructions (plan, input, output; device pointers) run as

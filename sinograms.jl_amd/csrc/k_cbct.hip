@@ -192,9 +192,9 @@ __device__ __forceinline__ ColView col_view(float x, float y, ViewCS v, float ds
 // [1, nt); otherwise voxels out of range load zeros (offset kMissOffset)
 // and weigh 0.
 template <int ZB, int FMT>
-__device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[ZB], float wt,
+__device__ __forceinline__ void gather_view(const ColView& cv, const float* z, float wt,
                                             float ntf, bool fast, __amdgpu_buffer_rsrc_t rs,
-                                            uint32_t rowb, float (&acc)[ZB])
+                                            uint32_t rowb, float* acc)
 {
     // byte offset of (is-1, it-1) is it*rowb + (is-1)*esz - rowb
     const uint32_t colb = ((uint32_t)cv.isf - 1u) * Fmt<FMT>::esz - rowb;
@@ -236,6 +236,7 @@ __device__ __forceinline__ void gather_view(const ColView& cv, const float (&z)[
 constexpr int kWaves = 4;      // waves per block, each on its own z-slab
 constexpr int kGatherVariant = 16;   // cbct_variant: the gather kernel
 constexpr int kPackedVariant = 32;   // cbct_variant: the packed-FP16 kernel
+constexpr int kMirrorVariant = 64;   // cbct_variant: the mirror form of the staged kernel
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
 // One block of the gather kernel: block `blk` (8 x 8 columns x 4 waves of
@@ -356,17 +357,38 @@ constexpr int kSB = SGX_STAGED_BATCH;
 // difference of two FP16 values is exact in FP16 whenever they are within a
 // factor 2 of each other, and otherwise rounded once).  The pitch keeps rows
 // 16-byte aligned and staggers the banks of neighbouring rows.
-template <int FMT> struct Lds {
+//
+// Mirror form (MIR).  When the source stays in the plane z = 0 (no per-view
+// source z), the detector's t centre is its middle row (offset_t = 0) and
+// the z axis is symmetric (z[nz-1-k] = -z[k]), the voxel at -z lands on the
+// detector row mirrored from the one z lands on: bt(-z) = nt + 1 - bt(z),
+// with the same s position and weights.  A mirror block owns 8 voxels z > 0
+// per lane and their 8 mirror images; its window (at most kHm rows) is the
+// one of the upper 8, and the lower voxels read a second window holding the
+// mirrored rows in reverse order, entries (q~[r], q~[r+1] - q~[r]) with
+// q~[r] = q[nt - 1 - r], at the same entry row and column.  The two windows
+// share each LDS row (the mirror one at entry column `half` of the row), so
+// the second read takes the first one's address with an immediate offset.
+// One position (FMA, floor, fraction, address) then serves two voxels: 6
+// VALU per update in the inner loop instead of 8.
+constexpr int kSZm = 8;               // mirror form: voxels z > 0 per lane (and 8 images)
+constexpr int kHm = 20;               // mirror form: window rows (each of the two windows)
+template <int FMT, bool MIR = false> struct Lds {
     static constexpr bool half = FMT == F_F16S;
     static constexpr uint32_t esz = half ? 4 : 8;
-    static constexpr int pitch = half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH;
-    static constexpr int buf = kHmax * pitch;                 // entries per buffer
-    static_assert(pitch >= kWmax + 2 && (pitch * esz) % 16 == 0, "pitch");
+    static constexpr int wpitch = half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH;  // per window
+    static constexpr int pitch = MIR ? 2 * wpitch : wpitch;   // entries per LDS row
+    static constexpr int rows = MIR ? kHm : kHmax;            // rows per buffer
+    static constexpr int buf = rows * pitch;                  // entries per buffer
+    // one stager per quad and row pair of each window
+    static constexpr int stagers = (kWmax / 4) * (rows / 2) * (MIR ? 2 : 1);
+    static_assert(wpitch >= kWmax + 2 && (wpitch * esz) % 16 == 0, "pitch");
+    static_assert(stagers <= 256, "one staging thread per quad and row pair");
+    // the second read of a mirror pair: an immediate offset of ds_read2
+    static_assert(!MIR || wpitch + 1 <= 255, "ds_read2 offset");
 };
-static_assert(kSZ % kSB == 0, "batch divides the voxels per lane");
+static_assert(kSZ % kSB == 0 && kSZm % (kSB / 2) == 0, "batch divides the voxels per lane");
 constexpr int kQuads = kWmax / 4;
-constexpr int kStagers = kQuads * (kHmax / 2);   // threads that stage
-static_assert(kStagers <= 256, "one staging thread per quad and row pair");
 
 // Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
 // stored as int4 {clo | interior, rlo, W, H}.  clo is a multiple of 4;
@@ -380,7 +402,8 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
                                    const float* __restrict__ zds, int nx, int ny, int nz, int ns,
                                    int nt, float dsd_ds, float dso_ds, float ws, float wt,
                                    float inv_dt_ds, int is_arc, const float* __restrict__ zs,
-                                   int arc_t_exact, float so_ds, int4* __restrict__ win)
+                                   int arc_t_exact, float so_ds, int mirror,
+                                   int4* __restrict__ win)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i >= (int64_t)nblocks * nviews) return;
@@ -388,7 +411,10 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     const int tiles_x = (nx + kSP - 1) / kSP;
     const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
     const int tile = blk % tiles, zblk = blk / tiles;
-    const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP, iz0 = zblk * kSZ;
+    // a mirror block's window is the one of its voxels z > 0
+    const int zn = mirror ? kSZm : kSZ, hmax = mirror ? kHm : kHmax;
+    const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP;
+    const int iz0 = mirror ? nz / 2 + zblk * kSZm : zblk * kSZ;
     // extent of the block's voxel centres (any axis order)
     float x0 = xds[ix0], x1 = x0, y0 = yds[iy0], y1 = y0, z0 = zds[min(iz0, nz - 1)], z1 = z0;
     for (int k = 1; k < kSP; ++k) {
@@ -396,7 +422,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         x0 = fminf(x0, xv); x1 = fmaxf(x1, xv);
         y0 = fminf(y0, yv); y1 = fmaxf(y1, yv);
     }
-    for (int k = 1; k < kSZ; ++k) {
+    for (int k = 1; k < zn; ++k) {
         const float zv = zds[min(iz0 + k, nz - 1)];
         z0 = fminf(z0, zv); z1 = fmaxf(z1, zv);
     }
@@ -447,16 +473,17 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         if (chi >= clo && rhi >= rlo) {
             const int W = chi - clo + 1, H = rhi - rlo + 1;
             const bool interior = smin >= 2.f && smax < (float)(ns - 1) && tmin >= 2.f &&
-                                  tmax < (float)(nt - 2) && W <= kWmax && H <= kHmax;
-            w = make_int4(clo | (interior ? 1 : 0), rlo, min(W, kWmax), min(H, kHmax));
+                                  tmax < (float)(nt - 2) && W <= kWmax && H <= hmax;
+            w = make_int4(clo | (interior ? 1 : 0), rlo, min(W, kWmax), min(H, hmax));
         }
     }
     win[i] = w;
 }
 
-// One block's work: voxel block `blk` (16 x 16 columns x 16 z), all views,
-// with the block's LDS `lds` (2 * Lds<FMT>::buf entries).
-template <bool ARC, int FMT, bool OPT>
+// One block's work: voxel block `blk` (16 x 16 columns x 16 z; for the
+// mirror form 16 x 16 columns x 8 z > 0 and their 8 images), all views, with
+// the block's LDS `lds` (2 * Lds<FMT, MIR>::buf entries).
+template <bool ARC, int FMT, bool OPT, bool MIR>
 __device__ __forceinline__ void cbct_staged_block(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
@@ -466,8 +493,13 @@ __device__ __forceinline__ void cbct_staged_block(
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
     int arc_t_exact, int narrow, float so_ds, int blk, char* lds)
 {
-    using L = Lds<FMT>;
+    using L = Lds<FMT, MIR>;
     constexpr int kLP = L::pitch;
+    constexpr int NZ = MIR ? kSZm : kSZ;        // voxels per lane whose position is computed
+    constexpr int NV = MIR ? 2 * kSZm : kSZ;    // voxels per lane
+    constexpr int SB = MIR ? kSB / 2 : kSB;     // positions per batch (kSB LDS reads)
+    // byte offset of the mirror window's entries within an LDS row
+    constexpr uint32_t kMirOff = (uint32_t)L::wpitch * L::esz;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int tiles_x = (nx + kSP - 1) / kSP;
@@ -475,27 +507,31 @@ __device__ __forceinline__ void cbct_staged_block(
     const int tile = blk % tiles, zblk = blk / tiles;
     const int ix = (tile % tiles_x) * kSP + (wave & 1) * 8 + (lane & 7);
     const int iy = (tile / tiles_x) * kSP + (wave >> 1) * 8 + (lane >> 3);
-    const int iz0 = zblk * kSZ;
+    // voxel j of the lane: izb + j; for the mirror form j < kSZm are z > 0
+    // and kSZm + j the image of j
+    const int izb = MIR ? nz / 2 + zblk * kSZm : zblk * kSZ;
+    auto vox = [&](int j) { return (MIR && j >= kSZm) ? nz - 1 - (izb + j - kSZm) : izb + j; };
     const bool inxy = ix < nx && iy < ny;
     const int64_t nxy = (int64_t)nx * ny;
     const int64_t col = inxy ? ix + (int64_t)nx * iy : 0;
 
     // Every thread stays to the end: all of them stage and meet the barriers.
-    // Bit j of inbits: voxel iz0 + j exists and is inside the mask.
-    float z[kSZ], acc[kSZ];
+    // Bit j of inbits: voxel j exists and is inside the mask.
+    float z[NV], acc[NV];
     uint32_t inbits = 0;
 #pragma unroll
-    for (int j = 0; j < kSZ; ++j) {
-        const int iz = iz0 + j;
+    for (int j = 0; j < NV; ++j) {
+        const int iz = vox(j);
         const bool in = inxy && iz < nz && (mask == nullptr || mask[col + nxy * iz]);
         inbits |= (uint32_t)in << j;
         z[j] = zds[min(iz, nz - 1)];
         acc[j] = 0.f;
     }
     const bool any = inbits != 0;
+    // z range of the voxels whose position is computed
     float zlo = z[0], zhi = z[0];
 #pragma unroll
-    for (int j = 1; j < kSZ; ++j) {
+    for (int j = 1; j < NZ; ++j) {
         zlo = fminf(zlo, z[j]);
         zhi = fmaxf(zhi, z[j]);
     }
@@ -508,42 +544,51 @@ __device__ __forceinline__ void cbct_staged_block(
     const int4* bwin = win + (int64_t)blk * nviews;
 
     // Staging role: thread (quad sq, group sg) moves bins clo+4sq .. +3 of
-    // entry rows 2sg, 2sg+1, i.e. data rows 2sg .. 2sg+2 of the window.
-    const bool stager = tid < kStagers;
+    // entry rows 2sg, 2sg+1 of its window, i.e. data rows 2sg .. 2sg+2 of it.
+    // Mirror form: groups sg >= kHm / 2 stage the mirror window, whose entry
+    // row r reads the data rows nt - 1 - (rlo + r) downwards.
+    const bool stager = tid < L::stagers;
     const int sq = tid % kQuads, sg = tid / kQuads;
-    const uint32_t toff = ((uint32_t)(2 * sg) * (uint32_t)pitch + 4u * sq) * Fmt<FMT>::esz;
-    char* const sdst = lds + ((2 * sg) * kLP + 4 * sq) * L::esz;
+    const bool mst = MIR && sg >= kHm / 2;
+    const int sgw = mst ? sg - kHm / 2 : sg;          // row pair within the window
+    const uint32_t toff = ((uint32_t)(2 * sgw) * (uint32_t)pitch + 4u * sq) * Fmt<FMT>::esz;
+    char* const sdst = lds + ((2 * sgw) * kLP + (mst ? L::wpitch : 0) + 4 * sq) * L::esz;
     float sv[3][4];          // FP32 / BF16: the window's values, widened
     uint2 hv[3];             // F16S: the raw FP16 values (4 bins per row)
     auto stage_load = [&](int ia, int4 w) {
-        if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
+        if (!stager || 4 * sq >= w.z || 2 * sgw >= w.w) return;
         const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
-        const uint32_t off = toff + ((uint32_t)w.y * (uint32_t)pitch + (uint32_t)(w.x & ~3)) *
-                                        Fmt<FMT>::esz;
-        const bool third = 2 * sg + 2 <= w.w;              // data row 2sg+2 needed
+        const uint32_t cb = (uint32_t)(w.x & ~3) * Fmt<FMT>::esz;
+        // offset of data row 2sgw (+ k rows in the window's direction)
+        const uint32_t off = mst ? ((uint32_t)(nt - 1 - w.y - 2 * sgw) * (uint32_t)pitch + 4u * sq) *
+                                           Fmt<FMT>::esz + cb
+                                 : toff + (uint32_t)w.y * (uint32_t)pitch * Fmt<FMT>::esz + cb;
+        const uint32_t step = mst ? 0u - rowb : rowb;
+        const bool third = 2 * sgw + 2 <= w.w;             // data row 2sgw+2 needed
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             if (k == 2 && !third) break;
+            const uint32_t o = off + (uint32_t)k * step;
             if (FMT == F_F32) {
-                const auto u = __builtin_amdgcn_raw_buffer_load_b128(rs, off + k * rowb, 0, 0);
+                const auto u = __builtin_amdgcn_raw_buffer_load_b128(rs, o, 0, 0);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) sv[k][e] = __uint_as_float(u[e]);
             } else if (FMT == F_BF16) {
-                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, off + k * rowb, 0, 0);
+                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0);
                 const float2 e01 = dec2<FMT>(u[0]), e23 = dec2<FMT>(u[1]);
                 sv[k][0] = e01.x; sv[k][1] = e01.y; sv[k][2] = e23.x; sv[k][3] = e23.y;
             } else {
-                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, off + k * rowb, 0, 0);
+                const auto u = __builtin_amdgcn_raw_buffer_load_b64(rs, o, 0, 0);
                 hv[k] = make_uint2(u[0], u[1]);
             }
         }
     };
     auto stage_store = [&](int buf, int4 w) {
-        if (!stager || 4 * sq >= w.z || 2 * sg >= w.w) return;
+        if (!stager || 4 * sq >= w.z || 2 * sgw >= w.w) return;
         char* d = sdst + buf * (L::buf * L::esz);
 #pragma unroll
         for (int k = 0; k < 2; ++k) {
-            if (2 * sg + k >= w.w) break;
+            if (2 * sgw + k >= w.w) break;
             if (L::half) {
                 // row differences of bin pairs in FP16, then entries (a, d) per bin
                 const uint32_t d01 = half2_sub(hv[k + 1].x, hv[k].x);
@@ -593,7 +638,9 @@ __device__ __forceinline__ void cbct_staged_block(
         // Window coordinates of the lane's taps: entry columns ce, ce+1 and
         // entry rows (int)bt - 1 - rlo for bt from blo to bhi.  The test
         // implies bs, blo, bhi >= 1 (truncation is floor there) and the
-        // detector bounds 1 <= floor(bs) < ns, 1 <= floor(bt) < nt.
+        // detector bounds 1 <= floor(bs) < ns, 1 <= floor(bt) < nt.  The
+        // mirror images sit at nt + 1 - bt: they need bt > 1 (their floor
+        // below nt), which an interior window gives with a row to spare.
         r.ce = (int)r.bs - 1 - (w.x & ~3);
         r.inwin = true;
         r.blo = r.bhi = 0.f;
@@ -603,7 +650,7 @@ __device__ __forceinline__ void cbct_staged_block(
             const int rl = (int)r.blo - 1 - w.y, rh = (int)r.bhi - 1 - w.y;
             const uint32_t wlim = w.z > 1 ? (uint32_t)(w.z - 1) : 0u;
             r.inwin = (uint32_t)r.ce < wlim && (uint32_t)rl < (uint32_t)w.w &&
-                      (uint32_t)rh < (uint32_t)w.w;
+                      (uint32_t)rh < (uint32_t)w.w && (!MIR || fminf(r.blo, r.bhi) > 1.f);
         }
         return r;
     };
@@ -626,35 +673,52 @@ __device__ __forceinline__ void cbct_staged_block(
                 const uint32_t lb = ((uint32_t)lv.ce +
                                      (uint32_t)(buf * L::buf - (wcur.y + 1) * kLP)) * L::esz;
 #pragma unroll
-                for (int h = 0; h < kSZ; h += kSB) {
-                    float4 tp[kSB];        // (a0, d0, a1, d1)
-                    float wu[kSB];
+                for (int h = 0; h < NZ; h += SB) {
+                    float4 tp[SB], tm[SB];    // (a0, d0, a1, d1); tm: the mirror window
+                    float wu[SB];
 #pragma unroll
-                    for (int j = 0; j < kSB; ++j) {
+                    for (int j = 0; j < SB; ++j) {
                         const float bt = fmaf(lv.mz, z[h + j], lv.wt);
                         wu[j] = __builtin_amdgcn_fractf(bt);
                         // one in-bounds offset (the 32-bit sum wraps to the entry's offset)
-                        const char* p = lds_bytes +
-                            (int)(__umul24((uint32_t)bt, (uint32_t)(kLP * L::esz)) + lb);
+                        const uint32_t o =
+                            __umul24((uint32_t)bt, (uint32_t)(kLP * L::esz)) + lb;
+                        const char* p = lds_bytes + (int)o;
                         if (L::half) {
                             const uint32_t* u = reinterpret_cast<const uint32_t*>(p);
                             const uint32_t u0 = u[0], u1 = u[1];
                             tp[j] = make_float4(f16f(u0 & 0xffffu), f16f(u0 >> 16),
                                                 f16f(u1 & 0xffffu), f16f(u1 >> 16));
+                            if (MIR) {
+                                const uint32_t* m = reinterpret_cast<const uint32_t*>(p + kMirOff);
+                                const uint32_t m0 = m[0], m1 = m[1];
+                                tm[j] = make_float4(f16f(m0 & 0xffffu), f16f(m0 >> 16),
+                                                    f16f(m1 & 0xffffu), f16f(m1 >> 16));
+                            }
                         } else {
                             const float2* f = reinterpret_cast<const float2*>(p);
                             const float2 e0 = f[0], e1 = f[1];
                             tp[j] = make_float4(e0.x, e0.y, e1.x, e1.y);
+                            if (MIR) {
+                                const float2* m = reinterpret_cast<const float2*>(p + kMirOff);
+                                const float2 m0 = m[0], m1 = m[1];
+                                tm[j] = make_float4(m0.x, m0.y, m1.x, m1.y);
+                            }
                         }
                     }
                     // every LDS read of the batch is issued before its
                     // arithmetic (the scheduler otherwise keeps two in flight)
                     if (SGX_STAGED_SCHED) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                    for (int j = 0; j < kSB; ++j) {
+                    for (int j = 0; j < SB; ++j) {
                         const float p0 = fmaf(wu[j], tp[j].y, tp[j].x);
                         const float p1 = fmaf(wu[j], tp[j].w, tp[j].z);
                         acc[h + j] = fmaf(wb, p1, fmaf(wa, p0, acc[h + j]));
+                        if (MIR) {
+                            const float m0 = fmaf(wu[j], tm[j].y, tm[j].x);
+                            const float m1 = fmaf(wu[j], tm[j].w, tm[j].z);
+                            acc[kSZm + h + j] = fmaf(wb, m1, fmaf(wa, m0, acc[kSZm + h + j]));
+                        }
                     }
                 }
             } else {
@@ -667,8 +731,17 @@ __device__ __forceinline__ void cbct_staged_block(
     }
 
     // Views whose taps missed the staged window (detector edges in t, windows
-    // larger than the LDS buffer): gathered from global memory, lane by lane.
+    // larger than the LDS buffer): gathered from global memory, lane by lane,
+    // each voxel at its own z (the mirror images too).
     if (missed) {
+        float zlo2 = z[NV - 1], zhi2 = z[NV - 1];
+        if (MIR) {
+#pragma unroll
+            for (int j = kSZm; j < NV; ++j) {
+                zlo2 = fminf(zlo2, z[j]);
+                zhi2 = fmaxf(zhi2, z[j]);
+            }
+        }
         for (int ia = 0; ia < nviews; ++ia) {
             const LaneView lv = lane_view(ia, bwin[ia], true);
             if (lv.inwin || !(lv.bs >= 1.f && lv.bs < nsf)) continue;    // 1 <= floor(bs) < ns
@@ -678,10 +751,14 @@ __device__ __forceinline__ void cbct_staged_block(
             cv.wa = lv.w2 - cv.wb;
             cv.mz = lv.mz;
             cv.wt = lv.wt;
+            const __amdgpu_buffer_rsrc_t rs = view_rsrc(qb + ia * viewb, vbytes);
             const float tlo = fminf(lv.blo, lv.bhi), thi = fmaxf(lv.blo, lv.bhi);
-            const bool fast = tlo >= 1.f && thi < ntf;
-            gather_view<kSZ, FMT>(cv, z, lv.wt, ntf, fast, view_rsrc(qb + ia * viewb, vbytes),
-                                  rowb, acc);
+            gather_view<NZ, FMT>(cv, z, lv.wt, ntf, tlo >= 1.f && thi < ntf, rs, rowb, acc);
+            if (MIR) {
+                const float b0 = fmaf(lv.mz, zlo2, lv.wt), b1 = fmaf(lv.mz, zhi2, lv.wt);
+                const bool fast = fminf(b0, b1) >= 1.f && fmaxf(b0, b1) < ntf;
+                gather_view<NZ, FMT>(cv, z + kSZm, lv.wt, ntf, fast, rs, rowb, acc + kSZm);
+            }
         }
     }
 
@@ -689,12 +766,14 @@ __device__ __forceinline__ void cbct_staged_block(
     // no second mask read inside a branch).
     if (inxy) {
 #pragma unroll
-        for (int j = 0; j < kSZ; ++j)
-            if (iz0 + j < nz) image[col + nxy * (iz0 + j)] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
+        for (int j = 0; j < NV; ++j) {
+            const int iz = vox(j);
+            if (iz < nz) image[col + nxy * iz] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
+        }
     }
 }
 
-template <bool ARC, int FMT, bool OPT>
+template <bool ARC, int FMT, bool OPT, bool MIR>
 __global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
 cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
@@ -705,12 +784,12 @@ cbct_staged_kernel(
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
     int arc_t_exact, int narrow, float so_ds)
 {
-    using L = Lds<FMT>;
+    using L = Lds<FMT, MIR>;
     __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
-    cbct_staged_block<ARC, FMT, OPT>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds, yds,
-                                     zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws, wt,
-                                     inv_dt_ds, win, zs, arc_t_exact, narrow, so_ds, blockIdx.x,
-                                     lds);
+    cbct_staged_block<ARC, FMT, OPT, MIR>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds,
+                                          yds, zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws,
+                                          wt, inv_dt_ds, win, zs, arc_t_exact, narrow, so_ds,
+                                          blockIdx.x, lds);
 }
 
 // Batch form: one launch over the tasks (item, voxel block) of many
@@ -718,22 +797,23 @@ cbct_staged_kernel(
 // (largest tasks first, interleaved so that block b % 8 keeps to one XCD's
 // share).  The hardware dispatches blocks in index order, so the launch
 // walks the table in order and backfills the GPU as blocks finish.
-template <bool ARC, int FMT, bool OPT>
+template <bool ARC, int FMT, bool OPT, bool MIR>
 __global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
 cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
 {
-    using L = Lds<FMT>;
+    using L = Lds<FMT, MIR>;
     __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
     const int2 tk = tasks[blockIdx.x];
     // through the constant address space: invariant loads, as for kernel
     // arguments
     using CItem = const __attribute__((address_space(4))) CbctItem;
     const CItem& it = ((CItem*)(items))[tk.x];
-    cbct_staged_block<ARC, FMT, OPT>(it.q, it.ns, it.nt, it.pitch, it.nviews, it.view_stride,
-                                     it.cs, it.vscale, it.xds, it.yds, it.zds, it.nx, it.ny, it.nz,
-                                     it.mask, it.image, it.dsd_ds, it.dso_ds, it.ws, it.wt,
-                                     it.inv_dt_ds, it.win, it.zs, it.arc_t_exact, it.narrow,
-                                     it.so_ds, tk.y, lds);
+    cbct_staged_block<ARC, FMT, OPT, MIR>(it.q, it.ns, it.nt, it.pitch, it.nviews,
+                                          it.view_stride, it.cs, it.vscale, it.xds, it.yds,
+                                          it.zds, it.nx, it.ny, it.nz, it.mask, it.image,
+                                          it.dsd_ds, it.dso_ds, it.ws, it.wt, it.inv_dt_ds,
+                                          it.win, it.zs, it.arc_t_exact, it.narrow, it.so_ds,
+                                          tk.y, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1063,6 +1143,9 @@ int arc_narrow(const CbctArgs& a)
                            0.75f * a.dsd_ds;
 }
 
+// Blocks of the staged (and packed) kernels: every 16 x 16 column tile
+// times the z blocks of 16 voxels (the mirror form: 8 voxels z > 0 and their
+// images, the same count for its even nz / 16).
 int64_t staged_blocks(const CbctArgs& a)
 {
     const int64_t tiles = (int64_t)((a.nx + kSP - 1) / kSP) * ((a.ny + kSP - 1) / kSP);
@@ -1084,13 +1167,14 @@ void launch_gather(const CbctArgs& a, hipStream_t st)
     SGX_HIP(hipGetLastError());
 }
 
-template <bool ARC, int FMT, bool OPT>
+template <bool ARC, int FMT, bool OPT, bool MIR>
 void launch_staged(const CbctArgs& a, hipStream_t st)
 {
     const int64_t nb = staged_blocks(a);
     if (nb >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
-    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT>), dim3((unsigned)nb), dim3(256), 0, st,
+    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT, MIR>), dim3((unsigned)nb), dim3(256), 0,
+                       st,
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
                        a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0,
@@ -1112,6 +1196,18 @@ void launch_packed(const CbctArgs& a, hipStream_t st)
     SGX_HIP(hipGetLastError());
 }
 
+}  // namespace
+
+bool cbct_mirror_ok(const CbctArgs& a, bool z_symmetric)
+{
+    // bt(-z) = nt + 1 - bt(z) needs the source in the plane z = 0 and the t
+    // centre on the middle row: wt = (nt + 1) / 2 exactly
+    return z_symmetric && a.zs == nullptr && a.nz % (2 * kSZm) == 0 &&
+           a.wt == (float)((a.nt + 1) / 2.0) && !(a.fmt == SGX_Q_FMT_F16S && a.f16_packed);
+}
+
+namespace {
+
 // The scaled-FP16 form takes the packed kernel when asked for it.
 bool packed_form(const CbctArgs& a)
 {
@@ -1123,9 +1219,12 @@ template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
     if (packed_form(a)) {
         if (a.is_arc) launch_packed<true, OPT>(a, st);
         else launch_packed<false, OPT>(a, st);
+    } else if (a.win && a.mirror) {
+        if (a.is_arc) launch_staged<true, FMT, OPT, true>(a, st);
+        else launch_staged<false, FMT, OPT, true>(a, st);
     } else if (a.win) {
-        if (a.is_arc) launch_staged<true, FMT, OPT>(a, st);
-        else launch_staged<false, FMT, OPT>(a, st);
+        if (a.is_arc) launch_staged<true, FMT, OPT, false>(a, st);
+        else launch_staged<false, FMT, OPT, false>(a, st);
     } else {
         if (a.is_arc) launch_gather<8, true, FMT, OPT>(a, st);
         else launch_gather<8, false, FMT, OPT>(a, st);
@@ -1176,7 +1275,7 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
     hipLaunchKernelGGL(cbct_window_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        (int)nb, a.nviews, a.cs, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.ns, a.nt,
                        a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, a.zs,
-                       a.arc_t_exact ? 1 : 0, a.so_ds, win);
+                       a.arc_t_exact ? 1 : 0, a.so_ds, a.mirror && !packed_form(a) ? 1 : 0, win);
     SGX_HIP(hipGetLastError());
 }
 
@@ -1185,7 +1284,8 @@ int cbct_variant(const CbctArgs& a)
     const int fmt = a.fmt == SGX_Q_FMT_BF16 ? F_BF16 : a.fmt == SGX_Q_FMT_F16S ? F_F16S : F_F32;
     const bool opt = a.zs || a.arc_t_exact || a.so_ds != 0.f;
     return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant) +
-           (packed_form(a) ? kPackedVariant : 0);
+           (packed_form(a) ? kPackedVariant : 0) +
+           (a.win && a.mirror && !packed_form(a) ? kMirrorVariant : 0);
 }
 
 CbctItem cbct_item(const CbctArgs& a)
@@ -1243,10 +1343,10 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_gather_queue_kernel(
                                         it.so_ds, tk.y);
 }
 
-template <bool ARC, int FMT, bool OPT>
+template <bool ARC, int FMT, bool OPT, bool MIR>
 static const void* cbq_fn()
 {
-    return reinterpret_cast<const void*>(&cbct_queue_kernel<ARC, FMT, OPT>);
+    return reinterpret_cast<const void*>(&cbct_queue_kernel<ARC, FMT, OPT, MIR>);
 }
 
 template <bool ARC, int FMT, bool OPT>
@@ -1277,13 +1377,19 @@ static const void* cbct_queue_fn(int variant)
         SGX_CBG(F_F32);
 #undef SGX_CBG
     }
+#define SGX_CBQ2(F, M)                                                               \
+    return arc ? (opt ? cbq_fn<true, F, true, M>() : cbq_fn<true, F, false, M>())    \
+               : (opt ? cbq_fn<false, F, true, M>() : cbq_fn<false, F, false, M>())
 #define SGX_CBQ(F)                                                                   \
-    return arc ? (opt ? cbq_fn<true, F, true>() : cbq_fn<true, F, false>())          \
-               : (opt ? cbq_fn<false, F, true>() : cbq_fn<false, F, false>())
+    do {                                                                             \
+        if (variant & kMirrorVariant) SGX_CBQ2(F, true);                             \
+        SGX_CBQ2(F, false);                                                          \
+    } while (0)
     if (fmt == F_BF16) SGX_CBQ(F_BF16);
     if (fmt == F_F16S) SGX_CBQ(F_F16S);
     SGX_CBQ(F_F32);
 #undef SGX_CBQ
+#undef SGX_CBQ2
 }
 
 void launch_cbct_queue(int variant, const CbctItem* items, const int2* tasks, int ntasks,

@@ -62,6 +62,7 @@ struct PlanTune {
     int cbct = 0;
     int want_blocks = 2048;
     int vc_max = 64;
+    int mirror = 0;      // cone beam, staged kernel: its mirror form (where the geometry allows)
 };
 
 // Largest view-grouping target and smallest LDS chunk the autotuner tries.
@@ -72,6 +73,7 @@ struct sgx_ctx {
     int device = 0;
     std::map<std::string, PlanTune> tune_cache;   // autotuned shapes per size bucket
     int cbct_kernel = 0;          // SGX_OPT_CBCT_KERNEL
+    int cbct_mirror = 1;          // SGX_OPT_CBCT_MIRROR
     std::string scratch_err;      // why the device scratch is unusable (empty: usable)
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -388,7 +390,13 @@ int sgx_ctx_synchronize(sgx_ctx* ctx)
 int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
 {
     return guard(ctx, [&] {
-        require(option == SGX_OPT_CBCT_KERNEL, SGX_ERR_ARG, "unknown option");
+        require(option == SGX_OPT_CBCT_KERNEL || option == SGX_OPT_CBCT_MIRROR, SGX_ERR_ARG,
+                "unknown option");
+        if (option == SGX_OPT_CBCT_MIRROR) {
+            require(value >= 0 && value <= 2, SGX_ERR_ARG, "SGX_OPT_CBCT_MIRROR is 0, 1 or 2");
+            ctx->cbct_mirror = (int)value;
+            return;
+        }
         require(value >= 0 && value <= 2, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0, 1 or 2");
         ctx->cbct_kernel = (int)value;
     });
@@ -396,8 +404,9 @@ int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
 
 int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value)
 {
-    if (!ctx || !value || option != SGX_OPT_CBCT_KERNEL) return SGX_ERR_ARG;
-    *value = ctx->cbct_kernel;
+    if (!ctx || !value || (option != SGX_OPT_CBCT_KERNEL && option != SGX_OPT_CBCT_MIRROR))
+        return SGX_ERR_ARG;
+    *value = option == SGX_OPT_CBCT_MIRROR ? ctx->cbct_mirror : ctx->cbct_kernel;
     return SGX_OK;
 }
 
@@ -435,7 +444,17 @@ struct CbctCall {
     size_t o_b = 0, o_x = 0, o_y = 0, o_z = 0, o_cs = 0, o_zs = 0;
     int64_t nv = 0;
     bool has_zs = false;
+    bool z_sym = false;     // the z axis (/ ds, FP32) is symmetric: z[nz-1-k] = -z[k]
 };
+
+// z[n-1-k] == -z[k] for every k, exactly in FP32
+bool axis_symmetric(const std::vector<float>& z)
+{
+    const size_t n = z.size();
+    for (size_t k = 0; k < n; ++k)
+        if (z[n - 1 - k] != -z[k]) return false;
+    return true;
+}
 
 void cbct_check(const void* proj, int64_t ns, int64_t nt, int64_t na, int64_t ia_skip,
                 const double* betas, double ds, double dt, const double* xc, int64_t nx,
@@ -465,6 +484,7 @@ CbctCall cbct_call_tables(int64_t na, int64_t ia_skip, const double* betas, doub
     T.o_x = T.pack.add(xv.data(), xv.size() * 4);
     T.o_y = T.pack.add(yv.data(), yv.size() * 4);
     T.o_z = T.pack.add(zv.data(), zv.size() * 4);
+    T.z_sym = axis_symmetric(zv);
     T.o_cs = T.pack.add(b.data(), T.nv * sizeof(sgx::ViewCS));   // space only
     T.has_zs = opts && opts->source_zs;
     if (T.has_zs) {
@@ -551,6 +571,9 @@ void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int6
     sgx::CbctArgs a = cbct_call_args(T, d, q, (flags & SGX_Q_BF16) != 0, ns, nt, pitch, ia_skip,
                                      dsd, dso, ds, dt, offset_s, offset_t, is_arc, nx, ny, nz, m,
                                      img, opts);
+    // the FP32 and BF16 data of this entry point take the mirror form only when
+    // it is asked for everywhere (it measured slower for them)
+    a.mirror = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym);
     if (ctx->cbct_kernel != 1) {
         int4* win = static_cast<int4*>(
             ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
@@ -614,6 +637,7 @@ int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t n
         sgx::CbctArgs a = cbct_call_args(T, base, proj, false, ns, nt, ns, ia_skip, dsd, dso, ds,
                                          dt, offset_s, offset_t, is_arc, nx, ny, nz, mask, image,
                                          nullptr);
+        a.mirror = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym);
         const bool staged = ctx->cbct_kernel != 1;
         const size_t need =
             tbytes + (staged ? sizeof(int4) * (size_t)sgx::cbct_window_entries(a) : 0);
@@ -865,6 +889,9 @@ struct sgx_plan {
     float* zs = nullptr;            // cone-beam option: per-view source z / ds (or null)
     bool arc_t_exact = false;       // cone-beam option: exact arc-detector t magnification
     float so_ds = 0.f;              // cone-beam option: source offset / ds
+    bool z_sym = false;             // cone-beam: the z axis is symmetric about 0
+    bool mirror_allowed = true;     // cone-beam: the autotuner may try the mirror form
+    bool win_mirror = false;        // the form `win` was computed for
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
     // Bumped by every change that a captured batch graph would not see
@@ -963,7 +990,25 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     a.arc_t_exact = P.arc_t_exact;
     a.so_ds = P.so_ds;
     a.f16_packed = P.tune.cbct == 2;
+    a.mirror = P.tune.mirror && sgx::cbct_mirror_ok(a, P.z_sym);
     return a;
+}
+
+// The staged backprojector's window table of a cone-beam plan, for its
+// current form (the mirror form's blocks have their own windows).
+void plan_windows(sgx_plan& P, hipStream_t st)
+{
+    const sgx::CbctArgs a = cbct_args_of(P, nullptr);
+    sgx::launch_cbct_windows(a, P.win, st);
+    P.win_mirror = a.mirror;
+}
+
+// Recompute the window table when the plan's form changed since (a tuning
+// choice): a no-op otherwise.
+void ensure_windows(sgx_plan& P, hipStream_t st)
+{
+    if (P.is3d() && P.win && cbct_args_of(P, nullptr).mirror != P.win_mirror)
+        plan_windows(P, st);
 }
 
 // One plan execute on stream `st` (device pointers): weights + filter, then
@@ -1014,6 +1059,7 @@ void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool tim
     plan_filter_phase(P, in, st);
     if (timed) SGX_HIP(hipEventRecord(P.ev[1], st));
     if (P.is3d()) {
+        ensure_windows(P, st);
         sgx::CbctArgs a = cbct_args_of(P, img);
         if (P.tune.cbct == 1) a.win = nullptr;
         sgx::launch_cbct_back(a, st);
@@ -1032,6 +1078,12 @@ std::vector<PlanTune> tune_candidates(const sgx_plan& P)
     std::vector<PlanTune> c;
     if (P.is3d()) {
         for (int k : {0, 1}) c.push_back(PlanTune{k, 2048, 64});
+        sgx_plan& Q = const_cast<sgx_plan&>(P);
+        const int m = Q.tune.mirror;
+        Q.tune.mirror = 1;
+        const bool can_mirror = P.mirror_allowed && cbct_args_of(Q, nullptr).mirror;
+        Q.tune.mirror = m;
+        if (can_mirror) c.push_back(PlanTune{0, 2048, 64, 1});
         if (P.prec == SGX_PREC_F16S) c.push_back(PlanTune{2, 2048, 64});
     } else {
         for (int want : {1024, 2048, kTuneMaxBlocks})
@@ -1147,6 +1199,10 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->nz = is3d ? nz : 1;
         P->npad = npad;
         P->tune.cbct = ctx->cbct_kernel;
+        // the mirror form: by default for the scaled-FP16 form, where it
+        // measured faster; the autotuner times it for the others
+        P->mirror_allowed = ctx->cbct_mirror != 0;
+        P->tune.mirror = ctx->cbct_mirror == 2 || (ctx->cbct_mirror == 1 && prec == SGX_PREC_F16S);
 
         const int64_t ns = G.ns, nt = G.nt, na = G.na;
         TablePack pack;
@@ -1160,6 +1216,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         if (is3d) {
             const auto zv = axis_over(zc, nz, G.ds);
             o_z = pack.add(zv.data(), nz * 4);
+            P->z_sym = axis_symmetric(zv);
         }
         const auto gd = filter_kernel_g(Hk, npad);
         std::vector<float> gf(gd.begin(), gd.end());
@@ -1301,7 +1358,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
             // the staged backprojector's detector windows, fixed by the geometry
             const int64_t nw = sgx::cbct_window_entries(cbct_args_of(*P, nullptr));
             SGX_HIP(hipMalloc(&P->win, sizeof(int4) * nw));
-            sgx::launch_cbct_windows(cbct_args_of(*P, nullptr), P->win, ctx->stream);
+            plan_windows(*P, ctx->stream);
         }
         for (hipEvent_t& e : P->ev) SGX_HIP(hipEventCreate(&e));
         SGX_HIP(hipStreamSynchronize(ctx->stream));
@@ -1367,7 +1424,7 @@ int sgx_plan_set_cbct_opts(sgx_plan* plan, const sgx_cbct_opts* opts)
         P.so_ds = opts ? (float)opts->source_offset / (float)P.geom.ds : 0.f;
         ++P.gen;
         // the staged backprojector's windows follow the t mapping
-        sgx::launch_cbct_windows(cbct_args_of(P, nullptr), P.win, plan->ctx->stream);
+        plan_windows(P, plan->ctx->stream);
         SGX_HIP(hipStreamSynchronize(plan->ctx->stream));
     });
 }
@@ -1477,7 +1534,8 @@ int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
 int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4])
 {
     if (!plan || !values) return SGX_ERR_ARG;
-    values[0] = plan->tune.cbct;
+    values[0] = plan->tune.cbct == 0 && plan->tune.mirror && plan->win_mirror ? 3
+                                                                                : plan->tune.cbct;
     values[1] = plan->tune.want_blocks;
     values[2] = plan->tune.vc_max;
     values[3] = plan->tuned ? 1 : 0;
@@ -1697,6 +1755,8 @@ struct sgx_batch {
     void capture(bool use_queues)
     {
         constexpr int kBranches = 4;
+        // every cone-beam plan's window table matches its form before capture
+        for (const Item& it : items) ensure_windows(*it.plan, ctx->stream);
         build_queues(use_queues);
         std::vector<bool> in_queue(items.size(), false);
         for (const Queue& Q : queues)

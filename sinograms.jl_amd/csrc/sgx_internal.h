@@ -118,8 +118,14 @@ struct CbctArgs {
     // scaled-FP16 data with windows: true takes the packed-FP16 kernel in
     // place of the staged kernel's FP32 interpolation
     bool f16_packed = false;
+    // staged kernel: the mirror form (pairs z and -z; see cbct_mirror_ok)
+    bool mirror = false;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
+// Whether the staged kernel may take its mirror form for `a`: the z axis
+// is symmetric (z[nz-1-k] = -z[k], as the caller checks on its axis), no
+// per-view source z, the t centre on the middle row, nz a multiple of 16.
+bool cbct_mirror_ok(const CbctArgs& a, bool z_symmetric);
 
 // Batch (task-table) form of the LDS-staged cone-beam backprojector: one
 // item per reconstruction (device array), tasks (item, voxel block); block b

@@ -185,8 +185,12 @@ def _declare(L):
 
 
 SGX_OPT_CBCT_KERNEL = 1
+SGX_OPT_CBCT_MIRROR = 2
+CBCT_MIRROR = ("never", "fp16", "always")    # values of SGX_OPT_CBCT_MIRROR
 # values of SGX_OPT_CBCT_KERNEL
 CBCT_KERNELS = ("staged", "gather", "staged_packed16")
+# cone-beam kernels a plan reports (sgx_plan_tuning): the above and the mirror form
+PLAN_CBCT_KERNELS = CBCT_KERNELS + ("staged_mirror",)
 SGX_BATCH_NO_TUNE = 0x100
 SGX_BATCH_NO_QUEUE = 0x200
 SGX_BATCH_QUEUE = 0x400
@@ -263,6 +267,21 @@ class Context:
     def cbct_kernel(self, name: str):
         self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_CBCT_KERNEL,
                                              CBCT_KERNELS.index(name)))
+
+    @property
+    def cbct_mirror(self) -> str:
+        """When the staged cone-beam backprojector takes its mirror form (each
+        voxel z > 0 paired with its image at -z, where the geometry allows):
+        "fp16" (for scaled-FP16 plans, the default; the autotuner times it for
+        the others), "always" or "never"."""
+        v = ctypes.c_int64()
+        self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_CBCT_MIRROR, ctypes.byref(v)))
+        return CBCT_MIRROR[v.value]
+
+    @cbct_mirror.setter
+    def cbct_mirror(self, when: str):
+        self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_CBCT_MIRROR,
+                                             CBCT_MIRROR.index(when)))
 
     def close(self):
         if self.h:

@@ -37,11 +37,11 @@ def autotune(plan: _Plan, sino, out, reps: int = 3, use_cache: bool = True) -> d
 
 def tuning(plan: _Plan) -> dict:
     """The plan's kernel shape: cone-beam backprojector (one of
-    ``_lib.CBCT_KERNELS``),
+    ``_lib.PLAN_CBCT_KERNELS``),
     2-D view-grouping target and LDS view chunk, and whether it was tuned."""
     v = (ctypes.c_int64 * 4)()
     plan._ctx.check(_lib.load().sgx_plan_tuning(plan.h, v))
-    return {"cbct_kernel": _lib.CBCT_KERNELS[v[0]], "want_blocks": v[1], "view_chunk": v[2],
+    return {"cbct_kernel": _lib.PLAN_CBCT_KERNELS[v[0]], "want_blocks": v[1], "view_chunk": v[2],
             "tuned": bool(v[3])}
 
 

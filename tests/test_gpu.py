@@ -86,6 +86,52 @@ def test_cbct_back_fp32_vs_oracle(sg, orc, torch_cuda, cbct_kernel, arc, dims, i
     assert np.array_equal(got_d, got)
 
 
+@pytest.mark.parametrize("arc", [False, True])
+@pytest.mark.parametrize("case", ["interior", "edges", "masked_skip", "exact_t"])
+def test_cbct_back_mirror_vs_oracle(sg, orc, torch_cuda, arc, case):
+    # The mirror form of the staged kernel: with the z axis symmetric about 0,
+    # offset_t = 0 and no source z, one detector position serves a voxel
+    # z > 0 and its image at -z (bt(-z) = nt + 1 - bt(z)).  "edges": a short
+    # detector, so that windows at the top and bottom reach its edge and
+    # lanes take the gather path; "exact_t": the exact arc t magnification,
+    # which keeps the symmetry.  Against the Float32 oracle, and against the
+    # same kernel with the mirror form off.
+    _say("cbct_back mirror", arc, case)
+    rng = np.random.default_rng(50)
+    nt = 20 if case == "edges" else 48
+    rg = (sg.CtFanArc if arc else sg.CtFanFlat)(ns=64, nt=nt, ds=1.1, dt=1.3, na=36,
+                                                 offset_s=0.25, offset_t=0.0, dsd=120.0,
+                                                 dod=50.0)
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    dims = (24, 20, 32)
+    ig = sg.ImageGeom(dims, (1.0, 0.9, 1.1), (0.3, 0.0, 0.0))
+    ia_skip = 1
+    if case == "masked_skip":
+        ig = ig.with_mask(rng.random(dims) > 0.3)
+        ia_skip = 2
+    exact = case == "exact_t"
+    xc, yc, zc = ig.axes()
+    assert np.array_equal(zc[::-1], -zc)
+    ctx = sg._lib.context(0)
+    try:
+        ctx.cbct_mirror = "always"
+        got = sg.cbct_back(proj, rg, ig, ia_skip=ia_skip, arc_t_exact=exact)
+        ctx.cbct_mirror = "never"
+        got0 = sg.cbct_back(proj, rg, ig, ia_skip=ia_skip, arc_t_exact=exact)
+    finally:
+        ctx.cbct_mirror = "fp16"
+    ref = orc.cbct_back(proj, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, rg.offset_s,
+                        rg.offset_t, arc, xc, yc, zc, ig.mask, ia_skip=ia_skip,
+                        dtype=np.float32, arc_t_exact=exact)
+    e, e0 = rel_rmse(got, ref), rel_rmse(got, got0)
+    _say(f"  vs Float32 oracle {e:.3g}, vs the form without mirror {e0:.3g}")
+    assert np.all(got[~ig.mask] == 0)
+    assert e < TOL32
+    assert e0 < 2e-6
+    if case == "edges":
+        assert np.any(ref == 0)          # voxels that see no detector row in some view
+
+
 def test_cbct_back_detector_edges(sg, orc, cbct_kernel):
     # voxels projecting beyond every detector edge, including NaN-free far field
     _say("cbct_back edges", cbct_kernel)
@@ -679,12 +725,21 @@ def test_fdk_stored_data_exact_vs_oracle(sg, orc, torch_cuda, prec):
     # The backprojection reads exactly the stored filtered data: widen the
     # plan's stored values exactly, apply the per-view scale and backproject
     # them with the Float32 oracle.
+    # The z axis is symmetric, so the staged kernel's mirror form would serve
+    # the lower half from mirrored windows; the FP16 form's pair model below
+    # (upper tap a + fp16(b - a)) is the non-mirror kernel's, so that form is
+    # checked with the mirror form off, and the mirror form after it.
     _say("fdk stored-data parity", prec)
     rg = sg.CtFanArc(ns=66, nt=48, ds=1.0, dt=1.2, na=40, dsd=300.0, dod=100.0)
     ig = sg.ImageGeom((24, 24, 16))
     rng = np.random.default_rng(7)
     proj = rng.standard_normal(rg.dims()).astype(np.float32)
-    plan = sg.plan_fbp(rg, ig, precision=prec)
+    ctx = sg._lib.context(0)
+    ctx.cbct_mirror = "never"
+    try:
+        plan = sg.plan_fbp(rg, ig, precision=prec)
+    finally:
+        ctx.cbct_mirror = "fp16"
     got = sg.fdk(plan, proj)
     q, scale = plan.read_filtered()
     raw = plan.read_stored()
@@ -704,6 +759,18 @@ def test_fdk_stored_data_exact_vs_oracle(sg, orc, torch_cuda, prec):
         ref = orc.cbct_back_pairs(a.astype(np.float64) * sc, d.astype(np.float64) * sc,
                                   rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, True,
                                   xc, yc, zc, ig.mask, dtype=np.float32)
+        # The mirror form reads the lower half's upper taps as stored values
+        # (b, not a + fp16(b - a)): against the exact interpolation of the
+        # stored values it is at least as close as the pair model.
+        exact = orc.cbct_back(a.astype(np.float64) * sc, rg.ar(f32=True), rg.dsd, rg.dso,
+                              rg.ds, rg.dt, 0.0, 0.0, True, xc, yc, zc, ig.mask)
+        pm = sg.plan_fbp(rg, ig, precision=prec)
+        assert sg.tuning(pm)["cbct_kernel"] == "staged_mirror"       # the form's default
+        got_m = sg.fdk(pm, proj)
+        e_m, e_p = rel_rmse(got_m, exact), rel_rmse(got, exact)
+        _say(f"  vs the exact interpolation of the stored values: mirror {e_m:.3g}, "
+             f"pair model {e_p:.3g}")
+        assert e_m <= 1.05 * e_p + 1e-6
     else:
         q = q.astype(np.float64) * scale[None, None, :]
         ref = orc.cbct_back(q, rg.ar(f32=True), rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, True,
@@ -908,7 +975,7 @@ def test_autotune_choice_and_cache(sg, orc, torch_cuda):
     p1 = sg.plan_fbp(rg, ig)
     y1 = sg.recon._empty_colmajor(ig.dims, dev)
     t = sg.autotune(p1, x, y1)
-    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather")
+    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather", "staged_mirror")
     ref = orc.fdk(proj.astype(np.float64), p1.filter,
                   orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] *
                   orc.angle_weight(rg.ar(f32=True).astype(np.float32)),
