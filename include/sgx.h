@@ -91,7 +91,11 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
  *                        1 = gather backprojector,
  *                        2 = LDS-staged, with scaled-FP16 plans interpolating
  *                            in packed FP16 arithmetic (otherwise as 0;
- *                            about twice the FP16 form's rounding error).
+ *                            about twice the FP16 form's rounding error),
+ *                        3 = LDS-staged with blocks 8 voxels deep (twice the
+ *                            blocks: for volumes too small to fill the GPU;
+ *                            the mirror form, where it applies, takes
+ *                            precedence).
  *   SGX_OPT_CBCT_MIRROR: the staged backprojector's mirror form, which pairs
  *                        each voxel z > 0 with its image at -z where the
  *                        geometry is symmetric (z axis symmetric about 0, no
@@ -356,15 +360,18 @@ int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms);
  * is cached in the context per size bucket (kind, precision and the
  * power-of-two ceilings of nx, ny, nz, ns, nt, na); with `use_cache` a plan
  * whose bucket is cached takes it without timing.  `image` holds the chosen
- * shape's result on return.  Candidates: cone-beam, the staged and gather
- * backprojectors; 2-D, view-grouping targets {1024, 2048, 4096} blocks x LDS
- * view chunks {16, 32, 64}. */
+ * shape's result on return.  Candidates: cone-beam, the staged
+ * backprojector with blocks 16 or 8 voxels deep and (symmetric geometries)
+ * its mirror form, the gather backprojector, and for scaled-FP16 plans the
+ * packed-FP16 interpolation; 2-D, view-grouping targets {1024, 2048, 4096}
+ * blocks x LDS view chunks {16, 32, 64}. */
 int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
                       int use_cache);
 /* The plan's kernel shape: values[0] = cone-beam kernel (0 staged, 1
  * gather, 2 staged with packed-FP16 interpolation, 3 the staged kernel's
- * mirror form), values[1] = 2-D view-grouping target, values[2] = 2-D view
- * chunk, values[3] = 1 when autotuned. */
+ * mirror form, 4 the staged kernel with blocks 8 voxels deep), values[1] =
+ * 2-D view-grouping target, values[2] = 2-D view chunk, values[3] = 1 when
+ * autotuned. */
 int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);
 
 /* A batch: many reconstructions (plan, input, output; device pointers) run as

@@ -237,6 +237,7 @@ constexpr int kWaves = 4;      // waves per block, each on its own z-slab
 constexpr int kGatherVariant = 16;   // cbct_variant: the gather kernel
 constexpr int kPackedVariant = 32;   // cbct_variant: the packed-FP16 kernel
 constexpr int kMirrorVariant = 64;   // cbct_variant: the mirror form of the staged kernel
+constexpr int kZ8Variant = 128;      // cbct_variant: the Z8 form of the staged kernel
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
 // One block of the gather kernel: block `blk` (8 x 8 columns x 4 waves of
@@ -371,22 +372,29 @@ constexpr int kSB = SGX_STAGED_BATCH;
 // the second read takes the first one's address with an immediate offset.
 // One position (FMA, floor, fraction, address) then serves two voxels: 6
 // VALU per update in the inner loop instead of 8.
-constexpr int kSZm = 8;               // mirror form: voxels z > 0 per lane (and 8 images)
-constexpr int kHm = 20;               // mirror form: window rows (each of the two windows)
-template <int FMT, bool MIR = false> struct Lds {
+//
+// Z8 form: blocks of 16 x 16 columns x 8 z (twice as many blocks, windows of
+// at most kHm rows), for volumes too small to fill the GPU with 16-deep
+// blocks; one of the autotuner's candidates.
+constexpr int kSZm = 8;               // mirror / Z8 forms: voxels per lane positioned
+constexpr int kHm = 20;               // mirror / Z8 forms: window rows (each window)
+constexpr int kZ16 = 0, kZMirror = 1, kZ8 = 2;   // block forms of the staged kernel (ZF)
+template <int FMT, int ZF = kZ16> struct Lds {
     static constexpr bool half = FMT == F_F16S;
+    static constexpr bool mir = ZF == kZMirror;
     static constexpr uint32_t esz = half ? 4 : 8;
     static constexpr int wpitch = half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH;  // per window
-    static constexpr int pitch = MIR ? 2 * wpitch : wpitch;   // entries per LDS row
-    static constexpr int rows = MIR ? kHm : kHmax;            // rows per buffer
+    static constexpr int pitch = mir ? 2 * wpitch : wpitch;   // entries per LDS row
+    static constexpr int rows = ZF == kZ16 ? kHmax : kHm;     // rows per buffer
     static constexpr int buf = rows * pitch;                  // entries per buffer
     // one stager per quad and row pair of each window
-    static constexpr int stagers = (kWmax / 4) * (rows / 2) * (MIR ? 2 : 1);
+    static constexpr int stagers = (kWmax / 4) * (rows / 2) * (mir ? 2 : 1);
     static_assert(wpitch >= kWmax + 2 && (wpitch * esz) % 16 == 0, "pitch");
     static_assert(stagers <= 256, "one staging thread per quad and row pair");
     // the second read of a mirror pair: an immediate offset of ds_read2
-    static_assert(!MIR || wpitch + 1 <= 255, "ds_read2 offset");
+    static_assert(!mir || wpitch + 1 <= 255, "ds_read2 offset");
 };
+
 static_assert(kSZ % kSB == 0 && kSZm % (kSB / 2) == 0, "batch divides the voxels per lane");
 constexpr int kQuads = kWmax / 4;
 
@@ -402,7 +410,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
                                    const float* __restrict__ zds, int nx, int ny, int nz, int ns,
                                    int nt, float dsd_ds, float dso_ds, float ws, float wt,
                                    float inv_dt_ds, int is_arc, const float* __restrict__ zs,
-                                   int arc_t_exact, float so_ds, int mirror,
+                                   int arc_t_exact, float so_ds, int zform,
                                    int4* __restrict__ win)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -412,9 +420,9 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
     const int tile = blk % tiles, zblk = blk / tiles;
     // a mirror block's window is the one of its voxels z > 0
-    const int zn = mirror ? kSZm : kSZ, hmax = mirror ? kHm : kHmax;
+    const int zn = zform == kZ16 ? kSZ : kSZm, hmax = zform == kZ16 ? kHmax : kHm;
     const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP;
-    const int iz0 = mirror ? nz / 2 + zblk * kSZm : zblk * kSZ;
+    const int iz0 = zform == kZMirror ? nz / 2 + zblk * kSZm : zblk * zn;
     // extent of the block's voxel centres (any axis order)
     float x0 = xds[ix0], x1 = x0, y0 = yds[iy0], y1 = y0, z0 = zds[min(iz0, nz - 1)], z1 = z0;
     for (int k = 1; k < kSP; ++k) {
@@ -481,9 +489,10 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
 }
 
 // One block's work: voxel block `blk` (16 x 16 columns x 16 z; for the
-// mirror form 16 x 16 columns x 8 z > 0 and their 8 images), all views, with
-// the block's LDS `lds` (2 * Lds<FMT, MIR>::buf entries).
-template <bool ARC, int FMT, bool OPT, bool MIR>
+// mirror form 16 x 16 columns x 8 z > 0 and their 8 images; for the Z8 form
+// 16 x 16 x 8), all views, with the block's LDS `lds` (2 * Lds<FMT, ZF>::buf
+// entries).
+template <bool ARC, int FMT, bool OPT, int ZF>
 __device__ __forceinline__ void cbct_staged_block(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
@@ -493,10 +502,11 @@ __device__ __forceinline__ void cbct_staged_block(
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
     int arc_t_exact, int narrow, float so_ds, int blk, char* lds)
 {
-    using L = Lds<FMT, MIR>;
+    using L = Lds<FMT, ZF>;
+    constexpr bool MIR = ZF == kZMirror;
     constexpr int kLP = L::pitch;
-    constexpr int NZ = MIR ? kSZm : kSZ;        // voxels per lane whose position is computed
-    constexpr int NV = MIR ? 2 * kSZm : kSZ;    // voxels per lane
+    constexpr int NZ = ZF == kZ16 ? kSZ : kSZm; // voxels per lane whose position is computed
+    constexpr int NV = MIR ? 2 * kSZm : NZ;     // voxels per lane
     constexpr int SB = MIR ? kSB / 2 : kSB;     // positions per batch (kSB LDS reads)
     // byte offset of the mirror window's entries within an LDS row
     constexpr uint32_t kMirOff = (uint32_t)L::wpitch * L::esz;
@@ -509,7 +519,7 @@ __device__ __forceinline__ void cbct_staged_block(
     const int iy = (tile / tiles_x) * kSP + (wave >> 1) * 8 + (lane >> 3);
     // voxel j of the lane: izb + j; for the mirror form j < kSZm are z > 0
     // and kSZm + j the image of j
-    const int izb = MIR ? nz / 2 + zblk * kSZm : zblk * kSZ;
+    const int izb = MIR ? nz / 2 + zblk * kSZm : zblk * NZ;
     auto vox = [&](int j) { return (MIR && j >= kSZm) ? nz - 1 - (izb + j - kSZm) : izb + j; };
     const bool inxy = ix < nx && iy < ny;
     const int64_t nxy = (int64_t)nx * ny;
@@ -773,7 +783,7 @@ __device__ __forceinline__ void cbct_staged_block(
     }
 }
 
-template <bool ARC, int FMT, bool OPT, bool MIR>
+template <bool ARC, int FMT, bool OPT, int ZF>
 __global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
 cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
@@ -784,9 +794,9 @@ cbct_staged_kernel(
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
     int arc_t_exact, int narrow, float so_ds)
 {
-    using L = Lds<FMT, MIR>;
+    using L = Lds<FMT, ZF>;
     __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
-    cbct_staged_block<ARC, FMT, OPT, MIR>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds,
+    cbct_staged_block<ARC, FMT, OPT, ZF>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds,
                                           yds, zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws,
                                           wt, inv_dt_ds, win, zs, arc_t_exact, narrow, so_ds,
                                           blockIdx.x, lds);
@@ -797,18 +807,18 @@ cbct_staged_kernel(
 // (largest tasks first, interleaved so that block b % 8 keeps to one XCD's
 // share).  The hardware dispatches blocks in index order, so the launch
 // walks the table in order and backfills the GPU as blocks finish.
-template <bool ARC, int FMT, bool OPT, bool MIR>
+template <bool ARC, int FMT, bool OPT, int ZF>
 __global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
 cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
 {
-    using L = Lds<FMT, MIR>;
+    using L = Lds<FMT, ZF>;
     __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
     const int2 tk = tasks[blockIdx.x];
     // through the constant address space: invariant loads, as for kernel
     // arguments
     using CItem = const __attribute__((address_space(4))) CbctItem;
     const CItem& it = ((CItem*)(items))[tk.x];
-    cbct_staged_block<ARC, FMT, OPT, MIR>(it.q, it.ns, it.nt, it.pitch, it.nviews,
+    cbct_staged_block<ARC, FMT, OPT, ZF>(it.q, it.ns, it.nt, it.pitch, it.nviews,
                                           it.view_stride, it.cs, it.vscale, it.xds, it.yds,
                                           it.zds, it.nx, it.ny, it.nz, it.mask, it.image,
                                           it.dsd_ds, it.dso_ds, it.ws, it.wt, it.inv_dt_ds,
@@ -1143,13 +1153,21 @@ int arc_narrow(const CbctArgs& a)
                            0.75f * a.dsd_ds;
 }
 
+// The staged kernel's block form for `a` (the packed kernel has the 16-deep
+// blocks).
+int zform_of(const CbctArgs& a)
+{
+    return (a.fmt == SGX_Q_FMT_F16S && a.f16_packed) ? kZ16 : a.zform;
+}
+
 // Blocks of the staged (and packed) kernels: every 16 x 16 column tile
-// times the z blocks of 16 voxels (the mirror form: 8 voxels z > 0 and their
-// images, the same count for its even nz / 16).
+// times the z blocks (16 voxels; the mirror form: 8 voxels z > 0 and their
+// images, the same count for its even nz / 16; the Z8 form: 8 voxels).
 int64_t staged_blocks(const CbctArgs& a)
 {
     const int64_t tiles = (int64_t)((a.nx + kSP - 1) / kSP) * ((a.ny + kSP - 1) / kSP);
-    return tiles * ((a.nz + kSZ - 1) / kSZ);
+    const int zb = zform_of(a) == kZ8 ? kSZm : kSZ;
+    return tiles * ((a.nz + zb - 1) / zb);
 }
 
 template <int ZB, bool ARC, int FMT, bool OPT>
@@ -1167,13 +1185,13 @@ void launch_gather(const CbctArgs& a, hipStream_t st)
     SGX_HIP(hipGetLastError());
 }
 
-template <bool ARC, int FMT, bool OPT, bool MIR>
+template <bool ARC, int FMT, bool OPT, int ZF>
 void launch_staged(const CbctArgs& a, hipStream_t st)
 {
     const int64_t nb = staged_blocks(a);
     if (nb >= (int64_t(1) << 31))
         throw Error(SGX_ERR_UNSUPPORTED, "cbct_back: volume too large for one launch");
-    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT, MIR>), dim3((unsigned)nb), dim3(256), 0,
+    hipLaunchKernelGGL((cbct_staged_kernel<ARC, FMT, OPT, ZF>), dim3((unsigned)nb), dim3(256), 0,
                        st,
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
@@ -1219,12 +1237,15 @@ template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
     if (packed_form(a)) {
         if (a.is_arc) launch_packed<true, OPT>(a, st);
         else launch_packed<false, OPT>(a, st);
-    } else if (a.win && a.mirror) {
-        if (a.is_arc) launch_staged<true, FMT, OPT, true>(a, st);
-        else launch_staged<false, FMT, OPT, true>(a, st);
+    } else if (a.win && a.zform == kZMirror) {
+        if (a.is_arc) launch_staged<true, FMT, OPT, kZMirror>(a, st);
+        else launch_staged<false, FMT, OPT, kZMirror>(a, st);
+    } else if (a.win && a.zform == kZ8) {
+        if (a.is_arc) launch_staged<true, FMT, OPT, kZ8>(a, st);
+        else launch_staged<false, FMT, OPT, kZ8>(a, st);
     } else if (a.win) {
-        if (a.is_arc) launch_staged<true, FMT, OPT, false>(a, st);
-        else launch_staged<false, FMT, OPT, false>(a, st);
+        if (a.is_arc) launch_staged<true, FMT, OPT, kZ16>(a, st);
+        else launch_staged<false, FMT, OPT, kZ16>(a, st);
     } else {
         if (a.is_arc) launch_gather<8, true, FMT, OPT>(a, st);
         else launch_gather<8, false, FMT, OPT>(a, st);
@@ -1275,7 +1296,7 @@ void launch_cbct_windows(const CbctArgs& a, int4* win, hipStream_t st)
     hipLaunchKernelGGL(cbct_window_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
                        (int)nb, a.nviews, a.cs, a.xds, a.yds, a.zds, a.nx, a.ny, a.nz, a.ns, a.nt,
                        a.dsd_ds, a.dso_ds, a.ws, a.wt, a.inv_dt_ds, a.is_arc ? 1 : 0, a.zs,
-                       a.arc_t_exact ? 1 : 0, a.so_ds, a.mirror && !packed_form(a) ? 1 : 0, win);
+                       a.arc_t_exact ? 1 : 0, a.so_ds, zform_of(a), win);
     SGX_HIP(hipGetLastError());
 }
 
@@ -1285,7 +1306,8 @@ int cbct_variant(const CbctArgs& a)
     const bool opt = a.zs || a.arc_t_exact || a.so_ds != 0.f;
     return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant) +
            (packed_form(a) ? kPackedVariant : 0) +
-           (a.win && a.mirror && !packed_form(a) ? kMirrorVariant : 0);
+           (a.win && zform_of(a) == kZMirror ? kMirrorVariant : 0) +
+           (a.win && zform_of(a) == kZ8 ? kZ8Variant : 0);
 }
 
 CbctItem cbct_item(const CbctArgs& a)
@@ -1343,10 +1365,10 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_gather_queue_kernel(
                                         it.so_ds, tk.y);
 }
 
-template <bool ARC, int FMT, bool OPT, bool MIR>
+template <bool ARC, int FMT, bool OPT, int ZF>
 static const void* cbq_fn()
 {
-    return reinterpret_cast<const void*>(&cbct_queue_kernel<ARC, FMT, OPT, MIR>);
+    return reinterpret_cast<const void*>(&cbct_queue_kernel<ARC, FMT, OPT, ZF>);
 }
 
 template <bool ARC, int FMT, bool OPT>
@@ -1382,8 +1404,9 @@ static const void* cbct_queue_fn(int variant)
                : (opt ? cbq_fn<false, F, true, M>() : cbq_fn<false, F, false, M>())
 #define SGX_CBQ(F)                                                                   \
     do {                                                                             \
-        if (variant & kMirrorVariant) SGX_CBQ2(F, true);                             \
-        SGX_CBQ2(F, false);                                                          \
+        if (variant & kMirrorVariant) SGX_CBQ2(F, kZMirror);                         \
+        if (variant & kZ8Variant) SGX_CBQ2(F, kZ8);                                  \
+        SGX_CBQ2(F, kZ16);                                                           \
     } while (0)
     if (fmt == F_BF16) SGX_CBQ(F_BF16);
     if (fmt == F_F16S) SGX_CBQ(F_F16S);

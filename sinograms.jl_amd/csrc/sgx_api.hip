@@ -62,7 +62,8 @@ struct PlanTune {
     int cbct = 0;
     int want_blocks = 2048;
     int vc_max = 64;
-    int mirror = 0;      // cone beam, staged kernel: its mirror form (where the geometry allows)
+    int zform = 0;       // cone beam, staged kernel: block form (0 16 z, 1 mirror where the
+                         // geometry allows it, 2 8 z)
 };
 
 // Largest view-grouping target and smallest LDS chunk the autotuner tries.
@@ -397,7 +398,7 @@ int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
             ctx->cbct_mirror = (int)value;
             return;
         }
-        require(value >= 0 && value <= 2, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0, 1 or 2");
+        require(value >= 0 && value <= 3, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0 to 3");
         ctx->cbct_kernel = (int)value;
     });
 }
@@ -573,7 +574,9 @@ void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int6
                                      img, opts);
     // the FP32 and BF16 data of this entry point take the mirror form only when
     // it is asked for everywhere (it measured slower for them)
-    a.mirror = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym);
+    a.zform = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym) ? 1
+              : ctx->cbct_kernel == 3                                  ? 2
+                                                                       : 0;
     if (ctx->cbct_kernel != 1) {
         int4* win = static_cast<int4*>(
             ctx->get(W_WIN, sizeof(int4) * sgx::cbct_window_entries(a)));
@@ -637,7 +640,9 @@ int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t n
         sgx::CbctArgs a = cbct_call_args(T, base, proj, false, ns, nt, ns, ia_skip, dsd, dso, ds,
                                          dt, offset_s, offset_t, is_arc, nx, ny, nz, mask, image,
                                          nullptr);
-        a.mirror = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym);
+        a.zform = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym) ? 1
+                  : ctx->cbct_kernel == 3                                  ? 2
+                                                                           : 0;
         const bool staged = ctx->cbct_kernel != 1;
         const size_t need =
             tbytes + (staged ? sizeof(int4) * (size_t)sgx::cbct_window_entries(a) : 0);
@@ -891,7 +896,8 @@ struct sgx_plan {
     float so_ds = 0.f;              // cone-beam option: source offset / ds
     bool z_sym = false;             // cone-beam: the z axis is symmetric about 0
     bool mirror_allowed = true;     // cone-beam: the autotuner may try the mirror form
-    bool win_mirror = false;        // the form `win` was computed for
+    int win_zform = 0;              // the block form `win` was computed for
+    int64_t win_cap = 0;            // entries `win` holds
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
     // Bumped by every change that a captured batch graph would not see
@@ -990,7 +996,7 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
     a.arc_t_exact = P.arc_t_exact;
     a.so_ds = P.so_ds;
     a.f16_packed = P.tune.cbct == 2;
-    a.mirror = P.tune.mirror && sgx::cbct_mirror_ok(a, P.z_sym);
+    a.zform = P.tune.zform == 1 ? (sgx::cbct_mirror_ok(a, P.z_sym) ? 1 : 0) : P.tune.zform;
     return a;
 }
 
@@ -999,15 +1005,25 @@ sgx::CbctArgs cbct_args_of(const sgx_plan& P, float* img)
 void plan_windows(sgx_plan& P, hipStream_t st)
 {
     const sgx::CbctArgs a = cbct_args_of(P, nullptr);
+    const int64_t n = sgx::cbct_window_entries(a);
+    if (n > P.win_cap) {
+        // a form with more blocks (Z8): the old table may still be in use
+        SGX_HIP(hipStreamSynchronize(st));
+        if (P.win) SGX_HIP(hipFree(P.win));
+        P.win = nullptr;
+        P.win_cap = 0;
+        SGX_HIP(hipMalloc(&P.win, sizeof(int4) * n));
+        P.win_cap = n;
+    }
     sgx::launch_cbct_windows(a, P.win, st);
-    P.win_mirror = a.mirror;
+    P.win_zform = a.zform;
 }
 
 // Recompute the window table when the plan's form changed since (a tuning
 // choice): a no-op otherwise.
 void ensure_windows(sgx_plan& P, hipStream_t st)
 {
-    if (P.is3d() && P.win && cbct_args_of(P, nullptr).mirror != P.win_mirror)
+    if (P.is3d() && P.win && cbct_args_of(P, nullptr).zform != P.win_zform)
         plan_windows(P, st);
 }
 
@@ -1079,11 +1095,12 @@ std::vector<PlanTune> tune_candidates(const sgx_plan& P)
     if (P.is3d()) {
         for (int k : {0, 1}) c.push_back(PlanTune{k, 2048, 64});
         sgx_plan& Q = const_cast<sgx_plan&>(P);
-        const int m = Q.tune.mirror;
-        Q.tune.mirror = 1;
-        const bool can_mirror = P.mirror_allowed && cbct_args_of(Q, nullptr).mirror;
-        Q.tune.mirror = m;
+        const int m = Q.tune.zform;
+        Q.tune.zform = 1;
+        const bool can_mirror = P.mirror_allowed && cbct_args_of(Q, nullptr).zform == 1;
+        Q.tune.zform = m;
         if (can_mirror) c.push_back(PlanTune{0, 2048, 64, 1});
+        c.push_back(PlanTune{0, 2048, 64, 2});
         if (P.prec == SGX_PREC_F16S) c.push_back(PlanTune{2, 2048, 64});
     } else {
         for (int want : {1024, 2048, kTuneMaxBlocks})
@@ -1198,11 +1215,14 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->ny = ny;
         P->nz = is3d ? nz : 1;
         P->npad = npad;
-        P->tune.cbct = ctx->cbct_kernel;
+        P->tune.cbct = ctx->cbct_kernel == 3 ? 0 : ctx->cbct_kernel;
         // the mirror form: by default for the scaled-FP16 form, where it
         // measured faster; the autotuner times it for the others
         P->mirror_allowed = ctx->cbct_mirror != 0;
-        P->tune.mirror = ctx->cbct_mirror == 2 || (ctx->cbct_mirror == 1 && prec == SGX_PREC_F16S);
+        P->tune.zform =
+            ctx->cbct_mirror == 2 || (ctx->cbct_mirror == 1 && prec == SGX_PREC_F16S) ? 1
+            : ctx->cbct_kernel == 3                                                     ? 2
+                                                                                        : 0;
 
         const int64_t ns = G.ns, nt = G.nt, na = G.na;
         TablePack pack;
@@ -1356,8 +1376,6 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         }
         if (is3d) {
             // the staged backprojector's detector windows, fixed by the geometry
-            const int64_t nw = sgx::cbct_window_entries(cbct_args_of(*P, nullptr));
-            SGX_HIP(hipMalloc(&P->win, sizeof(int4) * nw));
             plan_windows(*P, ctx->stream);
         }
         for (hipEvent_t& e : P->ev) SGX_HIP(hipEventCreate(&e));
@@ -1534,8 +1552,8 @@ int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
 int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4])
 {
     if (!plan || !values) return SGX_ERR_ARG;
-    values[0] = plan->tune.cbct == 0 && plan->tune.mirror && plan->win_mirror ? 3
-                                                                                : plan->tune.cbct;
+    values[0] = plan->tune.cbct != 0 ? plan->tune.cbct
+                : plan->win_zform == 1 ? 3 : plan->win_zform == 2 ? 4 : 0;
     values[1] = plan->tune.want_blocks;
     values[2] = plan->tune.vc_max;
     values[3] = plan->tuned ? 1 : 0;

@@ -118,8 +118,9 @@ struct CbctArgs {
     // scaled-FP16 data with windows: true takes the packed-FP16 kernel in
     // place of the staged kernel's FP32 interpolation
     bool f16_packed = false;
-    // staged kernel: the mirror form (pairs z and -z; see cbct_mirror_ok)
-    bool mirror = false;
+    // staged kernel's block form: 0 = 16 z per lane, 1 = the mirror form
+    // (pairs z and -z; see cbct_mirror_ok), 2 = 8 z per lane
+    int zform = 0;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 // Whether the staged kernel may take its mirror form for `a`: the z axis

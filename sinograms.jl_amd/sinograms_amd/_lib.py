@@ -188,9 +188,9 @@ SGX_OPT_CBCT_KERNEL = 1
 SGX_OPT_CBCT_MIRROR = 2
 CBCT_MIRROR = ("never", "fp16", "always")    # values of SGX_OPT_CBCT_MIRROR
 # values of SGX_OPT_CBCT_KERNEL
-CBCT_KERNELS = ("staged", "gather", "staged_packed16")
-# cone-beam kernels a plan reports (sgx_plan_tuning): the above and the mirror form
-PLAN_CBCT_KERNELS = CBCT_KERNELS + ("staged_mirror",)
+CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_z8")
+# cone-beam kernels a plan reports (sgx_plan_tuning)
+PLAN_CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_mirror", "staged_z8")
 SGX_BATCH_NO_TUNE = 0x100
 SGX_BATCH_NO_QUEUE = 0x200
 SGX_BATCH_QUEUE = 0x400
@@ -257,8 +257,9 @@ class Context:
     @property
     def cbct_kernel(self) -> str:
         """Cone-beam backprojector of this context: "staged" (LDS-staged
-        detector windows, the default), "gather", or "staged_packed16"
-        (staged, with scaled-FP16 plans interpolating in packed FP16)."""
+        detector windows, the default), "gather", "staged_packed16" (staged,
+        with scaled-FP16 plans interpolating in packed FP16) or "staged_z8"
+        (staged, blocks 8 voxels deep)."""
         v = ctypes.c_int64()
         self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_CBCT_KERNEL, ctypes.byref(v)))
         return CBCT_KERNELS[v.value]

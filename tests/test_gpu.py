@@ -45,7 +45,7 @@ def _bf16_round(x):
 
 # --- K3 cone-beam backprojection -------------------------------------------------------
 
-@pytest.fixture(params=["staged", "gather"])
+@pytest.fixture(params=["staged", "gather", "staged_z8"])
 def cbct_kernel(request, sg, torch_cuda):
     """Run a test with each cone-beam backprojector of the shared context."""
     ctx = sg._lib.context(0)
@@ -130,6 +130,31 @@ def test_cbct_back_mirror_vs_oracle(sg, orc, torch_cuda, arc, case):
     assert e0 < 2e-6
     if case == "edges":
         assert np.any(ref == 0)          # voxels that see no detector row in some view
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "f16s"])
+def test_fdk_block_forms_bit_identical(sg, torch_cuda, prec):
+    # The staged kernel's 8-deep blocks run every voxel through the same
+    # arithmetic, views in the same order, as the 16-deep blocks: the volumes
+    # are bit for bit the same (nz = 40: a partial block at the top).
+    _say("fdk block forms", prec)
+    rng = np.random.default_rng(51)
+    rg = sg.CtFanFlat(ns=96, nt=64, ds=1.2, dt=1.2, na=48, dsd=400.0, dod=150.0)
+    ig = sg.ImageGeom((40, 36, 40), (1.0, 1.0, 1.0))
+    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    ctx = sg._lib.context(0)
+    vols = {}
+    try:
+        ctx.cbct_mirror = "never"
+        for k in ("staged", "staged_z8"):
+            ctx.cbct_kernel = k
+            plan = sg.plan_fbp(rg, ig, precision=prec)
+            assert sg.tuning(plan)["cbct_kernel"] == k
+            vols[k] = sg.fdk(plan, proj)
+    finally:
+        ctx.cbct_kernel = "staged"
+        ctx.cbct_mirror = "fp16"
+    assert np.array_equal(vols["staged"], vols["staged_z8"])
 
 
 def test_cbct_back_detector_edges(sg, orc, cbct_kernel):
@@ -975,7 +1000,7 @@ def test_autotune_choice_and_cache(sg, orc, torch_cuda):
     p1 = sg.plan_fbp(rg, ig)
     y1 = sg.recon._empty_colmajor(ig.dims, dev)
     t = sg.autotune(p1, x, y1)
-    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather", "staged_mirror")
+    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather", "staged_mirror", "staged_z8")
     ref = orc.fdk(proj.astype(np.float64), p1.filter,
                   orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] *
                   orc.angle_weight(rg.ar(f32=True).astype(np.float32)),
