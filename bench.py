@@ -710,7 +710,7 @@ def main():
     t32 = step["fp32"]
     f32ms, b32ms = kt["fp32"]
     compulsory = 4 * rg.ns * rg.nt * rg.na + 4 * updates // rg.na
-    kname = "cbct_staged_kernel<false,0,false>" if args.cbct_kernel == "staged" \
+    kname = "cbct_staged_kernel<false,0,false,0>" if args.cbct_kernel == "staged" \
         else "cbct_back_kernel<8,false,0,false>"
     roof = roofline(kname, b32ms, compulsory, updates, "update", 16.0)
     roof["filter_ms"] = f32ms
