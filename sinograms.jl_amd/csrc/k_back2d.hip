@@ -15,8 +15,9 @@
 // group of views, processed in chunks of views.  For each view of a chunk one
 // thread computes the tile's detector window from the tile's corners (bb is
 // monotone along each axis of the pixel box while the source is outside it),
-// with a bin of margin; only the windows are staged in LDS, several loads in
-// flight per thread.  The two taps of each update are then LDS reads; a lane
+// with a bin of margin, its start rounded down to 4 bins; only the windows
+// are staged in LDS, in runs of 32 bins per thread (32 loads in flight, runs
+// past a window's end skipped).  The two taps of each update are then LDS reads; a lane
 // whose taps fall outside its view's window (windows wider than the LDS
 // slot, pixels at or behind the source) reads them from global memory, so
 // the result never depends on the window.  Per-view cos/sin are wave-uniform
@@ -40,9 +41,17 @@
 // bins, so FP32 keeps it to about 1e-6 of a bin.  The weight
 // w2 = dsd^2 / (d^2 + r^2) comes without a second reciprocal: by Lagrange's
 // identity num^2 + den^2 = (d^2 + r^2) R2, so w2 = dsd^2 R2 / den^2 / (1 + t^2),
-// with 1 / (1 + t^2) its series of NT terms.  About 21 VALU instructions per
-// update against about 36 for the per-pixel atan2 form; the per-pixel form stays for geometries where |t| is not small,
-// and for sgx_plan_back / sgx_plan_back_t, whose pair is an exact transpose.
+// with 1 / (1 + t^2) its series of NT terms (KR folded into its
+// coefficients).  The inner loop issues 20 VALU instructions per update (NT =
+// 3) against about 36 for the per-pixel atan2 form; the per-pixel form stays
+// for geometries where |t| is not small, and for sgx_plan_back /
+// sgx_plan_back_t, whose pair is an exact transpose.
+//
+// Per-update LDS traffic: the two taps (one ds_read2) and the view's
+// constants (one 16-byte and one 8-byte read shared by the wave).  Tap
+// addresses are absolute LDS addresses (no add of the dynamic LDS base per
+// update; the block reductions use words of the dynamic LDS, not the
+// runtime's static buffer).
 
 #include "sgx_internal.h"
 #include "sgx.h"
@@ -58,7 +67,13 @@ constexpr int kThreads = kTile * kTile;
 constexpr int kLdsBytes = 48 * 1024;
 constexpr int kU = 8;           // views per unrolled step of the backprojector
 constexpr int kWinMax = 128;    // staged window slot per view (bins), a power of two
-constexpr int kStage = 8;       // staging loads in flight per thread
+constexpr int kRunBins = 32;    // staging: bins per thread and run (loads in flight together)
+
+// Raw buffer resource over `bytes` bytes at p: loads past the end return 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t sino_rsrc(const float* p, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(p), 0, (int)bytes, 0x00020000);
+}
 
 // Detector position bb (1-based bins) and weight w2 of pixel (x, y) (/ ds) in
 // one view; shared by the backprojector and its transpose.
@@ -83,10 +98,11 @@ __device__ __forceinline__ void pixel_bin(float x, float y, ViewCS t, float wb, 
     }
 }
 
-// Window slot per view: the smallest power of two >= nb, at most kWinMax.
+// Window slot per view: the smallest power of two >= nb, at least kRunBins
+// (one staging run) and at most kWinMax.
 int window_bins(int nb)
 {
-    int w = 4;
+    int w = kRunBins;
     while (w < nb && w < kWinMax) w *= 2;
     return w;
 }
@@ -94,19 +110,30 @@ int window_bins(int nb)
 // Per-(tile, view) constants of the tile-relative arc form (see above), 32
 // bytes: A, B, R2, f0; KR = dsd^2 R2; the in-detector range of the fractional
 // coordinate [blo, bhi) = [1 - I0, nb - I0) (so I0 = 1 - blo); base = the LDS
-// byte offset of tap il - 1 when floor(f0 + dsd atan(t)) = 0.
-struct RelView {
+// byte address of tap il - 1 when floor(f0 + dsd atan(t)) = 0.
+struct alignas(16) RelView {
     float A, B, R2, f0, KR, blo, bhi;
     int base;
 };
 static_assert(sizeof(RelView) == 32, "RelView");
+
+// Block-wide reduction words after the per-view tables of the dynamic LDS
+// (the runtime's reductions would take static LDS ahead of the dynamic block,
+// and every LDS address of the inner loop would carry that offset).
+constexpr int kFlagBytes = 16;
+
+// Dynamic LDS of a chunk of vc views with slots of wm bins.
+size_t chunk_lds(int vc, int wm)
+{
+    return (size_t)vc * (wm * sizeof(float) + sizeof(int2) + sizeof(RelView)) + kFlagBytes;
+}
 
 // Views of one LDS chunk: at most vc_max and what fits in kLdsBytes with the
 // per-view tables; a multiple of kU once it reaches kU.
 int chunk_views(int nb, int vc_max)
 {
     const int per = 4 * window_bins(nb) + 8 + (int)sizeof(RelView);
-    int vc = std::max(1, std::min(vc_max, kLdsBytes / per));
+    int vc = std::max(1, std::min(vc_max, (kLdsBytes - kFlagBytes) / per));
     if (vc >= kU) vc -= vc % kU;
     return vc;
 }
@@ -127,15 +154,29 @@ template <int NT> struct RelPoly {
         for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, c[k]);
         return fmaf(t, p, f0);
     }
-    // 1 / (1 + t^2) = 1 - t^2 + t^4 - ..., NT terms
-    __device__ __forceinline__ static float inv1p(float t2)
+    // K / (1 + t^2) = K (1 - t^2 + t^4 - ...), NT terms (the signs are
+    // source modifiers of the FMAs)
+    __device__ __forceinline__ static float inv1p(float t2, float K)
     {
-        float p = (NT & 1) ? 1.f : -1.f;
+        float p = (NT & 1) ? K : -K;
 #pragma unroll
-        for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, (k & 1) ? -1.f : 1.f);
+        for (int k = NT - 2; k >= 0; --k) p = fmaf(t2, p, (k & 1) ? -K : K);
         return p;
     }
 };
+
+// LDS byte address of a pointer into the block's LDS, and a float pointer
+// in the LDS address space from one: the inner loop forms tap addresses from
+// absolute LDS addresses, so no add of the dynamic LDS base per update.
+using LdsFloat = __attribute__((address_space(3))) const float;
+__device__ __forceinline__ int lds_byte(const void* p)
+{
+    return (int)(uint32_t)reinterpret_cast<uintptr_t>(p);
+}
+__device__ __forceinline__ LdsFloat* lds_ptr(int byte)
+{
+    return reinterpret_cast<LdsFloat*>((uint32_t)byte);
+}
 
 // floor(x) as an integer in one instruction (v_cvt_flr_i32_f32) for the
 // small, finite coordinates of the tile-relative form
@@ -166,6 +207,7 @@ __device__ __forceinline__ void back2d_tile(
     const int wm = 1 << wshift;
     int2* win = reinterpret_cast<int2*>(lds + vc * wm);
     RelView* rel = reinterpret_cast<RelView*>(win + vc);
+    int* flags = reinterpret_cast<int*>(rel + vc);   // [0]: some pixel active, [1]: chunk interior
     const int tid = threadIdx.x;
     const int tiles_x = (nx + kTile - 1) / kTile;
     const int tx0 = (tile % tiles_x) * kTile, ty0 = (tile / tiles_x) * kTile;
@@ -176,7 +218,12 @@ __device__ __forceinline__ void back2d_tile(
     const bool active = inside && (mask == nullptr || mask[pix]);
     float* dst = out + out_group_stride * group;
 
-    if (!__syncthreads_or(active)) {          // whole tile outside the mask
+    // whole tile outside the mask: nothing to add
+    if (tid == 0) flags[0] = 0;
+    __syncthreads();
+    if (active) flags[0] = 1;
+    __syncthreads();
+    if (!flags[0]) {
         if (inside) dst[pix] = 0.f;
         return;
     }
@@ -198,6 +245,8 @@ __device__ __forceinline__ void back2d_tile(
     const float dx = x - xt, dy = y - yt;
     const RelPoly<NT> poly(dsd_ds);
     float acc = 0.f;
+    const float ndy = -dy;
+    const int lds0 = lds_byte(lds);
 
     for (int c0 = v_begin; c0 < v_end; c0 += vc) {
         const int nv = min(vc, v_end - c0);
@@ -220,7 +269,8 @@ __device__ __forceinline__ void back2d_tile(
                 bmax = fmaxf(bmax, bb);
             }
             if (ok && bmax >= 1.f && bmin < nb1) {      // false for NaN
-                lo = max((int)floorf(fmaxf(bmin, 0.f)) - 2, 0);
+                // the window starts at a multiple of 4 bins (16-byte staging loads)
+                lo = max((int)floorf(fmaxf(bmin, 0.f)) - 2, 0) & ~3;
                 const int hi = min((int)floorf(fminf(bmax, nb1)) + 1, nb - 1);
                 n = hi - lo + 1;
                 if (n > wm) n = 0;                       // too wide: taps from global memory
@@ -242,7 +292,7 @@ __device__ __forceinline__ void back2d_tile(
                     r.f0 = (float)(K - I0);
                     r.blo = (float)(1.0 - I0);
                     r.bhi = (float)((double)nb - I0);
-                    r.base = 4 * ((tid << wshift) + ((int)I0 - 1 - lo));
+                    r.base = lds_byte(lds) + 4 * ((tid << wshift) + ((int)I0 - 1 - lo));
                     // every pixel's bb is inside [bmin, bmax] up to FP32 rounding
                     // of the corners; one bin of margin on each side
                     interior = ok && n > 0 && bmin >= 2.f && bmax < nb1 - 1.f &&
@@ -254,7 +304,7 @@ __device__ __forceinline__ void back2d_tile(
                     r.KR = 0.f;
                     r.f0 = 0.5f;
                     r.blo = r.bhi = 0.f;      // I0 = 1
-                    r.base = 4 * (tid << wshift);
+                    r.base = lds_byte(lds) + 4 * (tid << wshift);
                 }
                 rel[tid] = r;
             }
@@ -262,28 +312,50 @@ __device__ __forceinline__ void back2d_tile(
         // GEOM 4: the chunk is interior when all its views are (a reduction
         // barrier only where it is used)
         bool chunk_interior = false;
-        if constexpr (REL)
-            chunk_interior =
-                __builtin_amdgcn_readfirstlane(__syncthreads_and(interior ? 1 : 0)) != 0;
-        else
+        if constexpr (REL) {
+            // flags[1] was last read before the barrier that opened this chunk
+            if (tid == 0) flags[1] = 1;
             __syncthreads();
-        const int total = nvp << wshift;
-        for (int i0 = tid; i0 < total; i0 += kThreads * kStage) {
-            float t[kStage];
-#pragma unroll
-            for (int u = 0; u < kStage; ++u) {
-                const int i = i0 + u * kThreads;
-                const int v = i >> wshift, k = i & (wm - 1);
-                t[u] = 0.f;
-                if (i < total) {
-                    const int2 w = win[v];
-                    if (k <= w.y) t[u] = q[view_stride * (int64_t)(c0 + v) + (w.x - 1) + k];
+            if (!interior) flags[1] = 0;
+            __syncthreads();
+            chunk_interior = __builtin_amdgcn_readfirstlane(flags[1]) != 0;
+        } else {
+            __syncthreads();
+        }
+        // Staging: slot v holds bins lo .. lo + wm - 1 of view c0 + v (lo a
+        // multiple of 4, wm a multiple of kRunBins).  A thread moves one run
+        // of kRunBins bins of a slot: dword loads at immediate offsets, all in
+        // flight together (rows need no alignment), then 16-byte LDS writes.
+        // Runs past a window's end are skipped; bins past a window's end
+        // within a run are data of the row or of the next view (0 past the
+        // sinogram's end), which no tap reads.  The padding views of the
+        // tile-relative form read bins 0 and 1 of their slot with weight 0:
+        // those are zeroed.
+        {
+            const int rshift = wshift - 5;                       // runs per slot: 2^rshift
+            const int64_t left = (int64_t)(nviews - 1 - c0) * view_stride + nb;  // floats
+            const __amdgpu_buffer_rsrc_t rs =
+                sino_rsrc(q + view_stride * (int64_t)c0,
+                          (uint32_t)std::min<int64_t>(left * 4, 0x7fffffff));
+            for (int rr = tid; rr < (nvp << rshift); rr += kThreads) {
+                const int v = rr >> rshift;
+                const int k0 = (rr & ((1 << rshift) - 1)) * kRunBins;
+                float* dst = lds + (v << wshift) + k0;
+                if (REL && v >= nv) {
+                    if (k0 == 0) *reinterpret_cast<float4*>(dst) = make_float4(0.f, 0.f, 0.f, 0.f);
+                    continue;
                 }
-            }
+                const int2 w = win[v];
+                if (k0 > w.y) continue;                          // past the window (or none)
+                const uint32_t off =
+                    (uint32_t)((view_stride * (int64_t)v + (w.x - 1) + k0) * 4);
+                float t[kRunBins];
 #pragma unroll
-            for (int u = 0; u < kStage; ++u) {
-                const int i = i0 + u * kThreads;
-                if (i < total) lds[i] = t[u];
+                for (int j = 0; j < kRunBins; ++j)
+                    t[j] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rs, off + 4 * j, 0, 0));
+#pragma unroll
+                for (int j = 0; j < kRunBins; j += 4)
+                    *reinterpret_cast<float4*>(dst + j) = make_float4(t[j], t[j + 1], t[j + 2], t[j + 3]);
             }
         }
         __syncthreads();
@@ -295,18 +367,17 @@ __device__ __forceinline__ void back2d_tile(
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     const RelView r = rel[v + u];
-                    const float E = fmaf(dx, r.B, -dy * r.A);
+                    const float den = fmaf(dx, r.B, fmaf(ndy, r.A, r.R2));   // R2 + E
                     const float num = fmaf(dx, r.A, dy * r.B);
-                    const float rt = __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float rt = __builtin_amdgcn_rcpf(den);
                     const float tt = num * rt;
                     const float t2 = tt * tt;
                     const float bbf = poly.bin(tt, t2, r.f0);
                     wr[u] = __builtin_amdgcn_fractf(bbf);
-                    const float* p = reinterpret_cast<const float*>(
-                        reinterpret_cast<const char*>(lds) + (r.base + 4 * cvt_floor(bbf)));
+                    LdsFloat* p = lds_ptr(r.base + 4 * cvt_floor(bbf));
                     a[u] = p[0];
                     b[u] = p[1];
-                    w[u] = r.KR * rt * rt * RelPoly<NT>::inv1p(t2);
+                    w[u] = RelPoly<NT>::inv1p(t2, r.KR) * rt * rt;
                 }
 #pragma unroll
                 for (int u = 0; u < kU; ++u) acc = fmaf(w[u], fmaf(wr[u], b[u] - a[u], a[u]), acc);
@@ -321,19 +392,19 @@ __device__ __forceinline__ void back2d_tile(
 #pragma unroll
                 for (int u = 0; u < kU; ++u) {
                     const RelView r = rel[v + u];
-                    const float E = fmaf(dx, r.B, -dy * r.A);
+                    const float den = fmaf(dx, r.B, fmaf(ndy, r.A, r.R2));   // R2 + E
                     const float num = fmaf(dx, r.A, dy * r.B);
-                    const float rt = __builtin_amdgcn_rcpf(r.R2 + E);
+                    const float rt = __builtin_amdgcn_rcpf(den);
                     const float tt = num * rt;
                     const float t2 = tt * tt;
                     const float bbf = poly.bin(tt, t2, r.f0);
                     const bool ok = bbf >= r.blo && bbf < r.bhi;   // 1 <= floor(bb) < nb
-                    w[u] = ok ? r.KR * rt * rt * RelPoly<NT>::inv1p(t2) : 0.f;
+                    w[u] = ok ? RelPoly<NT>::inv1p(t2, r.KR) * rt * rt : 0.f;
                     const float bbc = ok ? bbf : r.blo;
                     const int fi = (int)floorf(bbc);
                     wr[u] = __builtin_amdgcn_fractf(bbc);
                     il[u] = (int)(1.f - r.blo) + fi;              // taps il - 1 and il (0-based)
-                    const int local = (r.base >> 2) - ((v + u) << wshift) + fi;
+                    const int local = ((r.base - lds0) >> 2) - ((v + u) << wshift) + fi;
                     const int2 wv = win[v + u];
                     const bool in = local >= 0 && local < wv.y;
                     miss |= (uint32_t)(!in) << u;
@@ -587,7 +658,7 @@ int back2d_item_tiles(const Back2dItem& it)
 
 size_t back2d_item_lds(const Back2dItem& it)
 {
-    return (size_t)it.vc * ((sizeof(float) << it.wshift) + sizeof(int2) + sizeof(RelView));
+    return chunk_lds(it.vc, 1 << it.wshift);
 }
 
 template <int GEOM, int NT>
@@ -639,7 +710,7 @@ void launch_back2d(const Back2dArgs& a, hipStream_t st)
     const int wm = window_bins(a.nb);
     int wshift = 0;
     while ((1 << wshift) < wm) ++wshift;
-    const size_t lds = (size_t)vc * (wm * sizeof(float) + sizeof(int2) + sizeof(RelView));
+    const size_t lds = chunk_lds(vc, wm);
 #define SGX_B2D(G, NT)                                                                          \
     hipLaunchKernelGGL((back2d_kernel<G, NT>), grid, block, lds, st, a.q, a.nb, a.nviews,       \
                        a.view_stride, a.cs, a.xds, a.yds, a.nx, a.ny, a.mask, out, npix, vpg, vc, \
