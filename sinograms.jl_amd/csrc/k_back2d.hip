@@ -121,18 +121,24 @@ static_assert(sizeof(RelView) == 32, "RelView");
 // (the runtime's reductions would take static LDS ahead of the dynamic block,
 // and every LDS address of the inner loop would carry that offset).
 constexpr int kFlagBytes = 16;
+// Window slots are wm + kSlotPad floats apart: the staging writes of lanes
+// that move the same run of consecutive slots then start 16 bytes apart in
+// the LDS banks (no conflicts), where a power-of-two pitch put them all on
+// one bank.
+constexpr int kSlotPad = 4;
 
 // Dynamic LDS of a chunk of vc views with slots of wm bins.
 size_t chunk_lds(int vc, int wm)
 {
-    return (size_t)vc * (wm * sizeof(float) + sizeof(int2) + sizeof(RelView)) + kFlagBytes;
+    return (size_t)vc * ((wm + kSlotPad) * sizeof(float) + sizeof(int2) + sizeof(RelView)) +
+           kFlagBytes;
 }
 
 // Views of one LDS chunk: at most vc_max and what fits in kLdsBytes with the
 // per-view tables; a multiple of kU once it reaches kU.
 int chunk_views(int nb, int vc_max)
 {
-    const int per = 4 * window_bins(nb) + 8 + (int)sizeof(RelView);
+    const int per = 4 * (window_bins(nb) + kSlotPad) + 8 + (int)sizeof(RelView);
     int vc = std::max(1, std::min(vc_max, (kLdsBytes - kFlagBytes) / per));
     if (vc >= kU) vc -= vc % kU;
     return vc;
@@ -200,12 +206,12 @@ __device__ __forceinline__ void back2d_tile(
     float* __restrict__ out, int64_t out_group_stride, int views_per_group, int vc, int wshift,
     float wb, float dsd_ds, float dso_ds, float so_ds, int tile, int group, float* lds)
 {
-    // vc window slots of wm bins, then per view int2 {lo + 1, n - 1}: the
+    // vc window slots of wm bins (sp floats apart), then per view int2 {lo + 1, n - 1}: the
     // window holds 0-based bins lo .. lo + n - 1 (n = 0: nothing staged),
     // then (GEOM 4) per view a RelView
     constexpr bool REL = GEOM == 4;
-    const int wm = 1 << wshift;
-    int2* win = reinterpret_cast<int2*>(lds + vc * wm);
+    const int wm = 1 << wshift, sp = wm + kSlotPad;
+    int2* win = reinterpret_cast<int2*>(lds + vc * sp);
     RelView* rel = reinterpret_cast<RelView*>(win + vc);
     int* flags = reinterpret_cast<int*>(rel + vc);   // [0]: some pixel active, [1]: chunk interior
     const int tid = threadIdx.x;
@@ -292,7 +298,7 @@ __device__ __forceinline__ void back2d_tile(
                     r.f0 = (float)(K - I0);
                     r.blo = (float)(1.0 - I0);
                     r.bhi = (float)((double)nb - I0);
-                    r.base = lds_byte(lds) + 4 * ((tid << wshift) + ((int)I0 - 1 - lo));
+                    r.base = lds_byte(lds) + 4 * (tid * sp + ((int)I0 - 1 - lo));
                     // every pixel's bb is inside [bmin, bmax] up to FP32 rounding
                     // of the corners; one bin of margin on each side
                     interior = ok && n > 0 && bmin >= 2.f && bmax < nb1 - 1.f &&
@@ -304,7 +310,7 @@ __device__ __forceinline__ void back2d_tile(
                     r.KR = 0.f;
                     r.f0 = 0.5f;
                     r.blo = r.bhi = 0.f;      // I0 = 1
-                    r.base = lds_byte(lds) + 4 * (tid << wshift);
+                    r.base = lds_byte(lds) + 4 * (tid * sp);
                 }
                 rel[tid] = r;
             }
@@ -333,14 +339,18 @@ __device__ __forceinline__ void back2d_tile(
         // those are zeroed.
         {
             const int rshift = wshift - 5;                       // runs per slot: 2^rshift
+            int vbits = 0;                                       // slots: at most 2^vbits
+            while ((1 << vbits) < nvp) ++vbits;
             const int64_t left = (int64_t)(nviews - 1 - c0) * view_stride + nb;  // floats
             const __amdgpu_buffer_rsrc_t rs =
                 sino_rsrc(q + view_stride * (int64_t)c0,
                           (uint32_t)std::min<int64_t>(left * 4, 0x7fffffff));
-            for (int rr = tid; rr < (nvp << rshift); rr += kThreads) {
-                const int v = rr >> rshift;
-                const int k0 = (rr & ((1 << rshift) - 1)) * kRunBins;
-                float* dst = lds + (v << wshift) + k0;
+            // consecutive lanes take the same run of consecutive slots
+            for (int rr = tid; rr < (1 << (vbits + rshift)); rr += kThreads) {
+                const int v = rr & ((1 << vbits) - 1);
+                const int k0 = (rr >> vbits) * kRunBins;
+                if (v >= nvp) continue;
+                float* dst = lds + v * sp + k0;
                 if (REL && v >= nv) {
                     if (k0 == 0) *reinterpret_cast<float4*>(dst) = make_float4(0.f, 0.f, 0.f, 0.f);
                     continue;
@@ -404,11 +414,11 @@ __device__ __forceinline__ void back2d_tile(
                     const int fi = (int)floorf(bbc);
                     wr[u] = __builtin_amdgcn_fractf(bbc);
                     il[u] = (int)(1.f - r.blo) + fi;              // taps il - 1 and il (0-based)
-                    const int local = ((r.base - lds0) >> 2) - ((v + u) << wshift) + fi;
+                    const int local = ((r.base - lds0) >> 2) - (v + u) * sp + fi;
                     const int2 wv = win[v + u];
                     const bool in = local >= 0 && local < wv.y;
                     miss |= (uint32_t)(!in) << u;
-                    const float* p = lds + ((v + u) << wshift) + (in ? local : 0);
+                    const float* p = lds + (v + u) * sp + (in ? local : 0);
                     a[u] = p[0];
                     b[u] = p[1];
                 }
@@ -452,7 +462,7 @@ __device__ __forceinline__ void back2d_tile(
                     const int local = il[u] - wv.x;      // il - 1 - lo
                     const bool in = local >= 0 && local < wv.y;
                     miss |= (uint32_t)(!in) << u;
-                    const float* p = lds + (vu << wshift) + (in ? local : 0);
+                    const float* p = lds + vu * sp + (in ? local : 0);
                     a[u] = p[0];
                     b[u] = p[1];
                 }
