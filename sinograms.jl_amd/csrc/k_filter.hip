@@ -68,6 +68,7 @@ struct Params {
     const float* ptab;            // (nb, na) short-scan Parker weights, or null
     const unsigned int* vbound;   // F16S: per-view max |weighted input| (float bits)
     float g1;                     // F16S: sum |g|
+    float wmax;                   // F16S on the FFT form: bound of |bin x cylinder weight|
     float gscale;                 // split-FP16: power of two applied to g
     float* vscale;                // F16S: per-view output scale (written)
     int row_tiles_per_block;
@@ -996,6 +997,63 @@ __global__ __launch_bounds__(256) void bound_kernel(Params p, unsigned int* __re
     }
 }
 
+// Streaming range pass of the FFT form (scaled FP16): per view, max |input|
+// times the view's weight bound |scale| x wmax x |wview[view]| (an upper
+// bound of the weighted input, since every factor of the fused weight is
+// bounded by its part and Parker weights by 1).  A looser bound than the
+// per-sample weighted maximum only moves the power-of-two scale up, which
+// FP16 stores exactly (no value falls out of FP16's normal range above
+// 2^-24 of the view's largest).  One load and half a VALU max per sample:
+// HBM-bound.
+constexpr int kAbsRows = 64;     // rows per block of the streaming range pass
+__global__ __launch_bounds__(256) void absmax_kernel(Params p, unsigned int* __restrict__ vbound,
+                                                     int vec)
+{
+    const int tid = threadIdx.x;
+    const int view = blockIdx.x;
+    const int t0 = blockIdx.y * kAbsRows;
+    const int nrows = min(kAbsRows, p.nt - t0);
+    const float* base = p.in + ((int64_t)view * p.nt + t0) * p.nb;
+    const int n = nrows * p.nb;
+    float mx = 0.f;
+    if (vec) {
+        const float4* b4 = reinterpret_cast<const float4*>(base);
+        const int n4 = n >> 2;
+        for (int i0 = tid; i0 < n4; i0 += 256 * kBoundLoads) {
+            float4 v[kBoundLoads];
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i4 = i0 + 256 * k;
+                v[k] = i4 < n4 ? b4[i4] : make_float4(0.f, 0.f, 0.f, 0.f);
+            }
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k)
+                mx = fmaxf(fmaxf(mx, fmaxf(fabsf(v[k].x), fabsf(v[k].y))),
+                           fmaxf(fabsf(v[k].z), fabsf(v[k].w)));
+        }
+    } else {
+        for (int i0 = tid; i0 < n; i0 += 256 * kBoundLoads) {
+            float v[kBoundLoads];
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) {
+                const int i = i0 + 256 * k;
+                v[k] = i < n ? base[i] : 0.f;
+            }
+#pragma unroll
+            for (int k = 0; k < kBoundLoads; ++k) mx = fmaxf(mx, fabsf(v[k]));
+        }
+    }
+    for (int o = 32; o >= 1; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    __shared__ float red[4];
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        const float w = fabsf(p.scale) * p.wmax * (p.wview ? fabsf(p.wview[view]) : 1.f);
+        atomicMax(vbound + view, __float_as_uint(mx * w));     // >= 0: bit order = value order
+    }
+}
+
 __global__ void zero_u32_kernel(unsigned int* p, int64_t n)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
@@ -1037,6 +1095,7 @@ Params make_params(const FilterArgs& a)
     p.ptab = a.ptab;
     p.vbound = a.vbound;
     p.g1 = a.g1;
+    p.wmax = a.wmax;
     p.gscale = a.gscale;
     p.vscale = a.vscale;
     p.row_tiles_per_block = 0;
@@ -1145,7 +1204,11 @@ void launch_filter_bound(const FilterArgs& a, unsigned int* vbound, hipStream_t 
         throw Error(SGX_ERR_UNSUPPORTED, "sino_filter: too many rows per view");
     const int vec = a.nb % 4 == 0 && (reinterpret_cast<uintptr_t>(p.in) & 15u) == 0;
     const dim3 grid((unsigned)na, (unsigned)((a.nt + kBoundRows - 1) / kBoundRows));
-    if (a.nb > kNbSmall)
+    if (a.fft && a.wmax > 0.f)
+        hipLaunchKernelGGL(absmax_kernel,
+                           dim3((unsigned)na, (unsigned)((a.nt + kAbsRows - 1) / kAbsRows)),
+                           dim3(256), 0, st, p, vbound, vec);
+    else if (a.nb > kNbSmall)
         hipLaunchKernelGGL(bound_kernel<kNbWide>, grid, dim3(256), 0, st, p, vbound, vec);
     else
         hipLaunchKernelGGL(bound_kernel<kNbSmall>, grid, dim3(256), 0, st, p, vbound, vec);

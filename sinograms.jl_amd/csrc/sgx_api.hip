@@ -876,6 +876,7 @@ struct sgx_plan {
     unsigned int* vbound = nullptr; // per-view max |weighted input| (F16S form)
     float* vscale = nullptr;        // per-view FP16 scale (F16S form)
     float g1 = 0.f;                 // sum |g| (F16S output bound)
+    float wmax = 0.f;               // bound of |bin x cylinder weight| (F16S, FFT form)
     float gscale = 1.f;             // power of two with max|g| * gscale <= 2^14 (split FP16)
     float* partial = nullptr;
     int64_t partial_cap = 0;
@@ -942,6 +943,7 @@ struct sgx_plan {
             a.gscale = gscale;
             a.vbound = vbound;
             a.g1 = g1;
+            a.wmax = fft ? wmax : 0.f;
             a.vscale = vscale;
         }
         a.scale = (float)G.angle_weight;
@@ -1294,6 +1296,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         // cone beam on the FFT form: wbin x cylinder weight per (bin, row), in
         // FP64 then rounded (fdk_weight_cyl, src/fbp3/fdk.jl:21-45)
         size_t o_wtab = 0;
+        double wtmax = 0.0;
         if (!ft.empty() && is3d) {
             std::vector<float> wt((size_t)ns * nt);
             const bool arc = G.kind == SGX_CT_FAN_ARC;
@@ -1304,8 +1307,12 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
                                                std::sqrt((tt / G.dsd) * (tt / G.dsd) + 1.0)
                                          : G.dso / std::sqrt(s[i] * s[i] + tt * tt + G.dsd * G.dsd);
                     wt[it * ns + i] = (float)((wbin_host.empty() ? 1.0 : (double)wbin_host[i]) * w);
+                    wtmax = std::max(wtmax, std::fabs((double)wt[it * ns + i]));
                 }
             }
+            // the streaming range pass multiplies max |input| by this bound
+            // (rounded up: every table entry is at most wtmax)
+            P->wmax = (float)(wtmax * (1.0 + 1e-6));
             o_wtab = pack.add(wt.data(), wt.size() * 4);
         }
         SGX_HIP(hipMalloc(&P->tables, pack.host.size()));

@@ -309,6 +309,11 @@ struct FilterArgs {
     // FFT form, cone beam: the per-(bin, row) part of the fused weight,
     // wbin[m] times the cylinder weight of (s[m], t[row]), (nb, nt), or null
     const float* wtab = nullptr;
+    // scaled FP16 on the FFT form: a bound of |wbin[m] x cylinder weight|
+    // over all (bin, row); > 0 selects the streaming range pass, whose view
+    // bound is max |input| x |scale| x wmax x |wview[view]| (Parker weights
+    // are <= 1), instead of the per-sample weighted maximum
+    float wmax = 0.f;
 };
 // Host: the FFT form's table for the response Hk, 5 x N floats with the
 // transform size N = max(npad, 1024): the twiddles W_N^(n1 k2) as
