@@ -95,7 +95,10 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
  *                        3 = LDS-staged with blocks 8 voxels deep (twice the
  *                            blocks: for volumes too small to fill the GPU;
  *                            the mirror form, where it applies, takes
- *                            precedence).
+ *                            precedence),
+ *                        4 = as 3, with detector windows of up to 64 bins x
+ *                            32 rows (coarse voxels, whose blocks span more
+ *                            of the detector than the others' windows hold).
  *   SGX_OPT_CBCT_MIRROR: the staged backprojector's mirror form, which pairs
  *                        each voxel z > 0 with its image at -z where the
  *                        geometry is symmetric (z axis symmetric about 0, no
@@ -361,15 +364,16 @@ int sgx_plan_last_times(const sgx_plan* plan, float* filter_ms, float* back_ms);
  * power-of-two ceilings of nx, ny, nz, ns, nt, na); with `use_cache` a plan
  * whose bucket is cached takes it without timing.  `image` holds the chosen
  * shape's result on return.  Candidates: cone-beam, the staged
- * backprojector with blocks 16 or 8 voxels deep and (symmetric geometries)
- * its mirror form, the gather backprojector, and for scaled-FP16 plans the
+ * backprojector with blocks 16 or 8 voxels deep (the latter also with wide
+ * windows) and (symmetric geometries) its mirror form, the gather backprojector, and for scaled-FP16 plans the
  * packed-FP16 interpolation; 2-D, view-grouping targets {1024, 2048, 4096}
  * blocks x LDS view chunks {16, 32, 64}. */
 int sgx_plan_autotune(sgx_plan* plan, const float* sino, float* image, int reps,
                       int use_cache);
 /* The plan's kernel shape: values[0] = cone-beam kernel (0 staged, 1
  * gather, 2 staged with packed-FP16 interpolation, 3 the staged kernel's
- * mirror form, 4 the staged kernel with blocks 8 voxels deep), values[1] =
+ * mirror form, 4 the staged kernel with blocks 8 voxels deep, 5 the same
+ * with wide windows), values[1] =
  * 2-D view-grouping target, values[2] = 2-D view chunk, values[3] = 1 when
  * autotuned. */
 int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4]);

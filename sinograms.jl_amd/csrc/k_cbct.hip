@@ -238,6 +238,7 @@ constexpr int kGatherVariant = 16;   // cbct_variant: the gather kernel
 constexpr int kPackedVariant = 32;   // cbct_variant: the packed-FP16 kernel
 constexpr int kMirrorVariant = 64;   // cbct_variant: the mirror form of the staged kernel
 constexpr int kZ8Variant = 128;      // cbct_variant: the Z8 form of the staged kernel
+constexpr int kZ8WVariant = 256;     // cbct_variant: the wide Z8 form of the staged kernel
 constexpr int kPatch = 8;      // 8 x 8 columns per wave
 
 // One block of the gather kernel: block `blk` (8 x 8 columns x 4 waves of
@@ -341,6 +342,14 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #ifndef SGX_STAGED_PITCH
 #define SGX_STAGED_PITCH 58
 #endif
+// the wide 8-deep form's LDS row pitch (entries): 16 bytes of bank stagger
+// per row for both entry sizes
+#ifndef SGX_STAGED_PITCH_W
+#define SGX_STAGED_PITCH_W 66
+#endif
+#ifndef SGX_STAGED_PITCH16_W
+#define SGX_STAGED_PITCH16_W 68
+#endif
 #ifndef SGX_STAGED_SCHED
 #define SGX_STAGED_SCHED 1
 #endif
@@ -376,27 +385,35 @@ constexpr int kSB = SGX_STAGED_BATCH;
 // Z8 form: blocks of 16 x 16 columns x 8 z (twice as many blocks, windows of
 // at most kHm rows), for volumes too small to fill the GPU with 16-deep
 // blocks; one of the autotuner's candidates.
+//
+// Wide Z8 form (Z8W): 8-deep blocks with windows of up to kWmaxW bins x
+// kHmax rows, for coarse voxels (a 16 x 16 x 8 block of 2 mm voxels seen at
+// magnification 2 spans about 50 bins and 27 rows of 1.2 mm, past the other
+// forms' windows, whose blocks would then gather).  Every thread stages.
 constexpr int kSZm = 8;               // mirror / Z8 forms: voxels per lane positioned
 constexpr int kHm = 20;               // mirror / Z8 forms: window rows (each window)
-constexpr int kZ16 = 0, kZMirror = 1, kZ8 = 2;   // block forms of the staged kernel (ZF)
+constexpr int kWmaxW = 64;            // Z8W form: window bins
+constexpr int kZ16 = 0, kZMirror = 1, kZ8 = 2, kZ8W = 3;   // block forms of the staged kernel (ZF)
 template <int FMT, int ZF = kZ16> struct Lds {
     static constexpr bool half = FMT == F_F16S;
     static constexpr bool mir = ZF == kZMirror;
+    static constexpr bool wide = ZF == kZ8W;
     static constexpr uint32_t esz = half ? 4 : 8;
-    static constexpr int wpitch = half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH;  // per window
+    static constexpr int wmax = wide ? kWmaxW : kWmax;        // window bins
+    static constexpr int wpitch = wide ? (half ? SGX_STAGED_PITCH16_W : SGX_STAGED_PITCH_W)
+                                       : (half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH);
     static constexpr int pitch = mir ? 2 * wpitch : wpitch;   // entries per LDS row
-    static constexpr int rows = ZF == kZ16 ? kHmax : kHm;     // rows per buffer
+    static constexpr int rows = (ZF == kZ16 || wide) ? kHmax : kHm;   // rows per buffer
     static constexpr int buf = rows * pitch;                  // entries per buffer
     // one stager per quad and row pair of each window
-    static constexpr int stagers = (kWmax / 4) * (rows / 2) * (mir ? 2 : 1);
-    static_assert(wpitch >= kWmax + 2 && (wpitch * esz) % 16 == 0, "pitch");
+    static constexpr int stagers = (wmax / 4) * (rows / 2) * (mir ? 2 : 1);
+    static_assert(wpitch >= wmax + 2 && (wpitch * esz) % 16 == 0, "pitch");
     static_assert(stagers <= 256, "one staging thread per quad and row pair");
     // the second read of a mirror pair: an immediate offset of ds_read2
     static_assert(!mir || wpitch + 1 <= 255, "ds_read2 offset");
 };
 
 static_assert(kSZ % kSB == 0 && kSZm % (kSB / 2) == 0, "batch divides the voxels per lane");
-constexpr int kQuads = kWmax / 4;
 
 // Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
 // stored as int4 {clo | interior, rlo, W, H}.  clo is a multiple of 4;
@@ -420,7 +437,9 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     const int tiles = tiles_x * ((ny + kSP - 1) / kSP);
     const int tile = blk % tiles, zblk = blk / tiles;
     // a mirror block's window is the one of its voxels z > 0
-    const int zn = zform == kZ16 ? kSZ : kSZm, hmax = zform == kZ16 ? kHmax : kHm;
+    const int zn = zform == kZ16 ? kSZ : kSZm;
+    const int hmax = (zform == kZ16 || zform == kZ8W) ? kHmax : kHm;
+    const int wmax = zform == kZ8W ? kWmaxW : kWmax;
     const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP;
     const int iz0 = zform == kZMirror ? nz / 2 + zblk * kSZm : zblk * zn;
     // extent of the block's voxel centres (any axis order)
@@ -481,8 +500,8 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
         if (chi >= clo && rhi >= rlo) {
             const int W = chi - clo + 1, H = rhi - rlo + 1;
             const bool interior = smin >= 2.f && smax < (float)(ns - 1) && tmin >= 2.f &&
-                                  tmax < (float)(nt - 2) && W <= kWmax && H <= hmax;
-            w = make_int4(clo | (interior ? 1 : 0), rlo, min(W, kWmax), min(H, hmax));
+                                  tmax < (float)(nt - 2) && W <= wmax && H <= hmax;
+            w = make_int4(clo | (interior ? 1 : 0), rlo, min(W, wmax), min(H, hmax));
         }
     }
     win[i] = w;
@@ -558,7 +577,8 @@ __device__ __forceinline__ void cbct_staged_block(
     // Mirror form: groups sg >= kHm / 2 stage the mirror window, whose entry
     // row r reads the data rows nt - 1 - (rlo + r) downwards.
     const bool stager = tid < L::stagers;
-    const int sq = tid % kQuads, sg = tid / kQuads;
+    constexpr int QD = L::wmax / 4;                   // quads per window row
+    const int sq = tid % QD, sg = tid / QD;
     const bool mst = MIR && sg >= kHm / 2;
     const int sgw = mst ? sg - kHm / 2 : sg;          // row pair within the window
     const uint32_t toff = ((uint32_t)(2 * sgw) * (uint32_t)pitch + 4u * sq) * Fmt<FMT>::esz;
@@ -1166,7 +1186,8 @@ int zform_of(const CbctArgs& a)
 int64_t staged_blocks(const CbctArgs& a)
 {
     const int64_t tiles = (int64_t)((a.nx + kSP - 1) / kSP) * ((a.ny + kSP - 1) / kSP);
-    const int zb = zform_of(a) == kZ8 ? kSZm : kSZ;
+    const int zf = zform_of(a);
+    const int zb = (zf == kZ8 || zf == kZ8W) ? kSZm : kSZ;
     return tiles * ((a.nz + zb - 1) / zb);
 }
 
@@ -1243,6 +1264,9 @@ template <int FMT, bool OPT> void launch_opt(const CbctArgs& a, hipStream_t st)
     } else if (a.win && a.zform == kZ8) {
         if (a.is_arc) launch_staged<true, FMT, OPT, kZ8>(a, st);
         else launch_staged<false, FMT, OPT, kZ8>(a, st);
+    } else if (a.win && a.zform == kZ8W) {
+        if (a.is_arc) launch_staged<true, FMT, OPT, kZ8W>(a, st);
+        else launch_staged<false, FMT, OPT, kZ8W>(a, st);
     } else if (a.win) {
         if (a.is_arc) launch_staged<true, FMT, OPT, kZ16>(a, st);
         else launch_staged<false, FMT, OPT, kZ16>(a, st);
@@ -1307,7 +1331,8 @@ int cbct_variant(const CbctArgs& a)
     return (a.is_arc ? 1 : 0) + (opt ? 2 : 0) + 4 * fmt + (a.win ? 0 : kGatherVariant) +
            (packed_form(a) ? kPackedVariant : 0) +
            (a.win && zform_of(a) == kZMirror ? kMirrorVariant : 0) +
-           (a.win && zform_of(a) == kZ8 ? kZ8Variant : 0);
+           (a.win && zform_of(a) == kZ8 ? kZ8Variant : 0) +
+           (a.win && zform_of(a) == kZ8W ? kZ8WVariant : 0);
 }
 
 CbctItem cbct_item(const CbctArgs& a)
@@ -1406,6 +1431,7 @@ static const void* cbct_queue_fn(int variant)
     do {                                                                             \
         if (variant & kMirrorVariant) SGX_CBQ2(F, kZMirror);                         \
         if (variant & kZ8Variant) SGX_CBQ2(F, kZ8);                                  \
+        if (variant & kZ8WVariant) SGX_CBQ2(F, kZ8W);                                \
         SGX_CBQ2(F, kZ16);                                                           \
     } while (0)
     if (fmt == F_BF16) SGX_CBQ(F_BF16);

@@ -63,7 +63,7 @@ struct PlanTune {
     int want_blocks = 2048;
     int vc_max = 64;
     int zform = 0;       // cone beam, staged kernel: block form (0 16 z, 1 mirror where the
-                         // geometry allows it, 2 8 z)
+                         // geometry allows it, 2 8 z, 3 8 z with wide windows)
 };
 
 // Largest view-grouping target and smallest LDS chunk the autotuner tries.
@@ -398,7 +398,7 @@ int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
             ctx->cbct_mirror = (int)value;
             return;
         }
-        require(value >= 0 && value <= 3, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0 to 3");
+        require(value >= 0 && value <= 4, SGX_ERR_ARG, "SGX_OPT_CBCT_KERNEL is 0 to 4");
         ctx->cbct_kernel = (int)value;
     });
 }
@@ -576,6 +576,7 @@ void cbct_back_impl(sgx_ctx* ctx, const void* proj, int64_t ns, int64_t nt, int6
     // it is asked for everywhere (it measured slower for them)
     a.zform = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym) ? 1
               : ctx->cbct_kernel == 3                                  ? 2
+              : ctx->cbct_kernel == 4                                  ? 3
                                                                        : 0;
     if (ctx->cbct_kernel != 1) {
         int4* win = static_cast<int4*>(
@@ -642,6 +643,7 @@ int sgx_cbct_back_noalloc(sgx_ctx* ctx, const float* proj, int64_t ns, int64_t n
                                          nullptr);
         a.zform = ctx->cbct_mirror == 2 && sgx::cbct_mirror_ok(a, T.z_sym) ? 1
                   : ctx->cbct_kernel == 3                                  ? 2
+                  : ctx->cbct_kernel == 4                                  ? 3
                                                                            : 0;
         const bool staged = ctx->cbct_kernel != 1;
         const size_t need =
@@ -1103,6 +1105,7 @@ std::vector<PlanTune> tune_candidates(const sgx_plan& P)
         Q.tune.zform = m;
         if (can_mirror) c.push_back(PlanTune{0, 2048, 64, 1});
         c.push_back(PlanTune{0, 2048, 64, 2});
+        c.push_back(PlanTune{0, 2048, 64, 3});
         if (P.prec == SGX_PREC_F16S) c.push_back(PlanTune{2, 2048, 64});
     } else {
         for (int want : {1024, 2048, kTuneMaxBlocks})
@@ -1217,13 +1220,14 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         P->ny = ny;
         P->nz = is3d ? nz : 1;
         P->npad = npad;
-        P->tune.cbct = ctx->cbct_kernel == 3 ? 0 : ctx->cbct_kernel;
+        P->tune.cbct = ctx->cbct_kernel >= 3 ? 0 : ctx->cbct_kernel;
         // the mirror form: by default for the scaled-FP16 form, where it
         // measured faster; the autotuner times it for the others
         P->mirror_allowed = ctx->cbct_mirror != 0;
         P->tune.zform =
             ctx->cbct_mirror == 2 || (ctx->cbct_mirror == 1 && prec == SGX_PREC_F16S) ? 1
             : ctx->cbct_kernel == 3                                                     ? 2
+            : ctx->cbct_kernel == 4                                                     ? 3
                                                                                         : 0;
 
         const int64_t ns = G.ns, nt = G.nt, na = G.na;
@@ -1560,7 +1564,8 @@ int sgx_plan_tuning(const sgx_plan* plan, int64_t values[4])
 {
     if (!plan || !values) return SGX_ERR_ARG;
     values[0] = plan->tune.cbct != 0 ? plan->tune.cbct
-                : plan->win_zform == 1 ? 3 : plan->win_zform == 2 ? 4 : 0;
+                : plan->win_zform == 1 ? 3 : plan->win_zform == 2 ? 4
+                : plan->win_zform == 3 ? 5 : 0;
     values[1] = plan->tune.want_blocks;
     values[2] = plan->tune.vc_max;
     values[3] = plan->tuned ? 1 : 0;

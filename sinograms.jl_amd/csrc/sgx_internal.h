@@ -119,7 +119,8 @@ struct CbctArgs {
     // place of the staged kernel's FP32 interpolation
     bool f16_packed = false;
     // staged kernel's block form: 0 = 16 z per lane, 1 = the mirror form
-    // (pairs z and -z; see cbct_mirror_ok), 2 = 8 z per lane
+    // (pairs z and -z; see cbct_mirror_ok), 2 = 8 z per lane, 3 = 8 z per
+    // lane with windows of up to 64 bins x 32 rows (coarse voxels)
     int zform = 0;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);

@@ -188,9 +188,10 @@ SGX_OPT_CBCT_KERNEL = 1
 SGX_OPT_CBCT_MIRROR = 2
 CBCT_MIRROR = ("never", "fp16", "always")    # values of SGX_OPT_CBCT_MIRROR
 # values of SGX_OPT_CBCT_KERNEL
-CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_z8")
+CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_z8", "staged_z8w")
 # cone-beam kernels a plan reports (sgx_plan_tuning)
-PLAN_CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_mirror", "staged_z8")
+PLAN_CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_mirror", "staged_z8",
+                     "staged_z8w")
 SGX_BATCH_NO_TUNE = 0x100
 SGX_BATCH_NO_QUEUE = 0x200
 SGX_BATCH_QUEUE = 0x400
@@ -258,8 +259,9 @@ class Context:
     def cbct_kernel(self) -> str:
         """Cone-beam backprojector of this context: "staged" (LDS-staged
         detector windows, the default), "gather", "staged_packed16" (staged,
-        with scaled-FP16 plans interpolating in packed FP16) or "staged_z8"
-        (staged, blocks 8 voxels deep)."""
+        with scaled-FP16 plans interpolating in packed FP16), "staged_z8"
+        (staged, blocks 8 voxels deep) or "staged_z8w" (8-deep blocks with
+        windows of up to 64 bins x 32 rows, for coarse voxels)."""
         v = ctypes.c_int64()
         self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_CBCT_KERNEL, ctypes.byref(v)))
         return CBCT_KERNELS[v.value]

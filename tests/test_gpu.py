@@ -45,7 +45,7 @@ def _bf16_round(x):
 
 # --- K3 cone-beam backprojection -------------------------------------------------------
 
-@pytest.fixture(params=["staged", "gather", "staged_z8"])
+@pytest.fixture(params=["staged", "gather", "staged_z8", "staged_z8w"])
 def cbct_kernel(request, sg, torch_cuda):
     """Run a test with each cone-beam backprojector of the shared context."""
     ctx = sg._lib.context(0)
@@ -133,20 +133,34 @@ def test_cbct_back_mirror_vs_oracle(sg, orc, torch_cuda, arc, case):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "f16s"])
-def test_fdk_block_forms_bit_identical(sg, torch_cuda, prec):
-    # The staged kernel's 8-deep blocks run every voxel through the same
-    # arithmetic, views in the same order, as the 16-deep blocks: the volumes
-    # are bit for bit the same (nz = 40: a partial block at the top).
-    _say("fdk block forms", prec)
+@pytest.mark.parametrize("voxels", ["fine", "coarse"])
+def test_fdk_block_forms_bit_identical(sg, orc, torch_cuda, prec, voxels):
+    # The staged kernel's 8-deep blocks (with the usual or the wide windows)
+    # run every voxel through the same arithmetic as the 16-deep blocks.
+    # "fine": every view of every block is staged, views summed in order, and
+    # the volumes are bit for bit the same (nz = 40: a partial block at the
+    # top).  "coarse": 1.8 mm voxels at magnification 2 on 1 mm bins; a
+    # 16 x 16 column block spans about 55 bins, past the 48-bin windows (those
+    # lanes gather the views they miss after the staged ones, so the sums'
+    # order differs) and inside the wide windows, which stage every view.
+    _say("fdk block forms", prec, voxels)
     rng = np.random.default_rng(51)
-    rg = sg.CtFanFlat(ns=96, nt=64, ds=1.2, dt=1.2, na=48, dsd=400.0, dod=150.0)
-    ig = sg.ImageGeom((40, 36, 40), (1.0, 1.0, 1.0))
-    proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    if voxels == "fine":
+        rg = sg.CtFanFlat(ns=96, nt=64, ds=1.2, dt=1.2, na=48, dsd=400.0, dod=150.0)
+        ig = sg.ImageGeom((40, 36, 40), (1.0, 1.0, 1.0))
+    else:
+        rg = sg.CtFanFlat(ns=160, nt=96, ds=1.0, dt=1.0, na=36, dsd=400.0, dod=200.0)
+        ig = sg.ImageGeom((40, 36, 24), (1.8, 1.8, 1.8))
+    if voxels == "fine":
+        proj = rng.standard_normal(rg.dims()).astype(np.float32)
+    else:                       # a phantom: the oracle check below at the FP32 bar
+        ell = sg.phantom.random_ellipsoids(51, 8, 60.0)
+        proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
     ctx = sg._lib.context(0)
     vols = {}
     try:
         ctx.cbct_mirror = "never"
-        for k in ("staged", "staged_z8"):
+        for k in ("staged", "staged_z8", "staged_z8w"):
             ctx.cbct_kernel = k
             plan = sg.plan_fbp(rg, ig, precision=prec)
             assert sg.tuning(plan)["cbct_kernel"] == k
@@ -154,7 +168,20 @@ def test_fdk_block_forms_bit_identical(sg, torch_cuda, prec):
     finally:
         ctx.cbct_kernel = "staged"
         ctx.cbct_mirror = "fp16"
-    assert np.array_equal(vols["staged"], vols["staged_z8"])
+    if voxels == "fine":
+        assert np.array_equal(vols["staged"], vols["staged_z8"])
+        assert np.array_equal(vols["staged"], vols["staged_z8w"])
+        return
+    for k in ("staged_z8", "staged_z8w"):
+        assert rel_rmse(vols[k], vols["staged"]) < 1e-6
+    if prec == "fp32":
+        ar = rg.ar(f32=True).astype(np.float32)
+        vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
+            orc.angle_weight(ar)
+        xc, yc, zc = ig.axes()
+        ref = orc.fdk(proj, plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False,
+                      xc, yc, zc, ig.mask, dtype=np.float32)
+        assert rel_rmse(vols["staged_z8w"], ref) < TOL32
 
 
 def test_cbct_back_detector_edges(sg, orc, cbct_kernel):
@@ -1000,7 +1027,8 @@ def test_autotune_choice_and_cache(sg, orc, torch_cuda):
     p1 = sg.plan_fbp(rg, ig)
     y1 = sg.recon._empty_colmajor(ig.dims, dev)
     t = sg.autotune(p1, x, y1)
-    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather", "staged_mirror", "staged_z8")
+    assert t["tuned"] and t["cbct_kernel"] in ("staged", "gather", "staged_mirror", "staged_z8",
+                                               "staged_z8w")
     ref = orc.fdk(proj.astype(np.float64), p1.filter,
                   orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] *
                   orc.angle_weight(rg.ar(f32=True).astype(np.float32)),
