@@ -326,16 +326,22 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 
 // Build-time tuning of the staged kernel (defaults measured best on the
 // 256^2 x 128 FDK workload): LDS loads in flight per batch, minimum waves
-// per SIMD (caps the VGPRs), LDS row pitch in entries.  The arc detector's
-// 16-bit forms ask for 4 waves: uncapped they take 131 VGPRs, one step past
-// 128, and run at 3 waves per SIMD 26 % slower (3 VGPRs spill, outside the
-// view loop).
+// per SIMD of the flat-detector forms (caps the VGPRs), LDS row pitch in
+// entries.  At 5 waves per SIMD (96 VGPRs; about 27 spill to scratch, the
+// LDS allows 5 blocks of the 16-deep form per CU) the waves hide the staging
+// and LDS latency the per-view barrier exposes: FP32 1.096 -> 0.990 ms,
+// scaled FP16 0.922 -> 0.880 ms, BF16 1.071 -> 1.051 ms against 3 waves and
+// batches of 8 (119 VGPRs, 4 waves).  The arc detector's forms keep their
+// bounds (not measured at 5): the 16-bit forms ask for 4 waves (uncapped
+// they take 131 VGPRs, one step past 128, and run at 3 waves per SIMD 26 %
+// slower), FP32 for 3.
 #ifndef SGX_STAGED_BATCH
-#define SGX_STAGED_BATCH 8
+#define SGX_STAGED_BATCH 4
 #endif
 #ifndef SGX_STAGED_WAVES
-#define SGX_STAGED_WAVES 3
+#define SGX_STAGED_WAVES 5
 #endif
+
 #ifndef SGX_STAGED_PITCH16
 #define SGX_STAGED_PITCH16 52
 #endif
@@ -414,6 +420,15 @@ template <int FMT, int ZF = kZ16> struct Lds {
 };
 
 static_assert(kSZ % kSB == 0 && kSZm % (kSB / 2) == 0, "batch divides the voxels per lane");
+
+// Minimum waves per SIMD of a staged kernel instance: SGX_STAGED_WAVES for
+// the flat detector, capped by the blocks of 4 waves that the 160 KiB of LDS
+// per CU holds (4 for the FP32 and BF16 mirror and wide forms).
+template <bool ARC, int FMT, int ZF> constexpr int staged_waves()
+{
+    constexpr int by_lds = (160 * 1024) / (2 * Lds<FMT, ZF>::buf * (int)Lds<FMT, ZF>::esz);
+    return ARC ? (FMT != F_F32 ? 4 : 3) : (SGX_STAGED_WAVES < by_lds ? SGX_STAGED_WAVES : by_lds);
+}
 
 // Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
 // stored as int4 {clo | interior, rlo, W, H}.  clo is a multiple of 4;
@@ -804,7 +819,7 @@ __device__ __forceinline__ void cbct_staged_block(
 }
 
 template <bool ARC, int FMT, bool OPT, int ZF>
-__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
+__global__ __launch_bounds__(256, (staged_waves<ARC, FMT, ZF>())) void
 cbct_staged_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
@@ -828,7 +843,7 @@ cbct_staged_kernel(
 // share).  The hardware dispatches blocks in index order, so the launch
 // walks the table in order and backfills the GPU as blocks finish.
 template <bool ARC, int FMT, bool OPT, int ZF>
-__global__ __launch_bounds__(256, (ARC && FMT != F_F32) ? 4 : SGX_STAGED_WAVES) void
+__global__ __launch_bounds__(256, (staged_waves<ARC, FMT, ZF>())) void
 cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
 {
     using L = Lds<FMT, ZF>;
@@ -1135,7 +1150,7 @@ __device__ __forceinline__ void cbct_packed_block(
 }
 
 template <bool ARC, bool OPT>
-__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_packed_kernel(
+__global__ __launch_bounds__(256, 3) void cbct_packed_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
@@ -1152,7 +1167,7 @@ __global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_packed_kernel(
 
 // Batch form (as cbct_queue_kernel).
 template <bool ARC, bool OPT>
-__global__ __launch_bounds__(256, SGX_STAGED_WAVES) void cbct_packed_queue_kernel(
+__global__ __launch_bounds__(256, 3) void cbct_packed_queue_kernel(
     const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
 {
     __shared__ __attribute__((aligned(16))) char lds[2 * kPkBuf];
