@@ -327,19 +327,23 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 // Build-time tuning of the staged kernel (defaults measured best on the
 // 256^2 x 128 FDK workload): LDS loads in flight per batch, minimum waves
 // per SIMD of the flat-detector forms (caps the VGPRs), LDS row pitch in
-// entries.  At 5 waves per SIMD (96 VGPRs; about 27 spill to scratch, the
-// LDS allows 5 blocks of the 16-deep form per CU) the waves hide the staging
-// and LDS latency the per-view barrier exposes: FP32 1.096 -> 0.990 ms,
-// scaled FP16 0.922 -> 0.880 ms, BF16 1.071 -> 1.051 ms against 3 waves and
-// batches of 8 (119 VGPRs, 4 waves).  The arc detector's forms keep their
-// bounds (not measured at 5): the 16-bit forms ask for 4 waves (uncapped
-// they take 131 VGPRs, one step past 128, and run at 3 waves per SIMD 26 %
-// slower), FP32 for 3.
+// entries.  More waves per SIMD hide the staging and LDS latency that the
+// per-view barrier exposes, though the VGPR cap spills some registers (to
+// scratch, outside the inner loop's working set).  Against 3 waves and
+// batches of 8 (119 VGPRs, 4 waves): FP32 1.096 -> 0.986 ms at 5 waves (96
+// VGPRs, 27 spilled; the LDS holds 5 blocks of its 16-deep form per CU),
+// scaled FP16 (mirror form) 0.922 -> 0.878 ms at 5 and 0.737 ms at 6 (80
+// VGPRs, 12 spilled; 7 waves: 0.732 ms with 25 spilled), BF16 1.071 -> 1.052
+// ms.  Squeezing the FP32 rows to 6 blocks per CU (pitch 52, 80 VGPRs, 59
+// spilled) measured 0.995 ms.  The arc detector's forms keep their bounds
+// (not measured here): the 16-bit forms ask for 4 waves (uncapped they take
+// 131 VGPRs, one step past 128, and run at 3 waves per SIMD 26 % slower),
+// FP32 for 3.
 #ifndef SGX_STAGED_BATCH
 #define SGX_STAGED_BATCH 4
 #endif
 #ifndef SGX_STAGED_WAVES
-#define SGX_STAGED_WAVES 5
+#define SGX_STAGED_WAVES 6
 #endif
 
 #ifndef SGX_STAGED_PITCH16
