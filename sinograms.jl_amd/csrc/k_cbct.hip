@@ -349,6 +349,11 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 #ifndef SGX_STAGED_PITCH16
 #define SGX_STAGED_PITCH16 52
 #endif
+// minimum waves per SIMD of the packed-FP16 kernel (1.079 ms at 3, 1.016 at
+// 5, 0.943 at 6 on the 256^2 x 128 workload)
+#ifndef SGX_PACKED_WAVES
+#define SGX_PACKED_WAVES 6
+#endif
 #ifndef SGX_STAGED_PITCH
 #define SGX_STAGED_PITCH 58
 #endif
@@ -892,7 +897,8 @@ cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ t
 //   the 8-byte entries repeat every bin (twice the staging work and LDS
 //   writes of the FP32 interpolation's 4-byte entries), so the whole kernel
 //   issues 9.7 VALU per update against 10.1 and runs 1.09 ms against
-//   0.98 ms.  The rounding of the position to 1/1024 row, of v and of W each
+//   0.98 ms for the 16-deep FP32-interpolation blocks, both at 3 waves per
+//   SIMD; at 6 waves 0.94 ms against 0.74 ms for the mirror form.  The rounding of the position to 1/1024 row, of v and of W each
 //   add about as much error as the FP16 storage itself: 6.7e-5 from Float64
 //   against 3.4e-5.
 // * Lanes whose taps leave the window, or whose w2 would leave FP16's range,
@@ -1154,7 +1160,7 @@ __device__ __forceinline__ void cbct_packed_block(
 }
 
 template <bool ARC, bool OPT>
-__global__ __launch_bounds__(256, 3) void cbct_packed_kernel(
+__global__ __launch_bounds__(256, SGX_PACKED_WAVES) void cbct_packed_kernel(
     const void* __restrict__ q, int ns, int nt, int pitch, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ vscale,
     const float* __restrict__ xds, const float* __restrict__ yds,
@@ -1171,7 +1177,7 @@ __global__ __launch_bounds__(256, 3) void cbct_packed_kernel(
 
 // Batch form (as cbct_queue_kernel).
 template <bool ARC, bool OPT>
-__global__ __launch_bounds__(256, 3) void cbct_packed_queue_kernel(
+__global__ __launch_bounds__(256, SGX_PACKED_WAVES) void cbct_packed_queue_kernel(
     const CbctItem* __restrict__ items, const int2* __restrict__ tasks)
 {
     __shared__ __attribute__((aligned(16))) char lds[2 * kPkBuf];
