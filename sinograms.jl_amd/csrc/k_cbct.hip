@@ -334,16 +334,24 @@ __global__ __launch_bounds__(64 * kWaves) void cbct_back_kernel(
 // VGPRs, 27 spilled; the LDS holds 5 blocks of its 16-deep form per CU),
 // scaled FP16 (mirror form) 0.922 -> 0.878 ms at 5 and 0.737 ms at 6 (80
 // VGPRs, 12 spilled; 7 waves: 0.732 ms with 25 spilled), BF16 1.071 -> 1.052
-// ms.  Squeezing the FP32 rows to 6 blocks per CU (pitch 52, 80 VGPRs, 59
-// spilled) measured 0.995 ms.  The arc detector's forms keep their bounds
-// (not measured here): the 16-bit forms ask for 4 waves (uncapped they take
-// 131 VGPRs, one step past 128, and run at 3 waves per SIMD 26 % slower),
-// FP32 for 3.
+// ms.  Squeezing the FP32 blocks to 6 per CU measured slower: pitch 52,
+// 80 VGPRs with 59 spilled, 0.995 ms; 28-row windows (SGX_STAGED_ROWS, 0.3 %
+// of the workload's windows then overflow), 80 VGPRs, 1.245 ms.  The arc
+// detector's forms (the same volume from a 512 x 384 arc panel): FP32
+// 1.110 -> 1.083 ms at 5 waves (from 3), scaled FP16 1.018 -> 0.815 ms at 6
+// (from 4; uncapped they take 131 VGPRs and run at 3 waves), BF16 1.13 ->
+// 1.11 ms.
 #ifndef SGX_STAGED_BATCH
 #define SGX_STAGED_BATCH 4
 #endif
 #ifndef SGX_STAGED_WAVES
 #define SGX_STAGED_WAVES 6
+#endif
+#ifndef SGX_STAGED_WAVES_ARC16
+#define SGX_STAGED_WAVES_ARC16 6
+#endif
+#ifndef SGX_STAGED_WAVES_ARC32
+#define SGX_STAGED_WAVES_ARC32 5
 #endif
 
 #ifndef SGX_STAGED_PITCH16
@@ -375,6 +383,13 @@ constexpr int kSZ = 16;               // voxels per lane along z
 #endif
 constexpr int kWmax = SGX_STAGED_WMAX; // staged window: bins (quads of 4) ...
 constexpr int kHmax = 32;             // ... x rows (16 groups of 2)
+// rows of the 16-deep form's windows (at most kHmax; the packed kernel
+// shares its window table)
+#ifndef SGX_STAGED_ROWS
+#define SGX_STAGED_ROWS 32
+#endif
+constexpr int kH16 = SGX_STAGED_ROWS;
+static_assert(kH16 <= kHmax && kH16 % 2 == 0, "16-deep window rows");
 constexpr int kSB = SGX_STAGED_BATCH;
 
 // LDS entry of the staged window: (q[is, it], q[is, it+1] - q[is, it]) as two
@@ -418,7 +433,7 @@ template <int FMT, int ZF = kZ16> struct Lds {
     static constexpr int wpitch = wide ? (half ? SGX_STAGED_PITCH16_W : SGX_STAGED_PITCH_W)
                                        : (half ? SGX_STAGED_PITCH16 : SGX_STAGED_PITCH);
     static constexpr int pitch = mir ? 2 * wpitch : wpitch;   // entries per LDS row
-    static constexpr int rows = (ZF == kZ16 || wide) ? kHmax : kHm;   // rows per buffer
+    static constexpr int rows = ZF == kZ16 ? kH16 : wide ? kHmax : kHm;   // rows per buffer
     static constexpr int buf = rows * pitch;                  // entries per buffer
     // one stager per quad and row pair of each window
     static constexpr int stagers = (wmax / 4) * (rows / 2) * (mir ? 2 : 1);
@@ -436,7 +451,9 @@ static_assert(kSZ % kSB == 0 && kSZm % (kSB / 2) == 0, "batch divides the voxels
 template <bool ARC, int FMT, int ZF> constexpr int staged_waves()
 {
     constexpr int by_lds = (160 * 1024) / (2 * Lds<FMT, ZF>::buf * (int)Lds<FMT, ZF>::esz);
-    return ARC ? (FMT != F_F32 ? 4 : 3) : (SGX_STAGED_WAVES < by_lds ? SGX_STAGED_WAVES : by_lds);
+    constexpr int want = ARC ? (FMT != F_F32 ? SGX_STAGED_WAVES_ARC16 : SGX_STAGED_WAVES_ARC32)
+                             : SGX_STAGED_WAVES;
+    return want < by_lds ? want : by_lds;
 }
 
 // Window of one (block, view): entries (clo + c, rlo + r), c < W, r < H,
@@ -462,7 +479,7 @@ __global__ void cbct_window_kernel(int nblocks, int nviews, const ViewCS* __rest
     const int tile = blk % tiles, zblk = blk / tiles;
     // a mirror block's window is the one of its voxels z > 0
     const int zn = zform == kZ16 ? kSZ : kSZm;
-    const int hmax = (zform == kZ16 || zform == kZ8W) ? kHmax : kHm;
+    const int hmax = zform == kZ16 ? kH16 : zform == kZ8W ? kHmax : kHm;
     const int wmax = zform == kZ8W ? kWmaxW : kWmax;
     const int ix0 = (tile % tiles_x) * kSP, iy0 = (tile / tiles_x) * kSP;
     const int iz0 = zform == kZMirror ? nz / 2 + zblk * kSZm : zblk * zn;
