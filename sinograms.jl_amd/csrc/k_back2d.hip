@@ -65,7 +65,19 @@ namespace {
 constexpr int kTile = 16;
 constexpr int kThreads = kTile * kTile;
 constexpr int kLdsBytes = 48 * 1024;
-constexpr int kU = 8;           // views per unrolled step of the backprojector
+// Views per unrolled step of the backprojector, and a minimum of waves per
+// SIMD for the launch bound (1: none).  Measured on C1 (888 x 984 -> 512^2):
+// 8 views per step take 160 VGPRs (3 waves per SIMD), 0.188 ms; 4 views 125
+// VGPRs (4 waves, the most the 36 KB of LDS per block allows), 0.162 ms; 4
+// views held to 5 waves 96 VGPRs with 10 spilled, 0.166 ms; 2 views at 5
+// waves 0.176 ms.
+#ifndef SGX_B2D_U
+#define SGX_B2D_U 4
+#endif
+#ifndef SGX_B2D_WAVES
+#define SGX_B2D_WAVES 1
+#endif
+constexpr int kU = SGX_B2D_U;   // views per unrolled step of the backprojector
 constexpr int kWinMax = 128;    // staged window slot per view (bins), a power of two
 constexpr int kRunBins = 32;    // staging: bins per thread and run (loads in flight together)
 
@@ -486,7 +498,7 @@ __device__ __forceinline__ void back2d_tile(
 }
 
 template <int GEOM, int NT = 1>
-__global__ __launch_bounds__(kThreads) void back2d_kernel(
+__global__ __launch_bounds__(kThreads, SGX_B2D_WAVES) void back2d_kernel(
     const float* __restrict__ q, int nb, int nviews, int64_t view_stride,
     const ViewCS* __restrict__ cs, const float* __restrict__ xds,
     const float* __restrict__ yds, int nx, int ny, const uint8_t* __restrict__ mask,
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(kThreads) void back2d_kernel(
 // largest tasks first, interleaved over the XCDs); a task backprojects
 // every view of its tile (no view groups) into the item's image.
 template <int GEOM, int NT>
-__global__ __launch_bounds__(kThreads) void back2d_queue_kernel(
+__global__ __launch_bounds__(kThreads, SGX_B2D_WAVES) void back2d_queue_kernel(
     const Back2dItem* __restrict__ items, const int2* __restrict__ tasks)
 {
     extern __shared__ float lds[];
