@@ -538,9 +538,11 @@ void launch_mode(const Params& p, hipStream_t st)
 // the spectrum table.  Element M k1 + l + 32 h of lane l is element
 // l + 32 n2 with n2 = (M/32) k1 + h: exactly the layout step A takes, so the
 // inverse needs no extra data movement, and its output R[M j1 + 32 h + l]
-// lands at lane l: rows stored coalesced.  Step A is radix-2 decimation in
-// time (inputs in bit-reversed registers), step D decimation in frequency
-// (outputs in bit-reversed registers): renamings resolved at compile time.
+// lands at lane l: rows stored coalesced.  Both steps are radix-2
+// decimation in time, A with its inputs in bit-reversed registers, D on
+// registers renamed so that its natural input gives outputs in bit-reversed
+// registers: renamings resolved at compile time.  Each twiddled butterfly
+// takes the scaled form (6 VALU instead of 8; see bfly).
 //
 // Bound: VALU (about 680 wave instructions per column at M = 32, both
 // transforms and the weights) and HBM (each column read once and written once).
@@ -573,39 +575,64 @@ __host__ __device__ constexpr float tw64_cos(int t)
 }
 __host__ __device__ constexpr float tw64_sin(int t) { return t <= 16 ? tw64_cos(16 - t) : tw64_cos(t - 16); }
 
-// (re, im) *= W_M^t = e^(-2 pi i t / M), 0 <= t < M / 2
+// Radix-2 decimation-in-time butterfly (a, b) -> (a + W b, a - W b) with
+// W = W_M^t = c - i s, c = cos(2 pi t / M), s = sin(2 pi t / M), t known at
+// compile time.  A general twiddle takes the scaled (tangent) form: with
+// |c| >= |s|, W b = c (b_r + r b_i, b_i - r b_r), r = s / c, and the
+// outputs are a +- c (u, v): two FMAs for (u, v) and four for the outputs,
+// 6 VALU instead of 8 for the product and the two sums (with |s| > |c| the
+// same with s and r = c / s).
 template <int M>
-__device__ __forceinline__ void rot(float& re, float& im, int t)
+__device__ __forceinline__ void bfly(float& ar, float& ai, float& br, float& bi, int t)
 {
-    if (t == 0) return;
-    const float r = re, i = im;
-    if (4 * t == M) {                         // -i
-        re = i;
-        im = -r;
+    if (t == 0) {
+        const float r = br, i = bi;
+        br = ar - r;
+        bi = ai - i;
+        ar += r;
+        ai += i;
+    } else if (4 * t == M) {                  // W = -i: W b = (b_i, -b_r)
+        const float r = br, i = bi;
+        br = ar - i;
+        bi = ai + r;
+        ar += i;
+        ai -= r;
     } else {
         const float c = tw64_cos(t * (64 / M)), sn = tw64_sin(t * (64 / M));
-        re = fmaf(r, c, i * sn);
-        im = fmaf(i, c, -(r * sn));
+        const float r = br, i = bi;
+        float u, v, k;
+        if (fabsf(c) >= fabsf(sn)) {
+            const float q = sn / c;
+            u = fmaf(q, i, r);
+            v = fmaf(-q, r, i);
+            k = c;
+        } else {
+            const float q = c / sn;
+            u = fmaf(q, r, i);
+            v = fmaf(q, i, -r);
+            k = sn;
+        }
+        br = fmaf(-k, u, ar);
+        bi = fmaf(-k, v, ai);
+        ar = fmaf(k, u, ar);
+        ai = fmaf(k, v, ai);
     }
 }
 
-// 32-point forward DFT, radix-2 decimation in frequency, in place: natural
-// input order, X[k] ends in register bitrev<5>(k).
-__device__ __forceinline__ void dft32_dif(float* re, float* im)
+// 32-point forward DFT, radix-2 decimation in time on registers renamed at
+// compile time: natural input (element n in register n), X[k] ends in
+// register bitrev<5>(k) (position p of the textbook DIT lives in register
+// bitrev<5>(p)).
+__device__ __forceinline__ void dft32_dit_br(float* re, float* im)
 {
 #pragma unroll
-    for (int span = 16; span >= 1; span /= 2) {
+    for (int span = 1; span < 32; span *= 2) {
 #pragma unroll
         for (int g = 0; g < 32; g += 2 * span) {
 #pragma unroll
             for (int j = 0; j < span; ++j) {
-                const int a = g + j, b = a + span;
-                float dr = re[a] - re[b], di = im[a] - im[b];
-                re[a] += re[b];
-                im[a] += im[b];
-                rot<32>(dr, di, j * (16 / span));
-                re[b] = dr;
-                im[b] = di;
+                const int a = bitrev<5>(g + j), b = bitrev<5>(g + j + span);
+                bfly<32>(re[a], im[a], re[b], im[b], j * (32 / (2 * span)));
             }
         }
     }
@@ -629,12 +656,7 @@ __device__ __forceinline__ void dft_dit(float (&re)[M], float (&im)[M])
                     im[b] = im[a];
                     continue;
                 }
-                float br = re[b], bi = im[b];
-                rot<M>(br, bi, j * (M / (2 * span)));
-                re[b] = re[a] - br;
-                im[b] = im[a] - bi;
-                re[a] += br;
-                im[a] += bi;
+                bfly<M>(re[a], im[a], re[b], im[b], j * (M / (2 * span)));
             }
         }
     }
@@ -802,7 +824,7 @@ __global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params
     dft_dit<M, HALF>(re, im);                                   // A
     twiddle_transpose<M>(re, im, tw_re, tw_im, tb, l);          // B, C
 #pragma unroll
-    for (int h = 0; h < H; ++h) dft32_dif(re + 32 * h, im + 32 * h);   // D
+    for (int h = 0; h < H; ++h) dft32_dit_br(re + 32 * h, im + 32 * h);   // D
     // X[M k1 + l + 32 h] is in register 32 h + bitrev5(k1).  Spectrum (He / N)
     // and swap give the inverse's step-A input: element n2 = H k1 + h of lane
     // l, in register bitrev(n2).
@@ -820,7 +842,7 @@ __global__ __launch_bounds__(256, M == 64 ? 2 : 1) void fft_filter_kernel(Params
     dft_dit<M, false>(ar, ai);
     twiddle_transpose<M>(ar, ai, tw_re, tw_im, tb, l);
 #pragma unroll
-    for (int h = 0; h < H; ++h) dft32_dif(ar + 32 * h, ai + 32 * h);
+    for (int h = 0; h < H; ++h) dft32_dit_br(ar + 32 * h, ai + 32 * h);
     // R[M j1 + l + 32 h] in register 32 h + bitrev5(j1)
 
     // Output: column a is Im R, column b is Re R; rows nb .. nrows_out - 1 are 0.
