@@ -107,9 +107,19 @@ int sgx_ctx_synchronize(sgx_ctx* ctx);
  *                        1 = for scaled-FP16 plans (default; the autotuner
  *                            times it for the other forms),
  *                        2 = wherever the geometry allows, 0 = never.
+ *   SGX_OPT_VIEW_CHUNKS: 1 to 8 (default 1): a cone-beam plan execute with
+ *                        the staged backprojector (FP32 interpolation)
+ *                        filters and backprojects its views in this many
+ *                        chunks of at least 32 views; the filters of later
+ *                        chunks run on a stream of the plan beside the
+ *                        backprojections of earlier ones (in chunk order on
+ *                        the caller's stream under stream capture, with the
+ *                        same result).  The chunks' sums are added in chunk
+ *                        order; 1 = one filter launch, then one
+ *                        backprojection over all views.
  * Plans take the context's choices when they are created (and an autotuned
  * kernel after sgx_plan_autotune). */
-enum { SGX_OPT_CBCT_KERNEL = 1, SGX_OPT_CBCT_MIRROR = 2 };
+enum { SGX_OPT_CBCT_KERNEL = 1, SGX_OPT_CBCT_MIRROR = 2, SGX_OPT_VIEW_CHUNKS = 3 };
 int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value);
 int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value);
 

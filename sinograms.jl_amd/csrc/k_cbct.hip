@@ -560,7 +560,7 @@ __device__ __forceinline__ void cbct_staged_block(
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
-    int arc_t_exact, int narrow, float so_ds, int blk, char* lds)
+    int arc_t_exact, int narrow, float so_ds, int blk, char* lds, int win_stride, int accumulate)
 {
     using L = Lds<FMT, ZF>;
     constexpr bool MIR = ZF == kZMirror;
@@ -611,7 +611,9 @@ __device__ __forceinline__ void cbct_staged_block(
     const uint32_t vbytes = (uint32_t)nt * rowb + Fmt<FMT>::pad;
     const int64_t viewb = view_stride * (int64_t)Fmt<FMT>::esz;
     const char* qb = static_cast<const char*>(q);
-    const int4* bwin = win + (int64_t)blk * nviews;
+    // the block's row of the window table (win_stride views per block; a
+    // launch over a chunk of the views starts at its first view's entry)
+    const int4* bwin = win + (int64_t)blk * win_stride;
 
     // Staging role: thread (quad sq, group sg) moves bins clo+4sq .. +3 of
     // entry rows 2sg, 2sg+1 of its window, i.e. data rows 2sg .. 2sg+2 of it.
@@ -834,12 +836,16 @@ __device__ __forceinline__ void cbct_staged_block(
     }
 
     // Masked-out voxels are written as 0 (a select on the prologue's bits:
-    // no second mask read inside a branch).
+    // no second mask read inside a branch).  A launch over a later chunk of
+    // the views adds its sum to the image.
     if (inxy) {
 #pragma unroll
         for (int j = 0; j < NV; ++j) {
             const int iz = vox(j);
-            if (iz < nz) image[col + nxy * iz] = ((inbits >> j) & 1u) ? acc[j] : 0.f;
+            if (iz >= nz) continue;
+            float* o = image + col + nxy * iz;
+            const float v = accumulate ? *o + acc[j] : acc[j];
+            *o = ((inbits >> j) & 1u) ? v : 0.f;
         }
     }
 }
@@ -853,14 +859,14 @@ cbct_staged_kernel(
     const float* __restrict__ zds, int nx, int ny, int nz, const uint8_t* __restrict__ mask,
     float* __restrict__ image, float dsd_ds, float dso_ds, float ws, float wt,
     float inv_dt_ds, const int4* __restrict__ win, const float* __restrict__ zs,
-    int arc_t_exact, int narrow, float so_ds)
+    int arc_t_exact, int narrow, float so_ds, int win_stride, int accumulate)
 {
     using L = Lds<FMT, ZF>;
     __shared__ __attribute__((aligned(16))) char lds[2 * L::buf * L::esz];
     cbct_staged_block<ARC, FMT, OPT, ZF>(q, ns, nt, pitch, nviews, view_stride, cs, vscale, xds,
                                           yds, zds, nx, ny, nz, mask, image, dsd_ds, dso_ds, ws,
                                           wt, inv_dt_ds, win, zs, arc_t_exact, narrow, so_ds,
-                                          blockIdx.x, lds);
+                                          blockIdx.x, lds, win_stride, accumulate);
 }
 
 // Batch form: one launch over the tasks (item, voxel block) of many
@@ -884,7 +890,7 @@ cbct_queue_kernel(const CbctItem* __restrict__ items, const int2* __restrict__ t
                                           it.zds, it.nx, it.ny, it.nz, it.mask, it.image,
                                           it.dsd_ds, it.dso_ds, it.ws, it.wt, it.inv_dt_ds,
                                           it.win, it.zs, it.arc_t_exact, it.narrow, it.so_ds,
-                                          tk.y, lds);
+                                          tk.y, lds, it.nviews, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1259,7 +1265,8 @@ void launch_staged(const CbctArgs& a, hipStream_t st)
                        a.q, a.ns, a.nt, a.pitch, a.nviews, a.view_stride, a.cs, a.vscale, a.xds,
                        a.yds, a.zds, a.nx, a.ny, a.nz, a.mask, a.image, a.dsd_ds, a.dso_ds,
                        a.ws, a.wt, a.inv_dt_ds, a.win, a.zs, a.arc_t_exact ? 1 : 0,
-                       arc_narrow(a), a.so_ds);
+                       arc_narrow(a), a.so_ds, a.win_stride > 0 ? a.win_stride : a.nviews,
+                       a.accumulate ? 1 : 0);
     SGX_HIP(hipGetLastError());
 }
 

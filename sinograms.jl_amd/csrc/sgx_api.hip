@@ -68,6 +68,15 @@ struct PlanTune {
 
 // Largest view-grouping target and smallest LDS chunk the autotuner tries.
 constexpr int kTuneMaxBlocks = 4096;
+// View chunks of a cone-beam plan execute (SGX_OPT_VIEW_CHUNKS): the default,
+// the most, and the fewest views a chunk may hold.  One chunk by default: on
+// the 256^2 x 128 FDK workload (360 views) 2, 3 and 4 chunks took 1.27-1.30 ms
+// per FP32 execute against 1.17 ms (the scaled-FP16 form 0.97-1.02 against
+// 0.98 ms): the overlapped filters do not make up for the extra
+// backprojection launches.
+constexpr int kViewChunks = 1;
+constexpr int kMaxViewChunks = 8;
+constexpr int kMinChunkViews = 32;
 constexpr int kTuneMinChunk = 16;
 
 struct sgx_ctx {
@@ -75,6 +84,7 @@ struct sgx_ctx {
     std::map<std::string, PlanTune> tune_cache;   // autotuned shapes per size bucket
     int cbct_kernel = 0;          // SGX_OPT_CBCT_KERNEL
     int cbct_mirror = 1;          // SGX_OPT_CBCT_MIRROR
+    int view_chunks = kViewChunks;  // SGX_OPT_VIEW_CHUNKS
     std::string scratch_err;      // why the device scratch is unusable (empty: usable)
     hipStream_t stream = nullptr;
     bool own_stream = false;
@@ -391,8 +401,15 @@ int sgx_ctx_synchronize(sgx_ctx* ctx)
 int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
 {
     return guard(ctx, [&] {
-        require(option == SGX_OPT_CBCT_KERNEL || option == SGX_OPT_CBCT_MIRROR, SGX_ERR_ARG,
-                "unknown option");
+        require(option == SGX_OPT_CBCT_KERNEL || option == SGX_OPT_CBCT_MIRROR ||
+                    option == SGX_OPT_VIEW_CHUNKS,
+                SGX_ERR_ARG, "unknown option");
+        if (option == SGX_OPT_VIEW_CHUNKS) {
+            require(value >= 1 && value <= kMaxViewChunks, SGX_ERR_ARG,
+                    "SGX_OPT_VIEW_CHUNKS is 1 to 8");
+            ctx->view_chunks = (int)value;
+            return;
+        }
         if (option == SGX_OPT_CBCT_MIRROR) {
             require(value >= 0 && value <= 2, SGX_ERR_ARG, "SGX_OPT_CBCT_MIRROR is 0, 1 or 2");
             ctx->cbct_mirror = (int)value;
@@ -405,9 +422,13 @@ int sgx_ctx_set_option(sgx_ctx* ctx, int option, int64_t value)
 
 int sgx_ctx_get_option(const sgx_ctx* ctx, int option, int64_t* value)
 {
-    if (!ctx || !value || (option != SGX_OPT_CBCT_KERNEL && option != SGX_OPT_CBCT_MIRROR))
+    if (!ctx || !value ||
+        (option != SGX_OPT_CBCT_KERNEL && option != SGX_OPT_CBCT_MIRROR &&
+         option != SGX_OPT_VIEW_CHUNKS))
         return SGX_ERR_ARG;
-    *value = option == SGX_OPT_CBCT_MIRROR ? ctx->cbct_mirror : ctx->cbct_kernel;
+    *value = option == SGX_OPT_CBCT_MIRROR  ? ctx->cbct_mirror
+             : option == SGX_OPT_VIEW_CHUNKS ? ctx->view_chunks
+                                             : ctx->cbct_kernel;
     return SGX_OK;
 }
 
@@ -903,6 +924,12 @@ struct sgx_plan {
     int64_t win_cap = 0;            // entries `win` holds
     hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
     bool timed = false;
+    // cone beam: view chunks of an execute (SGX_OPT_VIEW_CHUNKS at creation),
+    // and the side stream and events that run later chunks' filters beside
+    // earlier chunks' backprojections (made on first use)
+    int view_chunks = 1;
+    hipStream_t side = nullptr;
+    hipEvent_t pev[kMaxViewChunks] = {};
     // Bumped by every change that a captured batch graph would not see
     // (options, view weights, kernel shapes): batches recapture on a change.
     uint64_t gen = 0;
@@ -911,6 +938,9 @@ struct sgx_plan {
     {
         for (hipEvent_t& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (hipEvent_t& e : pev)
+            if (e) (void)hipEventDestroy(e);
+        if (side) (void)hipStreamDestroy(side);
         if (wview_owned) (void)hipFree(wview_owned);
         if (ptab) (void)hipFree(ptab);
         if (zs) (void)hipFree(zs);
@@ -1073,8 +1103,105 @@ sgx::Back2dArgs back2d_run_args(const sgx_plan& P, float* img)
     return a;
 }
 
+// The views [v0, v1) of a filter launch (pointers offset to view v0).
+sgx::FilterArgs filter_chunk(const sgx_plan& P, sgx::FilterArgs f, int64_t v0, int64_t v1)
+{
+    const int64_t nt = P.geom.nt, nb = P.geom.ns;
+    const size_t esz = P.prec == SGX_PREC_FP32 ? 4 : 2;
+    f.in += v0 * nt * nb;
+    f.out = static_cast<char*>(f.out) + v0 * nt * (int64_t)P.pitch * esz;
+    f.ncols = nt * (v1 - v0);
+    if (f.wview) f.wview += v0;
+    if (f.ptab) f.ptab += v0 * nb;
+    if (f.vscale) f.vscale += v0;
+    if (f.vbound) f.vbound += v0;
+    return f;
+}
+
+// Number of view chunks of a cone-beam execute: the plan's setting, for the
+// staged backprojector's FP32-interpolation forms, at least kMinChunkViews
+// views a chunk; 1 otherwise.
+int view_chunks_of(const sgx_plan& P, const sgx::CbctArgs& a)
+{
+    if (!P.is3d() || P.tune.cbct != 0 || !a.win || a.f16_packed || P.view_chunks <= 1) return 1;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(P.view_chunks, P.geom.na / kMinChunkViews));
+}
+
+// Cone-beam execute in view chunks: chunk k's views are filtered, then
+// backprojected, the first backprojection storing and the others adding to
+// the image (the same sums in the same order whatever runs beside them).
+// Outside stream capture, the filters of chunks 1.. run on the plan's side
+// stream, each backprojection waiting for its own chunk's filter, so that
+// they overlap the backprojections of earlier chunks; the side stream starts
+// after the work queued on `st` before the call (which includes earlier
+// executes' reads of the plan's buffers) and `st` has joined it at the end.
+// Under capture every launch goes on `st` in chunk order.
+void plan_run_chunked(sgx_plan& P, const float* in, float* img, hipStream_t st, bool timed,
+                      int K)
+{
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    SGX_HIP(hipStreamIsCapturing(st, &cap));
+    const bool overlap = cap == hipStreamCaptureStatusNone;
+    if (overlap && !P.side) {
+        SGX_HIP(hipStreamCreateWithFlags(&P.side, hipStreamNonBlocking));
+        for (hipEvent_t& e : P.pev) SGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    const int64_t na = P.geom.na;
+    auto v_at = [&](int k) { return na * k / K; };
+    const sgx::FilterArgs fa = P.filter_args(in, P.q);
+    auto filter = [&](int k, hipStream_t s) {
+        const sgx::FilterArgs f = filter_chunk(P, fa, v_at(k), v_at(k + 1));
+        if (P.prec == SGX_PREC_F16S) sgx::launch_filter_bound(f, P.vbound + v_at(k), s);
+        sgx::launch_sino_filter(f, s);
+    };
+    if (timed) SGX_HIP(hipEventRecord(P.ev[0], st));
+    if (overlap) {
+        SGX_HIP(hipEventRecord(P.pev[0], st));
+        SGX_HIP(hipStreamWaitEvent(P.side, P.pev[0], 0));
+    }
+    filter(0, st);
+    if (timed) SGX_HIP(hipEventRecord(P.ev[1], st));
+    if (overlap) {
+        for (int k = 1; k < K; ++k) {
+            filter(k, P.side);
+            SGX_HIP(hipEventRecord(P.pev[k], P.side));
+        }
+    }
+    const sgx::CbctArgs a = cbct_args_of(P, img);
+    const size_t esz = P.prec == SGX_PREC_FP32 ? 4 : 2;
+    for (int k = 0; k < K; ++k) {
+        if (k > 0) {
+            if (overlap) SGX_HIP(hipStreamWaitEvent(st, P.pev[k], 0));
+            else filter(k, st);
+        }
+        const int64_t v0 = v_at(k);
+        sgx::CbctArgs c = a;
+        c.q = static_cast<const char*>(a.q) + v0 * a.view_stride * (int64_t)esz;
+        c.cs += v0;
+        if (c.vscale) c.vscale += v0;
+        if (c.zs) c.zs += v0;
+        c.win += v0;
+        c.win_stride = (int)na;
+        c.nviews = (int)(v_at(k + 1) - v0);
+        c.accumulate = k > 0;
+        sgx::launch_cbct_back(c, st);
+    }
+    if (timed) {
+        SGX_HIP(hipEventRecord(P.ev[2], st));
+        P.timed = true;
+    }
+}
+
 void plan_run(sgx_plan& P, const float* in, float* img, hipStream_t st, bool timed)
 {
+    if (P.is3d()) {
+        ensure_windows(P, st);
+        const int K = view_chunks_of(P, cbct_args_of(P, img));
+        if (K > 1) {
+            plan_run_chunked(P, in, img, st, timed, K);
+            return;
+        }
+    }
     if (timed) SGX_HIP(hipEventRecord(P.ev[0], st));
     plan_filter_phase(P, in, st);
     if (timed) SGX_HIP(hipEventRecord(P.ev[1], st));
@@ -1224,6 +1351,7 @@ int sgx_plan_create(sgx_ctx* ctx, const sgx_geom* geom, const double* betas,
         // the mirror form: by default for the scaled-FP16 form, where it
         // measured faster; the autotuner times it for the others
         P->mirror_allowed = ctx->cbct_mirror != 0;
+        P->view_chunks = ctx->view_chunks;
         P->tune.zform =
             ctx->cbct_mirror == 2 || (ctx->cbct_mirror == 1 && prec == SGX_PREC_F16S) ? 1
             : ctx->cbct_kernel == 3                                                     ? 2

@@ -122,6 +122,12 @@ struct CbctArgs {
     // (pairs z and -z; see cbct_mirror_ok), 2 = 8 z per lane, 3 = 8 z per
     // lane with windows of up to 64 bins x 32 rows (coarse voxels)
     int zform = 0;
+    // a staged launch over a chunk of a plan's views (q, cs, vscale, zs and
+    // win already offset to its first view): the window table's views per
+    // block (0: nviews), and whether the launch adds its sum to image
+    // instead of storing it (the chunks after the first)
+    int win_stride = 0;
+    bool accumulate = false;
 };
 void launch_cbct_back(const CbctArgs& a, hipStream_t st);
 // Whether the staged kernel may take its mirror form for `a`: the z axis

@@ -186,6 +186,7 @@ def _declare(L):
 
 SGX_OPT_CBCT_KERNEL = 1
 SGX_OPT_CBCT_MIRROR = 2
+SGX_OPT_VIEW_CHUNKS = 3
 CBCT_MIRROR = ("never", "fp16", "always")    # values of SGX_OPT_CBCT_MIRROR
 # values of SGX_OPT_CBCT_KERNEL
 CBCT_KERNELS = ("staged", "gather", "staged_packed16", "staged_z8", "staged_z8w")
@@ -285,6 +286,19 @@ class Context:
     def cbct_mirror(self, when: str):
         self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_CBCT_MIRROR,
                                              CBCT_MIRROR.index(when)))
+
+    @property
+    def view_chunks(self) -> int:
+        """View chunks of a cone-beam plan execute (1 to 8, default 1): the
+        filters of later chunks run beside the backprojections of earlier
+        ones; 1 runs one filter launch, then one backprojection."""
+        v = ctypes.c_int64()
+        self.check(load().sgx_ctx_get_option(self.h, SGX_OPT_VIEW_CHUNKS, ctypes.byref(v)))
+        return v.value
+
+    @view_chunks.setter
+    def view_chunks(self, k: int):
+        self.check(load().sgx_ctx_set_option(self.h, SGX_OPT_VIEW_CHUNKS, int(k)))
 
     def close(self):
         if self.h:

@@ -132,6 +132,59 @@ def test_cbct_back_mirror_vs_oracle(sg, orc, torch_cuda, arc, case):
         assert np.any(ref == 0)          # voxels that see no detector row in some view
 
 
+@pytest.mark.parametrize("prec", ["fp32", "f16s"])
+def test_fdk_view_chunks(sg, orc, torch_cuda, prec):
+    # A cone-beam execute in view chunks (the filters of later chunks on the
+    # plan's side stream, beside the backprojections of earlier chunks)
+    # against one filter launch and one backprojection: per-chunk sums added
+    # in chunk order, so equal to FP32 rounding; repeated device executes,
+    # one with its input produced on another stream, equal the first bit for
+    # bit; and the chunked result stays within the FP32 bar of the oracle.
+    import torch
+    _say("fdk view chunks", prec)
+    rg = sg.CtFanFlat(ns=256, nt=48, ds=1.2, dt=1.2, na=100, dsd=600.0, dod=200.0)
+    ig = sg.ImageGeom((48, 44, 32), (1.0, 1.0, 1.0))
+    ell = sg.phantom.random_ellipsoids(61, 8, 48.0)
+    proj = sg.phantom.proj_ellipsoids(ell, rg).numpy().astype(np.float32)
+    dev = _colmajor(torch_cuda, proj)
+    ctx = sg._lib.context(0)
+    vols, plans = {}, {}
+    before = ctx.view_chunks
+    try:
+        for k in (1, 2, 3):
+            ctx.view_chunks = k
+            plans[k] = sg.plan_fbp(rg, ig, precision=prec)
+            vols[k] = sg.fdk(plans[k], proj)
+    finally:
+        ctx.view_chunks = before
+    for k in (2, 3):
+        e = rel_rmse(vols[k], vols[1])
+        _say(f"  {k} chunks vs 1: rel-RMSE {e:.3g}")
+        assert e < 1e-6, (k, e)
+    plan = plans[3]
+    outs = [plan.execute(dev) for _ in range(3)]
+    for o in outs:
+        assert np.array_equal(o.cpu().numpy(), vols[3])
+    side = torch.cuda.Stream()
+    late = torch.zeros_like(dev)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(side):
+        torch.cuda._sleep(20_000_000)          # keeps `side` busy for several ms
+        late.copy_(dev)
+        got = plan.execute(late).cpu().numpy()
+    assert np.array_equal(got, vols[3])
+    if prec == "fp32":
+        ar = rg.ar(f32=True).astype(np.float32)
+        vw = orc.fdk_weight_cyl(rg.s(), rg.t(), rg.dsd, rg.dso, False)[:, :, None] * \
+            orc.angle_weight(ar)
+        xc, yc, zc = ig.axes()
+        ref = orc.fdk(proj, plan.filter, vw, ar, rg.dsd, rg.dso, rg.ds, rg.dt, 0.0, 0.0, False,
+                      xc, yc, zc, ig.mask, dtype=np.float32)
+        e = rel_rmse(vols[3], ref)
+        _say(f"  3 chunks vs Float32 oracle: rel-RMSE {e:.3g}")
+        assert e < TOL32
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "f16s"])
 @pytest.mark.parametrize("voxels", ["fine", "coarse"])
 def test_fdk_block_forms_bit_identical(sg, orc, torch_cuda, prec, voxels):
